@@ -1,0 +1,127 @@
+/* dc_hip.h — C ABI of the MI355X dual-contouring backend (libdc_hip.so).
+ *
+ * This is the drop-in boundary for the reference's dense GPU backend. Each entry point names the
+ * reference interface it replaces (file:line in the Rizzlord/Phanty-SDF-Cuda tree). Plain
+ * pointers and sizes only; every call returns a dc_hip_status and never exits the process.
+ *
+ * Semantics follow the reference CPU backend (CpuDualContouringBackend, src/cpp/dc_backend.cpp:
+ * 51-125): active cells are the cells with a strictly negative and a strictly positive corner;
+ * vertex ids are their rank in (k, j, i) order; quads are emitted per axis in (i, j, k) order
+ * with k fastest, oriented by normalized central differences; vertices are the truncated-SVD
+ * QEF solution about the mass point with 4-cell averaged trilinear hermite normals.
+ */
+#ifndef DC_HIP_H
+#define DC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dc_hip_status {
+  DC_HIP_OK = 0,
+  DC_HIP_ERR_NO_DEVICE = 1,      /* no HIP device, or the device index is out of range */
+  DC_HIP_ERR_INVALID_ARG = 2,    /* NULL pointer, unknown option or precision */
+  DC_HIP_ERR_INVALID_GRID = 3,   /* nx, ny or nz < 2, or sizes beyond the 32-bit index limits */
+  DC_HIP_ERR_NONFINITE = 4,      /* a sample is NaN or +-Inf */
+  DC_HIP_ERR_OUT_OF_MEMORY = 5,  /* device allocation failed */
+  DC_HIP_ERR_HIP = 6,            /* a HIP runtime call failed (see dc_hip_last_error) */
+  DC_HIP_ERR_NO_FORWARD = 7,     /* backward or result query without a successful forward */
+  DC_HIP_ERR_INTERNAL = 8        /* a device-side consistency check failed */
+} dc_hip_status;
+
+typedef enum dc_hip_precision {
+  /* FP64 hermite data and solve, the CPU backend's arithmetic; FP64 vertices available. */
+  DC_HIP_PLAIN_FP64 = 0,
+  /* FP32 assembly in mass-point-centred coordinates with compensated sums; cells whose condition
+   * estimate or truncation margin is poor are solved again in FP64. */
+  DC_HIP_FP32_FALLBACK = 1
+} dc_hip_precision;
+
+/* Replaces DenseSdfGrid (include/dc_backend.h:13-21) and DenseSdfGridDevice
+ * (include/dc_cuda.h:24-29): values[i + nx*(j + ny*k)], sample (i,j,k) at
+ * (ox + i*vx, oy + j*vy, oz + k*vz) evaluated in FP32. */
+typedef struct dc_hip_grid {
+  int32_t nx, ny, nz;
+  float vx, vy, vz;
+  float ox, oy, oz;
+  const float* values;
+} dc_hip_grid;
+
+/* Mirrors the DualContouringStats fields the dense backends fill (include/dc_backend.h:44-78,
+ * src/cuda/dc_cuda.cu:828-844), plus the per-pass timings of this backend. Times are device
+ * (hipEvent) milliseconds except total_ms, which is host wall time of the call. */
+typedef struct dc_hip_stats {
+  int32_t nx, ny, nz;
+  int64_t total_cells;
+  int64_t active_cells;
+  float upload_ms;
+  float marking_ms;        /* k_mark: active + face bit planes */
+  float compaction_ms;     /* active-cell scan */
+  float face_prefix_sum_ms;/* pencil scan */
+  float qef_ms;            /* vertex solve */
+  float face_fill_ms;      /* quad emission */
+  float face_emission_ms;  /* face_prefix_sum_ms + face_fill_ms */
+  float download_ms;
+  float total_ms;
+  float backward_ms;
+  int64_t vertex_count;
+  int64_t face_count;
+  int64_t qef_fallback_count; /* FP32 mode: cells solved again in FP64 */
+  int64_t invalid_count;
+} dc_hip_stats;
+
+typedef struct dc_hip_context dc_hip_context;
+
+/* Creates a context on HIP device `device` with its own stream. */
+int dc_hip_create(int device, dc_hip_context** out);
+void dc_hip_destroy(dc_hip_context* ctx);
+/* Runs all later work on `stream` (a hipStream_t; NULL restores the context's own stream). */
+int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
+/* Options: "mark_rows" (2, 4, 8), "mark_waves" (1, 2, 4), "timing" (0/1, per-pass events),
+ * "qef_blocks" (persistent blocks of the solve). */
+int dc_hip_set_option(dc_hip_context* ctx, const char* key, int64_t value);
+
+/* Replaces CudaDualContouringBackend::extract (src/cuda/dc_cuda.cu:614-649): host values are
+ * uploaded, extracted, and the mesh stays in the context (copy it with dc_hip_copy_result). */
+int dc_hip_extract(dc_hip_context* ctx, const dc_hip_grid* grid, int precision,
+                   dc_hip_stats* stats);
+/* Replaces CudaDualContouringBackend::extract_device (src/cuda/dc_cuda.cu:651-847): values is a
+ * device pointer, read on the context's stream; the call returns when the mesh is complete. */
+int dc_hip_extract_device(dc_hip_context* ctx, const dc_hip_grid* grid, int precision,
+                          dc_hip_stats* stats);
+/* Counts of the last successful extraction (single grid or whole batch). */
+int dc_hip_result_counts(dc_hip_context* ctx, int64_t* n_vertices, int64_t* n_faces);
+/* Copies the last result (any pointer may be NULL; host or device memory). vertices:
+ * 3*n_vertices floats; vertices_f64: 3*n_vertices doubles (plain FP64 mode only); faces:
+ * 4*n_faces int32. Returns when the copies are complete. */
+int dc_hip_copy_result(dc_hip_context* ctx, float* vertices, double* vertices_f64,
+                       int32_t* faces);
+/* Device pointers of the last result, owned by the context and valid until its next call. */
+int dc_hip_result_device(dc_hip_context* ctx, const float** vertices,
+                         const double** vertices_f64, const int32_t** faces);
+
+/* Batched extraction of n_grids device-resident grids of any sizes in one pipeline (no
+ * counterpart in the reference, which extracts one grid per call). Results are concatenated
+ * grid-major; per-grid vertex and face offsets (n_grids + 1 entries each, may be NULL) give the
+ * ranges; face vertex ids are local to their grid. */
+int dc_hip_extract_batch(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t n_grids,
+                         int precision, int64_t* vertex_offsets, int64_t* face_offsets,
+                         dc_hip_stats* stats);
+
+/* Adjoint of the last extraction at fixed topology (no counterpart in the reference):
+ * grad_values[N_s] (device, overwritten) = dL/dvalues for grad_vertices[3*n_vertices]
+ * (device) = dL/dvertices. Deterministic: per-cell stencil contributions are gathered in a
+ * fixed order, no floating-point atomics. */
+int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad_values,
+                    dc_hip_stats* stats);
+
+const char* dc_hip_last_error(dc_hip_context* ctx);
+const char* dc_hip_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DC_HIP_H */

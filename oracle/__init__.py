@@ -1,0 +1,119 @@
+"""ctypes access to the CPU restatement of the reference CpuDualContouringBackend.
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package. It checks the product; it is never the product.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "build", "liboracle_dc.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"oracle not built: {LIB} (run make -C oracle)")
+    lib = ctypes.CDLL(LIB)
+    P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    lib.oracle_dc_run.restype = P
+    lib.oracle_dc_run.argtypes = [I, I, I, F, F, F, F, F, F, P]
+    lib.oracle_dc_run_f64.restype = P
+    lib.oracle_dc_run_f64.argtypes = [I, I, I, F, F, F, F, F, F, P]
+    lib.oracle_dc_free.argtypes = [P]
+    lib.oracle_dc_counts.argtypes = [P, P, P, P]
+    lib.oracle_dc_copy.argtypes = [P, P, P, P, P, P]
+    lib.oracle_dc_hermite.argtypes = [P, I, P, P, P]
+    lib.oracle_dc_backward.argtypes = [P, P, P]
+    lib.oracle_sym3_eig.argtypes = [P, P, P]
+    lib.oracle_qef_solve.argtypes = [I, P, P, P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Result:
+    """Outputs of one oracle extraction (the handle is kept for hermite data and the adjoint)."""
+
+    def __init__(self, handle, dims):
+        self._h = handle
+        self.dims = dims
+        lib = load()
+        nv, nf, navg = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        lib.oracle_dc_counts(handle, ctypes.byref(nv), ctypes.byref(nf), ctypes.byref(navg))
+        self.n_vertices, self.n_faces, self.averaged_edges = nv.value, nf.value, navg.value
+        self.V32 = np.empty((self.n_vertices, 3), np.float32)
+        self.V64 = np.empty((self.n_vertices, 3), np.float64)
+        self.F = np.empty((self.n_faces, 4), np.int32)
+        self.active = np.empty(self.n_vertices, np.int64)
+        self.dots = np.empty(self.n_faces, np.float64)
+        lib.oracle_dc_copy(handle, _p(self.V32), _p(self.V64), _p(self.F), _p(self.active),
+                           _p(self.dots))
+
+    def hermite(self, averaged=True):
+        n = self.n_vertices
+        mask = np.empty(n, np.uint16)
+        pts = np.empty((n, 12, 3), np.float64)
+        nrm = np.empty((n, 12, 3), np.float64)
+        load().oracle_dc_hermite(self._h, 1 if averaged else 0, _p(mask), _p(pts), _p(nrm))
+        return mask, pts, nrm
+
+    def backward(self, gV):
+        gV = np.ascontiguousarray(gV, dtype=np.float64).reshape(-1, 3)
+        assert gV.shape[0] == self.n_vertices
+        nx, ny, nz = self.dims
+        out = np.empty(nx * ny * nz, np.float64)
+        load().oracle_dc_backward(self._h, _p(gV), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().oracle_dc_free(self._h)
+            self._h = None
+
+
+def run(values, nx, ny, nz, vx, vy, vz, ox=0.0, oy=0.0, oz=0.0, f64=False):
+    lib = load()
+    if f64:
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        h = lib.oracle_dc_run_f64(nx, ny, nz, vx, vy, vz, ox, oy, oz, _p(v))
+    else:
+        v = np.ascontiguousarray(values, dtype=np.float32).ravel()
+        h = lib.oracle_dc_run(nx, ny, nz, vx, vy, vz, ox, oy, oz, _p(v))
+    if not h:
+        raise ValueError("oracle rejected the grid")
+    return Result(h, (nx, ny, nz))
+
+
+def run_grid(grid, f64=False):
+    return run(grid.values, grid.nx, grid.ny, grid.nz, grid.vx, grid.vy, grid.vz, grid.ox,
+               grid.oy, grid.oz, f64=f64)
+
+
+def sym3_eig(M):
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    lam = np.empty(3, np.float64)
+    Q = np.empty((3, 3), np.float64)
+    load().oracle_sym3_eig(_p(M), _p(lam), _p(Q))
+    return lam, Q
+
+
+def qef_solve(normals, points):
+    n = np.ascontiguousarray(normals, dtype=np.float64)
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    x = np.empty(3, np.float64)
+    load().oracle_qef_solve(n.shape[0], _p(n), _p(p), _p(x))
+    return x

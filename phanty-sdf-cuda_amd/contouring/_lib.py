@@ -1,0 +1,115 @@
+"""ctypes binding of libdc_hip.so (include/dc_hip.h) and libdc_synth.so.
+
+The product path has no fallback: if the HIP library is missing, loading raises.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(os.path.dirname(_PKG), "build")
+
+DC_HIP_OK = 0
+DC_HIP_ERR_NO_DEVICE = 1
+DC_HIP_ERR_INVALID_ARG = 2
+DC_HIP_ERR_INVALID_GRID = 3
+DC_HIP_ERR_NONFINITE = 4
+DC_HIP_ERR_OUT_OF_MEMORY = 5
+DC_HIP_ERR_HIP = 6
+DC_HIP_ERR_NO_FORWARD = 7
+DC_HIP_ERR_INTERNAL = 8
+
+DC_HIP_PLAIN_FP64 = 0
+DC_HIP_FP32_FALLBACK = 1
+
+# Every symbol include/dc_hip.h declares; the CPU tests check the library exports all of them.
+EXPORTS = (
+    "dc_hip_create", "dc_hip_destroy", "dc_hip_set_stream", "dc_hip_set_option",
+    "dc_hip_extract", "dc_hip_extract_device", "dc_hip_result_counts", "dc_hip_copy_result",
+    "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_last_error",
+    "dc_hip_status_string",
+)
+
+
+class DcHipGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("vx", ctypes.c_float), ("vy", ctypes.c_float), ("vz", ctypes.c_float),
+                ("ox", ctypes.c_float), ("oy", ctypes.c_float), ("oz", ctypes.c_float),
+                ("values", ctypes.c_void_p)]
+
+
+class DcHipStats(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("total_cells", ctypes.c_int64), ("active_cells", ctypes.c_int64),
+                ("upload_ms", ctypes.c_float), ("marking_ms", ctypes.c_float),
+                ("compaction_ms", ctypes.c_float), ("face_prefix_sum_ms", ctypes.c_float),
+                ("qef_ms", ctypes.c_float), ("face_fill_ms", ctypes.c_float),
+                ("face_emission_ms", ctypes.c_float), ("download_ms", ctypes.c_float),
+                ("total_ms", ctypes.c_float), ("backward_ms", ctypes.c_float),
+                ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
+                ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64)]
+
+
+_hip = None
+_synth = None
+
+
+def lib_path():
+    return os.path.join(BUILD_DIR, "libdc_hip.so")
+
+
+def load_hip():
+    """Load libdc_hip.so; raises RuntimeError when the HIP extension has not been built."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP extension not built: {path} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    sig = {
+        "dc_hip_create": (I, [I, ctypes.POINTER(P)]),
+        "dc_hip_destroy": (None, [P]),
+        "dc_hip_set_stream": (I, [P, P]),
+        "dc_hip_set_option": (I, [P, ctypes.c_char_p, I64]),
+        "dc_hip_extract": (I, [P, ctypes.POINTER(DcHipGrid), I, ctypes.POINTER(DcHipStats)]),
+        "dc_hip_extract_device": (I, [P, ctypes.POINTER(DcHipGrid), I,
+                                      ctypes.POINTER(DcHipStats)]),
+        "dc_hip_result_counts": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "dc_hip_copy_result": (I, [P, P, P, P]),
+        "dc_hip_result_device": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "dc_hip_extract_batch": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, P, P,
+                                     ctypes.POINTER(DcHipStats)]),
+        "dc_hip_backward": (I, [P, P, P, ctypes.POINTER(DcHipStats)]),
+        "dc_hip_last_error": (ctypes.c_char_p, [P]),
+        "dc_hip_status_string": (ctypes.c_char_p, [I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _hip = lib
+    return lib
+
+
+def load_synth():
+    global _synth
+    if _synth is not None:
+        return _synth
+    path = os.path.join(BUILD_DIR, "libdc_synth.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"synth library not built: {path} is missing")
+    lib = ctypes.CDLL(path)
+    lib.dc_synth_csg.restype = ctypes.c_int
+    lib.dc_synth_csg.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    lib.dc_synth_cli.restype = ctypes.c_int
+    lib.dc_synth_cli.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    _synth = lib
+    return lib
+
+
+class DcHipError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"dc_hip error {status}: {message}")
+        self.status = status

@@ -1,0 +1,273 @@
+"""Python mirror of the reference backend classes, on top of the C ABI (include/dc_hip.h).
+
+Reference interface (file:line in the reference tree):
+  DenseSdfGrid, DualContouringMesh, DualContouringStats   include/dc_backend.h:13-78,
+                                                          src/cpp/bindings.cpp:135-185
+  <Backend>().extract(grid) -> (mesh, stats)              src/cpp/bindings.cpp:186-222
+  DenseSdfGridDevice, extract_device(grid)                include/dc_cuda.h:24-35,
+                                                          src/cpp/bindings.cpp:204-227
+HipDualContouringBackend takes the place of CudaDualContouringBackend. There is no CPU fallback:
+without the HIP library or a device every call raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {"plain-fp64": _lib.DC_HIP_PLAIN_FP64, "fp32-fallback": _lib.DC_HIP_FP32_FALLBACK}
+
+
+class DenseSdfGrid:
+    """include/dc_backend.h:13-21. values: nx*ny*nz floats, index i + nx*(j + ny*k)."""
+
+    def __init__(self, nx=0, ny=0, nz=0, vx=0.0, vy=0.0, vz=0.0, ox=0.0, oy=0.0, oz=0.0,
+                 values=None):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.vx, self.vy, self.vz = vx, vy, vz
+        self.ox, self.oy, self.oz = ox, oy, oz
+        self.values = values if values is not None else []
+
+
+class DenseSdfGridDevice:
+    """include/dc_cuda.h:24-29: d_values is a device pointer (int) or a float32 torch tensor."""
+
+    def __init__(self, nx=0, ny=0, nz=0, vx=0.0, vy=0.0, vz=0.0, ox=0.0, oy=0.0, oz=0.0,
+                 d_values=None):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.vx, self.vy, self.vz = vx, vy, vz
+        self.ox, self.oy, self.oz = ox, oy, oz
+        self.d_values = d_values
+
+
+class DualContouringMesh:
+    """include/dc_backend.h:23-26: vertices (3 floats per vertex), faces (4 ints per quad)."""
+
+    def __init__(self, vertices=None, faces=None, vertices_f64=None):
+        self.vertices = vertices if vertices is not None else np.zeros(0, np.float32)
+        self.faces = faces if faces is not None else np.zeros(0, np.int32)
+        self.vertices_f64 = vertices_f64
+
+
+_STATS_FIELDS = (
+    "backend", "nx", "ny", "nz", "total_cells", "active_cells", "total_bricks", "active_bricks",
+    "upload_ms", "marking_ms", "compaction_ms", "active_brick_marking_ms",
+    "active_brick_compaction_ms", "active_cell_marking_ms", "active_cell_compaction_ms", "qef_ms",
+    "face_emission_ms", "face_count_ms", "face_prefix_sum_ms", "face_fill_ms", "download_ms",
+    "total_ms", "vertex_count", "face_count", "qef_fallback_count", "clamp_count", "invalid_count",
+    "ambiguous_cells", "multi_vertex_cells", "one_vertex_cells", "two_vertex_cells",
+    "split_rejection_count", "bad_qef_count", "faces_skipped_due_to_missing_cluster",
+)
+
+
+class DualContouringStats:
+    """include/dc_backend.h:44-78 (every field, zero-initialised) plus backward_ms."""
+
+    def __init__(self):
+        for f in _STATS_FIELDS:
+            setattr(self, f, "" if f == "backend" else 0)
+        self.backward_ms = 0.0
+
+    @classmethod
+    def from_c(cls, s, backend="hip"):
+        st = cls()
+        st.backend = backend
+        for f in ("nx", "ny", "nz", "total_cells", "active_cells", "upload_ms", "marking_ms",
+                  "compaction_ms", "face_prefix_sum_ms", "qef_ms", "face_fill_ms",
+                  "face_emission_ms", "download_ms", "total_ms", "backward_ms", "vertex_count",
+                  "face_count", "qef_fallback_count", "invalid_count"):
+            setattr(st, f, getattr(s, f))
+        st.active_cell_marking_ms = s.marking_ms
+        st.active_cell_compaction_ms = s.compaction_ms
+        return st
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in _STATS_FIELDS + ("backward_ms",)}
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor or an int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class HipDualContouringBackend:
+    """MI355X dense dual contouring (replaces CudaDualContouringBackend, dc_cuda.h:31-35).
+
+    precision: "plain-fp64" (the CPU backend's arithmetic, FP64 vertices available) or
+    "fp32-fallback" (FP32 assembly, FP64 only for ill-conditioned cells).
+    """
+
+    def __init__(self, precision="plain-fp64", device=0, mark_rows=None, mark_waves=None):
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
+        self.device = device
+        self._lib = _lib.load_hip()
+        ctx = ctypes.c_void_p()
+        rc = self._lib.dc_hip_create(device, ctypes.byref(ctx))
+        if rc != _lib.DC_HIP_OK:
+            raise _lib.DcHipError(rc, self._lib.dc_hip_status_string(rc).decode())
+        self._ctx = ctx
+        self.generation = 0
+        if mark_rows is not None:
+            self.set_option("mark_rows", mark_rows)
+        if mark_waves is not None:
+            self.set_option("mark_waves", mark_waves)
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            self._lib.dc_hip_destroy(ctx)
+            self._ctx = None
+
+    # -- plumbing ---------------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != _lib.DC_HIP_OK:
+            msg = self._lib.dc_hip_last_error(self._ctx).decode()
+            raise _lib.DcHipError(rc, msg or self._lib.dc_hip_status_string(rc).decode())
+
+    def set_option(self, key, value):
+        self._check(self._lib.dc_hip_set_option(self._ctx, key.encode(), int(value)))
+
+    def set_stream(self, stream_handle):
+        self._check(self._lib.dc_hip_set_stream(self._ctx, ctypes.c_void_p(stream_handle or 0)))
+
+    def _use_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    @staticmethod
+    def _cgrid(g, ptr):
+        return _lib.DcHipGrid(int(g.nx), int(g.ny), int(g.nz), float(g.vx), float(g.vy),
+                              float(g.vz), float(g.ox), float(g.oy), float(g.oz),
+                              ctypes.c_void_p(ptr))
+
+    def counts(self):
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.dc_hip_result_counts(self._ctx, ctypes.byref(nv), ctypes.byref(nf)))
+        return nv.value, nf.value
+
+    def _download(self, with_f64):
+        nv, nf = self.counts()
+        V = np.empty(3 * nv, np.float32)
+        F = np.empty(4 * nf, np.int32)
+        V64 = np.empty(3 * nv, np.float64) if with_f64 else None
+        self._check(self._lib.dc_hip_copy_result(
+            self._ctx, V.ctypes.data_as(ctypes.c_void_p),
+            V64.ctypes.data_as(ctypes.c_void_p) if with_f64 else None,
+            F.ctypes.data_as(ctypes.c_void_p)))
+        return DualContouringMesh(V, F, V64)
+
+    # -- reference API ------------------------------------------------------------------------
+    def extract(self, grid, with_f64=None):
+        """Host grid -> (DualContouringMesh, DualContouringStats) (dc_cuda.cu:614-649)."""
+        import time
+        vals = np.ascontiguousarray(np.asarray(grid.values, dtype=np.float32))
+        if vals.size != int(grid.nx) * int(grid.ny) * int(grid.nz):
+            raise ValueError("values size does not match nx*ny*nz")
+        cg = self._cgrid(grid, vals.ctypes.data)
+        st = _lib.DcHipStats()
+        self.set_stream(None)
+        self._check(self._lib.dc_hip_extract(self._ctx, ctypes.byref(cg),
+                                             PRECISIONS[self.precision], ctypes.byref(st)))
+        self.generation += 1
+        t0 = time.perf_counter()
+        if with_f64 is None:
+            with_f64 = self.precision == "plain-fp64"
+        mesh = self._download(with_f64)
+        stats = DualContouringStats.from_c(st)
+        stats.download_ms = (time.perf_counter() - t0) * 1e3
+        stats.total_ms += stats.download_ms
+        return mesh, stats
+
+    def extract_device(self, grid, with_f64=None):
+        """Device grid -> host mesh (dc_cuda.cu:651-847). grid: DenseSdfGridDevice."""
+        import time
+        st = _lib.DcHipStats()
+        cg = self._cgrid(grid, _ptr(grid.d_values))
+        if not isinstance(grid.d_values, int):
+            self._use_torch_stream()
+        self._check(self._lib.dc_hip_extract_device(self._ctx, ctypes.byref(cg),
+                                                    PRECISIONS[self.precision], ctypes.byref(st)))
+        self.generation += 1
+        t0 = time.perf_counter()
+        if with_f64 is None:
+            with_f64 = self.precision == "plain-fp64"
+        mesh = self._download(with_f64)
+        stats = DualContouringStats.from_c(st)
+        stats.download_ms = (time.perf_counter() - t0) * 1e3
+        stats.total_ms += stats.download_ms
+        return mesh, stats
+
+    # -- tensor API (device in, device out) ---------------------------------------------------
+    def extract_tensor(self, phi, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
+                       with_f64=False):
+        """phi: contiguous float32 CUDA tensor of shape (nz, ny, nx). Returns (V [A,3] float32,
+        F [F,4] int32[, V64 [A,3] float64]) as new CUDA tensors, plus the stats."""
+        import torch
+        if phi.dtype != torch.float32 or not phi.is_cuda or phi.dim() != 3:
+            raise ValueError("phi must be a float32 CUDA tensor of shape (nz, ny, nx)")
+        phi = phi.contiguous()
+        nz, ny, nx = phi.shape
+        g = DenseSdfGridDevice(nx, ny, nz, *spacing, *origin, d_values=phi)
+        self._use_torch_stream()
+        st = _lib.DcHipStats()
+        cg = self._cgrid(g, phi.data_ptr())
+        self._check(self._lib.dc_hip_extract_device(self._ctx, ctypes.byref(cg),
+                                                    PRECISIONS[self.precision], ctypes.byref(st)))
+        self.generation += 1
+        nv, nf = self.counts()
+        V = torch.empty((nv, 3), dtype=torch.float32, device=phi.device)
+        F = torch.empty((nf, 4), dtype=torch.int32, device=phi.device)
+        V64 = torch.empty((nv, 3), dtype=torch.float64, device=phi.device) if with_f64 else None
+        self._check(self._lib.dc_hip_copy_result(
+            self._ctx, ctypes.c_void_p(V.data_ptr() if nv else 0),
+            ctypes.c_void_p(V64.data_ptr() if (with_f64 and nv) else 0),
+            ctypes.c_void_p(F.data_ptr() if nf else 0)))
+        out = (V, F, V64) if with_f64 else (V, F)
+        return out, DualContouringStats.from_c(st)
+
+    def extract_batch(self, grids, with_f64=False):
+        """List of DenseSdfGridDevice (tensor-backed) -> list of (V, F) CUDA tensor pairs."""
+        import torch
+        n = len(grids)
+        arr = (_lib.DcHipGrid * n)(*[self._cgrid(g, _ptr(g.d_values)) for g in grids])
+        voff = np.zeros(n + 1, np.int64)
+        foff = np.zeros(n + 1, np.int64)
+        st = _lib.DcHipStats()
+        self._use_torch_stream()
+        self._check(self._lib.dc_hip_extract_batch(
+            self._ctx, arr, n, PRECISIONS[self.precision], voff.ctypes.data_as(ctypes.c_void_p),
+            foff.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
+        self.generation += 1
+        nv, nf = self.counts()
+        dev = torch.device("cuda", self.device)
+        V = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        F = torch.empty((nf, 4), dtype=torch.int32, device=dev)
+        V64 = torch.empty((nv, 3), dtype=torch.float64, device=dev) if with_f64 else None
+        self._check(self._lib.dc_hip_copy_result(
+            self._ctx, ctypes.c_void_p(V.data_ptr() if nv else 0),
+            ctypes.c_void_p(V64.data_ptr() if (with_f64 and nv) else 0),
+            ctypes.c_void_p(F.data_ptr() if nf else 0)))
+        out = []
+        for q in range(n):
+            a, b = int(voff[q]), int(voff[q + 1])
+            c, d = int(foff[q]), int(foff[q + 1])
+            item = (V[a:b], F[c:d]) + ((V64[a:b],) if with_f64 else ())
+            out.append(item)
+        return out, DualContouringStats.from_c(st)
+
+    def backward(self, grad_vertices, grad_phi):
+        """dL/dphi (written into grad_phi, float32 CUDA) for dL/dV of the last forward."""
+        import torch
+        self._use_torch_stream()
+        st = _lib.DcHipStats()
+        gv = grad_vertices.contiguous().to(torch.float32)
+        self._check(self._lib.dc_hip_backward(self._ctx, ctypes.c_void_p(gv.data_ptr()),
+                                              ctypes.c_void_p(grad_phi.data_ptr()),
+                                              ctypes.byref(st)))
+        return DualContouringStats.from_c(st)

@@ -1,0 +1,543 @@
+// dc_hip.cpp — C ABI of the MI355X dual-contouring backend: context, HBM buffers, pipeline.
+//
+// One extraction (or one batch) is a fixed sequence of launches on the context's stream with a
+// single host synchronization at the end (the counters readback). Output sizes are data
+// dependent: buffers are sized from the previous call (or a fraction of the cell count), every
+// kernel bounds its writes by the capacity it was given, and a run whose counts exceed the
+// capacities is repeated once with exact capacities.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dc_hip.h"
+#include "dc_internal.h"
+#include "dc_kernels.h"
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+constexpr int kNumEvents = 8;
+
+}  // namespace
+
+struct dc_hip_context {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int mark_rows = 4;
+  int mark_waves = 4;
+  bool timing = true;
+  int64_t qef_blocks = 0;
+  int cu_count = 256;
+  DevBuf grids, bounds, act, ft, pcount, pbase, wprefix, alist, v64, v32, faces, status, tilectr,
+      counters, upload, flags, bwd_contrib, bwd_grad;
+  DcCounters* h_counters = nullptr;
+  hipEvent_t ev[kNumEvents] = {};
+  // last forward
+  bool have = false;
+  int precision = 0;
+  int64_t nv = 0, nf = 0;
+  uint64_t capA = 0, capF = 0;
+  std::vector<DcGridDev> hgrids;
+  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, hbounds;
+  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0;
+};
+
+namespace {
+
+int fail(dc_hip_context* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(dc_hip_context* c, hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory)
+    return fail(c, DC_HIP_ERR_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
+  return fail(c, DC_HIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define DC_TRY(ctx, expr, what)                  \
+  do {                                           \
+    hipError_t e_ = (expr);                      \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+  } while (0)
+
+int ensure(dc_hip_context* c, DevBuf& b, size_t bytes, const char* name) {
+  if (bytes == 0) bytes = 256;
+  if (b.cap >= bytes) return DC_HIP_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    if (e != hipSuccess) return hip_fail(c, e, name);
+  }
+  const size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return hip_fail(c, e, name);
+  }
+  b.cap = want;
+  return DC_HIP_OK;
+}
+
+template <typename T>
+T* as(DevBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+// Layout of a batch; returns an error for degenerate or oversized grids.
+int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
+  c->hgrids.resize(n);
+  c->tile_b.assign(n + 1, 0);
+  c->act_b.assign(n + 1, 0);
+  c->pen_b.assign(n + 1, 0);
+  c->samp_b.assign(n + 1, 0);
+  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0;
+  for (int q = 0; q < n; ++q) {
+    const dc_hip_grid& s = grids[q];
+    if (s.nx < 2 || s.ny < 2 || s.nz < 2)
+      return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimensions must be >= 2");
+    if (s.nx > 65536 || s.ny > 65536 || s.nz > 65536)
+      return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimension above 65536");
+    if (!s.values) return fail(c, DC_HIP_ERR_INVALID_ARG, "grid values pointer is NULL");
+    DcGridDev& g = c->hgrids[q];
+    g.val = s.values;
+    g.nx = s.nx;
+    g.ny = s.ny;
+    g.nz = s.nz;
+    g.cx = s.nx - 1;
+    g.cy = s.ny - 1;
+    g.cz = s.nz - 1;
+    g.W = (g.cx + 63) / 64;
+    g.Wk = (g.cz + 63) / 64;
+    g.vx = s.vx;
+    g.vy = s.vy;
+    g.vz = s.vz;
+    g.ox = s.ox;
+    g.oy = s.oy;
+    g.oz = s.oz;
+    g.act_off = words;
+    g.ft_off = ftw;
+    g.pen_off = pen;
+    g.samp_off = samp;
+    c->tile_b[q] = tiles;
+    c->act_b[q] = words;
+    c->pen_b[q] = pen;
+    c->samp_b[q] = samp;
+    words += (int64_t)g.W * g.cy * g.cz;
+    ftw += 3ll * g.Wk * g.cy * g.cx;
+    pen += 3ll * g.cy * g.cx;
+    tiles += dc_mark_tiles(g, c->mark_rows);
+    samp += (int64_t)g.nx * g.ny * g.nz;
+  }
+  c->tile_b[n] = tiles;
+  c->act_b[n] = words;
+  c->pen_b[n] = pen;
+  c->samp_b[n] = samp;
+  if ((uint64_t)words * 64ull >= (1ull << 32) * 4ull)
+    return fail(c, DC_HIP_ERR_INVALID_GRID, "batch has too many cells for 32-bit vertex ids");
+  c->n_words = words;
+  c->n_ft = ftw;
+  c->n_pen = pen;
+  c->n_tiles = tiles;
+  c->n_samples = samp;
+  return DC_HIP_OK;
+}
+
+float elapsed(dc_hip_context* c, int a, int b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// The launch sequence. Returns DC_HIP_OK after the counters have been read back.
+int run_once(dc_hip_context* c, int precision) {
+  hipStream_t s = c->stream;
+  const int n = (int)c->hgrids.size();
+  int rc;
+  if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
+  if ((rc = ensure(c, c->bounds, 4 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
+  if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
+  if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
+  if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
+  if ((rc = ensure(c, c->pcount, c->n_pen * 4, "pencil counts"))) return rc;
+  if ((rc = ensure(c, c->pbase, c->n_pen * 4, "pencil bases"))) return rc;
+  const int64_t st_a = dc_scan_tiles(c->n_words), st_p = dc_scan_tiles(c->n_pen);
+  if ((rc = ensure(c, c->status, (st_a + st_p) * 8, "scan status"))) return rc;
+  if ((rc = ensure(c, c->tilectr, 64, "scan tile counters"))) return rc;
+  if ((rc = ensure(c, c->counters, sizeof(DcCounters), "counters"))) return rc;
+  if ((rc = ensure(c, c->alist, c->capA * 8, "active list"))) return rc;
+  if ((rc = ensure(c, c->v64, c->capA * 24, "fp64 vertices"))) return rc;
+  if ((rc = ensure(c, c->v32, c->capA * 12, "vertices"))) return rc;
+  if ((rc = ensure(c, c->flags, c->capA * 4, "vertex flags"))) return rc;
+  if ((rc = ensure(c, c->faces, c->capF * 16, "faces"))) return rc;
+
+  // host staging stays alive in the context until the copies have run
+  std::vector<int64_t>& bounds = c->hbounds;
+  bounds.clear();
+  bounds.insert(bounds.end(), c->tile_b.begin(), c->tile_b.end());
+  bounds.insert(bounds.end(), c->act_b.begin(), c->act_b.end());
+  bounds.insert(bounds.end(), c->pen_b.begin(), c->pen_b.end());
+  bounds.insert(bounds.end(), c->samp_b.begin(), c->samp_b.end());
+  DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
+                           hipMemcpyHostToDevice, s), "upload descriptors");
+  DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, s), "upload bounds");
+  const int64_t* d_tile_b = as<int64_t>(c->bounds);
+  const int64_t* d_act_b = d_tile_b + (n + 1);
+  const int64_t* d_pen_b = d_act_b + (n + 1);
+  const DcGridDev* d_grids = as<DcGridDev>(c->grids);
+  DcCounters* d_ctr = as<DcCounters>(c->counters);
+  uint64_t* st = as<uint64_t>(c->status);
+  uint32_t* tc = as<uint32_t>(c->tilectr);
+
+  // clears: kernels rather than memset nodes, so the sequence can be captured in a graph
+  DC_TRY(c, dc_launch_fill_u32(as<uint32_t>(c->pcount), c->n_pen, 0u, s), "clear pencils");
+  DC_TRY(c, dc_launch_fill_u64(st, st_a + st_p, 0ull, s), "clear scan status");
+  DC_TRY(c, dc_launch_fill_u32(tc, 16, 0u, s), "clear tile counters");
+  DC_TRY(c, dc_launch_fill_u32(reinterpret_cast<uint32_t*>(d_ctr),
+                               (int64_t)(sizeof(DcCounters) / 4), 0u, s), "clear counters");
+
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[0], s), "event");
+  DcMarkArgs ma;
+  ma.grids = d_grids;
+  ma.n_grids = n;
+  ma.tile_bounds = d_tile_b;
+  ma.n_tiles = c->n_tiles;
+  ma.rows = c->mark_rows;
+  ma.waves_per_block = c->mark_waves;
+  ma.act = as<uint64_t>(c->act);
+  ma.ft = as<uint64_t>(c->ft);
+  ma.pcount = as<uint32_t>(c->pcount);
+  ma.ctr = d_ctr;
+  DC_TRY(c, dc_launch_mark(ma, s), "mark");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
+  DC_TRY(c, dc_launch_scan_act(as<uint64_t>(c->act), c->n_words, as<uint32_t>(c->wprefix),
+                               as<uint64_t>(c->alist), c->capA, st, tc, d_ctr, s),
+         "active scan");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[2], s), "event");
+  DC_TRY(c, dc_launch_scan_pen(d_grids, n, d_pen_b, c->n_pen, as<uint32_t>(c->pcount),
+                               as<uint32_t>(c->pbase), st + st_a, tc + 1, d_ctr, s),
+         "pencil scan");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[3], s), "event");
+  DcQefArgs qa;
+  qa.grids = d_grids;
+  qa.n_grids = n;
+  qa.act_bounds = d_act_b;
+  qa.act = as<uint64_t>(c->act);
+  qa.wprefix = as<uint32_t>(c->wprefix);
+  qa.alist = as<uint64_t>(c->alist);
+  qa.ctr = d_ctr;
+  qa.capA = c->capA;
+  qa.blocks = c->qef_blocks > 0 ? c->qef_blocks : (int64_t)c->cu_count * 8;
+  qa.precision = precision;
+  qa.v64 = as<double>(c->v64);
+  qa.v32 = as<float>(c->v32);
+  qa.flags = as<uint32_t>(c->flags);
+  qa.ctr_rw = d_ctr;
+  DC_TRY(c, dc_launch_qef(qa, s), "qef");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[4], s), "event");
+  DcEmitArgs ea;
+  ea.grids = d_grids;
+  ea.n_grids = n;
+  ea.pen_bounds = d_pen_b;
+  ea.n_pen = c->n_pen;
+  ea.act = as<uint64_t>(c->act);
+  ea.wprefix = as<uint32_t>(c->wprefix);
+  ea.ft = as<uint64_t>(c->ft);
+  ea.pbase = as<uint32_t>(c->pbase);
+  ea.v64 = as<double>(c->v64);
+  ea.v32 = as<float>(c->v32);
+  ea.precision = precision;
+  ea.ctr = d_ctr;
+  ea.capF = c->capF;
+  ea.faces = as<int32_t>(c->faces);
+  ea.capA = c->capA;
+  DC_TRY(c, dc_launch_emit(ea, s), "emit");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
+  DC_TRY(c, hipMemcpyAsync(c->h_counters, d_ctr, sizeof(DcCounters), hipMemcpyDeviceToHost, s),
+         "read counters");
+  DC_TRY(c, hipStreamSynchronize(s), "pipeline");
+  return DC_HIP_OK;
+}
+
+int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precision,
+                 dc_hip_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  c->have = false;
+  if (precision != DC_HIP_PLAIN_FP64 && precision != DC_HIP_FP32_FALLBACK)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "unknown precision");
+  int rc = plan(c, grids, n);
+  if (rc) return rc;
+  int64_t cells = 0;
+  for (const auto& g : c->hgrids) cells += (int64_t)g.cx * g.cy * g.cz;
+  if (c->capA == 0) {
+    c->capA = (uint64_t)std::min<int64_t>(cells, std::max<int64_t>(1 << 14, cells / 16));
+    c->capF = c->capA * 2;
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rc = run_once(c, precision);
+    if (rc) return rc;
+    const DcCounters& h = *c->h_counters;
+    if (h.error) return fail(c, DC_HIP_ERR_INTERNAL, "device scan wait limit exceeded");
+    if (h.nonfinite) return fail(c, DC_HIP_ERR_NONFINITE, "grid contains NaN or Inf samples");
+    if (h.n_active >= (1ull << 31) || h.n_faces >= (1ull << 31))
+      return fail(c, DC_HIP_ERR_INVALID_GRID, "mesh exceeds 32-bit vertex or face ids");
+    if (h.n_active <= c->capA && h.n_faces <= c->capF) {
+      c->nv = (int64_t)h.n_active;
+      c->nf = (int64_t)h.n_faces;
+      c->precision = precision;
+      c->have = true;
+      break;
+    }
+    c->capA = std::max<uint64_t>(c->capA, h.n_active + h.n_active / 16 + 64);
+    c->capF = std::max<uint64_t>(c->capF, h.n_faces + h.n_faces / 16 + 64);
+  }
+  if (!c->have) return fail(c, DC_HIP_ERR_INTERNAL, "output capacity not reached after resize");
+  if (stats) {
+    dc_hip_stats& st = *stats;
+    float up = st.upload_ms;
+    std::memset(&st, 0, sizeof(st));
+    st.upload_ms = up;
+    st.nx = c->hgrids[0].nx;
+    st.ny = c->hgrids[0].ny;
+    st.nz = c->hgrids[0].nz;
+    st.total_cells = cells;
+    st.active_cells = c->nv;
+    st.vertex_count = c->nv;
+    st.face_count = c->nf;
+    st.qef_fallback_count = (int64_t)c->h_counters->fallback;
+    if (c->timing) {
+      st.marking_ms = elapsed(c, 0, 1);
+      st.compaction_ms = elapsed(c, 1, 2);
+      st.face_prefix_sum_ms = elapsed(c, 2, 3);
+      st.qef_ms = elapsed(c, 3, 4);
+      st.face_fill_ms = elapsed(c, 4, 5);
+      st.face_emission_ms = st.face_prefix_sum_ms + st.face_fill_ms;
+    }
+    st.total_ms =
+        std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() +
+        up;
+  }
+  return DC_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dc_hip_status_string(int status) {
+  switch (status) {
+    case DC_HIP_OK: return "ok";
+    case DC_HIP_ERR_NO_DEVICE: return "no HIP device";
+    case DC_HIP_ERR_INVALID_ARG: return "invalid argument";
+    case DC_HIP_ERR_INVALID_GRID: return "invalid grid";
+    case DC_HIP_ERR_NONFINITE: return "non-finite sample";
+    case DC_HIP_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case DC_HIP_ERR_HIP: return "HIP runtime error";
+    case DC_HIP_ERR_NO_FORWARD: return "no forward result";
+    case DC_HIP_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+int dc_hip_create(int device, dc_hip_context** out) {
+  if (!out) return DC_HIP_ERR_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DC_HIP_ERR_NO_DEVICE;
+  if (device < 0 || device >= count) return DC_HIP_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return DC_HIP_ERR_NO_DEVICE;
+  dc_hip_context* c = new dc_hip_context();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->cu_count = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return DC_HIP_ERR_HIP;
+  }
+  c->stream = c->own;
+  for (int q = 0; q < kNumEvents; ++q) {
+    if (hipEventCreate(&c->ev[q]) != hipSuccess) {
+      dc_hip_destroy(c);
+      return DC_HIP_ERR_HIP;
+    }
+  }
+  if (hipHostMalloc((void**)&c->h_counters, sizeof(DcCounters), hipHostMallocDefault) !=
+      hipSuccess) {
+    dc_hip_destroy(c);
+    return DC_HIP_ERR_OUT_OF_MEMORY;
+  }
+  std::memset(c->h_counters, 0, sizeof(DcCounters));
+  *out = c;
+  return DC_HIP_OK;
+}
+
+void dc_hip_destroy(dc_hip_context* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
+                    &c->alist, &c->v64, &c->v32, &c->faces, &c->status, &c->tilectr,
+                    &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad};
+  for (DevBuf* b : bufs)
+    if (b->p) hipFree(b->p);
+  for (int q = 0; q < kNumEvents; ++q)
+    if (c->ev[q]) hipEventDestroy(c->ev[q]);
+  if (c->h_counters) hipHostFree(c->h_counters);
+  if (c->own) hipStreamDestroy(c->own);
+  delete c;
+}
+
+int dc_hip_set_stream(dc_hip_context* c, void* stream) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
+  return DC_HIP_OK;
+}
+
+int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
+  if (!c || !key) return DC_HIP_ERR_INVALID_ARG;
+  const std::string k(key);
+  if (k == "mark_rows") {
+    if (value != 2 && value != 4 && value != 8)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_rows must be 2, 4 or 8");
+    c->mark_rows = (int)value;
+  } else if (k == "mark_waves") {
+    if (value != 1 && value != 2 && value != 4)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_waves must be 1, 2 or 4");
+    c->mark_waves = (int)value;
+  } else if (k == "timing") {
+    c->timing = value != 0;
+  } else if (k == "qef_blocks") {
+    if (value < 0) return fail(c, DC_HIP_ERR_INVALID_ARG, "qef_blocks must be >= 0");
+    c->qef_blocks = value;
+  } else {
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "unknown option " + k);
+  }
+  return DC_HIP_OK;
+}
+
+int dc_hip_extract_device(dc_hip_context* c, const dc_hip_grid* grid, int precision,
+                          dc_hip_stats* stats) {
+  if (!c || !grid) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (stats) stats->upload_ms = 0.f;
+  return run_pipeline(c, grid, 1, precision, stats);
+}
+
+int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
+                   dc_hip_stats* stats) {
+  if (!c || !grid || !grid->values) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2)
+    return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimensions must be >= 2");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  const size_t bytes = (size_t)grid->nx * grid->ny * grid->nz * sizeof(float);
+  int rc = ensure(c, c->upload, bytes, "upload buffer");
+  if (rc) return rc;
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], c->stream), "event");
+  DC_TRY(c, hipMemcpyAsync(c->upload.p, grid->values, bytes, hipMemcpyHostToDevice, c->stream),
+         "upload");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], c->stream), "event");
+  DC_TRY(c, hipStreamSynchronize(c->stream), "upload");
+  dc_hip_grid dg = *grid;
+  dg.values = as<float>(c->upload);
+  if (stats) stats->upload_ms = c->timing ? elapsed(c, 6, 7) : 0.f;
+  return run_pipeline(c, &dg, 1, precision, stats);
+}
+
+int dc_hip_extract_batch(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_grids,
+                         int precision, int64_t* vertex_offsets, int64_t* face_offsets,
+                         dc_hip_stats* stats) {
+  if (!c || !grids || n_grids <= 0) return fail(c, DC_HIP_ERR_INVALID_ARG, "bad batch");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (stats) stats->upload_ms = 0.f;
+  int rc = run_pipeline(c, grids, n_grids, precision, stats);
+  if (rc) return rc;
+  if (vertex_offsets || face_offsets) {
+    // per-grid offsets: wprefix at each grid's first act word, pbase at its first pencil
+    std::vector<uint32_t> vw(n_grids), fw(n_grids);
+    for (int q = 0; q < n_grids; ++q) {
+      DC_TRY(c, hipMemcpyAsync(&vw[q], as<uint32_t>(c->wprefix) + c->act_b[q], 4,
+                               hipMemcpyDeviceToHost, c->stream), "offsets");
+      DC_TRY(c, hipMemcpyAsync(&fw[q], as<uint32_t>(c->pbase) + c->pen_b[q], 4,
+                               hipMemcpyDeviceToHost, c->stream), "offsets");
+    }
+    DC_TRY(c, hipStreamSynchronize(c->stream), "offsets");
+    for (int q = 0; q < n_grids; ++q) {
+      if (vertex_offsets) vertex_offsets[q] = vw[q];
+      if (face_offsets) face_offsets[q] = fw[q];
+    }
+    if (vertex_offsets) vertex_offsets[n_grids] = c->nv;
+    if (face_offsets) face_offsets[n_grids] = c->nf;
+  }
+  return DC_HIP_OK;
+}
+
+int dc_hip_result_counts(dc_hip_context* c, int64_t* nv, int64_t* nf) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+  if (nv) *nv = c->nv;
+  if (nf) *nf = c->nf;
+  return DC_HIP_OK;
+}
+
+int dc_hip_copy_result(dc_hip_context* c, float* vertices, double* vertices_f64, int32_t* faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  // hipMemcpyDefault: destinations may be host or device memory (unified addressing)
+  if (vertices && c->nv)
+    DC_TRY(c, hipMemcpyAsync(vertices, c->v32.p, c->nv * 12, hipMemcpyDefault, c->stream),
+           "copy vertices");
+  if (vertices_f64 && c->nv) {
+    if (c->precision != DC_HIP_PLAIN_FP64)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "FP64 vertices exist only in plain FP64 mode");
+    DC_TRY(c, hipMemcpyAsync(vertices_f64, c->v64.p, c->nv * 24, hipMemcpyDefault, c->stream),
+           "copy fp64 vertices");
+  }
+  if (faces && c->nf)
+    DC_TRY(c, hipMemcpyAsync(faces, c->faces.p, c->nf * 16, hipMemcpyDefault, c->stream),
+           "copy faces");
+  DC_TRY(c, hipStreamSynchronize(c->stream), "download");
+  return DC_HIP_OK;
+}
+
+int dc_hip_result_device(dc_hip_context* c, const float** vertices, const double** vertices_f64,
+                         const int32_t** faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+  if (vertices) *vertices = as<float>(c->v32);
+  if (vertices_f64) *vertices_f64 = (c->precision == DC_HIP_PLAIN_FP64) ? as<double>(c->v64) : nullptr;
+  if (faces) *faces = as<int32_t>(c->faces);
+  return DC_HIP_OK;
+}
+
+int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_values,
+                    dc_hip_stats* stats) {
+  (void)grad_vertices;
+  (void)grad_values;
+  (void)stats;
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+  return fail(c, DC_HIP_ERR_INTERNAL, "backward not available");
+}
+
+const char* dc_hip_last_error(dc_hip_context* c) {
+  if (!c) return "no context";
+  return c->err.c_str();
+}
+
+}  // extern "C"

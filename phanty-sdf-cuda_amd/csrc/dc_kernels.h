@@ -1,0 +1,69 @@
+// dc_kernels.h — host launchers for the kernels in dc_kernels.hip (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dc_internal.h"
+
+struct DcMarkArgs {
+  const DcGridDev* grids;
+  int n_grids;
+  const int64_t* tile_bounds;  // n_grids + 1
+  int64_t n_tiles;
+  int rows;             // cell rows per wave tile: 2, 4 or 8
+  int waves_per_block;  // 1, 2 or 4
+  uint64_t* act;
+  uint64_t* ft;
+  uint32_t* pcount;
+  DcCounters* ctr;
+};
+
+struct DcQefArgs {
+  const DcGridDev* grids;
+  int n_grids;
+  const int64_t* act_bounds;  // n_grids + 1, in act words
+  const uint64_t* act;
+  const uint32_t* wprefix;
+  const uint64_t* alist;
+  const DcCounters* ctr;
+  uint64_t capA;
+  int64_t blocks;
+  int precision;
+  double* v64;
+  float* v32;
+  uint32_t* flags;  // per vertex, FP32 mode: 1 when the cell was solved in FP64
+  DcCounters* ctr_rw;
+};
+
+struct DcEmitArgs {
+  const DcGridDev* grids;
+  int n_grids;
+  const int64_t* pen_bounds;
+  int64_t n_pen;
+  const uint64_t* act;
+  const uint32_t* wprefix;
+  const uint64_t* ft;
+  const uint32_t* pbase;
+  const double* v64;
+  const float* v32;
+  int precision;
+  const DcCounters* ctr;
+  uint64_t capA;
+  uint64_t capF;
+  int32_t* faces;
+};
+
+hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
+hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+int64_t dc_mark_tiles(const DcGridDev& g, int rows);
+hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s);
+int64_t dc_scan_tiles(int64_t n);
+hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* wprefix,
+                              uint64_t* alist, uint64_t cap, uint64_t* status, uint32_t* tile_ctr,
+                              DcCounters* ctr, hipStream_t s);
+hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t* pen_bounds,
+                              int64_t n_pen, const uint32_t* pcount, uint32_t* pbase,
+                              uint64_t* status, uint32_t* tile_ctr, DcCounters* ctr,
+                              hipStream_t s);
+hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
+hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);

@@ -1,0 +1,894 @@
+// dc_kernels.hip — CDNA4 (gfx950) kernels of the dense dual-contouring path.
+//
+// Pipeline per batch of grids (a single extraction is a batch of one):
+//   k_mark      every sample read once: active-cell bit plane + per-axis face bit planes + pencil
+//               face counts. Wave64 tile = 64 cells along x x J cell rows x 64 cell layers; the
+//               one-sample halo in x comes from a lane shuffle (lane 63 loads one extra sample),
+//               the halo in y/z from the next row/layer already in registers.
+//   k_scan_act  decoupled look-back scan over the active plane: per-word vertex-id prefix and the
+//               active-cell list in vertex order.
+//   k_scan_pen  decoupled look-back scan over pencil face counts in the CPU backend's face
+//               order (axis, i, j): first face index of every pencil.
+//   k_qef64     one thread per active cell: hermite samples, trilinear normals, 4-cell normal
+//               averaging, QEF solve (FP64, cyclic Jacobi) -> vertex.
+//   k_emit      one thread per pencil: walks the face bits along k and writes oriented quads in
+//               the CPU backend's order.
+//
+// Reference semantics (file:line in the reference tree) are cited at each device function.
+// Compiled with -ffp-contract=off so that FP64 results match the CPU restatement bit for bit.
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "dc_internal.h"
+#include "dc_kernels.h"
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int find_bucket(const int64_t* __restrict__ bounds, int n, int64_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bounds[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Workgroup b runs on XCD b % 8. Remap so each XCD gets a contiguous range of tiles, keeping
+// neighbouring tiles (which share halo rows) in one L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
+  const int64_t q = nb >> 3, r = nb & 7, x = b & 7, s = b >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_mark
+// ---------------------------------------------------------------------------------------------
+template <int J>
+__global__ void __launch_bounds__(256)
+    k_mark(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ tile_bounds,
+           int64_t n_tiles, uint64_t* __restrict__ act, uint64_t* __restrict__ ft,
+           uint32_t* __restrict__ pcount, DcCounters* __restrict__ ctr) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpb = blockDim.x / kWave;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t tile = blk * wpb + (threadIdx.x / kWave);
+  if (tile >= n_tiles) return;  // uniform per wave
+  const int g = find_bucket(tile_bounds, n_grids, tile);
+  const DcGridDev G = grids[g];
+  int64_t lt = tile - tile_bounds[g];
+  const int nJB = (G.cy + J - 1) / J;
+  const int c = (int)(lt % G.W);
+  lt /= G.W;
+  const int jb = (int)(lt % nJB);
+  const int kw = (int)(lt / nJB);
+  const int i = 64 * c + lane;
+  const int j0 = jb * J;
+  const int k0 = kw * 64;
+  const int k1 = min(k0 + 64, G.cz);
+  const bool lane_in = i < G.nx;
+  const int ix = 64 * c + 64;
+  const bool extra = (lane == 63) && (ix < G.nx);
+  const int64_t rs = G.nx;
+  const int64_t ls = (int64_t)G.nx * G.ny;
+
+  uint32_t rows_s = 0, rows_c = 0;
+#pragma unroll
+  for (int r = 0; r <= J; ++r) rows_s |= (uint32_t)(j0 + r < G.ny) << r;
+#pragma unroll
+  for (int r = 0; r < J; ++r) rows_c |= (uint32_t)(j0 + r < G.cy) << r;
+  const uint32_t rows_j1 = (j0 == 0) ? (rows_c & ~1u) : rows_c;
+  const bool cin = i < G.cx;
+  const bool cin1 = cin && i >= 1;
+
+  float va[J + 1], ea[J + 1];
+  bool bad = false;
+  auto fetch = [&](int kk, float* v, float* e) {
+    const float* base = G.val + (int64_t)kk * ls + (int64_t)j0 * rs;
+#pragma unroll
+    for (int r = 0; r <= J; ++r) {
+      v[r] = 0.f;
+      e[r] = 0.f;
+      if (rows_s & (1u << r)) {
+        if (lane_in) v[r] = base[(int64_t)r * rs + i];
+        if (extra) e[r] = base[(int64_t)r * rs + ix];
+      }
+    }
+  };
+  auto bits = [&](const float* v, const float* e, uint32_t& N, uint32_t& P, uint32_t& N1,
+                  uint32_t& P1) {
+    uint32_t ne = 0, pe = 0;
+    N = 0;
+    P = 0;
+#pragma unroll
+    for (int r = 0; r <= J; ++r) {
+      N |= (uint32_t)(v[r] < 0.f) << r;
+      P |= (uint32_t)(v[r] > 0.f) << r;
+      ne |= (uint32_t)(e[r] < 0.f) << r;
+      pe |= (uint32_t)(e[r] > 0.f) << r;
+      bad |= !(fabsf(v[r]) <= FLT_MAX);
+    }
+    N1 = __shfl_down(N, 1);
+    P1 = __shfl_down(P, 1);
+    if (lane == 63) {
+      N1 = ne;
+      P1 = pe;
+    }
+  };
+
+  uint32_t Np, Pp, N1p, P1p;
+  fetch(k0, va, ea);
+  bits(va, ea, Np, Pp, N1p, P1p);
+  float vb[J + 1], eb[J + 1];
+  fetch(k0 + 1, vb, eb);
+
+  uint64_t fx[J], fy[J], fz[J];
+#pragma unroll
+  for (int r = 0; r < J; ++r) fx[r] = fy[r] = fz[r] = 0;
+
+  for (int k = k0; k < k1; ++k) {
+    // layer k+1 was fetched last iteration; issue layer k+2 now so it is in flight meanwhile
+#pragma unroll
+    for (int r = 0; r <= J; ++r) {
+      va[r] = vb[r];
+      ea[r] = eb[r];
+    }
+    if (k + 2 <= k1) fetch(k + 2, vb, eb);
+    uint32_t Nc, Pc, N1c, P1c;
+    bits(va, ea, Nc, Pc, N1c, P1c);
+    const uint32_t XNp = Np | N1p, XPp = Pp | P1p, XNc = Nc | N1c, XPc = Pc | P1c;
+    const uint32_t cn = XNp | (XNp >> 1) | XNc | (XNc >> 1);
+    const uint32_t cp = XPp | (XPp >> 1) | XPc | (XPc >> 1);
+    // Cell.cpp:66-76 has_vertex; contouring.cpp:534-539 strict sign change for quads
+    const uint32_t ab = cin ? (cn & cp & rows_c) : 0u;
+    const uint32_t fxb = (cin && k >= 1) ? (((Np & P1p) | (Pp & N1p)) & rows_j1) : 0u;
+    const uint32_t fyb = (cin1 && k >= 1) ? (((Np & (Pp >> 1)) | (Pp & (Np >> 1))) & rows_c) : 0u;
+    const uint32_t fzb = cin1 ? (((Np & Pc) | (Pp & Nc)) & rows_j1) : 0u;
+    const int kk = k - k0;
+    uint64_t aw = 0;
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+      fx[r] |= (uint64_t)((fxb >> r) & 1u) << kk;
+      fy[r] |= (uint64_t)((fyb >> r) & 1u) << kk;
+      fz[r] |= (uint64_t)((fzb >> r) & 1u) << kk;
+      const uint64_t w = __ballot((ab >> r) & 1u);
+      if (lane == r) aw = w;
+    }
+    if (lane < J && ((rows_c >> lane) & 1u))
+      act[G.act_off + ((int64_t)k * G.cy + j0 + lane) * G.W + c] = aw;
+    Np = Nc;
+    Pp = Pc;
+    N1p = N1c;
+    P1p = P1c;
+  }
+  if (__any(bad) && lane == 0) atomicOr(&ctr->nonfinite, 1u);
+  if (cin) {
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+      if (!((rows_c >> r) & 1u)) continue;
+      const int j = j0 + r;
+      const uint64_t w3[3] = {fx[r], fy[r], fz[r]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ft[G.ft_off + (((int64_t)a * G.Wk + kw) * G.cy + j) * G.cx + i] = w3[a];
+        const uint32_t n = (uint32_t)__popcll(w3[a]);
+        if (n) atomicAdd(&pcount[G.pen_off + ((int64_t)a * G.cy + j) * G.cx + i], n);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Decoupled look-back scan helpers (single pass; tiles taken in launch order from a counter so
+// every tile a block waits on has already started).
+// ---------------------------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr uint64_t kSpinLimit = 1ull << 22;
+
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t* s_wave, uint64_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wave[wv] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int q = 0; q < kScanThreads / 64; ++q) {
+    if (q < wv) before += s_wave[q];
+    all += s_wave[q];
+  }
+  *total = all;
+  return before + x - v;
+}
+
+__device__ __forceinline__ uint64_t lookback(uint64_t* __restrict__ status, int64_t tile,
+                                             uint64_t agg, DcCounters* __restrict__ ctr) {
+  if (tile == 0) {
+    __hip_atomic_store(&status[0], (DC_SCAN_FLAG_PREFIX << 62) | agg, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_AGG << 62) | agg, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t excl = 0;
+  int64_t t = tile - 1;
+  uint64_t spins = 0;
+  while (t >= 0) {
+    const uint64_t s =
+        __hip_atomic_load(&status[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t flag = s >> 62;
+    if (flag == 0) {
+      if (++spins > kSpinLimit) {  // bounded wait: report instead of hanging
+        atomicOr(&ctr->error, DC_ERR_SCAN_WAIT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & DC_SCAN_VALUE_MASK;
+    if (flag == DC_SCAN_FLAG_PREFIX) break;
+    --t;
+  }
+  __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_PREFIX << 62) | (excl + agg), __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// contouring.cpp:509-524: vertex id = rank of the active cell in (k, j, i) order.
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_act(const uint64_t* __restrict__ act, int64_t n_words, uint32_t* __restrict__ wprefix,
+               uint64_t* __restrict__ alist, uint64_t cap, uint64_t* __restrict__ status,
+               uint32_t* __restrict__ tile_ctr, DcCounters* __restrict__ ctr) {
+  __shared__ int64_t s_tile;
+  __shared__ uint64_t s_wave[kScanThreads / 64];
+  __shared__ uint64_t s_prefix;
+  if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint64_t w[kScanItems];
+  uint64_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    w[q] = (base + q < n_words) ? act[base + q] : 0ull;
+    cnt += (uint64_t)__popcll(w[q]);
+  }
+  uint64_t total;
+  const uint64_t excl = block_exclusive(cnt, s_wave, &total);
+  if (threadIdx.x == 0) s_prefix = lookback(status, tile, total, ctr);
+  __syncthreads();
+  uint64_t run = s_prefix + excl;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q >= n_words) break;
+    wprefix[base + q] = (uint32_t)run;
+    uint64_t bitsw = w[q];
+    uint64_t rk = run;
+    while (bitsw) {
+      const int b = __ffsll((long long)bitsw) - 1;
+      if (rk < cap) alist[rk] = (uint64_t)(base + q) * 64u + (uint64_t)b;
+      ++rk;
+      bitsw &= bitsw - 1;
+    }
+    run = rk;
+  }
+  if (threadIdx.x == 0 && (tile + 1) * kScanTile >= n_words) ctr->n_active = s_prefix + total;
+}
+
+// Pencil face counts in the CPU backend's face order: grid, axis, i, j (k is inside a pencil).
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_pen(const DcGridDev* __restrict__ grids, int n_grids,
+               const int64_t* __restrict__ pen_bounds, int64_t n_pen,
+               const uint32_t* __restrict__ pcount, uint32_t* __restrict__ pbase,
+               uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
+               DcCounters* __restrict__ ctr) {
+  __shared__ int64_t s_tile;
+  __shared__ uint64_t s_wave[kScanThreads / 64];
+  __shared__ uint64_t s_prefix;
+  if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t sidx[kScanItems];
+  uint32_t val[kScanItems];
+  uint64_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    const int64_t e = base + q;
+    sidx[q] = -1;
+    val[q] = 0;
+    if (e < n_pen) {
+      const int g = find_bucket(pen_bounds, n_grids, e);
+      const int cx = grids[g].cx, cy = grids[g].cy;
+      const int64_t local = e - pen_bounds[g];
+      const int64_t plane = (int64_t)cx * cy;
+      const int64_t a = local / plane;
+      const int64_t rem = local - a * plane;
+      const int64_t ii = rem / cy, jj = rem - (rem / cy) * cy;
+      sidx[q] = pen_bounds[g] + (a * cy + jj) * cx + ii;
+      val[q] = pcount[sidx[q]];
+      cnt += val[q];
+    }
+  }
+  uint64_t total;
+  const uint64_t excl = block_exclusive(cnt, s_wave, &total);
+  if (threadIdx.x == 0) s_prefix = lookback(status, tile, total, ctr);
+  __syncthreads();
+  uint64_t run = s_prefix + excl;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (sidx[q] < 0) break;
+    pbase[sidx[q]] = (uint32_t)run;
+    run += val[q];
+  }
+  if (threadIdx.x == 0 && (tile + 1) * kScanTile >= n_pen) ctr->n_faces = s_prefix + total;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-cell geometry shared by the solve and the emission (FP64, reference operation order)
+// ---------------------------------------------------------------------------------------------
+// contouring.cpp:232-241 corner order; include/Cell.h:26-30 edge pairs.
+__device__ __constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__device__ __constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                                {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// dc_backend.cpp:63-65: FP32 multiply then FP32 add, widened.
+__device__ __forceinline__ double coordd(float o, int i, float v) {
+  return (double)__fadd_rn(o, __fmul_rn((float)i, v));
+}
+
+__device__ __forceinline__ double sq3(double x, double y, double z) {
+  return (x * x + y * y) + z * z;
+}
+
+struct DCell {
+  double S[8];
+  double X0, X1, Y0, Y1, Z0, Z1;
+};
+
+__device__ __forceinline__ float sample(const DcGridDev& G, int i, int j, int k) {
+  return G.val[(int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k)];
+}
+
+__device__ __forceinline__ void load_dcell(const DcGridDev& G, int i, int j, int k, DCell& c) {
+  const int64_t b = (int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k);
+  const int64_t sx = 1, sy = G.nx, sz = (int64_t)G.nx * G.ny;
+  c.S[0] = (double)G.val[b];
+  c.S[1] = (double)G.val[b + sx];
+  c.S[2] = (double)G.val[b + sx + sy];
+  c.S[3] = (double)G.val[b + sy];
+  c.S[4] = (double)G.val[b + sz];
+  c.S[5] = (double)G.val[b + sx + sz];
+  c.S[6] = (double)G.val[b + sx + sy + sz];
+  c.S[7] = (double)G.val[b + sy + sz];
+  c.X0 = coordd(G.ox, i, G.vx);
+  c.X1 = coordd(G.ox, i + 1, G.vx);
+  c.Y0 = coordd(G.oy, j, G.vy);
+  c.Y1 = coordd(G.oy, j + 1, G.vy);
+  c.Z0 = coordd(G.oz, k, G.vz);
+  c.Z1 = coordd(G.oz, k + 1, G.vz);
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double clamp01d(double t) { return dmax(0.0, dmin(1.0, t)); }
+
+// Cell.cpp:80-155 gradientAtPointUsingTrilinear
+__device__ __forceinline__ void trilinear_grad(const DCell& c, const double p[3], double g[3]) {
+  const double S000 = c.S[0], S100 = c.S[1], S110 = c.S[2], S010 = c.S[3];
+  const double S001 = c.S[4], S101 = c.S[5], S111 = c.S[6], S011 = c.S[7];
+  const double dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
+  const double eps = 1e-12;
+  double u = 0.5, v = 0.5, w = 0.5;
+  if (fabs(dx) > eps) u = (p[0] - c.X0) / dx;
+  if (fabs(dy) > eps) v = (p[1] - c.Y0) / dy;
+  if (fabs(dz) > eps) w = (p[2] - c.Z0) / dz;
+  u = clamp01d(u);
+  v = clamp01d(v);
+  w = clamp01d(w);
+  const double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
+  const double du = (S100 - S000) * omv * omw + (S110 - S010) * v * omw +
+                    (S101 - S001) * omv * w + (S111 - S011) * v * w;
+  const double dv = (S010 - S000) * omu * omw + (S110 - S100) * u * omw +
+                    (S011 - S001) * omu * w + (S111 - S101) * u * w;
+  const double dw = (S001 - S000) * omu * omv + (S101 - S100) * u * omv +
+                    (S011 - S010) * omu * v + (S111 - S110) * u * v;
+  g[0] = 0.0;
+  g[1] = 0.0;
+  g[2] = 0.0;
+  if (fabs(dx) > eps) g[0] = du / dx;
+  if (fabs(dy) > eps) g[1] = dv / dy;
+  if (fabs(dz) > eps) g[2] = dw / dz;
+}
+
+__device__ __forceinline__ void normalize3(double v[3]) {
+  const double z = sq3(v[0], v[1], v[2]);
+  if (z > 0.0) {
+    const double s = sqrt(z);
+    v[0] = v[0] / s;
+    v[1] = v[1] / s;
+    v[2] = v[2] / s;
+  }
+}
+
+__device__ __forceinline__ double cornerx(const DCell& c, int q) { return c_corner[q][0] ? c.X1 : c.X0; }
+__device__ __forceinline__ double cornery(const DCell& c, int q) { return c_corner[q][1] ? c.Y1 : c.Y0; }
+__device__ __forceinline__ double cornerz(const DCell& c, int q) { return c_corner[q][2] ? c.Z1 : c.Z0; }
+
+// Cell.cpp:180-209 for one edge
+__device__ __forceinline__ bool hermite_edge(const DCell& c, int e, double p[3], double n[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  const double fa = c.S[a], fb = c.S[b];
+  if (fa * fb > 0.0) return false;
+  const double t = fa / (fa - fb + 1e-16);
+  p[0] = (1.0 - t) * cornerx(c, a) + t * cornerx(c, b);
+  p[1] = (1.0 - t) * cornery(c, a) + t * cornery(c, b);
+  p[2] = (1.0 - t) * cornerz(c, a) + t * cornerz(c, b);
+  trilinear_grad(c, p, n);
+  normalize3(n);
+  return true;
+}
+
+__device__ __forceinline__ bool cell_active(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                            int i, int j, int k) {
+  const uint64_t w = act[G.act_off + ((int64_t)k * G.cy + j) * G.W + (i >> 6)];
+  return (w >> (i & 63)) & 1ull;
+}
+
+// The 4 cells around the grid edge carried by local edge e, in the order process_edge visits
+// them (hermite_update.cpp:171-332): [base, base+dir1, base+dir2, base+dir1+dir2] with base the
+// cell whose min corner is the edge's lower endpoint and (dir1, dir2) from
+// get_face_normals_from_edge_idx (Cell.cpp:26-45). Local edge ids per axis:
+//   X: base edge 0, -Y edge 2, -Z edge 4, -Y-Z edge 6
+//   Y: base edge 3, -X edge 1, -Z edge 7, -X-Z edge 5
+//   Z: base edge 8, -X edge 9, -Y edge 11, -X-Y edge 10
+__device__ __constant__ int8_t c_grp_edge[3][4] = {{0, 2, 4, 6}, {3, 1, 7, 5}, {8, 9, 11, 10}};
+// (d1, d2) axes per edge axis: X -> (-Y, -Z), Y -> (-X, -Z), Z -> (-X, -Y)
+__device__ __constant__ int8_t c_grp_dirs[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+
+// Returns true and the averaged normal when local edge e of cell (i,j,k) is averaged
+// (hermite_update.cpp:334-438). own_n is the cell's own (normalized) hermite normal for e.
+__device__ bool averaged_normal(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                                int k, int e, const DCell& own, const double own_n[3],
+                                double out[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  int P0[3] = {i + c_corner[a][0], j + c_corner[a][1], k + c_corner[a][2]};
+  int P1[3] = {i + c_corner[b][0], j + c_corner[b][1], k + c_corner[b][2]};
+  int axis = (P0[0] != P1[0]) ? 0 : ((P0[1] != P1[1]) ? 1 : 2);
+  if (P1[axis] < P0[axis]) {
+    for (int d = 0; d < 3; ++d) {
+      const int t = P0[d];
+      P0[d] = P1[d];
+      P1[d] = t;
+    }
+  }
+  const bool in0 = P0[0] > 0 && P0[0] < G.nx - 1 && P0[1] > 0 && P0[1] < G.ny - 1 && P0[2] > 0 &&
+                   P0[2] < G.nz - 1;
+  const bool in1 = P1[0] > 0 && P1[0] < G.nx - 1 && P1[1] > 0 && P1[1] < G.ny - 1 && P1[2] > 0 &&
+                   P1[2] < G.nz - 1;
+  if (!in0 && !in1) return false;
+  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
+  int cells[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cells[q][0] = P0[0];
+    cells[q][1] = P0[1];
+    cells[q][2] = P0[2];
+  }
+  cells[1][d1] -= 1;
+  cells[2][d2] -= 1;
+  cells[3][d1] -= 1;
+  cells[3][d2] -= 1;
+  // A strict sign change on the edge makes all four cells active; otherwise look them up.
+  const double fa = own.S[a], fb = own.S[b];
+  if (!(fa * fb < 0.0)) {
+    for (int q = 0; q < 4; ++q)
+      if (!cell_active(G, act, cells[q][0], cells[q][1], cells[q][2])) return false;
+  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < 4; ++q) {
+    double n[3];
+    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
+      n[0] = own_n[0];
+      n[1] = own_n[1];
+      n[2] = own_n[2];
+    } else {
+      DCell nc;
+      load_dcell(G, cells[q][0], cells[q][1], cells[q][2], nc);
+      double p[3];
+      hermite_edge(nc, c_grp_edge[axis][q], p, n);
+    }
+    if (sq3(n[0], n[1], n[2]) > 0.0) {
+      acc[0] = acc[0] + n[0];
+      acc[1] = acc[1] + n[1];
+      acc[2] = acc[2] + n[2];
+    }
+  }
+  normalize3(acc);
+  out[0] = acc[0];
+  out[1] = acc[1];
+  out[2] = acc[2];
+  return true;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5);
+// the same operation sequence as the CPU restatement.
+__device__ void sym3_eig(const double Min[3][3], double lam[3], double Q[3][3]) {
+  double a[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      a[r][s] = Min[r][s];
+      Q[r][s] = (r == s) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    const double off = (a[0][1] * a[0][1] + a[0][2] * a[0][2]) + a[1][2] * a[1][2];
+    const double dia = (a[0][0] * a[0][0] + a[1][1] * a[1][1]) + a[2][2] * a[2][2];
+    if (!(off > 1e-36 * dia)) break;
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+      const int p = (pr == 2) ? 1 : 0;
+      const int q = (pr == 0) ? 1 : 2;
+      const int r = 3 - p - q;
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+      const double at = fabs(theta);
+      double t;
+      if (at > 1e150)
+        t = 0.5 / theta;
+      else
+        t = 1.0 / (at + sqrt(theta * theta + 1.0));
+      if (theta < 0.0) t = -t;
+      const double c = 1.0 / sqrt(t * t + 1.0);
+      const double s = t * c;
+      const double arp = a[r][p], arq = a[r][q];
+      a[p][p] = a[p][p] - t * apq;
+      a[q][q] = a[q][q] + t * apq;
+      a[p][q] = 0.0;
+      a[q][p] = 0.0;
+      const double nrp = c * arp - s * arq;
+      const double nrq = s * arp + c * arq;
+      a[r][p] = nrp;
+      a[p][r] = nrp;
+      a[r][q] = nrq;
+      a[q][r] = nrq;
+#pragma unroll
+      for (int row = 0; row < 3; ++row) {
+        const double qp = Q[row][p], qq = Q[row][q];
+        Q[row][p] = c * qp - s * qq;
+        Q[row][q] = s * qp + c * qq;
+      }
+    }
+  }
+  lam[0] = a[0][0];
+  lam[1] = a[1][1];
+  lam[2] = a[2][2];
+}
+
+// Cell::update + solve_quadratic_system + getCentroid (Cell.cpp:284-361, 586-600), FP64.
+// Rows are streamed in ascending edge order, so the sums match the CPU restatement exactly.
+__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                                int k, double x[3]) {
+  DCell own;
+  load_dcell(G, i, j, k, own);
+  double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
+  double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
+  int m = 0;
+  for (int e = 0; e < 12; ++e) {
+    double p[3], n[3];
+    if (!hermite_edge(own, e, p, n)) continue;
+    double nf[3];
+    if (!averaged_normal(G, act, i, j, k, e, own, n, nf)) {
+      nf[0] = n[0];
+      nf[1] = n[1];
+      nf[2] = n[2];
+    }
+    if (sqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
+    const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
+    M00 = M00 + nf[0] * nf[0];
+    M01 = M01 + nf[0] * nf[1];
+    M02 = M02 + nf[0] * nf[2];
+    M11 = M11 + nf[1] * nf[1];
+    M12 = M12 + nf[1] * nf[2];
+    M22 = M22 + nf[2] * nf[2];
+    r0 = r0 + nf[0] * b;
+    r1 = r1 + nf[1] * b;
+    r2 = r2 + nf[2] * b;
+    c0 = c0 + p[0];
+    c1 = c1 + p[1];
+    c2 = c2 + p[2];
+    ++m;
+  }
+  const double M[3][3] = {{M00, M01, M02}, {M01, M11, M12}, {M02, M12, M22}};
+  double lam[3], Q[3][3];
+  sym3_eig(M, lam, Q);
+  const double dm = (double)m;
+  c0 = c0 / dm;
+  c1 = c1 / dm;
+  c2 = c2 / dm;
+  const double g0 = r0 - ((M00 * c0 + M01 * c1) + M02 * c2);
+  const double g1 = r1 - ((M01 * c0 + M11 * c1) + M12 * c2);
+  const double g2 = r2 - ((M02 * c0 + M12 * c1) + M22 * c2);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(fabs(lam[q]) > 0.01)) continue;
+    const double qg = (Q[0][q] * g0 + Q[1][q] * g1) + Q[2][q] * g2;
+    const double coef = qg / lam[q];
+    y0 = y0 + coef * Q[0][q];
+    y1 = y1 + coef * Q[1][q];
+    y2 = y2 + coef * Q[2][q];
+  }
+  x[0] = c0 + y0;
+  x[1] = c1 + y1;
+  x[2] = c2 + y2;
+}
+
+__device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, int& i, int& j,
+                                            int& k) {
+  const int64_t lw = (int64_t)(entry >> 6) - G.act_off;
+  const int64_t row = lw / G.W;
+  const int c = (int)(lw - row * G.W);
+  i = 64 * c + (int)(entry & 63);
+  j = (int)(row % G.cy);
+  k = (int)(row / G.cy);
+}
+
+__global__ void __launch_bounds__(256)
+    k_qef64(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
+            const uint64_t* __restrict__ act, const uint64_t* __restrict__ alist,
+            const DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
+            float* __restrict__ v32) {
+  const uint64_t n = min(ctr->n_active, cap);
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t entry = alist[v];
+    const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
+    const DcGridDev G = grids[g];
+    int i, j, k;
+    decode_cell(G, entry, i, j, k);
+    double x[3];
+    solve_cell_fp64(G, act, i, j, k, x);
+    v64[3 * v + 0] = x[0];
+    v64[3 * v + 1] = x[1];
+    v64[3 * v + 2] = x[2];
+    v32[3 * v + 0] = (float)x[0];
+    v32[3 * v + 1] = (float)x[1];
+    v32[3 * v + 2] = (float)x[2];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_emit — quads in the CPU backend's order (contouring.cpp:607-729)
+// ---------------------------------------------------------------------------------------------
+// contouring.cpp:141-207 gradientAt: central differences with clamped neighbours, normalized.
+__device__ void grid_gradient(const DcGridDev& G, int i, int j, int k, double out[3]) {
+  const int im = max(i - 1, 0), ip = min(i + 1, G.nx - 1);
+  const int jm = max(j - 1, 0), jp = min(j + 1, G.ny - 1);
+  const int km = max(k - 1, 0), kp = min(k + 1, G.nz - 1);
+  out[0] = out[1] = out[2] = 0.0;
+  const double dx = coordd(G.ox, ip, G.vx) - coordd(G.ox, im, G.vx);
+  if (fabs(dx) > 1e-12)
+    out[0] = ((double)sample(G, ip, j, k) - (double)sample(G, im, j, k)) / dx;
+  const double dy = coordd(G.oy, jp, G.vy) - coordd(G.oy, jm, G.vy);
+  if (fabs(dy) > 1e-12)
+    out[1] = ((double)sample(G, i, jp, k) - (double)sample(G, i, jm, k)) / dy;
+  const double dz = coordd(G.oz, kp, G.vz) - coordd(G.oz, km, G.vz);
+  if (fabs(dz) > 1e-12)
+    out[2] = ((double)sample(G, i, j, kp) - (double)sample(G, i, j, km)) / dz;
+  const double nrm = sqrt(sq3(out[0], out[1], out[2]));
+  if (nrm > 0.0) {
+    out[0] = out[0] / nrm;
+    out[1] = out[1] / nrm;
+    out[2] = out[2] / nrm;
+  }
+}
+
+__device__ __forceinline__ int32_t vertex_id(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                             const uint32_t* __restrict__ wprefix, uint32_t vbase,
+                                             int i, int j, int k) {
+  const int64_t w = G.act_off + ((int64_t)k * G.cy + j) * G.W + (i >> 6);
+  const uint64_t below = (1ull << (i & 63)) - 1ull;
+  return (int32_t)(wprefix[w] - vbase + (uint32_t)__popcll(act[w] & below));
+}
+
+__global__ void __launch_bounds__(256)
+    k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ pen_bounds,
+           int64_t n_pen, const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
+           const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pbase,
+           const double* __restrict__ v64, const DcCounters* __restrict__ ctr, uint64_t capA,
+           uint64_t capF, int32_t* __restrict__ faces) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pen) return;
+  // an overflowed run is repeated by the host; never index vertices past the buffer
+  if (ctr->n_active > capA) return;
+  const int g = find_bucket(pen_bounds, n_grids, p);
+  const DcGridDev G = grids[g];
+  const int64_t local = p - G.pen_off;
+  const int64_t plane = (int64_t)G.cy * G.cx;
+  const int a = (int)(local / plane);
+  const int64_t rem = local - (int64_t)a * plane;
+  const int j = (int)(rem / G.cx);
+  const int i = (int)(rem - (int64_t)j * G.cx);
+  uint64_t pos = pbase[p];
+  const uint32_t vbase = wprefix[G.act_off];
+  const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
+  for (int kw = 0; kw < G.Wk; ++kw) {
+    uint64_t w = ft[G.ft_off + (((int64_t)a * G.Wk + kw) * G.cy + j) * G.cx + i];
+    while (w) {
+      const int b = __ffsll((long long)w) - 1;
+      w &= w - 1;
+      const int k = 64 * kw + b;
+      int c0, c1, c2, c3, e1[3];
+      if (a == 0) {
+        c0 = vertex_id(G, act, wprefix, vbase, i, j - 1, k - 1);
+        c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
+        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+        c3 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
+        e1[0] = i + 1; e1[1] = j; e1[2] = k;
+      } else if (a == 1) {
+        c0 = vertex_id(G, act, wprefix, vbase, i - 1, j, k - 1);
+        c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
+        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+        c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
+        e1[0] = i; e1[1] = j + 1; e1[2] = k;
+      } else {
+        c0 = vertex_id(G, act, wprefix, vbase, i - 1, j - 1, k);
+        c1 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
+        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+        c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
+        e1[0] = i; e1[1] = j; e1[2] = k + 1;
+      }
+      double g0[3], g1[3];
+      grid_gradient(G, i, j, k, g0);
+      grid_gradient(G, e1[0], e1[1], e1[2], g1);
+      const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
+      const double* pa = v64 + 3 * (vglob + c0);
+      const double* pb = v64 + 3 * (vglob + c1);
+      const double* pc = v64 + 3 * (vglob + c2);
+      const double* pd = v64 + 3 * (vglob + c3);
+      const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
+      const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
+      double q0 = ba1 * da2 - ba2 * da1;
+      double q1 = ba2 * da0 - ba0 * da2;
+      double q2 = ba0 * da1 - ba1 * da0;
+      if (sq3(q0, q1, q2) < 1e-24) {
+        const double ca0 = pc[0] - pa[0], ca1 = pc[1] - pa[1], ca2 = pc[2] - pa[2];
+        q0 = ca1 * da2 - ca2 * da1;
+        q1 = ca2 * da0 - ca0 * da2;
+        q2 = ca0 * da1 - ca1 * da0;
+      }
+      if (sq3(gn0, gn1, gn2) > 0 && sq3(q0, q1, q2) > 0) {
+        if ((q0 * gn0 + q1 * gn1) + q2 * gn2 < 0.0) {
+          const int t = c1;
+          c1 = c3;
+          c3 = t;
+        }
+      }
+      if (pos < capF) {
+        int4 quad = make_int4(c0, c1, c2, c3);
+        reinterpret_cast<int4*>(faces)[pos] = quad;
+      }
+      ++pos;
+    }
+  }
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
+int64_t fill_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------------
+hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)fill_blocks(n)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)fill_blocks(n)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+int64_t dc_mark_tiles(const DcGridDev& g, int rows) {
+  const int64_t nJB = (g.cy + rows - 1) / rows;
+  return (int64_t)g.W * nJB * g.Wk;
+}
+
+hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
+  const int wpb = a.waves_per_block;
+  if (wpb != 1 && wpb != 2 && wpb != 4) return hipErrorInvalidValue;
+  const int64_t blocks = (a.n_tiles + wpb - 1) / wpb;
+  if (blocks <= 0) return hipSuccess;
+  const dim3 grid((unsigned)blocks), block((unsigned)(64 * wpb));
+  switch (a.rows) {
+    case 2:
+      hipLaunchKernelGGL(k_mark<2>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds,
+                         a.n_tiles, a.act, a.ft, a.pcount, a.ctr);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_mark<4>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds,
+                         a.n_tiles, a.act, a.ft, a.pcount, a.ctr);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_mark<8>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds,
+                         a.n_tiles, a.act, a.ft, a.pcount, a.ctr);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int64_t dc_scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* wprefix,
+                              uint64_t* alist, uint64_t cap, uint64_t* status, uint32_t* tile_ctr,
+                              DcCounters* ctr, hipStream_t s) {
+  const int64_t tiles = dc_scan_tiles(n_words);
+  if (tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_act, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, act, n_words,
+                     wprefix, alist, cap, status, tile_ctr, ctr);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t* pen_bounds,
+                              int64_t n_pen, const uint32_t* pcount, uint32_t* pbase,
+                              uint64_t* status, uint32_t* tile_ctr, DcCounters* ctr,
+                              hipStream_t s) {
+  const int64_t tiles = dc_scan_tiles(n_pen);
+  if (tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_pen, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, grids, n_grids,
+                     pen_bounds, n_pen, pcount, pbase, status, tile_ctr, ctr);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
+  const unsigned blocks = (unsigned)a.blocks;
+  hipLaunchKernelGGL(k_qef64, dim3(blocks), dim3(256), 0, s, a.grids, a.n_grids, a.act_bounds,
+                     a.act, a.alist, a.ctr, a.capA, a.v64, a.v32);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
+  if (a.n_pen <= 0) return hipSuccess;
+  const int64_t blocks = (a.n_pen + 255) / 256;
+  hipLaunchKernelGGL(k_emit, dim3((unsigned)blocks), dim3(256), 0, s, a.grids, a.n_grids,
+                     a.pen_bounds, a.n_pen, a.act, a.wprefix, a.ft, a.pbase, a.v64, a.ctr, a.capA,
+                     a.capF, a.faces);
+  return hipGetLastError();
+}

@@ -1,0 +1,70 @@
+"""The C ABI without a GPU: the library loads, exports every symbol include/dc_hip.h declares,
+and reports a missing device through its status codes (never exits)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, gpu_available
+from contouring import _lib
+
+HEADER = os.path.join(ROOT, "include", "dc_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(dc_hip_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_what_binding_expects():
+    assert declared_symbols() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load_hip()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.lib_path()], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (dc_hip_\w+)", out))
+    assert set(declared_symbols()) <= exported
+
+
+def test_header_is_plain_c():
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", HEADER],
+                   check=True)
+
+
+def test_status_strings():
+    lib = _lib.load_hip()
+    for code in range(9):
+        assert lib.dc_hip_status_string(code)
+    assert lib.dc_hip_status_string(99) == b"unknown status"
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device path")
+def test_create_without_device_reports_no_device():
+    lib = _lib.load_hip()
+    ctx = ctypes.c_void_p()
+    rc = lib.dc_hip_create(0, ctypes.byref(ctx))
+    assert rc == _lib.DC_HIP_ERR_NO_DEVICE
+    assert not ctx.value
+
+
+def test_null_arguments_are_rejected():
+    lib = _lib.load_hip()
+    assert lib.dc_hip_create(0, None) == _lib.DC_HIP_ERR_INVALID_ARG
+    assert lib.dc_hip_set_option(None, b"timing", 1) == _lib.DC_HIP_ERR_INVALID_ARG
+    assert lib.dc_hip_extract(None, None, 0, None) == _lib.DC_HIP_ERR_INVALID_ARG
+    assert lib.dc_hip_result_counts(None, None, None) == _lib.DC_HIP_ERR_INVALID_ARG
+    assert lib.dc_hip_last_error(None) == b"no context"
+
+
+def test_backend_raises_without_device():
+    if gpu_available():
+        pytest.skip("checks the no-device path")
+    from contouring import HipDualContouringBackend
+    with pytest.raises(_lib.DcHipError):
+        HipDualContouringBackend()
