@@ -1,0 +1,42 @@
+// dc_hip_backend.h — C++ backend class over the C ABI (dc_hip.h); takes the place of
+// CudaDualContouringBackend (reference include/dc_cuda.h:31-35) behind IDualContouringBackend.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dc_backend.h"
+#include "dc_hip.h"
+
+// include/dc_cuda.h:24-29: a grid whose values already live in device memory (caller-owned).
+struct DenseSdfGridDevice {
+  int nx = 0, ny = 0, nz = 0;
+  float vx = 0.f, vy = 0.f, vz = 0.f;
+  float ox = 0.f, oy = 0.f, oz = 0.f;
+  const float* d_values = nullptr;
+};
+
+class HipDualContouringBackend : public IDualContouringBackend {
+ public:
+  // Throws std::runtime_error when no HIP device is available.
+  explicit HipDualContouringBackend(int device = 0, int precision = DC_HIP_PLAIN_FP64);
+  ~HipDualContouringBackend() override;
+  HipDualContouringBackend(const HipDualContouringBackend&) = delete;
+  HipDualContouringBackend& operator=(const HipDualContouringBackend&) = delete;
+
+  // dc_cuda.cu:614-649 — host grid in, host mesh out.
+  DualContouringMesh extract(const DenseSdfGrid& grid, DualContouringStats& stats) override;
+  // dc_cuda.cu:651-847 — device grid in, host mesh out.
+  DualContouringMesh extract_device(const DenseSdfGridDevice& grid, DualContouringStats& stats);
+  // FP64 vertices of the last extraction (plain FP64 mode).
+  std::vector<double> last_vertices_f64();
+  void set_option(const std::string& key, int64_t value);
+  int precision() const { return precision_; }
+
+ private:
+  DualContouringMesh download(const dc_hip_stats& st, DualContouringStats& stats);
+  void check(int rc);
+  dc_hip_context* ctx_ = nullptr;
+  int precision_;
+};

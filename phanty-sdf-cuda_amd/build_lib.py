@@ -45,11 +45,10 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs], verbose)
     _run(["g++", "-std=c++17", "-O3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
           "-o", os.path.join(BUILD, "libdc_synth.so"), os.path.join(CSRC, "synth.cpp")], verbose)
-    cli_srcs = [os.path.join(CSRC, "dc_cli.cpp"), os.path.join(CSRC, "dc_hip_backend.cpp")]
-    if all(os.path.exists(p) for p in cli_srcs):
-        _run(["g++", "-std=c++17", "-O2", "-I", INCLUDE, "-I", CSRC, "-o",
-              os.path.join(BUILD, "dc_cli"), *cli_srcs, "-L", BUILD, "-ldc_hip",
-              "-Wl,-rpath,$ORIGIN", "-fopenmp", "-ldl"], verbose)
+    cli_srcs = [os.path.join(CSRC, f) for f in ("dc_cli.cpp", "dc_hip_backend.cpp", "synth.cpp")]
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-I", INCLUDE, "-I", CSRC,
+          "-o", os.path.join(BUILD, "dc_cli"), *cli_srcs, "-L", BUILD, "-ldc_hip",
+          "-Wl,-rpath,$ORIGIN"], verbose)
     return lib
 
 

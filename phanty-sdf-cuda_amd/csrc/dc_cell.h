@@ -1,0 +1,797 @@
+// dc_cell.h — per-active-cell dual-contouring math for the CDNA4 kernels (device code).
+//
+// Two forward forms:
+//   FP64 "plain": the reference CPU backend's arithmetic in its global coordinates, with the same
+//     operation order as the CPU restatement, so vertices agree bit for bit.
+//   FP32: every cell works in its own frame (min corner at the origin), the QEF is assembled
+//     about the mass point with Neumaier-compensated sums, and a cell whose eigenvalues sit near
+//     the 0.01 truncation threshold, whose smallest kept eigenvalue is small (poor conditioning)
+//     or whose normals are degenerate is recomputed in FP64.
+// The adjoint (backward_cell) differentiates the same function at fixed topology and rank.
+//
+// Reference semantics (file:line in the reference tree):
+//   corner order contouring.cpp:232-241; edge pairs include/Cell.h:26-30
+//   hermite data Cell.cpp:162-212 (lerp_t :49-52, trilinear gradient :80-155)
+//   normal averaging hermite_update.cpp:171-438 (cells per edge: Cell.cpp:26-45)
+//   solve Cell.cpp:284-361, centroid :586-600
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dc_internal.h"
+
+namespace dccell {
+
+__device__ __constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__device__ __constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                                {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// The 4 cells around a grid edge in the order process_edge visits them: [base, base+dir1,
+// base+dir2, base+dir1+dir2], base = cell whose min corner is the edge's lower endpoint,
+// (dir1, dir2) from get_face_normals_from_edge_idx. Local edge ids per edge axis:
+//   X: 0, 2 (-Y), 4 (-Z), 6 (-Y-Z)   Y: 3, 1 (-X), 7 (-Z), 5 (-X-Z)   Z: 8, 9 (-X), 11 (-Y), 10
+__device__ __constant__ int8_t c_grp_edge[3][4] = {{0, 2, 4, 6}, {3, 1, 7, 5}, {8, 9, 11, 10}};
+__device__ __constant__ int8_t c_grp_dirs[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+
+constexpr double kSvdThreshold = 0.01;  // Cell.cpp:314
+// FP32 -> FP64 fallback: eigenvalue within this of the threshold, smallest kept eigenvalue
+// below kFbKept, or a hermite normal with squared norm below kFbTiny.
+constexpr float kFbBand = 5e-4f;
+constexpr float kFbKept = 0.05f;
+constexpr float kFbTiny = 1e-20f;
+
+// dc_backend.cpp:63-65: FP32 multiply then FP32 add, widened.
+__device__ __forceinline__ double coordd(float o, int i, float v) {
+  return (double)__fadd_rn(o, __fmul_rn((float)i, v));
+}
+
+template <typename T>
+__device__ __forceinline__ T tsqrt(T x);
+template <>
+__device__ __forceinline__ double tsqrt<double>(double x) {
+  return sqrt(x);
+}
+template <>
+__device__ __forceinline__ float tsqrt<float>(float x) {
+  return __fsqrt_rn(x);
+}
+
+template <typename T>
+__device__ __forceinline__ T sq3(T x, T y, T z) {
+  return (x * x + y * y) + z * z;
+}
+template <typename T>
+__device__ __forceinline__ T dot3(const T a[3], const T b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) {
+  return (b < a) ? b : a;
+}
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) {
+  return (a < b) ? b : a;
+}
+template <typename T>
+__device__ __forceinline__ T clamp01(T t) {
+  return tmax(T(0), tmin(T(1), t));
+}
+
+// fa*fb > 0 in double is exact for float inputs; FP32 uses the equivalent sign logic.
+__device__ __forceinline__ bool skip_edge(double fa, double fb) { return fa * fb > 0.0; }
+__device__ __forceinline__ bool skip_edge(float fa, float fb) {
+  return (fa > 0.f && fb > 0.f) || (fa < 0.f && fb < 0.f);
+}
+__device__ __forceinline__ bool strict_change(double fa, double fb) { return fa * fb < 0.0; }
+__device__ __forceinline__ bool strict_change(float fa, float fb) {
+  return (fa < 0.f && fb > 0.f) || (fa > 0.f && fb < 0.f);
+}
+
+template <typename T>
+struct Cell {
+  T S[8];
+  T X0, X1, Y0, Y1, Z0, Z1;
+  __device__ __forceinline__ T cx(int q) const { return c_corner[q][0] ? X1 : X0; }
+  __device__ __forceinline__ T cy(int q) const { return c_corner[q][1] ? Y1 : Y0; }
+  __device__ __forceinline__ T cz(int q) const { return c_corner[q][2] ? Z1 : Z0; }
+};
+
+__device__ __forceinline__ const float* sample_ptr(const DcGridDev& G, int i, int j, int k) {
+  return G.val + ((int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k));
+}
+
+template <typename T>
+__device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, int k, T S[8]) {
+  const float* b = sample_ptr(G, i, j, k);
+  const int64_t sy = G.nx, sz = (int64_t)G.nx * G.ny;
+  S[0] = (T)b[0];
+  S[1] = (T)b[1];
+  S[2] = (T)b[1 + sy];
+  S[3] = (T)b[sy];
+  S[4] = (T)b[sz];
+  S[5] = (T)b[1 + sz];
+  S[6] = (T)b[1 + sy + sz];
+  S[7] = (T)b[sy + sz];
+}
+
+// FP64: global frame (the CPU backend's coordinates).
+__device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<double>& c) {
+  load_values(G, i, j, k, c.S);
+  c.X0 = coordd(G.ox, i, G.vx);
+  c.X1 = coordd(G.ox, i + 1, G.vx);
+  c.Y0 = coordd(G.oy, j, G.vy);
+  c.Y1 = coordd(G.oy, j + 1, G.vy);
+  c.Z0 = coordd(G.oz, k, G.vz);
+  c.Z1 = coordd(G.oz, k + 1, G.vz);
+}
+
+// FP32: the cell's own frame, min corner at 0 (edge lengths from the FP32 grid coordinates).
+__device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<float>& c) {
+  load_values(G, i, j, k, c.S);
+  c.X0 = 0.f;
+  c.Y0 = 0.f;
+  c.Z0 = 0.f;
+  c.X1 = (float)(coordd(G.ox, i + 1, G.vx) - coordd(G.ox, i, G.vx));
+  c.Y1 = (float)(coordd(G.oy, j + 1, G.vy) - coordd(G.oy, j, G.vy));
+  c.Z1 = (float)(coordd(G.oz, k + 1, G.vz) - coordd(G.oz, k, G.vz));
+}
+
+// Cell.cpp:80-155 gradientAtPointUsingTrilinear
+template <typename T>
+__device__ __forceinline__ void trilinear_grad(const Cell<T>& c, const T p[3], T g[3]) {
+  const T S000 = c.S[0], S100 = c.S[1], S110 = c.S[2], S010 = c.S[3];
+  const T S001 = c.S[4], S101 = c.S[5], S111 = c.S[6], S011 = c.S[7];
+  const T dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
+  const T eps = T(1e-12);
+  T u = T(0.5), v = T(0.5), w = T(0.5);
+  if (fabs(dx) > eps) u = (p[0] - c.X0) / dx;
+  if (fabs(dy) > eps) v = (p[1] - c.Y0) / dy;
+  if (fabs(dz) > eps) w = (p[2] - c.Z0) / dz;
+  u = clamp01(u);
+  v = clamp01(v);
+  w = clamp01(w);
+  const T omu = T(1) - u, omv = T(1) - v, omw = T(1) - w;
+  const T du = (S100 - S000) * omv * omw + (S110 - S010) * v * omw + (S101 - S001) * omv * w +
+               (S111 - S011) * v * w;
+  const T dv = (S010 - S000) * omu * omw + (S110 - S100) * u * omw + (S011 - S001) * omu * w +
+               (S111 - S101) * u * w;
+  const T dw = (S001 - S000) * omu * omv + (S101 - S100) * u * omv + (S011 - S010) * omu * v +
+               (S111 - S110) * u * v;
+  g[0] = T(0);
+  g[1] = T(0);
+  g[2] = T(0);
+  if (fabs(dx) > eps) g[0] = du / dx;
+  if (fabs(dy) > eps) g[1] = dv / dy;
+  if (fabs(dz) > eps) g[2] = dw / dz;
+}
+
+template <typename T>
+__device__ __forceinline__ void normalize3(T v[3]) {
+  const T z = sq3(v[0], v[1], v[2]);
+  if (z > T(0)) {
+    const T s = tsqrt(z);
+    v[0] = v[0] / s;
+    v[1] = v[1] / s;
+    v[2] = v[2] / s;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void hermite_point(const Cell<T>& c, int e, T p[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  const T fa = c.S[a], fb = c.S[b];
+  const T t = fa / (fa - fb + T(1e-16));
+  p[0] = (T(1) - t) * c.cx(a) + t * c.cx(b);
+  p[1] = (T(1) - t) * c.cy(a) + t * c.cy(b);
+  p[2] = (T(1) - t) * c.cz(a) + t * c.cz(b);
+}
+
+// Cell.cpp:180-209 for one edge: hermite point p and normalized trilinear normal n; G is the
+// gradient before normalization.
+template <typename T>
+__device__ __forceinline__ bool hermite_edge(const Cell<T>& c, int e, T p[3], T n[3], T G[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  if (skip_edge(c.S[a], c.S[b])) return false;
+  hermite_point(c, e, p);
+  trilinear_grad(c, p, G);
+  n[0] = G[0];
+  n[1] = G[1];
+  n[2] = G[2];
+  normalize3(n);
+  return true;
+}
+
+__device__ __forceinline__ bool cell_active(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                            int i, int j, int k) {
+  const uint64_t w = act[G.act_off + ((int64_t)k * G.cy + j) * G.W + (i >> 6)];
+  return (w >> (i & 63)) & 1ull;
+}
+
+// hermite_update.cpp:171-332: is local edge e of cell (i,j,k) averaged, and over which cells.
+// The caller has established that e carries a hermite sample (fa*fb <= 0).
+__device__ __forceinline__ bool edge_group(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                           int i, int j, int k, int e, bool strict,
+                                           int cells[4][3], int& axis) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  int P0[3] = {i + c_corner[a][0], j + c_corner[a][1], k + c_corner[a][2]};
+  int P1[3] = {i + c_corner[b][0], j + c_corner[b][1], k + c_corner[b][2]};
+  axis = (P0[0] != P1[0]) ? 0 : ((P0[1] != P1[1]) ? 1 : 2);
+  if (P1[axis] < P0[axis]) P0[axis] = P1[axis];
+  const bool in0 = P0[0] > 0 && P0[0] < G.nx - 1 && P0[1] > 0 && P0[1] < G.ny - 1 && P0[2] > 0 &&
+                   P0[2] < G.nz - 1;
+  int Q1[3] = {P0[0], P0[1], P0[2]};
+  Q1[axis] += 1;
+  const bool in1 = Q1[0] > 0 && Q1[0] < G.nx - 1 && Q1[1] > 0 && Q1[1] < G.ny - 1 && Q1[2] > 0 &&
+                   Q1[2] < G.nz - 1;
+  if (!in0 && !in1) return false;
+  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cells[q][0] = P0[0];
+    cells[q][1] = P0[1];
+    cells[q][2] = P0[2];
+  }
+  cells[1][d1] -= 1;
+  cells[2][d2] -= 1;
+  cells[3][d1] -= 1;
+  cells[3][d2] -= 1;
+  // a strict sign change on the edge makes all four cells active; otherwise look them up
+  if (!strict) {
+    for (int q = 0; q < 4; ++q)
+      if (!cell_active(G, act, cells[q][0], cells[q][1], cells[q][2])) return false;
+  }
+  return true;
+}
+
+// Final (averaged or own) hermite normal of local edge e of cell (i,j,k).
+template <typename T>
+__device__ __forceinline__ void final_normal(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                             int i, int j, int k, int e, const Cell<T>& own,
+                                             const T own_n[3], T out[3]) {
+  int cells[4][3], axis;
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  if (!edge_group(G, act, i, j, k, e, strict_change(own.S[a], own.S[b]), cells, axis)) {
+    out[0] = own_n[0];
+    out[1] = own_n[1];
+    out[2] = own_n[2];
+    return;
+  }
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int q = 0; q < 4; ++q) {
+    T n[3];
+    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
+      n[0] = own_n[0];
+      n[1] = own_n[1];
+      n[2] = own_n[2];
+    } else {
+      Cell<T> nc;
+      load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
+      T p[3], Gr[3];
+      hermite_edge(nc, c_grp_edge[axis][q], p, n, Gr);
+    }
+    if (sq3(n[0], n[1], n[2]) > T(0)) {
+      acc[0] = acc[0] + n[0];
+      acc[1] = acc[1] + n[1];
+      acc[2] = acc[2] + n[2];
+    }
+  }
+  normalize3(acc);
+  out[0] = acc[0];
+  out[1] = acc[1];
+  out[2] = acc[2];
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5). The
+// FP64 instance has the CPU restatement's exact operation sequence.
+template <typename T>
+__device__ __forceinline__ T eig_tol();
+template <>
+__device__ __forceinline__ double eig_tol<double>() {
+  return 1e-36;
+}
+template <>
+__device__ __forceinline__ float eig_tol<float>() {
+  return 1e-14f;
+}
+
+// |theta| above which theta*theta + 1 would overflow: use t = 1/(2 theta).
+template <typename T>
+__device__ __forceinline__ T eig_big();
+template <>
+__device__ __forceinline__ double eig_big<double>() {
+  return 1e150;
+}
+template <>
+__device__ __forceinline__ float eig_big<float>() {
+  return 1e18f;
+}
+
+template <typename T>
+__device__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3]) {
+  T a[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      a[r][s] = Min[r][s];
+      Q[r][s] = (r == s) ? T(1) : T(0);
+    }
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    const T off = (a[0][1] * a[0][1] + a[0][2] * a[0][2]) + a[1][2] * a[1][2];
+    const T dia = (a[0][0] * a[0][0] + a[1][1] * a[1][1]) + a[2][2] * a[2][2];
+    if (!(off > eig_tol<T>() * dia)) break;
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+      const int p = (pr == 2) ? 1 : 0;
+      const int q = (pr == 0) ? 1 : 2;
+      const int r = 3 - p - q;
+      const T apq = a[p][q];
+      if (apq == T(0)) continue;
+      const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+      const T at = fabs(theta);
+      T t;
+      if (at > eig_big<T>())
+        t = T(0.5) / theta;
+      else
+        t = T(1) / (at + tsqrt(theta * theta + T(1)));
+      if (theta < T(0)) t = -t;
+      const T c = T(1) / tsqrt(t * t + T(1));
+      const T s = t * c;
+      const T arp = a[r][p], arq = a[r][q];
+      a[p][p] = a[p][p] - t * apq;
+      a[q][q] = a[q][q] + t * apq;
+      a[p][q] = T(0);
+      a[q][p] = T(0);
+      const T nrp = c * arp - s * arq;
+      const T nrq = s * arp + c * arq;
+      a[r][p] = nrp;
+      a[p][r] = nrp;
+      a[r][q] = nrq;
+      a[q][r] = nrq;
+#pragma unroll
+      for (int row = 0; row < 3; ++row) {
+        const T qp = Q[row][p], qq = Q[row][q];
+        Q[row][p] = c * qp - s * qq;
+        Q[row][q] = s * qp + c * qq;
+      }
+    }
+  }
+  lam[0] = a[0][0];
+  lam[1] = a[1][1];
+  lam[2] = a[2][2];
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 plain solve: Cell::update (Cell.cpp:284-361) with rows streamed in ascending edge order,
+// matching the CPU restatement's sums term by term.
+// ---------------------------------------------------------------------------------------------
+__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                                int k, double x[3]) {
+  Cell<double> own;
+  load_cell(G, i, j, k, own);
+  double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
+  double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
+  int m = 0;
+  for (int e = 0; e < 12; ++e) {
+    double p[3], n[3], Gr[3];
+    if (!hermite_edge(own, e, p, n, Gr)) continue;
+    double nf[3];
+    final_normal(G, act, i, j, k, e, own, n, nf);
+    if (tsqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
+    const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
+    M00 = M00 + nf[0] * nf[0];
+    M01 = M01 + nf[0] * nf[1];
+    M02 = M02 + nf[0] * nf[2];
+    M11 = M11 + nf[1] * nf[1];
+    M12 = M12 + nf[1] * nf[2];
+    M22 = M22 + nf[2] * nf[2];
+    r0 = r0 + nf[0] * b;
+    r1 = r1 + nf[1] * b;
+    r2 = r2 + nf[2] * b;
+    c0 = c0 + p[0];
+    c1 = c1 + p[1];
+    c2 = c2 + p[2];
+    ++m;
+  }
+  const double M[3][3] = {{M00, M01, M02}, {M01, M11, M12}, {M02, M12, M22}};
+  double lam[3], Q[3][3];
+  sym3_eig(M, lam, Q);
+  const double dm = (double)m;
+  c0 = c0 / dm;
+  c1 = c1 / dm;
+  c2 = c2 / dm;
+  const double g0 = r0 - ((M00 * c0 + M01 * c1) + M02 * c2);
+  const double g1 = r1 - ((M01 * c0 + M11 * c1) + M12 * c2);
+  const double g2 = r2 - ((M02 * c0 + M12 * c1) + M22 * c2);
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!(fabs(lam[q]) > kSvdThreshold)) continue;
+    const double qg = (Q[0][q] * g0 + Q[1][q] * g1) + Q[2][q] * g2;
+    const double coef = qg / lam[q];
+    y0 = y0 + coef * Q[0][q];
+    y1 = y1 + coef * Q[1][q];
+    y2 = y2 + coef * Q[2][q];
+  }
+  x[0] = c0 + y0;
+  x[1] = c1 + y1;
+  x[2] = c2 + y2;
+}
+
+// Neumaier compensated sum.
+struct NSum {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+      c += (s - t) + x;
+    else
+      c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ float val() const { return s + c; }
+};
+
+// Mass-point-centred assembly in the cell frame: centroid c, M = sum n n^T, g = sum n (n.(p-c)).
+// Returns false when a hermite normal is degenerate (the FP32 caller then falls back).
+template <typename T>
+struct Qef {
+  T c[3];
+  T M[3][3];
+  T g[3];
+  int m;
+};
+
+__device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                              int k, const Cell<float>& own, Qef<float>& Q) {
+  NSum s0, s1, s2;
+  int m = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int a = c_edge[e][0], b = c_edge[e][1];
+    if (skip_edge(own.S[a], own.S[b])) continue;
+    float p[3];
+    hermite_point(own, e, p);
+    s0.add(p[0]);
+    s1.add(p[1]);
+    s2.add(p[2]);
+    ++m;
+  }
+  const float fm = (float)m;
+  Q.m = m;
+  Q.c[0] = s0.val() / fm;
+  Q.c[1] = s1.val() / fm;
+  Q.c[2] = s2.val() / fm;
+  NSum M00, M01, M02, M11, M12, M22, g0, g1, g2;
+  bool ok = true;
+  for (int e = 0; e < 12; ++e) {
+    float p[3], n[3], Gr[3];
+    if (!hermite_edge(own, e, p, n, Gr)) continue;
+    if (!(sq3(Gr[0], Gr[1], Gr[2]) > kFbTiny)) ok = false;
+    float nf[3];
+    final_normal(G, act, i, j, k, e, own, n, nf);
+    normalize3(nf);
+    const float q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
+    const float d = dot3(nf, q);
+    M00.add(nf[0] * nf[0]);
+    M01.add(nf[0] * nf[1]);
+    M02.add(nf[0] * nf[2]);
+    M11.add(nf[1] * nf[1]);
+    M12.add(nf[1] * nf[2]);
+    M22.add(nf[2] * nf[2]);
+    g0.add(nf[0] * d);
+    g1.add(nf[1] * d);
+    g2.add(nf[2] * d);
+  }
+  Q.M[0][0] = M00.val();
+  Q.M[0][1] = Q.M[1][0] = M01.val();
+  Q.M[0][2] = Q.M[2][0] = M02.val();
+  Q.M[1][1] = M11.val();
+  Q.M[1][2] = Q.M[2][1] = M12.val();
+  Q.M[2][2] = M22.val();
+  Q.g[0] = g0.val();
+  Q.g[1] = g1.val();
+  Q.g[2] = g2.val();
+  return ok;
+}
+
+// FP32 solve in the cell frame. Returns false when the cell must be solved in FP64.
+__device__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                                int k, float xl[3]) {
+  Cell<float> own;
+  load_cell(G, i, j, k, own);
+  Qef<float> Q;
+  if (!assemble_fp32(G, act, i, j, k, own, Q)) return false;
+  float lam[3], V[3][3];
+  sym3_eig(Q.M, lam, V);
+  float y[3] = {0.f, 0.f, 0.f};
+  bool ok = true;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float l = lam[q];
+    if (fabsf(fabsf(l) - (float)kSvdThreshold) < kFbBand) ok = false;
+    if (!(fabsf(l) > (float)kSvdThreshold)) continue;
+    if (l < kFbKept) ok = false;
+    const float coef = (V[0][q] * Q.g[0] + V[1][q] * Q.g[1] + V[2][q] * Q.g[2]) / l;
+    y[0] += coef * V[0][q];
+    y[1] += coef * V[1][q];
+    y[2] += coef * V[2][q];
+  }
+  xl[0] = Q.c[0] + y[0];
+  xl[1] = Q.c[1] + y[1];
+  xl[2] = Q.c[2] + y[2];
+  return ok;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Adjoint
+// ---------------------------------------------------------------------------------------------
+// VJP of the trilinear gradient with respect to the 8 corner values and the point.
+template <typename T>
+__device__ void trilinear_vjp(const Cell<T>& c, const T p[3], const T Gb[3], T Sb[8], T pb[3]) {
+  const T S000 = c.S[0], S100 = c.S[1], S110 = c.S[2], S010 = c.S[3];
+  const T S001 = c.S[4], S101 = c.S[5], S111 = c.S[6], S011 = c.S[7];
+  const T dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
+  const T eps = T(1e-12);
+  const bool hx = fabs(dx) > eps, hy = fabs(dy) > eps, hz = fabs(dz) > eps;
+  T u = T(0.5), v = T(0.5), w = T(0.5);
+  if (hx) u = (p[0] - c.X0) / dx;
+  if (hy) v = (p[1] - c.Y0) / dy;
+  if (hz) w = (p[2] - c.Z0) / dz;
+  const bool iu = hx && u >= T(0) && u <= T(1);
+  const bool iv = hy && v >= T(0) && v <= T(1);
+  const bool iw = hz && w >= T(0) && w <= T(1);
+  u = clamp01(u);
+  v = clamp01(v);
+  w = clamp01(w);
+  const T omu = T(1) - u, omv = T(1) - v, omw = T(1) - w;
+  const T a = hx ? Gb[0] / dx : T(0);
+  const T bb = hy ? Gb[1] / dy : T(0);
+  const T cc = hz ? Gb[2] / dz : T(0);
+  Sb[1] += a * omv * omw; Sb[0] -= a * omv * omw;
+  Sb[2] += a * v * omw;   Sb[3] -= a * v * omw;
+  Sb[5] += a * omv * w;   Sb[4] -= a * omv * w;
+  Sb[6] += a * v * w;     Sb[7] -= a * v * w;
+  Sb[3] += bb * omu * omw; Sb[0] -= bb * omu * omw;
+  Sb[2] += bb * u * omw;   Sb[1] -= bb * u * omw;
+  Sb[7] += bb * omu * w;   Sb[4] -= bb * omu * w;
+  Sb[6] += bb * u * w;     Sb[5] -= bb * u * w;
+  Sb[4] += cc * omu * omv; Sb[0] -= cc * omu * omv;
+  Sb[5] += cc * u * omv;   Sb[1] -= cc * u * omv;
+  Sb[7] += cc * omu * v;   Sb[3] -= cc * omu * v;
+  Sb[6] += cc * u * v;     Sb[2] -= cc * u * v;
+  const T ddu_dv = ((S110 - S010) - (S100 - S000)) * omw + ((S111 - S011) - (S101 - S001)) * w;
+  const T ddu_dw = ((S101 - S001) - (S100 - S000)) * omv + ((S111 - S011) - (S110 - S010)) * v;
+  const T ddv_du = ((S110 - S100) - (S010 - S000)) * omw + ((S111 - S101) - (S011 - S001)) * w;
+  const T ddv_dw = ((S011 - S001) - (S010 - S000)) * omu + ((S111 - S101) - (S110 - S100)) * u;
+  const T ddw_du = ((S101 - S100) - (S001 - S000)) * omv + ((S111 - S110) - (S011 - S010)) * v;
+  const T ddw_dv = ((S011 - S010) - (S001 - S000)) * omu + ((S111 - S110) - (S101 - S100)) * u;
+  const T ub = bb * ddv_du + cc * ddw_du;
+  const T vb = a * ddu_dv + cc * ddw_dv;
+  const T wb = a * ddu_dw + bb * ddv_dw;
+  if (iu) pb[0] += ub / dx;
+  if (iv) pb[1] += vb / dy;
+  if (iw) pb[2] += wb / dz;
+}
+
+// VJP of n = v/|v| (identity when |v| = 0, as Eigen's normalize leaves v unchanged).
+template <typename T>
+__device__ __forceinline__ void normalize_vjp(const T v[3], const T nb[3], T vb[3]) {
+  const T z = sq3(v[0], v[1], v[2]);
+  if (!(z > T(0))) {
+    vb[0] = nb[0];
+    vb[1] = nb[1];
+    vb[2] = nb[2];
+    return;
+  }
+  const T s = tsqrt(z);
+  const T n[3] = {v[0] / s, v[1] / s, v[2] / s};
+  const T d = dot3(n, nb);
+  vb[0] = (nb[0] - n[0] * d) / s;
+  vb[1] = (nb[1] - n[1] * d) / s;
+  vb[2] = (nb[2] - n[2] * d) / s;
+}
+
+// Adjoint of t = fa/(fa-fb+1e-16) given t-bar.
+template <typename T>
+__device__ __forceinline__ void lerp_vjp(const Cell<T>& c, int e, T tb, T Sb[8]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  const T fa = c.S[a], fb = c.S[b];
+  const T D = fa - fb + T(1e-16);
+  const T D2 = D * D;
+  Sb[a] += tb * (T(1e-16) - fb) / D2;
+  Sb[b] += tb * fa / D2;
+}
+
+template <typename T>
+__device__ __forceinline__ T edge_dir_dot(const Cell<T>& c, int e, const T pb[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  return (pb[0] * (c.cx(b) - c.cx(a)) + pb[1] * (c.cy(b) - c.cy(a))) +
+         pb[2] * (c.cz(b) - c.cz(a));
+}
+
+// hermite sample (cell, e): normalized-normal adjoint nb plus point adjoint pb -> corner values.
+template <typename T>
+__device__ void hermite_vjp(const Cell<T>& c, int e, const T nb[3], const T pb_in[3], T Sb[8]) {
+  T p[3], G[3];
+  hermite_point(c, e, p);
+  trilinear_grad(c, p, G);
+  T Gb[3];
+  normalize_vjp(G, nb, Gb);
+  T pb[3] = {pb_in[0], pb_in[1], pb_in[2]};
+  trilinear_vjp(c, p, Gb, Sb, pb);
+  lerp_vjp(c, e, edge_dir_dot(c, e, pb), Sb);
+}
+
+// Adds cell-corner adjoints of the cell at offset (oi,oj,ok) from the active cell into the
+// 4x4x4 local stencil (sample offsets -1..2 per axis).
+template <typename T>
+__device__ __forceinline__ void add_corners(float* lc, int oi, int oj, int ok, const T Sb[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = (ok + c_corner[q][2] + 1) * 16 + (oj + c_corner[q][1] + 1) * 4 +
+                    (oi + c_corner[q][0] + 1);
+    lc[idx] += (float)Sb[q];
+  }
+}
+
+// Full assembly in T precision for the adjoint (cell frame for float, global for double).
+template <typename T>
+__device__ void assemble_any(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                             int k, const Cell<T>& own, Qef<T>& Q) {
+  T c[3] = {T(0), T(0), T(0)};
+  int m = 0;
+  for (int e = 0; e < 12; ++e) {
+    const int a = c_edge[e][0], b = c_edge[e][1];
+    if (skip_edge(own.S[a], own.S[b])) continue;
+    T p[3];
+    hermite_point(own, e, p);
+    c[0] += p[0];
+    c[1] += p[1];
+    c[2] += p[2];
+    ++m;
+  }
+  Q.m = m;
+  for (int d = 0; d < 3; ++d) Q.c[d] = c[d] / T(m);
+  for (int r = 0; r < 3; ++r) {
+    Q.g[r] = T(0);
+    for (int s = 0; s < 3; ++s) Q.M[r][s] = T(0);
+  }
+  for (int e = 0; e < 12; ++e) {
+    T p[3], n[3], Gr[3];
+    if (!hermite_edge(own, e, p, n, Gr)) continue;
+    T nf[3];
+    final_normal(G, act, i, j, k, e, own, n, nf);
+    normalize3(nf);
+    const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
+    const T d = dot3(nf, q);
+    for (int r = 0; r < 3; ++r) {
+      Q.g[r] += nf[r] * d;
+      for (int s = 0; s < 3; ++s) Q.M[r][s] += nf[r] * nf[s];
+    }
+  }
+}
+
+// dL/dphi contributions of one active cell into lc[64] (the 4x4x4 stencil around it).
+template <typename T>
+__device__ void backward_cell(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
+                              int k, const float xb_in[3], float* lc) {
+  Cell<T> own;
+  load_cell(G, i, j, k, own);
+  Qef<T> Q;
+  assemble_any(G, act, i, j, k, own, Q);
+  T lam[3], V[3][3];
+  sym3_eig(Q.M, lam, V);
+  const T xb[3] = {(T)xb_in[0], (T)xb_in[1], (T)xb_in[2]};
+  bool kept[3];
+  T gb[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    kept[q] = fabs(lam[q]) > T(kSvdThreshold);
+    if (!kept[q]) continue;
+    const T qx = (V[0][q] * xb[0] + V[1][q] * xb[1]) + V[2][q] * xb[2];
+    for (int d = 0; d < 3; ++d) gb[d] += V[d][q] * qx / lam[q];
+  }
+  // Daleckii-Krein: M-bar = V (F o (V^T sym(xb g^T) V)) V^T
+  T B[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      T s = T(0);
+      for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w)
+          s += V[u][a] * (T(0.5) * (xb[u] * Q.g[w] + Q.g[u] * xb[w])) * V[w][b];
+      T F;
+      if (kept[a] && kept[b])
+        F = T(-1) / (lam[a] * lam[b]);
+      else if (!kept[a] && !kept[b])
+        F = T(0);
+      else
+        F = ((kept[a] ? T(1) / lam[a] : T(0)) - (kept[b] ? T(1) / lam[b] : T(0))) /
+            (lam[a] - lam[b]);
+      B[a][b] = F * s;
+    }
+  T Mb[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      T s = T(0);
+      for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w) s += V[a][u] * B[u][w] * V[b][w];
+      Mb[a][b] = s;
+    }
+  // c-bar total = xb - M gb (every row's q = p - c pulls on c)
+  T cbm[3];
+  for (int a = 0; a < 3; ++a)
+    cbm[a] = (xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2])) / T(Q.m);
+  T Sown[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  const T zero3[3] = {T(0), T(0), T(0)};
+  for (int e = 0; e < 12; ++e) {
+    T p[3], n_raw[3], Gr[3];
+    if (!hermite_edge(own, e, p, n_raw, Gr)) continue;
+    // forward of the normal chain
+    int cells[4][3], axis;
+    const int ea = c_edge[e][0], eb = c_edge[e][1];
+    const bool avg = edge_group(G, act, i, j, k, e, strict_change(own.S[ea], own.S[eb]), cells,
+                                axis);
+    T acc[3] = {T(0), T(0), T(0)};
+    bool contrib[4] = {false, false, false, false};
+    T nf[3];
+    if (avg) {
+      for (int q = 0; q < 4; ++q) {
+        T n[3];
+        if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
+          n[0] = n_raw[0];
+          n[1] = n_raw[1];
+          n[2] = n_raw[2];
+        } else {
+          Cell<T> nc;
+          load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
+          T pp[3], GG[3];
+          hermite_edge(nc, c_grp_edge[axis][q], pp, n, GG);
+        }
+        contrib[q] = sq3(n[0], n[1], n[2]) > T(0);
+        if (contrib[q]) {
+          acc[0] += n[0];
+          acc[1] += n[1];
+          acc[2] += n[2];
+        }
+      }
+      nf[0] = acc[0];
+      nf[1] = acc[1];
+      nf[2] = acc[2];
+      normalize3(nf);
+    } else {
+      nf[0] = n_raw[0];
+      nf[1] = n_raw[1];
+      nf[2] = n_raw[2];
+    }
+    T n[3] = {nf[0], nf[1], nf[2]};
+    normalize3(n);
+    const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
+    const T nq = dot3(n, q), ng = dot3(n, gb);
+    T nb[3], pb[3];
+    for (int d = 0; d < 3; ++d) {
+      const T Mn = (T(2) * Mb[d][0] * n[0] + T(2) * Mb[d][1] * n[1]) + T(2) * Mb[d][2] * n[2];
+      nb[d] = Mn + gb[d] * nq + q[d] * ng;
+      pb[d] = n[d] * ng + cbm[d];
+    }
+    T ub[3];
+    normalize_vjp(nf, nb, ub);  // row renormalization
+    if (avg) {
+      T accb[3];
+      normalize_vjp(acc, ub, accb);
+      for (int cq = 0; cq < 4; ++cq) {
+        if (!contrib[cq]) continue;
+        Cell<T> nc;
+        load_cell(G, cells[cq][0], cells[cq][1], cells[cq][2], nc);
+        T Sb[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+        hermite_vjp(nc, (int)c_grp_edge[axis][cq], accb, zero3, Sb);
+        add_corners(lc, cells[cq][0] - i, cells[cq][1] - j, cells[cq][2] - k, Sb);
+      }
+      lerp_vjp(own, e, edge_dir_dot(own, e, pb), Sown);
+    } else {
+      hermite_vjp(own, e, ub, pb, Sown);
+    }
+  }
+  add_corners(lc, 0, 0, 0, Sown);
+}
+
+}  // namespace dccell

@@ -50,8 +50,8 @@ struct dc_hip_context {
   int64_t nv = 0, nf = 0;
   uint64_t capA = 0, capF = 0;
   std::vector<DcGridDev> hgrids;
-  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, hbounds;
-  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0;
+  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, dil_b, hbounds;
+  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_dil = 0;
 };
 
 namespace {
@@ -104,7 +104,8 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->act_b.assign(n + 1, 0);
   c->pen_b.assign(n + 1, 0);
   c->samp_b.assign(n + 1, 0);
-  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0;
+  c->dil_b.assign(n + 1, 0);
+  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, dil = 0;
   for (int q = 0; q < n; ++q) {
     const dc_hip_grid& s = grids[q];
     if (s.nx < 2 || s.ny < 2 || s.nz < 2)
@@ -122,6 +123,8 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     g.cz = s.nz - 1;
     g.W = (g.cx + 63) / 64;
     g.Wk = (g.cz + 63) / 64;
+    g.Ws = (g.nx + 63) / 64;
+    g.pad_ = 0;
     g.vx = s.vx;
     g.vy = s.vy;
     g.vz = s.vz;
@@ -132,20 +135,24 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     g.ft_off = ftw;
     g.pen_off = pen;
     g.samp_off = samp;
+    g.dil_off = dil;
     c->tile_b[q] = tiles;
     c->act_b[q] = words;
     c->pen_b[q] = pen;
     c->samp_b[q] = samp;
+    c->dil_b[q] = dil;
     words += (int64_t)g.W * g.cy * g.cz;
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
     tiles += dc_mark_tiles(g, c->mark_rows);
     samp += (int64_t)g.nx * g.ny * g.nz;
+    dil += (int64_t)g.Ws * g.ny * g.nz;
   }
   c->tile_b[n] = tiles;
   c->act_b[n] = words;
   c->pen_b[n] = pen;
   c->samp_b[n] = samp;
+  c->dil_b[n] = dil;
   if ((uint64_t)words * 64ull >= (1ull << 32) * 4ull)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "batch has too many cells for 32-bit vertex ids");
   c->n_words = words;
@@ -153,6 +160,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->n_pen = pen;
   c->n_tiles = tiles;
   c->n_samples = samp;
+  c->n_dil = dil;
   return DC_HIP_OK;
 }
 
@@ -168,7 +176,7 @@ int run_once(dc_hip_context* c, int precision) {
   const int n = (int)c->hgrids.size();
   int rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
-  if ((rc = ensure(c, c->bounds, 4 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
+  if ((rc = ensure(c, c->bounds, 5 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
   if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
   if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
@@ -191,6 +199,7 @@ int run_once(dc_hip_context* c, int precision) {
   bounds.insert(bounds.end(), c->act_b.begin(), c->act_b.end());
   bounds.insert(bounds.end(), c->pen_b.begin(), c->pen_b.end());
   bounds.insert(bounds.end(), c->samp_b.begin(), c->samp_b.end());
+  bounds.insert(bounds.end(), c->dil_b.begin(), c->dil_b.end());
   DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
                            hipMemcpyHostToDevice, s), "upload descriptors");
   DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
@@ -527,12 +536,53 @@ int dc_hip_result_device(dc_hip_context* c, const float** vertices, const double
 
 int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_values,
                     dc_hip_stats* stats) {
-  (void)grad_vertices;
-  (void)grad_values;
-  (void)stats;
-  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c || !grad_values) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
   if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
-  return fail(c, DC_HIP_ERR_INTERNAL, "backward not available");
+  if (c->nv > 0 && !grad_vertices) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL grad_vertices");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = (int)c->hgrids.size();
+  int rc;
+  if ((rc = ensure(c, c->bwd_contrib, (size_t)std::max<int64_t>(c->nv, 1) * 64 * 4,
+                   "adjoint contributions")))
+    return rc;
+  if ((rc = ensure(c, c->bwd_grad, c->n_dil * 8, "candidate plane"))) return rc;
+  hipStream_t s = c->stream;
+  const int64_t* d_tile_b = as<int64_t>(c->bounds);
+  DcBwdArgs a;
+  a.grids = as<DcGridDev>(c->grids);
+  a.n_grids = n;
+  a.act_bounds = d_tile_b + (n + 1);
+  a.samp_bounds = d_tile_b + 3 * (n + 1);
+  a.dil_bounds = d_tile_b + 4 * (n + 1);
+  a.n_dil = c->n_dil;
+  a.n_samples = c->n_samples;
+  a.act = as<uint64_t>(c->act);
+  a.wprefix = as<uint32_t>(c->wprefix);
+  a.alist = as<uint64_t>(c->alist);
+  a.flags = as<uint32_t>(c->flags);
+  a.precision = c->precision;
+  a.ctr = as<DcCounters>(c->counters);
+  a.capA = c->capA;
+  a.cell_blocks = std::max<int64_t>(1, std::min<int64_t>((c->nv + 63) / 64,
+                                                          (int64_t)c->cu_count * 32));
+  a.grad_vertices = grad_vertices;
+  a.contrib = as<float>(c->bwd_contrib);
+  a.dil = as<uint64_t>(c->bwd_grad);
+  a.grad_values = grad_values;
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
+  DC_TRY(c, dc_launch_backward(a, s), "backward");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
+  DC_TRY(c, hipStreamSynchronize(s), "backward");
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->vertex_count = c->nv;
+    stats->face_count = c->nf;
+    stats->backward_ms = c->timing ? elapsed(c, 6, 7) : 0.f;
+    stats->total_ms =
+        std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return DC_HIP_OK;
 }
 
 const char* dc_hip_last_error(dc_hip_context* c) {

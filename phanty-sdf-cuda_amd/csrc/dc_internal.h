@@ -18,6 +18,9 @@
 //   alist     uint64 active cells in vertex order: global act word * 64 + bit
 //   v64/v32   vertices (FP64 for the plain solve, FP32 output), 3 per active cell
 //   faces     int32 quads, 4 per face, grid-local vertex ids
+//   contrib   float[64] per active cell: its dL/dphi contributions to the 4x4x4 samples around it
+//   dil       uint64 candidate-sample plane: bit x of word (z*ny + y)*Ws + x/64 set when an
+//             active cell's 4x4x4 stencil contains sample (x,y,z)
 #pragma once
 #include <stdint.h>
 
@@ -28,11 +31,14 @@ struct DcGridDev {
   int32_t nx, ny, nz;
   int32_t cx, cy, cz;
   int32_t W, Wk;
+  int32_t Ws;        // 64-bit words per sample row = ceil(nx/64) (candidate plane of the adjoint)
+  int32_t pad_;
   float vx, vy, vz, ox, oy, oz;
   int64_t act_off;   // first act word
   int64_t ft_off;    // first ft word
   int64_t pen_off;   // first pencil (pcount / pbase index)
   int64_t samp_off;  // first sample in a concatenated gradient buffer (backward)
+  int64_t dil_off;   // first word of the adjoint's candidate-sample plane
 };
 
 struct DcCounters {

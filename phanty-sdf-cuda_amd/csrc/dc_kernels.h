@@ -53,6 +53,28 @@ struct DcEmitArgs {
   int32_t* faces;
 };
 
+struct DcBwdArgs {
+  const DcGridDev* grids;
+  int n_grids;
+  const int64_t* act_bounds;
+  const int64_t* dil_bounds;
+  const int64_t* samp_bounds;
+  int64_t n_dil;
+  int64_t n_samples;
+  const uint64_t* act;
+  const uint32_t* wprefix;
+  const uint64_t* alist;
+  const uint32_t* flags;
+  int precision;
+  const DcCounters* ctr;
+  uint64_t capA;
+  int64_t cell_blocks;
+  const float* grad_vertices;
+  float* contrib;
+  uint64_t* dil;
+  float* grad_values;
+};
+
 hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int64_t dc_mark_tiles(const DcGridDev& g, int rows);
@@ -67,3 +89,4 @@ hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t
                               hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
+hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s);

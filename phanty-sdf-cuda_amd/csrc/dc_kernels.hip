@@ -9,10 +9,12 @@
 //               active-cell list in vertex order.
 //   k_scan_pen  decoupled look-back scan over pencil face counts in the CPU backend's face
 //               order (axis, i, j): first face index of every pencil.
-//   k_qef64     one thread per active cell: hermite samples, trilinear normals, 4-cell normal
-//               averaging, QEF solve (FP64, cyclic Jacobi) -> vertex.
+//   k_qef       one thread per active cell: hermite samples, trilinear normals, 4-cell normal
+//               averaging, QEF solve (FP64 plain, or FP32 with FP64 fallback) -> vertex.
 //   k_emit      one thread per pencil: walks the face bits along k and writes oriented quads in
 //               the CPU backend's order.
+//   k_bwd_*     adjoint: per-cell stencil contributions (k_bwd_cell), the candidate-sample plane
+//               (k_bwd_dilate) and a fixed-order per-sample gather (k_bwd_gather).
 //
 // Reference semantics (file:line in the reference tree) are cited at each device function.
 // Compiled with -ffp-contract=off so that FP64 results match the CPU restatement bit for bit.
@@ -335,308 +337,13 @@ __global__ void __launch_bounds__(kScanThreads)
   if (threadIdx.x == 0 && (tile + 1) * kScanTile >= n_pen) ctr->n_faces = s_prefix + total;
 }
 
-// ---------------------------------------------------------------------------------------------
-// Per-cell geometry shared by the solve and the emission (FP64, reference operation order)
-// ---------------------------------------------------------------------------------------------
-// contouring.cpp:232-241 corner order; include/Cell.h:26-30 edge pairs.
-__device__ __constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-__device__ __constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                                {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+}  // namespace
 
-// dc_backend.cpp:63-65: FP32 multiply then FP32 add, widened.
-__device__ __forceinline__ double coordd(float o, int i, float v) {
-  return (double)__fadd_rn(o, __fmul_rn((float)i, v));
-}
+#include "dc_cell.h"
 
-__device__ __forceinline__ double sq3(double x, double y, double z) {
-  return (x * x + y * y) + z * z;
-}
+namespace {
 
-struct DCell {
-  double S[8];
-  double X0, X1, Y0, Y1, Z0, Z1;
-};
-
-__device__ __forceinline__ float sample(const DcGridDev& G, int i, int j, int k) {
-  return G.val[(int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k)];
-}
-
-__device__ __forceinline__ void load_dcell(const DcGridDev& G, int i, int j, int k, DCell& c) {
-  const int64_t b = (int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k);
-  const int64_t sx = 1, sy = G.nx, sz = (int64_t)G.nx * G.ny;
-  c.S[0] = (double)G.val[b];
-  c.S[1] = (double)G.val[b + sx];
-  c.S[2] = (double)G.val[b + sx + sy];
-  c.S[3] = (double)G.val[b + sy];
-  c.S[4] = (double)G.val[b + sz];
-  c.S[5] = (double)G.val[b + sx + sz];
-  c.S[6] = (double)G.val[b + sx + sy + sz];
-  c.S[7] = (double)G.val[b + sy + sz];
-  c.X0 = coordd(G.ox, i, G.vx);
-  c.X1 = coordd(G.ox, i + 1, G.vx);
-  c.Y0 = coordd(G.oy, j, G.vy);
-  c.Y1 = coordd(G.oy, j + 1, G.vy);
-  c.Z0 = coordd(G.oz, k, G.vz);
-  c.Z1 = coordd(G.oz, k + 1, G.vz);
-}
-
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double clamp01d(double t) { return dmax(0.0, dmin(1.0, t)); }
-
-// Cell.cpp:80-155 gradientAtPointUsingTrilinear
-__device__ __forceinline__ void trilinear_grad(const DCell& c, const double p[3], double g[3]) {
-  const double S000 = c.S[0], S100 = c.S[1], S110 = c.S[2], S010 = c.S[3];
-  const double S001 = c.S[4], S101 = c.S[5], S111 = c.S[6], S011 = c.S[7];
-  const double dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
-  const double eps = 1e-12;
-  double u = 0.5, v = 0.5, w = 0.5;
-  if (fabs(dx) > eps) u = (p[0] - c.X0) / dx;
-  if (fabs(dy) > eps) v = (p[1] - c.Y0) / dy;
-  if (fabs(dz) > eps) w = (p[2] - c.Z0) / dz;
-  u = clamp01d(u);
-  v = clamp01d(v);
-  w = clamp01d(w);
-  const double omu = 1.0 - u, omv = 1.0 - v, omw = 1.0 - w;
-  const double du = (S100 - S000) * omv * omw + (S110 - S010) * v * omw +
-                    (S101 - S001) * omv * w + (S111 - S011) * v * w;
-  const double dv = (S010 - S000) * omu * omw + (S110 - S100) * u * omw +
-                    (S011 - S001) * omu * w + (S111 - S101) * u * w;
-  const double dw = (S001 - S000) * omu * omv + (S101 - S100) * u * omv +
-                    (S011 - S010) * omu * v + (S111 - S110) * u * v;
-  g[0] = 0.0;
-  g[1] = 0.0;
-  g[2] = 0.0;
-  if (fabs(dx) > eps) g[0] = du / dx;
-  if (fabs(dy) > eps) g[1] = dv / dy;
-  if (fabs(dz) > eps) g[2] = dw / dz;
-}
-
-__device__ __forceinline__ void normalize3(double v[3]) {
-  const double z = sq3(v[0], v[1], v[2]);
-  if (z > 0.0) {
-    const double s = sqrt(z);
-    v[0] = v[0] / s;
-    v[1] = v[1] / s;
-    v[2] = v[2] / s;
-  }
-}
-
-__device__ __forceinline__ double cornerx(const DCell& c, int q) { return c_corner[q][0] ? c.X1 : c.X0; }
-__device__ __forceinline__ double cornery(const DCell& c, int q) { return c_corner[q][1] ? c.Y1 : c.Y0; }
-__device__ __forceinline__ double cornerz(const DCell& c, int q) { return c_corner[q][2] ? c.Z1 : c.Z0; }
-
-// Cell.cpp:180-209 for one edge
-__device__ __forceinline__ bool hermite_edge(const DCell& c, int e, double p[3], double n[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  const double fa = c.S[a], fb = c.S[b];
-  if (fa * fb > 0.0) return false;
-  const double t = fa / (fa - fb + 1e-16);
-  p[0] = (1.0 - t) * cornerx(c, a) + t * cornerx(c, b);
-  p[1] = (1.0 - t) * cornery(c, a) + t * cornery(c, b);
-  p[2] = (1.0 - t) * cornerz(c, a) + t * cornerz(c, b);
-  trilinear_grad(c, p, n);
-  normalize3(n);
-  return true;
-}
-
-__device__ __forceinline__ bool cell_active(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                            int i, int j, int k) {
-  const uint64_t w = act[G.act_off + ((int64_t)k * G.cy + j) * G.W + (i >> 6)];
-  return (w >> (i & 63)) & 1ull;
-}
-
-// The 4 cells around the grid edge carried by local edge e, in the order process_edge visits
-// them (hermite_update.cpp:171-332): [base, base+dir1, base+dir2, base+dir1+dir2] with base the
-// cell whose min corner is the edge's lower endpoint and (dir1, dir2) from
-// get_face_normals_from_edge_idx (Cell.cpp:26-45). Local edge ids per axis:
-//   X: base edge 0, -Y edge 2, -Z edge 4, -Y-Z edge 6
-//   Y: base edge 3, -X edge 1, -Z edge 7, -X-Z edge 5
-//   Z: base edge 8, -X edge 9, -Y edge 11, -X-Y edge 10
-__device__ __constant__ int8_t c_grp_edge[3][4] = {{0, 2, 4, 6}, {3, 1, 7, 5}, {8, 9, 11, 10}};
-// (d1, d2) axes per edge axis: X -> (-Y, -Z), Y -> (-X, -Z), Z -> (-X, -Y)
-__device__ __constant__ int8_t c_grp_dirs[3][2] = {{1, 2}, {0, 2}, {0, 1}};
-
-// Returns true and the averaged normal when local edge e of cell (i,j,k) is averaged
-// (hermite_update.cpp:334-438). own_n is the cell's own (normalized) hermite normal for e.
-__device__ bool averaged_normal(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                                int k, int e, const DCell& own, const double own_n[3],
-                                double out[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  int P0[3] = {i + c_corner[a][0], j + c_corner[a][1], k + c_corner[a][2]};
-  int P1[3] = {i + c_corner[b][0], j + c_corner[b][1], k + c_corner[b][2]};
-  int axis = (P0[0] != P1[0]) ? 0 : ((P0[1] != P1[1]) ? 1 : 2);
-  if (P1[axis] < P0[axis]) {
-    for (int d = 0; d < 3; ++d) {
-      const int t = P0[d];
-      P0[d] = P1[d];
-      P1[d] = t;
-    }
-  }
-  const bool in0 = P0[0] > 0 && P0[0] < G.nx - 1 && P0[1] > 0 && P0[1] < G.ny - 1 && P0[2] > 0 &&
-                   P0[2] < G.nz - 1;
-  const bool in1 = P1[0] > 0 && P1[0] < G.nx - 1 && P1[1] > 0 && P1[1] < G.ny - 1 && P1[2] > 0 &&
-                   P1[2] < G.nz - 1;
-  if (!in0 && !in1) return false;
-  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
-  int cells[4][3];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    cells[q][0] = P0[0];
-    cells[q][1] = P0[1];
-    cells[q][2] = P0[2];
-  }
-  cells[1][d1] -= 1;
-  cells[2][d2] -= 1;
-  cells[3][d1] -= 1;
-  cells[3][d2] -= 1;
-  // A strict sign change on the edge makes all four cells active; otherwise look them up.
-  const double fa = own.S[a], fb = own.S[b];
-  if (!(fa * fb < 0.0)) {
-    for (int q = 0; q < 4; ++q)
-      if (!cell_active(G, act, cells[q][0], cells[q][1], cells[q][2])) return false;
-  }
-  double acc[3] = {0.0, 0.0, 0.0};
-  for (int q = 0; q < 4; ++q) {
-    double n[3];
-    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
-      n[0] = own_n[0];
-      n[1] = own_n[1];
-      n[2] = own_n[2];
-    } else {
-      DCell nc;
-      load_dcell(G, cells[q][0], cells[q][1], cells[q][2], nc);
-      double p[3];
-      hermite_edge(nc, c_grp_edge[axis][q], p, n);
-    }
-    if (sq3(n[0], n[1], n[2]) > 0.0) {
-      acc[0] = acc[0] + n[0];
-      acc[1] = acc[1] + n[1];
-      acc[2] = acc[2] + n[2];
-    }
-  }
-  normalize3(acc);
-  out[0] = acc[0];
-  out[1] = acc[1];
-  out[2] = acc[2];
-  return true;
-}
-
-// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5);
-// the same operation sequence as the CPU restatement.
-__device__ void sym3_eig(const double Min[3][3], double lam[3], double Q[3][3]) {
-  double a[3][3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      a[r][s] = Min[r][s];
-      Q[r][s] = (r == s) ? 1.0 : 0.0;
-    }
-  for (int sweep = 0; sweep < 24; ++sweep) {
-    const double off = (a[0][1] * a[0][1] + a[0][2] * a[0][2]) + a[1][2] * a[1][2];
-    const double dia = (a[0][0] * a[0][0] + a[1][1] * a[1][1]) + a[2][2] * a[2][2];
-    if (!(off > 1e-36 * dia)) break;
-#pragma unroll
-    for (int pr = 0; pr < 3; ++pr) {
-      const int p = (pr == 2) ? 1 : 0;
-      const int q = (pr == 0) ? 1 : 2;
-      const int r = 3 - p - q;
-      const double apq = a[p][q];
-      if (apq == 0.0) continue;
-      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-      const double at = fabs(theta);
-      double t;
-      if (at > 1e150)
-        t = 0.5 / theta;
-      else
-        t = 1.0 / (at + sqrt(theta * theta + 1.0));
-      if (theta < 0.0) t = -t;
-      const double c = 1.0 / sqrt(t * t + 1.0);
-      const double s = t * c;
-      const double arp = a[r][p], arq = a[r][q];
-      a[p][p] = a[p][p] - t * apq;
-      a[q][q] = a[q][q] + t * apq;
-      a[p][q] = 0.0;
-      a[q][p] = 0.0;
-      const double nrp = c * arp - s * arq;
-      const double nrq = s * arp + c * arq;
-      a[r][p] = nrp;
-      a[p][r] = nrp;
-      a[r][q] = nrq;
-      a[q][r] = nrq;
-#pragma unroll
-      for (int row = 0; row < 3; ++row) {
-        const double qp = Q[row][p], qq = Q[row][q];
-        Q[row][p] = c * qp - s * qq;
-        Q[row][q] = s * qp + c * qq;
-      }
-    }
-  }
-  lam[0] = a[0][0];
-  lam[1] = a[1][1];
-  lam[2] = a[2][2];
-}
-
-// Cell::update + solve_quadratic_system + getCentroid (Cell.cpp:284-361, 586-600), FP64.
-// Rows are streamed in ascending edge order, so the sums match the CPU restatement exactly.
-__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                                int k, double x[3]) {
-  DCell own;
-  load_dcell(G, i, j, k, own);
-  double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
-  double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
-  int m = 0;
-  for (int e = 0; e < 12; ++e) {
-    double p[3], n[3];
-    if (!hermite_edge(own, e, p, n)) continue;
-    double nf[3];
-    if (!averaged_normal(G, act, i, j, k, e, own, n, nf)) {
-      nf[0] = n[0];
-      nf[1] = n[1];
-      nf[2] = n[2];
-    }
-    if (sqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
-    const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
-    M00 = M00 + nf[0] * nf[0];
-    M01 = M01 + nf[0] * nf[1];
-    M02 = M02 + nf[0] * nf[2];
-    M11 = M11 + nf[1] * nf[1];
-    M12 = M12 + nf[1] * nf[2];
-    M22 = M22 + nf[2] * nf[2];
-    r0 = r0 + nf[0] * b;
-    r1 = r1 + nf[1] * b;
-    r2 = r2 + nf[2] * b;
-    c0 = c0 + p[0];
-    c1 = c1 + p[1];
-    c2 = c2 + p[2];
-    ++m;
-  }
-  const double M[3][3] = {{M00, M01, M02}, {M01, M11, M12}, {M02, M12, M22}};
-  double lam[3], Q[3][3];
-  sym3_eig(M, lam, Q);
-  const double dm = (double)m;
-  c0 = c0 / dm;
-  c1 = c1 / dm;
-  c2 = c2 / dm;
-  const double g0 = r0 - ((M00 * c0 + M01 * c1) + M02 * c2);
-  const double g1 = r1 - ((M01 * c0 + M11 * c1) + M12 * c2);
-  const double g2 = r2 - ((M02 * c0 + M12 * c1) + M22 * c2);
-  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (!(fabs(lam[q]) > 0.01)) continue;
-    const double qg = (Q[0][q] * g0 + Q[1][q] * g1) + Q[2][q] * g2;
-    const double coef = qg / lam[q];
-    y0 = y0 + coef * Q[0][q];
-    y1 = y1 + coef * Q[1][q];
-    y2 = y2 + coef * Q[2][q];
-  }
-  x[0] = c0 + y0;
-  x[1] = c1 + y1;
-  x[2] = c2 + y2;
-}
+using namespace dccell;
 
 __device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, int& i, int& j,
                                             int& k) {
@@ -648,28 +355,147 @@ __device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, 
   k = (int)(row / G.cy);
 }
 
+// One thread per active cell (persistent grid-stride over the device-side count).
+template <int PREC>
 __global__ void __launch_bounds__(256)
-    k_qef64(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
-            const uint64_t* __restrict__ act, const uint64_t* __restrict__ alist,
-            const DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
-            float* __restrict__ v32) {
+    k_qef(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
+          const uint64_t* __restrict__ act, const uint64_t* __restrict__ alist,
+          DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
+          float* __restrict__ v32, uint32_t* __restrict__ flags) {
+  const uint64_t n = min(ctr->n_active, cap);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // iterate whole waves so the fallback ballot below sees a converged wave
+  const uint64_t limit = ((n + stride - 1) / stride) * stride;
+  for (uint64_t v = first; v < limit; v += stride) {
+    bool fb = false;
+    if (v < n) {
+      const uint64_t entry = alist[v];
+      const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
+      const DcGridDev G = grids[g];
+      int i, j, k;
+      decode_cell(G, entry, i, j, k);
+      double x[3];
+      if (PREC == DC_PREC_FP64) {
+        solve_cell_fp64(G, act, i, j, k, x);
+        v64[3 * v + 0] = x[0];
+        v64[3 * v + 1] = x[1];
+        v64[3 * v + 2] = x[2];
+      } else {
+        float xl[3];
+        if (solve_cell_fp32(G, act, i, j, k, xl)) {
+          x[0] = coordd(G.ox, i, G.vx) + (double)xl[0];
+          x[1] = coordd(G.oy, j, G.vy) + (double)xl[1];
+          x[2] = coordd(G.oz, k, G.vz) + (double)xl[2];
+        } else {
+          solve_cell_fp64(G, act, i, j, k, x);
+          fb = true;
+        }
+        flags[v] = fb ? 1u : 0u;
+      }
+      v32[3 * v + 0] = (float)x[0];
+      v32[3 * v + 1] = (float)x[1];
+      v32[3 * v + 2] = (float)x[2];
+    }
+    if (PREC == DC_PREC_FP32) {
+      const uint64_t nfb = __popcll(__ballot(fb));
+      if (nfb && (threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&ctr->fallback, nfb);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Adjoint: per-cell stencil contributions, then a deterministic gather per sample.
+// ---------------------------------------------------------------------------------------------
+constexpr int kBwdThreads = 64;
+constexpr int kStencil = 64;  // 4x4x4 samples around a cell: offsets -1..2 per axis
+
+__global__ void __launch_bounds__(kBwdThreads)
+    k_bwd_cell(const DcGridDev* __restrict__ grids, int n_grids,
+               const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
+               const uint64_t* __restrict__ alist, const uint32_t* __restrict__ flags,
+               int precision, const DcCounters* __restrict__ ctr, uint64_t cap,
+               const float* __restrict__ gv, float* __restrict__ contrib) {
+  __shared__ float s_lc[kBwdThreads][kStencil + 1];
+  float* lc = s_lc[threadIdx.x];
   const uint64_t n = min(ctr->n_active, cap);
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * blockDim.x) {
+    for (int q = 0; q < kStencil; ++q) lc[q] = 0.f;
     const uint64_t entry = alist[v];
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
     decode_cell(G, entry, i, j, k);
-    double x[3];
-    solve_cell_fp64(G, act, i, j, k, x);
-    v64[3 * v + 0] = x[0];
-    v64[3 * v + 1] = x[1];
-    v64[3 * v + 2] = x[2];
-    v32[3 * v + 0] = (float)x[0];
-    v32[3 * v + 1] = (float)x[1];
-    v32[3 * v + 2] = (float)x[2];
+    const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
+    if (precision == DC_PREC_FP64 || flags[v])
+      backward_cell<double>(G, act, i, j, k, xb, lc);
+    else
+      backward_cell<float>(G, act, i, j, k, xb, lc);
+    float* out = contrib + v * kStencil;
+    for (int q = 0; q < kStencil; ++q) out[q] = lc[q];
   }
+}
+
+// Candidate samples: bit x of word (z*ny + y)*Ws + x/64 is set when some active cell lies in
+// [x-2, x+1] x [y-2, y+1] x [z-2, z+1] (the cells whose 4x4x4 stencil contains the sample).
+__global__ void __launch_bounds__(256)
+    k_bwd_dilate(const DcGridDev* __restrict__ grids, int n_grids,
+                 const int64_t* __restrict__ dil_bounds, int64_t n_dil,
+                 const uint64_t* __restrict__ act, uint64_t* __restrict__ dil) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_dil) return;
+  const int g = find_bucket(dil_bounds, n_grids, t);
+  const DcGridDev G = grids[g];
+  const int64_t local = t - dil_bounds[g];
+  const int64_t row = local / G.Ws;
+  const int c = (int)(local - row * G.Ws);
+  const int y = (int)(row % G.ny), z = (int)(row / G.ny);
+  uint64_t am = 0, a0 = 0, ap = 0;  // OR of chunk c-1, c, c+1 over the 16 cell rows
+  for (int kz = max(z - 2, 0); kz <= min(z + 1, G.cz - 1); ++kz)
+    for (int jy = max(y - 2, 0); jy <= min(y + 1, G.cy - 1); ++jy) {
+      const uint64_t* w = act + G.act_off + ((int64_t)kz * G.cy + jy) * G.W;
+      if (c - 1 >= 0 && c - 1 < G.W) am |= w[c - 1];
+      if (c < G.W) a0 |= w[c];
+      if (c + 1 < G.W) ap |= w[c + 1];
+    }
+  const uint64_t d = a0 | ((a0 >> 1) | (ap << 63)) | ((a0 << 1) | (am >> 63)) |
+                     ((a0 << 2) | (am >> 62));
+  dil[G.dil_off + local] = d;
+}
+
+__global__ void __launch_bounds__(256)
+    k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
+                 const int64_t* __restrict__ samp_bounds, int64_t n_samples,
+                 const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
+                 const uint64_t* __restrict__ dil, const float* __restrict__ contrib,
+                 float* __restrict__ grad) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_samples) return;
+  const int g = find_bucket(samp_bounds, n_grids, s);
+  const DcGridDev G = grids[g];
+  const int64_t local = s - samp_bounds[g];
+  const int64_t row = local / G.nx;
+  const int x = (int)(local - row * G.nx);
+  const int y = (int)(row % G.ny), z = (int)(row / G.ny);
+  const uint64_t d = dil[G.dil_off + row * G.Ws + (x >> 6)];
+  float acc = 0.f;
+  if ((d >> (x & 63)) & 1ull) {
+    // fixed order: cells by (k, j, i) ascending
+    for (int kz = max(z - 2, 0); kz <= min(z + 1, G.cz - 1); ++kz)
+      for (int jy = max(y - 2, 0); jy <= min(y + 1, G.cy - 1); ++jy) {
+        const int64_t rw = G.act_off + ((int64_t)kz * G.cy + jy) * G.W;
+        for (int ix = max(x - 2, 0); ix <= min(x + 1, G.cx - 1); ++ix) {
+          const uint64_t w = act[rw + (ix >> 6)];
+          const uint64_t bit = 1ull << (ix & 63);
+          if (!(w & bit)) continue;
+          const uint64_t vid = (uint64_t)wprefix[rw + (ix >> 6)] + __popcll(w & (bit - 1ull));
+          const int slot = (z - kz + 1) * 16 + (y - jy + 1) * 4 + (x - ix + 1);
+          acc += contrib[vid * kStencil + slot];
+        }
+      }
+  }
+  grad[s] = acc;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -683,13 +509,13 @@ __device__ void grid_gradient(const DcGridDev& G, int i, int j, int k, double ou
   out[0] = out[1] = out[2] = 0.0;
   const double dx = coordd(G.ox, ip, G.vx) - coordd(G.ox, im, G.vx);
   if (fabs(dx) > 1e-12)
-    out[0] = ((double)sample(G, ip, j, k) - (double)sample(G, im, j, k)) / dx;
+    out[0] = ((double)*sample_ptr(G, ip, j, k) - (double)*sample_ptr(G, im, j, k)) / dx;
   const double dy = coordd(G.oy, jp, G.vy) - coordd(G.oy, jm, G.vy);
   if (fabs(dy) > 1e-12)
-    out[1] = ((double)sample(G, i, jp, k) - (double)sample(G, i, jm, k)) / dy;
+    out[1] = ((double)*sample_ptr(G, i, jp, k) - (double)*sample_ptr(G, i, jm, k)) / dy;
   const double dz = coordd(G.oz, kp, G.vz) - coordd(G.oz, km, G.vz);
   if (fabs(dz) > 1e-12)
-    out[2] = ((double)sample(G, i, j, kp) - (double)sample(G, i, j, km)) / dz;
+    out[2] = ((double)*sample_ptr(G, i, j, kp) - (double)*sample_ptr(G, i, j, km)) / dz;
   const double nrm = sqrt(sq3(out[0], out[1], out[2]));
   if (nrm > 0.0) {
     out[0] = out[0] / nrm;
@@ -710,8 +536,9 @@ __global__ void __launch_bounds__(256)
     k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ pen_bounds,
            int64_t n_pen, const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
            const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pbase,
-           const double* __restrict__ v64, const DcCounters* __restrict__ ctr, uint64_t capA,
-           uint64_t capF, int32_t* __restrict__ faces) {
+           const double* __restrict__ v64, const float* __restrict__ v32, int precision,
+           const DcCounters* __restrict__ ctr, uint64_t capA, uint64_t capF,
+           int32_t* __restrict__ faces) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pen) return;
   // an overflowed run is repeated by the host; never index vertices past the buffer
@@ -757,10 +584,14 @@ __global__ void __launch_bounds__(256)
       grid_gradient(G, i, j, k, g0);
       grid_gradient(G, e1[0], e1[1], e1[2], g1);
       const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
-      const double* pa = v64 + 3 * (vglob + c0);
-      const double* pb = v64 + 3 * (vglob + c1);
-      const double* pc = v64 + 3 * (vglob + c2);
-      const double* pd = v64 + 3 * (vglob + c3);
+      double pa[3], pb[3], pc[3], pd[3];
+      const int64_t vv[4] = {vglob + c0, vglob + c1, vglob + c2, vglob + c3};
+      double* dst[4] = {pa, pb, pc, pd};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * vv[q] + d] : (double)v32[3 * vv[q] + d];
       const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
       const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
       double q0 = ba1 * da2 - ba2 * da1;
@@ -878,9 +709,31 @@ hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t
 }
 
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
-  const unsigned blocks = (unsigned)a.blocks;
-  hipLaunchKernelGGL(k_qef64, dim3(blocks), dim3(256), 0, s, a.grids, a.n_grids, a.act_bounds,
-                     a.act, a.alist, a.ctr, a.capA, a.v64, a.v32);
+  const dim3 grid((unsigned)a.blocks), block(256);
+  if (a.precision == DC_PREC_FP64)
+    hipLaunchKernelGGL(k_qef<DC_PREC_FP64>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
+                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags);
+  else
+    hipLaunchKernelGGL(k_qef<DC_PREC_FP32>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
+                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_bwd_cell, dim3((unsigned)a.cell_blocks), dim3(kBwdThreads), 0, s, a.grids,
+                     a.n_grids, a.act_bounds, a.act, a.alist, a.flags, a.precision, a.ctr, a.capA,
+                     a.grad_vertices, a.contrib);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (a.n_dil > 0)
+    hipLaunchKernelGGL(k_bwd_dilate, dim3((unsigned)((a.n_dil + 255) / 256)), dim3(256), 0, s,
+                       a.grids, a.n_grids, a.dil_bounds, a.n_dil, a.act, a.dil);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (a.n_samples > 0)
+    hipLaunchKernelGGL(k_bwd_gather, dim3((unsigned)((a.n_samples + 255) / 256)), dim3(256), 0, s,
+                       a.grids, a.n_grids, a.samp_bounds, a.n_samples, a.act, a.wprefix, a.dil,
+                       a.contrib, a.grad_values);
   return hipGetLastError();
 }
 
@@ -888,7 +741,7 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
   if (a.n_pen <= 0) return hipSuccess;
   const int64_t blocks = (a.n_pen + 255) / 256;
   hipLaunchKernelGGL(k_emit, dim3((unsigned)blocks), dim3(256), 0, s, a.grids, a.n_grids,
-                     a.pen_bounds, a.n_pen, a.act, a.wprefix, a.ft, a.pbase, a.v64, a.ctr, a.capA,
-                     a.capF, a.faces);
+                     a.pen_bounds, a.n_pen, a.act, a.wprefix, a.ft, a.pbase, a.v64, a.v32, a.precision,
+                     a.ctr, a.capA, a.capF, a.faces);
   return hipGetLastError();
 }
