@@ -11,6 +11,7 @@ duration. `cpu_baseline` is the oracle (the C++ restatement of the reference CPU
 same grid, single thread, on this host.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -43,8 +44,8 @@ def run_steps(be, lib, cg, prec, steps):
     import ctypes
     from contouring import _lib
     st = _lib.DcHipStats()
-    acc = {"marking_ms": 0.0, "compaction_ms": 0.0, "face_prefix_sum_ms": 0.0, "qef_ms": 0.0,
-           "face_fill_ms": 0.0}
+    acc = {"marking_ms": 0.0, "compaction_ms": 0.0, "face_prefix_sum_ms": 0.0, "normals_ms": 0.0,
+           "qef_ms": 0.0, "face_fill_ms": 0.0}
     for _ in range(steps):
         rc = lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st))
         be._check(rc)
@@ -92,6 +93,44 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     log(f"[bench] plain fp64: {ms:.3f} ms/extraction, passes {passes}")
 
+    # plain-mode FP64 vertices (bit-identical to the CPU backend's) for the FP32 error below
+    nv = int(st.vertex_count)
+    v64 = torch.empty((nv, 3), dtype=torch.float64, device="cuda:0")
+    be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(v64.data_ptr()), None))
+
+    # FP32 + FP64-fallback forward followed by the adjoint (dL/dphi for a fixed dL/dV)
+    prec32 = _lib.DC_HIP_FP32_FALLBACK
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(0)
+    gv = torch.randn((nv, 3), dtype=torch.float32, device="cuda:0", generator=gen)
+    gphi = torch.empty_like(phi)
+    bst = _lib.DcHipStats()
+
+    def fwd_bwd(k):
+        acc = {"fwd_ms": 0.0, "bwd_ms": 0.0}
+        for _ in range(k):
+            rc = lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec32, ctypes.byref(st))
+            be._check(rc)
+            rc = lib.dc_hip_backward(be._ctx, ctypes.c_void_p(gv.data_ptr()),
+                                     ctypes.c_void_p(gphi.data_ptr()), ctypes.byref(bst))
+            be._check(rc)
+        return acc
+
+    fwd_bwd(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd_bwd(args.steps)
+    torch.cuda.synchronize()
+    fb_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    fallback = int(st.qef_fallback_count)
+    v32 = torch.empty((nv, 3), dtype=torch.float32, device="cuda:0")
+    be._check(lib.dc_hip_copy_result(be._ctx, ctypes.c_void_p(v32.data_ptr()), None, None))
+    err = (v32.double() - v64).abs() / g.h
+    err_max, err_rms = float(err.max()), float(err.pow(2).mean().sqrt())
+    log(f"[bench] fp32+fallback fwd+bwd: {fb_ms:.3f} ms ({fb_ms / ms:.2f}x plain), fallback "
+        f"{fallback}/{nv}, vertex err max {err_max:.2e} h rms {err_rms:.2e} h, "
+        f"bwd {bst.backward_ms:.3f} ms")
+
     Ns = n ** 3
     Nc = (n - 1) ** 3
     mark_bytes = 4 * Ns + Nc / 8
@@ -132,7 +171,10 @@ def main():
         "faces": int(st.face_count),
         "active_cells_per_s": st.vertex_count / (ms * 1e-3),
         "pass_ms": passes,
-        "fp32_bwd_ms": None,
+        "fp32_bwd_ms": fb_ms,
+        "fp32_bwd_over_plain": fb_ms / ms,
+        "fp32_vertex_err_h": {"max": err_max, "rms": err_rms},
+        "fp32_fallback_cells": fallback,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "kernel": "k_mark",

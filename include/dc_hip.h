@@ -70,6 +70,7 @@ typedef struct dc_hip_stats {
   int64_t face_count;
   int64_t qef_fallback_count; /* FP32 mode: cells solved again in FP64 */
   int64_t invalid_count;
+  float normals_ms;        /* averaged hermite normals, once per face */
 } dc_hip_stats;
 
 typedef struct dc_hip_context dc_hip_context;

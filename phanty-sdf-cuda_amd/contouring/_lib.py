@@ -46,7 +46,8 @@ class DcHipStats(ctypes.Structure):
                 ("face_emission_ms", ctypes.c_float), ("download_ms", ctypes.c_float),
                 ("total_ms", ctypes.c_float), ("backward_ms", ctypes.c_float),
                 ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
-                ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64)]
+                ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64),
+                ("normals_ms", ctypes.c_float)]
 
 
 _hip = None

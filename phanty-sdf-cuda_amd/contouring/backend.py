@@ -67,6 +67,7 @@ class DualContouringStats:
         for f in _STATS_FIELDS:
             setattr(self, f, "" if f == "backend" else 0)
         self.backward_ms = 0.0
+        self.normals_ms = 0.0
 
     @classmethod
     def from_c(cls, s, backend="hip"):
@@ -79,10 +80,11 @@ class DualContouringStats:
             setattr(st, f, getattr(s, f))
         st.active_cell_marking_ms = s.marking_ms
         st.active_cell_compaction_ms = s.compaction_ms
+        st.normals_ms = s.normals_ms
         return st
 
     def as_dict(self):
-        return {f: getattr(self, f) for f in _STATS_FIELDS + ("backward_ms",)}
+        return {f: getattr(self, f) for f in _STATS_FIELDS + ("backward_ms", "normals_ms")}
 
 
 def _ptr(x):

@@ -34,11 +34,10 @@ __device__ __constant__ int8_t c_grp_edge[3][4] = {{0, 2, 4, 6}, {3, 1, 7, 5}, {
 __device__ __constant__ int8_t c_grp_dirs[3][2] = {{1, 2}, {0, 2}, {0, 1}};
 
 constexpr double kSvdThreshold = 0.01;  // Cell.cpp:314
-// FP32 -> FP64 fallback: eigenvalue within this of the threshold, smallest kept eigenvalue
-// below kFbKept, or a hermite normal with squared norm below kFbTiny.
+// FP32 -> FP64 fallback: an eigenvalue within kFbBand of the threshold, a kept eigenvalue below
+// kFbKept (poor conditioning), or a final hermite normal that is not of unit length.
 constexpr float kFbBand = 5e-4f;
 constexpr float kFbKept = 0.05f;
-constexpr float kFbTiny = 1e-20f;
 
 // dc_backend.cpp:63-65: FP32 multiply then FP32 add, widened.
 __device__ __forceinline__ double coordd(float o, int i, float v) {
@@ -281,6 +280,95 @@ __device__ __forceinline__ void final_normal(const DcGridDev& G, const uint64_t*
   out[2] = acc[2];
 }
 
+// Averaged normals precomputed once per face (k_face_normals): the face id of the strict-crossing
+// interior edge at base cell (i0,j0,k0) along `axis` is pw[w] + popc(ft[w] & below(k0)).
+template <typename T>
+struct FaceNormals {
+  const uint64_t* ft;
+  const uint32_t* pw;
+  const T* nrm;
+  uint64_t cap;  // valid entries; 0 disables the lookup
+};
+
+template <typename T>
+__device__ __forceinline__ void own_normal(const Cell<T>& c, int e, T n[3]) {
+  T p[3], Gr[3];
+  hermite_edge(c, e, p, n, Gr);
+}
+
+// Final hermite normal of local edge e of cell (i,j,k): the face table when the edge is an
+// averaged strict crossing, the 4-cell average computed here for an averaged edge with a zero
+// endpoint, the cell's own normal otherwise.
+template <typename T>
+__device__ void final_normal_src(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                 const FaceNormals<T>& fn, int i, int j, int k, int e,
+                                 const Cell<T>& own, T out[3]) {
+  const int a = c_edge[e][0], b = c_edge[e][1];
+  const bool strict = strict_change(own.S[a], own.S[b]);
+  int cells[4][3], axis;
+  if (!edge_group(G, act, i, j, k, e, strict, cells, axis)) {
+    own_normal(own, e, out);
+    return;
+  }
+  if (strict && fn.cap) {
+    const int64_t idx = G.ft_off + (((int64_t)axis * G.Wk + (cells[0][2] >> 6)) * G.cy +
+                                    cells[0][1]) * G.cx + cells[0][0];
+    const uint64_t below = (1ull << (cells[0][2] & 63)) - 1ull;
+    const uint64_t face = (uint64_t)fn.pw[idx] + (uint64_t)__popcll(fn.ft[idx] & below);
+    if (face < fn.cap) {
+      out[0] = fn.nrm[3 * face + 0];
+      out[1] = fn.nrm[3 * face + 1];
+      out[2] = fn.nrm[3 * face + 2];
+      return;
+    }
+  }
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int q = 0; q < 4; ++q) {
+    T n[3];
+    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
+      own_normal(own, e, n);
+    } else {
+      Cell<T> nc;
+      load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
+      own_normal(nc, (int)c_grp_edge[axis][q], n);
+    }
+    if (sq3(n[0], n[1], n[2]) > T(0)) {
+      acc[0] = acc[0] + n[0];
+      acc[1] = acc[1] + n[1];
+      acc[2] = acc[2] + n[2];
+    }
+  }
+  normalize3(acc);
+  out[0] = acc[0];
+  out[1] = acc[1];
+  out[2] = acc[2];
+}
+
+// Averaged normal of the face on the `axis` edge at base cell (i,j,k) (all 4 cells active).
+template <typename T>
+__device__ void face_group_normal(const DcGridDev& G, int axis, int i, int j, int k, T out[3]) {
+  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int q = 0; q < 4; ++q) {
+    int c[3] = {i, j, k};
+    if (q & 1) c[d1] -= 1;
+    if (q & 2) c[d2] -= 1;
+    Cell<T> nc;
+    load_cell(G, c[0], c[1], c[2], nc);
+    T n[3];
+    own_normal(nc, (int)c_grp_edge[axis][q], n);
+    if (sq3(n[0], n[1], n[2]) > T(0)) {
+      acc[0] = acc[0] + n[0];
+      acc[1] = acc[1] + n[1];
+      acc[2] = acc[2] + n[2];
+    }
+  }
+  normalize3(acc);
+  out[0] = acc[0];
+  out[1] = acc[1];
+  out[2] = acc[2];
+}
+
 // Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5). The
 // FP64 instance has the CPU restatement's exact operation sequence.
 template <typename T>
@@ -365,18 +453,19 @@ __device__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3]) {
 // FP64 plain solve: Cell::update (Cell.cpp:284-361) with rows streamed in ascending edge order,
 // matching the CPU restatement's sums term by term.
 // ---------------------------------------------------------------------------------------------
-__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                                int k, double x[3]) {
+__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                const FaceNormals<double>& fn, int i, int j, int k, double x[3]) {
   Cell<double> own;
   load_cell(G, i, j, k, own);
   double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
   double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
   int m = 0;
   for (int e = 0; e < 12; ++e) {
-    double p[3], n[3], Gr[3];
-    if (!hermite_edge(own, e, p, n, Gr)) continue;
+    if (skip_edge(own.S[c_edge[e][0]], own.S[c_edge[e][1]])) continue;
+    double p[3];
+    hermite_point(own, e, p);
     double nf[3];
-    final_normal(G, act, i, j, k, e, own, n, nf);
+    final_normal_src(G, act, fn, i, j, k, e, own, nf);
     if (tsqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
     const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
     M00 = M00 + nf[0] * nf[0];
@@ -442,8 +531,9 @@ struct Qef {
   int m;
 };
 
-__device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                              int k, const Cell<float>& own, Qef<float>& Q) {
+__device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
+                              const FaceNormals<float>& fn, int i, int j, int k,
+                              const Cell<float>& own, Qef<float>& Q) {
   NSum s0, s1, s2;
   int m = 0;
   for (int e = 0; e < 12; ++e) {
@@ -464,11 +554,12 @@ __device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ a
   NSum M00, M01, M02, M11, M12, M22, g0, g1, g2;
   bool ok = true;
   for (int e = 0; e < 12; ++e) {
-    float p[3], n[3], Gr[3];
-    if (!hermite_edge(own, e, p, n, Gr)) continue;
-    if (!(sq3(Gr[0], Gr[1], Gr[2]) > kFbTiny)) ok = false;
+    if (skip_edge(own.S[c_edge[e][0]], own.S[c_edge[e][1]])) continue;
+    float p[3];
+    hermite_point(own, e, p);
     float nf[3];
-    final_normal(G, act, i, j, k, e, own, n, nf);
+    final_normal_src(G, act, fn, i, j, k, e, own, nf);
+    if (!(sq3(nf[0], nf[1], nf[2]) > 0.5f)) ok = false;  // zero or degenerate normal
     normalize3(nf);
     const float q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
     const float d = dot3(nf, q);
@@ -495,12 +586,12 @@ __device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ a
 }
 
 // FP32 solve in the cell frame. Returns false when the cell must be solved in FP64.
-__device__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                                int k, float xl[3]) {
+__device__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                const FaceNormals<float>& fn, int i, int j, int k, float xl[3]) {
   Cell<float> own;
   load_cell(G, i, j, k, own);
   Qef<float> Q;
-  if (!assemble_fp32(G, act, i, j, k, own, Q)) return false;
+  if (!assemble_fp32(G, act, fn, i, j, k, own, Q)) return false;
   float lam[3], V[3][3];
   sym3_eig(Q.M, lam, V);
   float y[3] = {0.f, 0.f, 0.f};

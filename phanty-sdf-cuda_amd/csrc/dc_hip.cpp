@@ -26,7 +26,7 @@ struct DevBuf {
   size_t cap = 0;
 };
 
-constexpr int kNumEvents = 8;
+constexpr int kNumEvents = 10;  // 0..6 forward passes, 8..9 upload / backward
 
 }  // namespace
 
@@ -41,7 +41,7 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, ft, pcount, pbase, wprefix, alist, v64, v32, faces, status, tilectr,
-      counters, upload, flags, bwd_contrib, bwd_grad;
+      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm;
   DcCounters* h_counters = nullptr;
   hipEvent_t ev[kNumEvents] = {};
   // last forward
@@ -191,6 +191,8 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->v32, c->capA * 12, "vertices"))) return rc;
   if ((rc = ensure(c, c->flags, c->capA * 4, "vertex flags"))) return rc;
   if ((rc = ensure(c, c->faces, c->capF * 16, "faces"))) return rc;
+  if ((rc = ensure(c, c->pw, c->n_ft * 4, "face word prefix"))) return rc;
+  if ((rc = ensure(c, c->fnrm, c->capF * 24, "face normals"))) return rc;
 
   // host staging stays alive in the context until the copies have run
   std::vector<int64_t>& bounds = c->hbounds;
@@ -241,6 +243,20 @@ int run_once(dc_hip_context* c, int precision) {
                                as<uint32_t>(c->pbase), st + st_a, tc + 1, d_ctr, s),
          "pencil scan");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[3], s), "event");
+  DcFaceNormalArgs fa;
+  fa.grids = d_grids;
+  fa.n_grids = n;
+  fa.pen_bounds = d_pen_b;
+  fa.n_pen = c->n_pen;
+  fa.ft = as<uint64_t>(c->ft);
+  fa.pbase = as<uint32_t>(c->pbase);
+  fa.pw = as<uint32_t>(c->pw);
+  fa.ctr = d_ctr;
+  fa.capF = c->capF;
+  fa.precision = precision;
+  fa.nrm = c->fnrm.p;
+  DC_TRY(c, dc_launch_face_normals(fa, s), "face normals");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[4], s), "event");
   DcQefArgs qa;
   qa.grids = d_grids;
   qa.n_grids = n;
@@ -256,8 +272,12 @@ int run_once(dc_hip_context* c, int precision) {
   qa.v32 = as<float>(c->v32);
   qa.flags = as<uint32_t>(c->flags);
   qa.ctr_rw = d_ctr;
+  qa.ft = as<uint64_t>(c->ft);
+  qa.pw = as<uint32_t>(c->pw);
+  qa.nrm = c->fnrm.p;
+  qa.capF = c->capF;
   DC_TRY(c, dc_launch_qef(qa, s), "qef");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[4], s), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
   DcEmitArgs ea;
   ea.grids = d_grids;
   ea.n_grids = n;
@@ -275,7 +295,7 @@ int run_once(dc_hip_context* c, int precision) {
   ea.faces = as<int32_t>(c->faces);
   ea.capA = c->capA;
   DC_TRY(c, dc_launch_emit(ea, s), "emit");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
   DC_TRY(c, hipMemcpyAsync(c->h_counters, d_ctr, sizeof(DcCounters), hipMemcpyDeviceToHost, s),
          "read counters");
   DC_TRY(c, hipStreamSynchronize(s), "pipeline");
@@ -332,8 +352,9 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
       st.marking_ms = elapsed(c, 0, 1);
       st.compaction_ms = elapsed(c, 1, 2);
       st.face_prefix_sum_ms = elapsed(c, 2, 3);
-      st.qef_ms = elapsed(c, 3, 4);
-      st.face_fill_ms = elapsed(c, 4, 5);
+      st.normals_ms = elapsed(c, 3, 4);
+      st.qef_ms = elapsed(c, 4, 5);
+      st.face_fill_ms = elapsed(c, 5, 6);
       st.face_emission_ms = st.face_prefix_sum_ms + st.face_fill_ms;
     }
     st.total_ms =
@@ -401,7 +422,8 @@ void dc_hip_destroy(dc_hip_context* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->status, &c->tilectr,
-                    &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad};
+                    &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad, &c->pw,
+                    &c->fnrm};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -456,14 +478,14 @@ int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
   const size_t bytes = (size_t)grid->nx * grid->ny * grid->nz * sizeof(float);
   int rc = ensure(c, c->upload, bytes, "upload buffer");
   if (rc) return rc;
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], c->stream), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], c->stream), "event");
   DC_TRY(c, hipMemcpyAsync(c->upload.p, grid->values, bytes, hipMemcpyHostToDevice, c->stream),
          "upload");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], c->stream), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], c->stream), "event");
   DC_TRY(c, hipStreamSynchronize(c->stream), "upload");
   dc_hip_grid dg = *grid;
   dg.values = as<float>(c->upload);
-  if (stats) stats->upload_ms = c->timing ? elapsed(c, 6, 7) : 0.f;
+  if (stats) stats->upload_ms = c->timing ? elapsed(c, 8, 9) : 0.f;
   return run_pipeline(c, &dg, 1, precision, stats);
 }
 
@@ -570,15 +592,15 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.contrib = as<float>(c->bwd_contrib);
   a.dil = as<uint64_t>(c->bwd_grad);
   a.grad_values = grad_values;
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], s), "event");
   DC_TRY(c, dc_launch_backward(a, s), "backward");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], s), "event");
   DC_TRY(c, hipStreamSynchronize(s), "backward");
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->vertex_count = c->nv;
     stats->face_count = c->nf;
-    stats->backward_ms = c->timing ? elapsed(c, 6, 7) : 0.f;
+    stats->backward_ms = c->timing ? elapsed(c, 8, 9) : 0.f;
     stats->total_ms =
         std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

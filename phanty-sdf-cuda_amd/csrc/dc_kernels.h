@@ -33,6 +33,24 @@ struct DcQefArgs {
   float* v32;
   uint32_t* flags;  // per vertex, FP32 mode: 1 when the cell was solved in FP64
   DcCounters* ctr_rw;
+  const uint64_t* ft;
+  const uint32_t* pw;
+  const void* nrm;  // per-face averaged normals (double in FP64 mode, float in FP32 mode)
+  uint64_t capF;
+};
+
+struct DcFaceNormalArgs {
+  const DcGridDev* grids;
+  int n_grids;
+  const int64_t* pen_bounds;
+  int64_t n_pen;
+  const uint64_t* ft;
+  const uint32_t* pbase;
+  uint32_t* pw;
+  const DcCounters* ctr;
+  uint64_t capF;
+  int precision;
+  void* nrm;
 };
 
 struct DcEmitArgs {
@@ -87,6 +105,7 @@ hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t
                               int64_t n_pen, const uint32_t* pcount, uint32_t* pbase,
                               uint64_t* status, uint32_t* tile_ctr, DcCounters* ctr,
                               hipStream_t s);
+hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
 hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s);

@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256)
 // every tile a block waits on has already started).
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint64_t kSpinLimit = 1ull << 22;
 
@@ -214,36 +214,54 @@ __device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t* s_wave
   return before + x - v;
 }
 
-__device__ __forceinline__ uint64_t lookback(uint64_t* __restrict__ status, int64_t tile,
-                                             uint64_t agg, DcCounters* __restrict__ ctr) {
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// Decoupled look-back run by the block's first wave: each probe reads the status of the 64
+// nearest predecessors at once, sums aggregates up to the nearest inclusive prefix, and waits
+// (bounded) only when a tile before that prefix has not published its aggregate yet.
+__device__ uint64_t lookback_wave(uint64_t* __restrict__ status, int64_t tile, uint64_t agg,
+                                  DcCounters* __restrict__ ctr) {
+  const int lane = threadIdx.x & 63;
   if (tile == 0) {
-    __hip_atomic_store(&status[0], (DC_SCAN_FLAG_PREFIX << 62) | agg, __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0)
+      __hip_atomic_store(&status[0], (DC_SCAN_FLAG_PREFIX << 62) | agg, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_AGG << 62) | agg, __ATOMIC_RELEASE,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_AGG << 62) | agg, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
   uint64_t excl = 0;
-  int64_t t = tile - 1;
+  int64_t base = tile - 1;
   uint64_t spins = 0;
-  while (t >= 0) {
-    const uint64_t s =
-        __hip_atomic_load(&status[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t flag = s >> 62;
-    if (flag == 0) {
+  while (true) {
+    const int64_t t = base - lane;
+    uint64_t st = DC_SCAN_FLAG_PREFIX << 62;  // before tile 0: an empty prefix
+    if (t >= 0) st = __hip_atomic_load(&status[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t flag = st >> 62;
+    const uint64_t pm = __ballot(flag == DC_SCAN_FLAG_PREFIX);
+    const uint64_t zm = __ballot(flag == 0);
+    const int fp = pm ? __ffsll((long long)pm) - 1 : 64;
+    const int fz = zm ? __ffsll((long long)zm) - 1 : 64;
+    if (fz < fp) {
       if (++spins > kSpinLimit) {  // bounded wait: report instead of hanging
-        atomicOr(&ctr->error, DC_ERR_SCAN_WAIT);
+        if (lane == 0) atomicOr(&ctr->error, DC_ERR_SCAN_WAIT);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl += s & DC_SCAN_VALUE_MASK;
-    if (flag == DC_SCAN_FLAG_PREFIX) break;
-    --t;
+    excl += wave_sum_u64(lane <= fp ? (st & DC_SCAN_VALUE_MASK) : 0ull);
+    if (fp < 64) break;
+    base -= 64;
   }
-  __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_PREFIX << 62) | (excl + agg), __ATOMIC_RELEASE,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], (DC_SCAN_FLAG_PREFIX << 62) | (excl + agg),
+                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   return excl;
 }
 
@@ -268,7 +286,10 @@ __global__ void __launch_bounds__(kScanThreads)
   }
   uint64_t total;
   const uint64_t excl = block_exclusive(cnt, s_wave, &total);
-  if (threadIdx.x == 0) s_prefix = lookback(status, tile, total, ctr);
+  if (threadIdx.x < 64) {
+    const uint64_t pre = lookback_wave(status, tile, total, ctr);
+    if (threadIdx.x == 0) s_prefix = pre;
+  }
   __syncthreads();
   uint64_t run = s_prefix + excl;
 #pragma unroll
@@ -325,7 +346,10 @@ __global__ void __launch_bounds__(kScanThreads)
   }
   uint64_t total;
   const uint64_t excl = block_exclusive(cnt, s_wave, &total);
-  if (threadIdx.x == 0) s_prefix = lookback(status, tile, total, ctr);
+  if (threadIdx.x < 64) {
+    const uint64_t pre = lookback_wave(status, tile, total, ctr);
+    if (threadIdx.x == 0) s_prefix = pre;
+  }
   __syncthreads();
   uint64_t run = s_prefix + excl;
 #pragma unroll
@@ -361,8 +385,15 @@ __global__ void __launch_bounds__(256)
     k_qef(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
           const uint64_t* __restrict__ act, const uint64_t* __restrict__ alist,
           DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
-          float* __restrict__ v32, uint32_t* __restrict__ flags) {
+          float* __restrict__ v32, uint32_t* __restrict__ flags, const uint64_t* __restrict__ ft,
+          const uint32_t* __restrict__ pw, const void* __restrict__ nrm, uint64_t capF) {
   const uint64_t n = min(ctr->n_active, cap);
+  const uint64_t nf = min(ctr->n_faces, capF);
+  const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm),
+                                 PREC == DC_PREC_FP64 ? nf : 0ull};
+  const FaceNormals<double> inline64{ft, pw, nullptr, 0ull};
+  const FaceNormals<float> fn32{ft, pw, reinterpret_cast<const float*>(nrm),
+                                PREC == DC_PREC_FP32 ? nf : 0ull};
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // iterate whole waves so the fallback ballot below sees a converged wave
@@ -377,18 +408,18 @@ __global__ void __launch_bounds__(256)
       decode_cell(G, entry, i, j, k);
       double x[3];
       if (PREC == DC_PREC_FP64) {
-        solve_cell_fp64(G, act, i, j, k, x);
+        solve_cell_fp64(G, act, fn64, i, j, k, x);
         v64[3 * v + 0] = x[0];
         v64[3 * v + 1] = x[1];
         v64[3 * v + 2] = x[2];
       } else {
         float xl[3];
-        if (solve_cell_fp32(G, act, i, j, k, xl)) {
+        if (solve_cell_fp32(G, act, fn32, i, j, k, xl)) {
           x[0] = coordd(G.ox, i, G.vx) + (double)xl[0];
           x[1] = coordd(G.oy, j, G.vy) + (double)xl[1];
           x[2] = coordd(G.oz, k, G.vz) + (double)xl[2];
         } else {
-          solve_cell_fp64(G, act, i, j, k, x);
+          solve_cell_fp64(G, act, inline64, i, j, k, x);
           fb = true;
         }
         flags[v] = fb ? 1u : 0u;
@@ -532,6 +563,100 @@ __device__ __forceinline__ int32_t vertex_id(const DcGridDev& G, const uint64_t*
   return (int32_t)(wprefix[w] - vbase + (uint32_t)__popcll(act[w] & below));
 }
 
+// Faces of the wave's 64 pencils spread over all 64 lanes: lane counts, wave prefix, then each
+// lane takes face f = batch + lane, finds its pencil by a branch-free search over the prefixes and
+// its k by selecting the r-th set bit of the pencil's face words. fn(G, a, i, j, k, face) runs
+// once per face; face is the global face index (pbase of the pencil + rank in the pencil).
+template <typename Fn>
+__device__ __forceinline__ void for_each_face(const DcGridDev* __restrict__ grids, int n_grids,
+                                              const int64_t* __restrict__ pen_bounds,
+                                              int64_t n_pen, const uint64_t* __restrict__ ft,
+                                              const uint32_t* __restrict__ pbase,
+                                              uint32_t* __restrict__ pw, Fn&& fn) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0;
+  if (p < n_pen) {
+    const int g = find_bucket(pen_bounds, n_grids, p);
+    const DcGridDev& G = grids[g];
+    const int64_t local = p - G.pen_off;
+    const int64_t plane = (int64_t)G.cy * G.cx;
+    const int a = (int)(local / plane);
+    const int64_t rem = local - (int64_t)a * plane;
+    uint32_t pos = pbase[p];
+    for (int kw = 0; kw < G.Wk; ++kw) {
+      const int64_t idx = G.ft_off + ((int64_t)a * G.Wk + kw) * plane + rem;
+      if (pw) pw[idx] = pos;
+      const uint32_t c = (uint32_t)__popcll(ft[idx]);
+      pos += c;
+      cnt += c;
+    }
+  }
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t off = incl - cnt;
+  const uint32_t total = __shfl(incl, 63);
+  for (uint32_t batch = 0; batch < total; batch += 64) {
+    const uint32_t f = batch + lane;
+    int owner = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const int mid = owner + step;
+      const uint32_t om = __shfl(off, mid & 63);
+      if (mid < 64 && om <= f) owner = mid;
+    }
+    const uint32_t off_o = __shfl(off, owner);
+    const int64_t po = __shfl(p, owner);
+    if (f >= total) continue;
+    uint32_t r = f - off_o;
+    const int g = find_bucket(pen_bounds, n_grids, po);
+    const DcGridDev& G = grids[g];
+    const int64_t local = po - G.pen_off;
+    const int64_t plane = (int64_t)G.cy * G.cx;
+    const int a = (int)(local / plane);
+    const int64_t rem = local - (int64_t)a * plane;
+    const int j = (int)(rem / G.cx);
+    const int i = (int)(rem - (int64_t)j * G.cx);
+    int k = -1;
+    for (int kw = 0; kw < G.Wk; ++kw) {
+      uint64_t w = ft[G.ft_off + ((int64_t)a * G.Wk + kw) * plane + rem];
+      const uint32_t c = (uint32_t)__popcll(w);
+      if (r < c) {
+        for (uint32_t t = 0; t < r; ++t) w &= w - 1;
+        k = 64 * kw + __ffsll((long long)w) - 1;
+        break;
+      }
+      r -= c;
+    }
+    if (k < 0) continue;
+    fn(G, a, i, j, k, (uint64_t)pbase[po] + (f - off_o));
+  }
+}
+
+// Averaged hermite normal of every face edge, once per face, in face order (plain: FP64 in the
+// global frame; FP32 mode: FP32 in each cell's frame). Also writes the per-word face prefix pw.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_face_normals(const DcGridDev* __restrict__ grids, int n_grids,
+                   const int64_t* __restrict__ pen_bounds, int64_t n_pen,
+                   const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pbase,
+                   uint32_t* __restrict__ pw, const DcCounters* __restrict__ ctr, uint64_t capF,
+                   T* __restrict__ nrm) {
+  for_each_face(grids, n_grids, pen_bounds, n_pen, ft, pbase, pw,
+                [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
+                  if (face >= capF) return;
+                  T n[3];
+                  face_group_normal(G, a, i, j, k, n);
+                  nrm[3 * face + 0] = n[0];
+                  nrm[3 * face + 1] = n[1];
+                  nrm[3 * face + 2] = n[2];
+                });
+}
+
 __global__ void __launch_bounds__(256)
     k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ pen_bounds,
            int64_t n_pen, const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
@@ -539,84 +664,65 @@ __global__ void __launch_bounds__(256)
            const double* __restrict__ v64, const float* __restrict__ v32, int precision,
            const DcCounters* __restrict__ ctr, uint64_t capA, uint64_t capF,
            int32_t* __restrict__ faces) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pen) return;
   // an overflowed run is repeated by the host; never index vertices past the buffer
   if (ctr->n_active > capA) return;
-  const int g = find_bucket(pen_bounds, n_grids, p);
-  const DcGridDev G = grids[g];
-  const int64_t local = p - G.pen_off;
-  const int64_t plane = (int64_t)G.cy * G.cx;
-  const int a = (int)(local / plane);
-  const int64_t rem = local - (int64_t)a * plane;
-  const int j = (int)(rem / G.cx);
-  const int i = (int)(rem - (int64_t)j * G.cx);
-  uint64_t pos = pbase[p];
-  const uint32_t vbase = wprefix[G.act_off];
-  const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
-  for (int kw = 0; kw < G.Wk; ++kw) {
-    uint64_t w = ft[G.ft_off + (((int64_t)a * G.Wk + kw) * G.cy + j) * G.cx + i];
-    while (w) {
-      const int b = __ffsll((long long)w) - 1;
-      w &= w - 1;
-      const int k = 64 * kw + b;
-      int c0, c1, c2, c3, e1[3];
-      if (a == 0) {
-        c0 = vertex_id(G, act, wprefix, vbase, i, j - 1, k - 1);
-        c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
-        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-        c3 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
-        e1[0] = i + 1; e1[1] = j; e1[2] = k;
-      } else if (a == 1) {
-        c0 = vertex_id(G, act, wprefix, vbase, i - 1, j, k - 1);
-        c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
-        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-        c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
-        e1[0] = i; e1[1] = j + 1; e1[2] = k;
-      } else {
-        c0 = vertex_id(G, act, wprefix, vbase, i - 1, j - 1, k);
-        c1 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
-        c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-        c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
-        e1[0] = i; e1[1] = j; e1[2] = k + 1;
-      }
-      double g0[3], g1[3];
-      grid_gradient(G, i, j, k, g0);
-      grid_gradient(G, e1[0], e1[1], e1[2], g1);
-      const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
-      double pa[3], pb[3], pc[3], pd[3];
-      const int64_t vv[4] = {vglob + c0, vglob + c1, vglob + c2, vglob + c3};
-      double* dst[4] = {pa, pb, pc, pd};
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * vv[q] + d] : (double)v32[3 * vv[q] + d];
-      const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
-      const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
-      double q0 = ba1 * da2 - ba2 * da1;
-      double q1 = ba2 * da0 - ba0 * da2;
-      double q2 = ba0 * da1 - ba1 * da0;
-      if (sq3(q0, q1, q2) < 1e-24) {
-        const double ca0 = pc[0] - pa[0], ca1 = pc[1] - pa[1], ca2 = pc[2] - pa[2];
-        q0 = ca1 * da2 - ca2 * da1;
-        q1 = ca2 * da0 - ca0 * da2;
-        q2 = ca0 * da1 - ca1 * da0;
-      }
-      if (sq3(gn0, gn1, gn2) > 0 && sq3(q0, q1, q2) > 0) {
-        if ((q0 * gn0 + q1 * gn1) + q2 * gn2 < 0.0) {
-          const int t = c1;
-          c1 = c3;
-          c3 = t;
-        }
-      }
-      if (pos < capF) {
-        int4 quad = make_int4(c0, c1, c2, c3);
-        reinterpret_cast<int4*>(faces)[pos] = quad;
-      }
-      ++pos;
+  for_each_face(grids, n_grids, pen_bounds, n_pen, ft, pbase, nullptr,
+                [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
+    if (pos >= capF) return;
+    const uint32_t vbase = wprefix[G.act_off];
+    const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
+    int c0, c1, c2, c3, e1[3];
+    if (a == 0) {
+      c0 = vertex_id(G, act, wprefix, vbase, i, j - 1, k - 1);
+      c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
+      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+      c3 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
+      e1[0] = i + 1; e1[1] = j; e1[2] = k;
+    } else if (a == 1) {
+      c0 = vertex_id(G, act, wprefix, vbase, i - 1, j, k - 1);
+      c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
+      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+      c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
+      e1[0] = i; e1[1] = j + 1; e1[2] = k;
+    } else {
+      c0 = vertex_id(G, act, wprefix, vbase, i - 1, j - 1, k);
+      c1 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
+      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+      c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
+      e1[0] = i; e1[1] = j; e1[2] = k + 1;
     }
-  }
+    double g0[3], g1[3];
+    grid_gradient(G, i, j, k, g0);
+    grid_gradient(G, e1[0], e1[1], e1[2], g1);
+    const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
+    double pa[3], pb[3], pc[3], pd[3];
+    const int64_t vv[4] = {vglob + c0, vglob + c1, vglob + c2, vglob + c3};
+    double* dst[4] = {pa, pb, pc, pd};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * vv[q] + d] : (double)v32[3 * vv[q] + d];
+    const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
+    const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
+    double q0 = ba1 * da2 - ba2 * da1;
+    double q1 = ba2 * da0 - ba0 * da2;
+    double q2 = ba0 * da1 - ba1 * da0;
+    if (sq3(q0, q1, q2) < 1e-24) {
+      const double ca0 = pc[0] - pa[0], ca1 = pc[1] - pa[1], ca2 = pc[2] - pa[2];
+      q0 = ca1 * da2 - ca2 * da1;
+      q1 = ca2 * da0 - ca0 * da2;
+      q2 = ca0 * da1 - ca1 * da0;
+    }
+    if (sq3(gn0, gn1, gn2) > 0 && sq3(q0, q1, q2) > 0) {
+      if ((q0 * gn0 + q1 * gn1) + q2 * gn2 < 0.0) {
+        const int t = c1;
+        c1 = c3;
+        c3 = t;
+      }
+    }
+    reinterpret_cast<int4*>(faces)[pos] = make_int4(c0, c1, c2, c3);
+  });
 }
 
 __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
@@ -708,14 +814,30 @@ hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t
   return hipGetLastError();
 }
 
+hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s) {
+  if (a.n_pen <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((a.n_pen + 255) / 256)), block(256);
+  if (a.precision == DC_PREC_FP64)
+    hipLaunchKernelGGL(k_face_normals<double>, grid, block, 0, s, a.grids, a.n_grids,
+                       a.pen_bounds, a.n_pen, a.ft, a.pbase, a.pw, a.ctr, a.capF,
+                       reinterpret_cast<double*>(a.nrm));
+  else
+    hipLaunchKernelGGL(k_face_normals<float>, grid, block, 0, s, a.grids, a.n_grids,
+                       a.pen_bounds, a.n_pen, a.ft, a.pbase, a.pw, a.ctr, a.capF,
+                       reinterpret_cast<float*>(a.nrm));
+  return hipGetLastError();
+}
+
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.blocks), block(256);
   if (a.precision == DC_PREC_FP64)
     hipLaunchKernelGGL(k_qef<DC_PREC_FP64>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
-                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags);
+                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags, a.ft, a.pw,
+                       a.nrm, a.capF);
   else
     hipLaunchKernelGGL(k_qef<DC_PREC_FP32>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
-                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags);
+                       a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags, a.ft, a.pw,
+                       a.nrm, a.capF);
   return hipGetLastError();
 }
 
@@ -739,7 +861,7 @@ hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
 
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
   if (a.n_pen <= 0) return hipSuccess;
-  const int64_t blocks = (a.n_pen + 255) / 256;
+  const int64_t blocks = (a.n_pen + 255) / 256;  // one lane per pencil, faces balanced per wave
   hipLaunchKernelGGL(k_emit, dim3((unsigned)blocks), dim3(256), 0, s, a.grids, a.n_grids,
                      a.pen_bounds, a.n_pen, a.act, a.wprefix, a.ft, a.pbase, a.v64, a.v32, a.precision,
                      a.ctr, a.capA, a.capF, a.faces);
