@@ -7,7 +7,9 @@
 //     about the mass point with Neumaier-compensated sums, and a cell whose eigenvalues sit near
 //     the 0.01 truncation threshold, whose smallest kept eigenvalue is small (poor conditioning)
 //     or whose normals are degenerate is recomputed in FP64.
-// The adjoint (backward_cell) differentiates the same function at fixed topology and rank.
+// The adjoint differentiates the same function at fixed topology and truncation rank.
+//
+// Edge and corner geometry is arithmetic (no lookup tables), so cell data stays in registers.
 //
 // Reference semantics (file:line in the reference tree):
 //   corner order contouring.cpp:232-241; edge pairs include/Cell.h:26-30
@@ -22,22 +24,56 @@
 
 namespace dccell {
 
-__device__ __constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-__device__ __constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                                {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-// The 4 cells around a grid edge in the order process_edge visits them: [base, base+dir1,
-// base+dir2, base+dir1+dir2], base = cell whose min corner is the edge's lower endpoint,
-// (dir1, dir2) from get_face_normals_from_edge_idx. Local edge ids per edge axis:
-//   X: 0, 2 (-Y), 4 (-Z), 6 (-Y-Z)   Y: 3, 1 (-X), 7 (-Z), 5 (-X-Z)   Z: 8, 9 (-X), 11 (-Y), 10
-__device__ __constant__ int8_t c_grp_edge[3][4] = {{0, 2, 4, 6}, {3, 1, 7, 5}, {8, 9, 11, 10}};
-__device__ __constant__ int8_t c_grp_dirs[3][2] = {{1, 2}, {0, 2}, {0, 1}};
-
 constexpr double kSvdThreshold = 0.01;  // Cell.cpp:314
 // FP32 -> FP64 fallback: an eigenvalue within kFbBand of the threshold, a kept eigenvalue below
 // kFbKept (poor conditioning), or a final hermite normal that is not of unit length.
 constexpr float kFbBand = 5e-4f;
 constexpr float kFbKept = 0.05f;
+
+// Corners (contouring.cpp:232-241): 0 (0,0,0) 1 (1,0,0) 2 (1,1,0) 3 (0,1,0) 4..7 the same at z=1.
+__device__ __forceinline__ int cbx(int q) { return (((q & 3) + 1) >> 1) & 1; }
+__device__ __forceinline__ int cby(int q) { return (q >> 1) & 1; }
+__device__ __forceinline__ int cbz(int q) { return q >> 2; }
+// Edges (include/Cell.h:26-30): {0,1} {1,2} {2,3} {3,0} {4,5} {5,6} {6,7} {7,4} {0,4} {1,5} {2,6}
+// {3,7}.
+__device__ __forceinline__ int edge_a(int e) { return e < 8 ? e : e - 8; }
+__device__ __forceinline__ int edge_b(int e) {
+  return e < 8 ? (((e & 3) == 3) ? e - 3 : e + 1) : e - 4;
+}
+// Axis of edge e (0 x, 1 y, 2 z) and the offset of its lower endpoint in the cell.
+__device__ __forceinline__ int edge_axis(int e) { return e >= 8 ? 2 : (e & 1); }
+__device__ __forceinline__ void edge_lower(int e, int& dx, int& dy, int& dz) {
+  if (e >= 8) {
+    dx = cbx(e - 8);
+    dy = cby(e - 8);
+    dz = 0;
+  } else if ((e & 1) == 0) {  // x edges 0, 2, 4, 6
+    dx = 0;
+    dy = (e >> 1) & 1;
+    dz = e >> 2;
+  } else {  // y edges 1, 3, 5, 7
+    dx = 1 - ((e >> 1) & 1);
+    dy = 0;
+    dz = e >> 2;
+  }
+}
+// The 4 cells around a grid edge in the order process_edge visits them (hermite_update.cpp:
+// 171-332): [base, base+dir1, base+dir2, base+dir1+dir2], base = cell whose min corner is the
+// edge's lower endpoint, (dir1, dir2) = (-y,-z) for x edges, (-x,-z) for y, (-x,-y) for z
+// (get_face_normals_from_edge_idx, Cell.cpp:26-45). Local edge of the grid edge in member q:
+//   x: 0, 2, 4, 6     y: 3, 1, 7, 5     z: 8, 9, 11, 10
+__device__ __forceinline__ void group_member(int axis, int q, int x0, int y0, int z0, int& x,
+                                             int& y, int& z) {
+  const int m1 = q & 1, m2 = (q >> 1) & 1;
+  x = x0 - (axis == 0 ? 0 : m1);
+  y = y0 - (axis == 0 ? m1 : (axis == 2 ? m2 : 0));
+  z = z0 - (axis == 2 ? 0 : m2);
+}
+__device__ __forceinline__ int group_edge(int axis, int q) {
+  if (axis == 0) return 2 * q;
+  if (axis == 1) return ((q & 1) ? 1 : 3) + ((q & 2) ? 4 : 0);
+  return q == 0 ? 8 : (q == 1 ? 9 : (q == 2 ? 11 : 10));
+}
 
 // dc_backend.cpp:63-65: FP32 multiply then FP32 add, widened.
 __device__ __forceinline__ double coordd(float o, int i, float v) {
@@ -76,6 +112,13 @@ __device__ __forceinline__ T clamp01(T t) {
   return tmax(T(0), tmin(T(1), t));
 }
 
+// S[q] += x for a run-time q without indexing the register array.
+template <typename T>
+__device__ __forceinline__ void add8(T* S, int q, T x) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) S[t] = (q == t) ? S[t] + x : S[t];
+}
+
 // fa*fb > 0 in double is exact for float inputs; FP32 uses the equivalent sign logic.
 __device__ __forceinline__ bool skip_edge(double fa, double fb) { return fa * fb > 0.0; }
 __device__ __forceinline__ bool skip_edge(float fa, float fb) {
@@ -86,13 +129,20 @@ __device__ __forceinline__ bool strict_change(float fa, float fb) {
   return (fa < 0.f && fb > 0.f) || (fa > 0.f && fb < 0.f);
 }
 
+// S[] holds the corner values for constant-index use; s(q) for a run-time corner index re-reads
+// the sample through the cache (indexing the register array would move it to scratch memory).
 template <typename T>
 struct Cell {
   T S[8];
   T X0, X1, Y0, Y1, Z0, Z1;
-  __device__ __forceinline__ T cx(int q) const { return c_corner[q][0] ? X1 : X0; }
-  __device__ __forceinline__ T cy(int q) const { return c_corner[q][1] ? Y1 : Y0; }
-  __device__ __forceinline__ T cz(int q) const { return c_corner[q][2] ? Z1 : Z0; }
+  const float* base;
+  int64_t sy, sz;
+  __device__ __forceinline__ T s(int q) const {
+    return (T)base[cbx(q) + sy * cby(q) + sz * cbz(q)];
+  }
+  __device__ __forceinline__ T cx(int q) const { return cbx(q) ? X1 : X0; }
+  __device__ __forceinline__ T cy(int q) const { return cby(q) ? Y1 : Y0; }
+  __device__ __forceinline__ T cz(int q) const { return cbz(q) ? Z1 : Z0; }
 };
 
 __device__ __forceinline__ const float* sample_ptr(const DcGridDev& G, int i, int j, int k) {
@@ -100,9 +150,13 @@ __device__ __forceinline__ const float* sample_ptr(const DcGridDev& G, int i, in
 }
 
 template <typename T>
-__device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, int k, T S[8]) {
+__device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, int k, Cell<T>& c) {
   const float* b = sample_ptr(G, i, j, k);
   const int64_t sy = G.nx, sz = (int64_t)G.nx * G.ny;
+  c.base = b;
+  c.sy = sy;
+  c.sz = sz;
+  T* S = c.S;
   S[0] = (T)b[0];
   S[1] = (T)b[1];
   S[2] = (T)b[1 + sy];
@@ -115,7 +169,7 @@ __device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, in
 
 // FP64: global frame (the CPU backend's coordinates).
 __device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<double>& c) {
-  load_values(G, i, j, k, c.S);
+  load_values(G, i, j, k, c);
   c.X0 = coordd(G.ox, i, G.vx);
   c.X1 = coordd(G.ox, i + 1, G.vx);
   c.Y0 = coordd(G.oy, j, G.vy);
@@ -126,7 +180,7 @@ __device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int 
 
 // FP32: the cell's own frame, min corner at 0 (edge lengths from the FP32 grid coordinates).
 __device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<float>& c) {
-  load_values(G, i, j, k, c.S);
+  load_values(G, i, j, k, c);
   c.X0 = 0.f;
   c.Y0 = 0.f;
   c.Z0 = 0.f;
@@ -177,27 +231,27 @@ __device__ __forceinline__ void normalize3(T v[3]) {
 
 template <typename T>
 __device__ __forceinline__ void hermite_point(const Cell<T>& c, int e, T p[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  const T fa = c.S[a], fb = c.S[b];
+  const int a = edge_a(e), b = edge_b(e);
+  const T fa = c.s(a), fb = c.s(b);
   const T t = fa / (fa - fb + T(1e-16));
   p[0] = (T(1) - t) * c.cx(a) + t * c.cx(b);
   p[1] = (T(1) - t) * c.cy(a) + t * c.cy(b);
   p[2] = (T(1) - t) * c.cz(a) + t * c.cz(b);
 }
 
-// Cell.cpp:180-209 for one edge: hermite point p and normalized trilinear normal n; G is the
-// gradient before normalization.
 template <typename T>
-__device__ __forceinline__ bool hermite_edge(const Cell<T>& c, int e, T p[3], T n[3], T G[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  if (skip_edge(c.S[a], c.S[b])) return false;
+__device__ __forceinline__ bool edge_crosses(const Cell<T>& c, int e) {
+  return !skip_edge(c.s(edge_a(e)), c.s(edge_b(e)));
+}
+
+// Cell.cpp:180-209: the normalized trilinear normal at the hermite point of edge e (the caller
+// has checked that the edge carries a sample).
+template <typename T>
+__device__ __forceinline__ void raw_normal(const Cell<T>& c, int e, T n[3]) {
+  T p[3];
   hermite_point(c, e, p);
-  trilinear_grad(c, p, G);
-  n[0] = G[0];
-  n[1] = G[1];
-  n[2] = G[2];
+  trilinear_grad(c, p, n);
   normalize3(n);
-  return true;
 }
 
 __device__ __forceinline__ bool cell_active(const DcGridDev& G, const uint64_t* __restrict__ act,
@@ -206,167 +260,104 @@ __device__ __forceinline__ bool cell_active(const DcGridDev& G, const uint64_t* 
   return (w >> (i & 63)) & 1ull;
 }
 
-// hermite_update.cpp:171-332: is local edge e of cell (i,j,k) averaged, and over which cells.
-// The caller has established that e carries a hermite sample (fa*fb <= 0).
+// hermite_update.cpp:171-332: is local edge e of cell (i,j,k) averaged? Returns the edge axis and
+// base cell. The caller has established that e carries a hermite sample (fa*fb <= 0); a strict
+// sign change makes the 4 cells active, otherwise they are looked up.
 __device__ __forceinline__ bool edge_group(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                           int i, int j, int k, int e, bool strict,
-                                           int cells[4][3], int& axis) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  int P0[3] = {i + c_corner[a][0], j + c_corner[a][1], k + c_corner[a][2]};
-  int P1[3] = {i + c_corner[b][0], j + c_corner[b][1], k + c_corner[b][2]};
-  axis = (P0[0] != P1[0]) ? 0 : ((P0[1] != P1[1]) ? 1 : 2);
-  if (P1[axis] < P0[axis]) P0[axis] = P1[axis];
-  const bool in0 = P0[0] > 0 && P0[0] < G.nx - 1 && P0[1] > 0 && P0[1] < G.ny - 1 && P0[2] > 0 &&
-                   P0[2] < G.nz - 1;
-  int Q1[3] = {P0[0], P0[1], P0[2]};
-  Q1[axis] += 1;
-  const bool in1 = Q1[0] > 0 && Q1[0] < G.nx - 1 && Q1[1] > 0 && Q1[1] < G.ny - 1 && Q1[2] > 0 &&
-                   Q1[2] < G.nz - 1;
+                                           int i, int j, int k, int e, bool strict, int& axis,
+                                           int& x0, int& y0, int& z0) {
+  int dx, dy, dz;
+  edge_lower(e, dx, dy, dz);
+  axis = edge_axis(e);
+  x0 = i + dx;
+  y0 = j + dy;
+  z0 = k + dz;
+  const int x1 = x0 + (axis == 0), y1 = y0 + (axis == 1), z1 = z0 + (axis == 2);
+  const bool in0 = x0 > 0 && x0 < G.nx - 1 && y0 > 0 && y0 < G.ny - 1 && z0 > 0 && z0 < G.nz - 1;
+  const bool in1 = x1 > 0 && x1 < G.nx - 1 && y1 > 0 && y1 < G.ny - 1 && z1 > 0 && z1 < G.nz - 1;
   if (!in0 && !in1) return false;
-  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    cells[q][0] = P0[0];
-    cells[q][1] = P0[1];
-    cells[q][2] = P0[2];
-  }
-  cells[1][d1] -= 1;
-  cells[2][d2] -= 1;
-  cells[3][d1] -= 1;
-  cells[3][d2] -= 1;
-  // a strict sign change on the edge makes all four cells active; otherwise look them up
   if (!strict) {
-    for (int q = 0; q < 4; ++q)
-      if (!cell_active(G, act, cells[q][0], cells[q][1], cells[q][2])) return false;
+    for (int q = 0; q < 4; ++q) {
+      int x, y, z;
+      group_member(axis, q, x0, y0, z0, x, y, z);
+      if (!cell_active(G, act, x, y, z)) return false;
+    }
   }
   return true;
 }
 
-// Final (averaged or own) hermite normal of local edge e of cell (i,j,k).
+// Sum of the group's nonzero raw normals in process_edge order.
 template <typename T>
-__device__ __forceinline__ void final_normal(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                             int i, int j, int k, int e, const Cell<T>& own,
-                                             const T own_n[3], T out[3]) {
-  int cells[4][3], axis;
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  if (!edge_group(G, act, i, j, k, e, strict_change(own.S[a], own.S[b]), cells, axis)) {
-    out[0] = own_n[0];
-    out[1] = own_n[1];
-    out[2] = own_n[2];
-    return;
-  }
-  T acc[3] = {T(0), T(0), T(0)};
+__device__ __forceinline__ void group_sum(const DcGridDev& G, int axis, int x0, int y0, int z0,
+                                          T acc[3]) {
+  acc[0] = acc[1] = acc[2] = T(0);
   for (int q = 0; q < 4; ++q) {
+    int x, y, z;
+    group_member(axis, q, x0, y0, z0, x, y, z);
+    Cell<T> nc;
+    load_cell(G, x, y, z, nc);
     T n[3];
-    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
-      n[0] = own_n[0];
-      n[1] = own_n[1];
-      n[2] = own_n[2];
-    } else {
-      Cell<T> nc;
-      load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
-      T p[3], Gr[3];
-      hermite_edge(nc, c_grp_edge[axis][q], p, n, Gr);
-    }
+    raw_normal(nc, group_edge(axis, q), n);
     if (sq3(n[0], n[1], n[2]) > T(0)) {
       acc[0] = acc[0] + n[0];
       acc[1] = acc[1] + n[1];
       acc[2] = acc[2] + n[2];
     }
   }
-  normalize3(acc);
-  out[0] = acc[0];
-  out[1] = acc[1];
-  out[2] = acc[2];
 }
 
-// Averaged normals precomputed once per face (k_face_normals): the face id of the strict-crossing
-// interior edge at base cell (i0,j0,k0) along `axis` is pw[w] + popc(ft[w] & below(k0)).
+// Averaged normals precomputed once per face (k_face_normals): for face f, nrm[4f..4f+2] is the
+// normalized 4-cell sum and nrm[4f+3] the length of the sum before normalization. The face id of
+// the strict-crossing interior edge at base cell (x0,y0,z0) along `axis` is
+// pw[w] + popc(ft[w] & below(z0)).
 template <typename T>
 struct FaceNormals {
   const uint64_t* ft;
   const uint32_t* pw;
-  const T* nrm;
-  uint64_t cap;  // valid entries; 0 disables the lookup
+  const double* nrm64;  // the table in FP64 (plain mode) or
+  const float* nrm32;   // in FP32 (FP32 mode); read in either precision
+  uint64_t cap;         // valid entries; 0 disables the lookup
+  __device__ __forceinline__ T at(uint64_t q) const {
+    return nrm64 ? (T)nrm64[q] : (T)nrm32[q];
+  }
 };
 
-template <typename T>
-__device__ __forceinline__ void own_normal(const Cell<T>& c, int e, T n[3]) {
-  T p[3], Gr[3];
-  hermite_edge(c, e, p, n, Gr);
+__device__ __forceinline__ uint64_t face_id(const DcGridDev& G, const uint64_t* __restrict__ ft,
+                                            const uint32_t* __restrict__ pw, int axis, int x0,
+                                            int y0, int z0) {
+  const int64_t idx =
+      G.ft_off + (((int64_t)axis * G.Wk + (z0 >> 6)) * G.cy + y0) * G.cx + x0;
+  const uint64_t below = (1ull << (z0 & 63)) - 1ull;
+  return (uint64_t)pw[idx] + (uint64_t)__popcll(ft[idx] & below);
 }
 
-// Final hermite normal of local edge e of cell (i,j,k): the face table when the edge is an
-// averaged strict crossing, the 4-cell average computed here for an averaged edge with a zero
-// endpoint, the cell's own normal otherwise.
+// Final hermite normal of local edge e of cell (i,j,k): the face table for an averaged strict
+// crossing, the 4-cell average computed here for an averaged edge with a zero endpoint, the cell's
+// own normal otherwise. *face is set to the face id when the table was used, else ~0.
 template <typename T>
-__device__ void final_normal_src(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                 const FaceNormals<T>& fn, int i, int j, int k, int e,
-                                 const Cell<T>& own, T out[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  const bool strict = strict_change(own.S[a], own.S[b]);
-  int cells[4][3], axis;
-  if (!edge_group(G, act, i, j, k, e, strict, cells, axis)) {
-    own_normal(own, e, out);
+__device__ __forceinline__ void final_normal(const DcGridDev& G, const uint64_t* __restrict__ act,
+                             const FaceNormals<T>& fn, int i, int j, int k, int e,
+                             const Cell<T>& own, T out[3], uint64_t* face, bool* averaged) {
+  const bool strict = strict_change(own.s(edge_a(e)), own.s(edge_b(e)));
+  int axis, x0, y0, z0;
+  *face = ~0ull;
+  *averaged = edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0);
+  if (!*averaged) {
+    raw_normal(own, e, out);
     return;
   }
   if (strict && fn.cap) {
-    const int64_t idx = G.ft_off + (((int64_t)axis * G.Wk + (cells[0][2] >> 6)) * G.cy +
-                                    cells[0][1]) * G.cx + cells[0][0];
-    const uint64_t below = (1ull << (cells[0][2] & 63)) - 1ull;
-    const uint64_t face = (uint64_t)fn.pw[idx] + (uint64_t)__popcll(fn.ft[idx] & below);
-    if (face < fn.cap) {
-      out[0] = fn.nrm[3 * face + 0];
-      out[1] = fn.nrm[3 * face + 1];
-      out[2] = fn.nrm[3 * face + 2];
+    const uint64_t f = face_id(G, fn.ft, fn.pw, axis, x0, y0, z0);
+    if (f < fn.cap) {
+      out[0] = fn.at(4 * f + 0);
+      out[1] = fn.at(4 * f + 1);
+      out[2] = fn.at(4 * f + 2);
+      *face = f;
       return;
     }
   }
-  T acc[3] = {T(0), T(0), T(0)};
-  for (int q = 0; q < 4; ++q) {
-    T n[3];
-    if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
-      own_normal(own, e, n);
-    } else {
-      Cell<T> nc;
-      load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
-      own_normal(nc, (int)c_grp_edge[axis][q], n);
-    }
-    if (sq3(n[0], n[1], n[2]) > T(0)) {
-      acc[0] = acc[0] + n[0];
-      acc[1] = acc[1] + n[1];
-      acc[2] = acc[2] + n[2];
-    }
-  }
-  normalize3(acc);
-  out[0] = acc[0];
-  out[1] = acc[1];
-  out[2] = acc[2];
-}
-
-// Averaged normal of the face on the `axis` edge at base cell (i,j,k) (all 4 cells active).
-template <typename T>
-__device__ void face_group_normal(const DcGridDev& G, int axis, int i, int j, int k, T out[3]) {
-  const int d1 = c_grp_dirs[axis][0], d2 = c_grp_dirs[axis][1];
-  T acc[3] = {T(0), T(0), T(0)};
-  for (int q = 0; q < 4; ++q) {
-    int c[3] = {i, j, k};
-    if (q & 1) c[d1] -= 1;
-    if (q & 2) c[d2] -= 1;
-    Cell<T> nc;
-    load_cell(G, c[0], c[1], c[2], nc);
-    T n[3];
-    own_normal(nc, (int)c_grp_edge[axis][q], n);
-    if (sq3(n[0], n[1], n[2]) > T(0)) {
-      acc[0] = acc[0] + n[0];
-      acc[1] = acc[1] + n[1];
-      acc[2] = acc[2] + n[2];
-    }
-  }
-  normalize3(acc);
-  out[0] = acc[0];
-  out[1] = acc[1];
-  out[2] = acc[2];
+  group_sum(G, axis, x0, y0, z0, out);
+  normalize3(out);
 }
 
 // Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5). The
@@ -381,7 +372,6 @@ template <>
 __device__ __forceinline__ float eig_tol<float>() {
   return 1e-14f;
 }
-
 // |theta| above which theta*theta + 1 would overflow: use t = 1/(2 theta).
 template <typename T>
 __device__ __forceinline__ T eig_big();
@@ -395,7 +385,7 @@ __device__ __forceinline__ float eig_big<float>() {
 }
 
 template <typename T>
-__device__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3]) {
+__device__ __forceinline__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3]) {
   T a[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -453,7 +443,7 @@ __device__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3]) {
 // FP64 plain solve: Cell::update (Cell.cpp:284-361) with rows streamed in ascending edge order,
 // matching the CPU restatement's sums term by term.
 // ---------------------------------------------------------------------------------------------
-__device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act,
+__device__ __forceinline__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act,
                                 const FaceNormals<double>& fn, int i, int j, int k, double x[3]) {
   Cell<double> own;
   load_cell(G, i, j, k, own);
@@ -461,11 +451,13 @@ __device__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__
   double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
   int m = 0;
   for (int e = 0; e < 12; ++e) {
-    if (skip_edge(own.S[c_edge[e][0]], own.S[c_edge[e][1]])) continue;
+    if (!edge_crosses(own, e)) continue;
     double p[3];
     hermite_point(own, e, p);
     double nf[3];
-    final_normal_src(G, act, fn, i, j, k, e, own, nf);
+    uint64_t face;
+    bool avg;
+    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
     if (tsqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
     const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
     M00 = M00 + nf[0] * nf[0];
@@ -521,8 +513,6 @@ struct NSum {
   __device__ __forceinline__ float val() const { return s + c; }
 };
 
-// Mass-point-centred assembly in the cell frame: centroid c, M = sum n n^T, g = sum n (n.(p-c)).
-// Returns false when a hermite normal is degenerate (the FP32 caller then falls back).
 template <typename T>
 struct Qef {
   T c[3];
@@ -531,14 +521,15 @@ struct Qef {
   int m;
 };
 
-__device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
+// Mass-point-centred assembly in the cell frame: centroid c, M = sum n n^T, g = sum n (n.(p-c)),
+// Neumaier-compensated. Returns false when a final normal is degenerate (then solve in FP64).
+__device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
                               const FaceNormals<float>& fn, int i, int j, int k,
                               const Cell<float>& own, Qef<float>& Q) {
   NSum s0, s1, s2;
   int m = 0;
   for (int e = 0; e < 12; ++e) {
-    const int a = c_edge[e][0], b = c_edge[e][1];
-    if (skip_edge(own.S[a], own.S[b])) continue;
+    if (!edge_crosses(own, e)) continue;
     float p[3];
     hermite_point(own, e, p);
     s0.add(p[0]);
@@ -554,11 +545,13 @@ __device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ a
   NSum M00, M01, M02, M11, M12, M22, g0, g1, g2;
   bool ok = true;
   for (int e = 0; e < 12; ++e) {
-    if (skip_edge(own.S[c_edge[e][0]], own.S[c_edge[e][1]])) continue;
+    if (!edge_crosses(own, e)) continue;
     float p[3];
     hermite_point(own, e, p);
     float nf[3];
-    final_normal_src(G, act, fn, i, j, k, e, own, nf);
+    uint64_t face;
+    bool avg;
+    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
     if (!(sq3(nf[0], nf[1], nf[2]) > 0.5f)) ok = false;  // zero or degenerate normal
     normalize3(nf);
     const float q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
@@ -586,7 +579,7 @@ __device__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ a
 }
 
 // FP32 solve in the cell frame. Returns false when the cell must be solved in FP64.
-__device__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
+__device__ __forceinline__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
                                 const FaceNormals<float>& fn, int i, int j, int k, float xl[3]) {
   Cell<float> own;
   load_cell(G, i, j, k, own);
@@ -618,7 +611,7 @@ __device__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__
 // ---------------------------------------------------------------------------------------------
 // VJP of the trilinear gradient with respect to the 8 corner values and the point.
 template <typename T>
-__device__ void trilinear_vjp(const Cell<T>& c, const T p[3], const T Gb[3], T Sb[8], T pb[3]) {
+__device__ __forceinline__ void trilinear_vjp(const Cell<T>& c, const T p[3], const T Gb[3], T Sb[8], T pb[3]) {
   const T S000 = c.S[0], S100 = c.S[1], S110 = c.S[2], S010 = c.S[3];
   const T S001 = c.S[4], S101 = c.S[5], S111 = c.S[6], S011 = c.S[7];
   const T dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
@@ -682,58 +675,51 @@ __device__ __forceinline__ void normalize_vjp(const T v[3], const T nb[3], T vb[
   vb[2] = (nb[2] - n[2] * d) / s;
 }
 
-// Adjoint of t = fa/(fa-fb+1e-16) given t-bar.
+// Adjoint of t = fa/(fa-fb+1e-16) for a point adjoint pb (p = (1-t) pa + t pb).
 template <typename T>
-__device__ __forceinline__ void lerp_vjp(const Cell<T>& c, int e, T tb, T Sb[8]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  const T fa = c.S[a], fb = c.S[b];
+__device__ __forceinline__ void lerp_vjp(const Cell<T>& c, int e, const T pb[3], T Sb[8]) {
+  const int a = edge_a(e), b = edge_b(e);
+  const T tb = (pb[0] * (c.cx(b) - c.cx(a)) + pb[1] * (c.cy(b) - c.cy(a))) +
+               pb[2] * (c.cz(b) - c.cz(a));
+  const T fa = c.s(a), fb = c.s(b);
   const T D = fa - fb + T(1e-16);
   const T D2 = D * D;
-  Sb[a] += tb * (T(1e-16) - fb) / D2;
-  Sb[b] += tb * fa / D2;
+  add8(Sb, a, tb * (T(1e-16) - fb) / D2);
+  add8(Sb, b, tb * fa / D2);
 }
 
+// Hermite sample (cell, e): adjoint nb of its normalized normal plus point adjoint pb_in -> the
+// cell's 8 corner values.
 template <typename T>
-__device__ __forceinline__ T edge_dir_dot(const Cell<T>& c, int e, const T pb[3]) {
-  const int a = c_edge[e][0], b = c_edge[e][1];
-  return (pb[0] * (c.cx(b) - c.cx(a)) + pb[1] * (c.cy(b) - c.cy(a))) +
-         pb[2] * (c.cz(b) - c.cz(a));
-}
-
-// hermite sample (cell, e): normalized-normal adjoint nb plus point adjoint pb -> corner values.
-template <typename T>
-__device__ void hermite_vjp(const Cell<T>& c, int e, const T nb[3], const T pb_in[3], T Sb[8]) {
-  T p[3], G[3];
+__device__ __forceinline__ void hermite_vjp(const Cell<T>& c, int e, const T nb[3], const T pb_in[3], T Sb[8]) {
+  T p[3], Gr[3];
   hermite_point(c, e, p);
-  trilinear_grad(c, p, G);
+  trilinear_grad(c, p, Gr);
   T Gb[3];
-  normalize_vjp(G, nb, Gb);
+  normalize_vjp(Gr, nb, Gb);
   T pb[3] = {pb_in[0], pb_in[1], pb_in[2]};
   trilinear_vjp(c, p, Gb, Sb, pb);
-  lerp_vjp(c, e, edge_dir_dot(c, e, pb), Sb);
+  lerp_vjp(c, e, pb, Sb);
 }
 
-// Adds cell-corner adjoints of the cell at offset (oi,oj,ok) from the active cell into the
-// 4x4x4 local stencil (sample offsets -1..2 per axis).
+// Adds the corner adjoints of the cell at offset (oi,oj,ok) from the active cell into its 4x4x4
+// stencil (sample offsets -1..2 per axis).
 template <typename T>
 __device__ __forceinline__ void add_corners(float* lc, int oi, int oj, int ok, const T Sb[8]) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int idx = (ok + c_corner[q][2] + 1) * 16 + (oj + c_corner[q][1] + 1) * 4 +
-                    (oi + c_corner[q][0] + 1);
-    lc[idx] += (float)Sb[q];
-  }
+  for (int q = 0; q < 8; ++q)
+    lc[(ok + cbz(q) + 1) * 16 + (oj + cby(q) + 1) * 4 + (oi + cbx(q) + 1)] += (float)Sb[q];
 }
 
-// Full assembly in T precision for the adjoint (cell frame for float, global for double).
+// Assembly in T precision for the adjoint (cell frame for float, global frame for double).
 template <typename T>
-__device__ void assemble_any(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                             int k, const Cell<T>& own, Qef<T>& Q) {
+__device__ __forceinline__ void assemble_any(const DcGridDev& G, const uint64_t* __restrict__ act,
+                             const FaceNormals<T>& fn, int i, int j, int k, const Cell<T>& own,
+                             Qef<T>& Q) {
   T c[3] = {T(0), T(0), T(0)};
   int m = 0;
   for (int e = 0; e < 12; ++e) {
-    const int a = c_edge[e][0], b = c_edge[e][1];
-    if (skip_edge(own.S[a], own.S[b])) continue;
+    if (!edge_crosses(own, e)) continue;
     T p[3];
     hermite_point(own, e, p);
     c[0] += p[0];
@@ -742,34 +728,45 @@ __device__ void assemble_any(const DcGridDev& G, const uint64_t* __restrict__ ac
     ++m;
   }
   Q.m = m;
+#pragma unroll
   for (int d = 0; d < 3; ++d) Q.c[d] = c[d] / T(m);
+#pragma unroll
   for (int r = 0; r < 3; ++r) {
     Q.g[r] = T(0);
+#pragma unroll
     for (int s = 0; s < 3; ++s) Q.M[r][s] = T(0);
   }
   for (int e = 0; e < 12; ++e) {
-    T p[3], n[3], Gr[3];
-    if (!hermite_edge(own, e, p, n, Gr)) continue;
-    T nf[3];
-    final_normal(G, act, i, j, k, e, own, n, nf);
+    if (!edge_crosses(own, e)) continue;
+    T p[3], nf[3];
+    hermite_point(own, e, p);
+    uint64_t face;
+    bool avg;
+    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
     normalize3(nf);
     const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
     const T d = dot3(nf, q);
+#pragma unroll
     for (int r = 0; r < 3; ++r) {
       Q.g[r] += nf[r] * d;
+#pragma unroll
       for (int s = 0; s < 3; ++s) Q.M[r][s] += nf[r] * nf[s];
     }
   }
 }
 
-// dL/dphi contributions of one active cell into lc[64] (the 4x4x4 stencil around it).
+// Adjoint phase 1 for one active cell. For each row: the adjoint u-bar of its final hermite
+// normal and p-bar of its hermite point. Averaged strict rows store u-bar in the face's slot for
+// this cell (ubar[4*face + member]); their normal's dependence on the 4 cells is applied in
+// phase 2. Point adjoints and all other rows go straight to the cell's stencil lc[64].
 template <typename T>
-__device__ void backward_cell(const DcGridDev& G, const uint64_t* __restrict__ act, int i, int j,
-                              int k, const float xb_in[3], float* lc) {
+__device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
+                              const FaceNormals<T>& fn, int i, int j, int k, const float xb_in[3],
+                              float* lc, float* __restrict__ ubar) {
   Cell<T> own;
   load_cell(G, i, j, k, own);
   Qef<T> Q;
-  assemble_any(G, act, i, j, k, own, Q);
+  assemble_any(G, act, fn, i, j, k, own, Q);
   T lam[3], V[3][3];
   sym3_eig(Q.M, lam, V);
   const T xb[3] = {(T)xb_in[0], (T)xb_in[1], (T)xb_in[2]};
@@ -780,16 +777,26 @@ __device__ void backward_cell(const DcGridDev& G, const uint64_t* __restrict__ a
     kept[q] = fabs(lam[q]) > T(kSvdThreshold);
     if (!kept[q]) continue;
     const T qx = (V[0][q] * xb[0] + V[1][q] * xb[1]) + V[2][q] * xb[2];
+#pragma unroll
     for (int d = 0; d < 3; ++d) gb[d] += V[d][q] * qx / lam[q];
   }
-  // Daleckii-Krein: M-bar = V (F o (V^T sym(xb g^T) V)) V^T
+  // Daleckii-Krein: M-bar = V (F o (V^T sym(xb g^T) V)) V^T, F_ab = (f(l_a)-f(l_b))/(l_a-l_b),
+  // f(l) = 1/l above the threshold, 0 below.
+  T S[3][3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+#pragma unroll
+    for (int w = 0; w < 3; ++w) S[u][w] = T(0.5) * (xb[u] * Q.g[w] + Q.g[u] * xb[w]);
   T B[3][3];
+#pragma unroll
   for (int a = 0; a < 3; ++a)
+#pragma unroll
     for (int b = 0; b < 3; ++b) {
       T s = T(0);
+#pragma unroll
       for (int u = 0; u < 3; ++u)
-        for (int w = 0; w < 3; ++w)
-          s += V[u][a] * (T(0.5) * (xb[u] * Q.g[w] + Q.g[u] * xb[w])) * V[w][b];
+#pragma unroll
+        for (int w = 0; w < 3; ++w) s += V[u][a] * S[u][w] * V[w][b];
       T F;
       if (kept[a] && kept[b])
         F = T(-1) / (lam[a] * lam[b]);
@@ -801,86 +808,115 @@ __device__ void backward_cell(const DcGridDev& G, const uint64_t* __restrict__ a
       B[a][b] = F * s;
     }
   T Mb[3][3];
+#pragma unroll
   for (int a = 0; a < 3; ++a)
+#pragma unroll
     for (int b = 0; b < 3; ++b) {
       T s = T(0);
+#pragma unroll
       for (int u = 0; u < 3; ++u)
+#pragma unroll
         for (int w = 0; w < 3; ++w) s += V[a][u] * B[u][w] * V[b][w];
       Mb[a][b] = s;
     }
-  // c-bar total = xb - M gb (every row's q = p - c pulls on c)
+  // x = c + P g with g = sum n (n.(p-c)): c-bar = xb - M gb, shared by the rows' points.
   T cbm[3];
+#pragma unroll
   for (int a = 0; a < 3; ++a)
     cbm[a] = (xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2])) / T(Q.m);
   T Sown[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   const T zero3[3] = {T(0), T(0), T(0)};
   for (int e = 0; e < 12; ++e) {
-    T p[3], n_raw[3], Gr[3];
-    if (!hermite_edge(own, e, p, n_raw, Gr)) continue;
-    // forward of the normal chain
-    int cells[4][3], axis;
-    const int ea = c_edge[e][0], eb = c_edge[e][1];
-    const bool avg = edge_group(G, act, i, j, k, e, strict_change(own.S[ea], own.S[eb]), cells,
-                                axis);
-    T acc[3] = {T(0), T(0), T(0)};
-    bool contrib[4] = {false, false, false, false};
-    T nf[3];
-    if (avg) {
-      for (int q = 0; q < 4; ++q) {
-        T n[3];
-        if (cells[q][0] == i && cells[q][1] == j && cells[q][2] == k) {
-          n[0] = n_raw[0];
-          n[1] = n_raw[1];
-          n[2] = n_raw[2];
-        } else {
-          Cell<T> nc;
-          load_cell(G, cells[q][0], cells[q][1], cells[q][2], nc);
-          T pp[3], GG[3];
-          hermite_edge(nc, c_grp_edge[axis][q], pp, n, GG);
-        }
-        contrib[q] = sq3(n[0], n[1], n[2]) > T(0);
-        if (contrib[q]) {
-          acc[0] += n[0];
-          acc[1] += n[1];
-          acc[2] += n[2];
-        }
-      }
-      nf[0] = acc[0];
-      nf[1] = acc[1];
-      nf[2] = acc[2];
-      normalize3(nf);
-    } else {
-      nf[0] = n_raw[0];
-      nf[1] = n_raw[1];
-      nf[2] = n_raw[2];
-    }
+    if (!edge_crosses(own, e)) continue;
+    T p[3], nf[3];
+    hermite_point(own, e, p);
+    uint64_t face;
+    bool avg;
+    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
     T n[3] = {nf[0], nf[1], nf[2]};
     normalize3(n);
     const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
     const T nq = dot3(n, q), ng = dot3(n, gb);
     T nb[3], pb[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T Mn = (T(2) * Mb[d][0] * n[0] + T(2) * Mb[d][1] * n[1]) + T(2) * Mb[d][2] * n[2];
       nb[d] = Mn + gb[d] * nq + q[d] * ng;
       pb[d] = n[d] * ng + cbm[d];
     }
     T ub[3];
-    normalize_vjp(nf, nb, ub);  // row renormalization
-    if (avg) {
-      T accb[3];
-      normalize_vjp(acc, ub, accb);
-      for (int cq = 0; cq < 4; ++cq) {
-        if (!contrib[cq]) continue;
-        Cell<T> nc;
-        load_cell(G, cells[cq][0], cells[cq][1], cells[cq][2], nc);
-        T Sb[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-        hermite_vjp(nc, (int)c_grp_edge[axis][cq], accb, zero3, Sb);
-        add_corners(lc, cells[cq][0] - i, cells[cq][1] - j, cells[cq][2] - k, Sb);
-      }
-      lerp_vjp(own, e, edge_dir_dot(own, e, pb), Sown);
-    } else {
+    normalize_vjp(nf, nb, ub);  // the solve re-normalizes each row
+    if (!avg) {
       hermite_vjp(own, e, ub, pb, Sown);
+      continue;
     }
+    lerp_vjp(own, e, pb, Sown);  // the point enters the solve directly
+    int axis, x0, y0, z0;
+    const bool strict = strict_change(own.s(edge_a(e)), own.s(edge_b(e)));
+    edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0);
+    if (face != ~0ull) {
+      // member index of this cell in the group: bit 0 = moved along dir1, bit 1 = along dir2
+      const int d1moved = (axis == 0) ? (j != y0) : (i != x0);
+      const int d2moved = (axis == 2) ? (j != y0) : (k != z0);
+      float* slot = ubar + 3 * (4 * face + (d1moved + 2 * d2moved));
+      slot[0] = (float)ub[0];
+      slot[1] = (float)ub[1];
+      slot[2] = (float)ub[2];
+      continue;
+    }
+    // averaged edge without a face-table entry (a zero endpoint): the whole group here
+    T acc[3];
+    group_sum(G, axis, x0, y0, z0, acc);
+    T accb[3];
+    normalize_vjp(acc, ub, accb);
+    for (int m = 0; m < 4; ++m) {
+      int x, y, z;
+      group_member(axis, m, x0, y0, z0, x, y, z);
+      Cell<T> nc;
+      load_cell(G, x, y, z, nc);
+      T n2[3];
+      raw_normal(nc, group_edge(axis, m), n2);
+      if (!(sq3(n2[0], n2[1], n2[2]) > T(0))) continue;
+      T Sb[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+      hermite_vjp(nc, group_edge(axis, m), accb, zero3, Sb);
+      add_corners(lc, x - i, y - j, z - k, Sb);
+    }
+  }
+  add_corners(lc, 0, 0, 0, Sown);
+}
+
+// Adjoint phase 2 for one active cell: for each averaged strict row, the summed u-bar of the
+// face's 4 cells through the average's normalization, then this cell's own raw normal's VJP.
+template <typename T>
+__device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_t* __restrict__ act,
+                               const FaceNormals<T>& fn, int i, int j, int k,
+                               const float* __restrict__ ubar, float* lc) {
+  Cell<T> own;
+  load_cell(G, i, j, k, own);
+  T Sown[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  const T zero3[3] = {T(0), T(0), T(0)};
+  if (!fn.cap) return;
+  for (int e = 0; e < 12; ++e) {
+    if (!strict_change(own.s(edge_a(e)), own.s(edge_b(e)))) continue;
+    int axis, x0, y0, z0;
+    if (!edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0)) continue;
+    const uint64_t f = face_id(G, fn.ft, fn.pw, axis, x0, y0, z0);
+    if (f >= fn.cap) continue;
+    T n[3];
+    raw_normal(own, e, n);
+    if (!(sq3(n[0], n[1], n[2]) > T(0))) continue;  // excluded from the average
+    T us[3] = {T(0), T(0), T(0)};
+    for (int m = 0; m < 4; ++m) {
+      const float* slot = ubar + 3 * (4 * f + m);
+      us[0] += (T)slot[0];
+      us[1] += (T)slot[1];
+      us[2] += (T)slot[2];
+    }
+    const T len = fn.at(4 * f + 3);
+    const T acc[3] = {fn.at(4 * f + 0) * len, fn.at(4 * f + 1) * len, fn.at(4 * f + 2) * len};
+    T accb[3];
+    normalize_vjp(acc, us, accb);
+    hermite_vjp(own, e, accb, zero3, Sown);
   }
   add_corners(lc, 0, 0, 0, Sown);
 }

@@ -41,7 +41,7 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, ft, pcount, pbase, wprefix, alist, v64, v32, faces, status, tilectr,
-      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm;
+      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm, ubar;
   DcCounters* h_counters = nullptr;
   hipEvent_t ev[kNumEvents] = {};
   // last forward
@@ -192,7 +192,7 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->flags, c->capA * 4, "vertex flags"))) return rc;
   if ((rc = ensure(c, c->faces, c->capF * 16, "faces"))) return rc;
   if ((rc = ensure(c, c->pw, c->n_ft * 4, "face word prefix"))) return rc;
-  if ((rc = ensure(c, c->fnrm, c->capF * 24, "face normals"))) return rc;
+  if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
 
   // host staging stays alive in the context until the copies have run
   std::vector<int64_t>& bounds = c->hbounds;
@@ -423,7 +423,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->status, &c->tilectr,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad, &c->pw,
-                    &c->fnrm};
+                    &c->fnrm, &c->ubar};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -569,6 +569,8 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                    "adjoint contributions")))
     return rc;
   if ((rc = ensure(c, c->bwd_grad, c->n_dil * 8, "candidate plane"))) return rc;
+  if ((rc = ensure(c, c->ubar, (size_t)std::max<int64_t>(c->nf, 1) * 48, "face adjoint slots")))
+    return rc;
   hipStream_t s = c->stream;
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   DcBwdArgs a;
@@ -589,6 +591,11 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.cell_blocks = std::max<int64_t>(1, std::min<int64_t>((c->nv + 63) / 64,
                                                           (int64_t)c->cu_count * 32));
   a.grad_vertices = grad_vertices;
+  a.ft = as<uint64_t>(c->ft);
+  a.pw = as<uint32_t>(c->pw);
+  a.nrm = c->fnrm.p;
+  a.capF = c->capF;
+  a.ubar = as<float>(c->ubar);
   a.contrib = as<float>(c->bwd_contrib);
   a.dil = as<uint64_t>(c->bwd_grad);
   a.grad_values = grad_values;

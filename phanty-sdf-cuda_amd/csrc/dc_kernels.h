@@ -88,6 +88,11 @@ struct DcBwdArgs {
   uint64_t capA;
   int64_t cell_blocks;
   const float* grad_vertices;
+  const uint64_t* ft;
+  const uint32_t* pw;
+  const void* nrm;
+  uint64_t capF;
+  float* ubar;  // 4 slots x 3 floats per face
   float* contrib;
   uint64_t* dil;
   float* grad_values;

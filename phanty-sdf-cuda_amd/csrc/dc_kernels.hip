@@ -13,8 +13,10 @@
 //               averaging, QEF solve (FP64 plain, or FP32 with FP64 fallback) -> vertex.
 //   k_emit      one thread per pencil: walks the face bits along k and writes oriented quads in
 //               the CPU backend's order.
-//   k_bwd_*     adjoint: per-cell stencil contributions (k_bwd_cell), the candidate-sample plane
-//               (k_bwd_dilate) and a fixed-order per-sample gather (k_bwd_gather).
+//   k_bwd_*     adjoint: per-cell stencil contributions in two phases (k_bwd_cell<1>: rows and
+//               per-face normal adjoints; k_bwd_cell<2>: each cell's share of its faces'
+//               averaged-normal adjoints), the candidate-sample plane (k_bwd_dilate) and a
+//               fixed-order per-sample gather (k_bwd_gather).
 //
 // Reference semantics (file:line in the reference tree) are cited at each device function.
 // Compiled with -ffp-contract=off so that FP64 results match the CPU restatement bit for bit.
@@ -389,10 +391,10 @@ __global__ void __launch_bounds__(256)
           const uint32_t* __restrict__ pw, const void* __restrict__ nrm, uint64_t capF) {
   const uint64_t n = min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
-  const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm),
+  const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm), nullptr,
                                  PREC == DC_PREC_FP64 ? nf : 0ull};
-  const FaceNormals<double> inline64{ft, pw, nullptr, 0ull};
-  const FaceNormals<float> fn32{ft, pw, reinterpret_cast<const float*>(nrm),
+  const FaceNormals<double> inline64{ft, pw, nullptr, nullptr, 0ull};
+  const FaceNormals<float> fn32{ft, pw, nullptr, reinterpret_cast<const float*>(nrm),
                                 PREC == DC_PREC_FP32 ? nf : 0ull};
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,29 +443,49 @@ __global__ void __launch_bounds__(256)
 constexpr int kBwdThreads = 64;
 constexpr int kStencil = 64;  // 4x4x4 samples around a cell: offsets -1..2 per axis
 
+template <int PHASE>
 __global__ void __launch_bounds__(kBwdThreads)
     k_bwd_cell(const DcGridDev* __restrict__ grids, int n_grids,
                const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
                const uint64_t* __restrict__ alist, const uint32_t* __restrict__ flags,
                int precision, const DcCounters* __restrict__ ctr, uint64_t cap,
-               const float* __restrict__ gv, float* __restrict__ contrib) {
+               const float* __restrict__ gv, const uint64_t* __restrict__ ft,
+               const uint32_t* __restrict__ pw, const void* __restrict__ nrm, uint64_t capF,
+               float* __restrict__ ubar, float* __restrict__ contrib) {
   __shared__ float s_lc[kBwdThreads][kStencil + 1];
   float* lc = s_lc[threadIdx.x];
   const uint64_t n = min(ctr->n_active, cap);
+  const uint64_t nf = min(ctr->n_faces, capF);
+  const double* n64 = precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(nrm) : nullptr;
+  const float* n32 = precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(nrm);
+  const FaceNormals<double> fd{ft, pw, n64, n32, nf};
+  const FaceNormals<float> ff{ft, pw, n64, n32, nf};
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * blockDim.x) {
-    for (int q = 0; q < kStencil; ++q) lc[q] = 0.f;
+    float* out = contrib + v * kStencil;
+    if (PHASE == 1) {
+      for (int q = 0; q < kStencil; ++q) lc[q] = 0.f;
+    } else {
+      for (int q = 0; q < kStencil; ++q) lc[q] = out[q];
+    }
     const uint64_t entry = alist[v];
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
     decode_cell(G, entry, i, j, k);
-    const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
-    if (precision == DC_PREC_FP64 || flags[v])
-      backward_cell<double>(G, act, i, j, k, xb, lc);
-    else
-      backward_cell<float>(G, act, i, j, k, xb, lc);
-    float* out = contrib + v * kStencil;
+    const bool f64 = precision == DC_PREC_FP64 || flags[v];
+    if (PHASE == 1) {
+      const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
+      if (f64)
+        backward_rows<double>(G, act, fd, i, j, k, xb, lc, ubar);
+      else
+        backward_rows<float>(G, act, ff, i, j, k, xb, lc, ubar);
+    } else {
+      if (f64)
+        backward_faces<double>(G, act, fd, i, j, k, ubar, lc);
+      else
+        backward_faces<float>(G, act, ff, i, j, k, ubar, lc);
+    }
     for (int q = 0; q < kStencil; ++q) out[q] = lc[q];
   }
 }
@@ -533,7 +555,7 @@ __global__ void __launch_bounds__(256)
 // k_emit — quads in the CPU backend's order (contouring.cpp:607-729)
 // ---------------------------------------------------------------------------------------------
 // contouring.cpp:141-207 gradientAt: central differences with clamped neighbours, normalized.
-__device__ void grid_gradient(const DcGridDev& G, int i, int j, int k, double out[3]) {
+__device__ __forceinline__ void grid_gradient(const DcGridDev& G, int i, int j, int k, double out[3]) {
   const int im = max(i - 1, 0), ip = min(i + 1, G.nx - 1);
   const int jm = max(j - 1, 0), jp = min(j + 1, G.ny - 1);
   const int km = max(k - 1, 0), kp = min(k + 1, G.nz - 1);
@@ -650,10 +672,13 @@ __global__ void __launch_bounds__(256)
                 [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
                   if (face >= capF) return;
                   T n[3];
-                  face_group_normal(G, a, i, j, k, n);
-                  nrm[3 * face + 0] = n[0];
-                  nrm[3 * face + 1] = n[1];
-                  nrm[3 * face + 2] = n[2];
+                  group_sum(G, a, i, j, k, n);
+                  const T len = tsqrt(sq3(n[0], n[1], n[2]));
+                  normalize3(n);
+                  nrm[4 * face + 0] = n[0];
+                  nrm[4 * face + 1] = n[1];
+                  nrm[4 * face + 2] = n[2];
+                  nrm[4 * face + 3] = len;
                 });
 }
 
@@ -842,10 +867,15 @@ hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
 }
 
 hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_bwd_cell, dim3((unsigned)a.cell_blocks), dim3(kBwdThreads), 0, s, a.grids,
-                     a.n_grids, a.act_bounds, a.act, a.alist, a.flags, a.precision, a.ctr, a.capA,
-                     a.grad_vertices, a.contrib);
+  hipLaunchKernelGGL(k_bwd_cell<1>, dim3((unsigned)a.cell_blocks), dim3(kBwdThreads), 0, s,
+                     a.grids, a.n_grids, a.act_bounds, a.act, a.alist, a.flags, a.precision,
+                     a.ctr, a.capA, a.grad_vertices, a.ft, a.pw, a.nrm, a.capF, a.ubar, a.contrib);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bwd_cell<2>, dim3((unsigned)a.cell_blocks), dim3(kBwdThreads), 0, s,
+                     a.grids, a.n_grids, a.act_bounds, a.act, a.alist, a.flags, a.precision,
+                     a.ctr, a.capA, a.grad_vertices, a.ft, a.pw, a.nrm, a.capF, a.ubar, a.contrib);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.n_dil > 0)
     hipLaunchKernelGGL(k_bwd_dilate, dim3((unsigned)((a.n_dil + 255) / 256)), dim3(256), 0, s,
