@@ -36,7 +36,7 @@ def _run(cmd, verbose):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("dc_kernels.hip", "dc_hip.cpp"):
+    for src in ("dc_kernels.hip", "dc_cells.hip", "dc_hip.cpp"):
         obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"]
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj], verbose)

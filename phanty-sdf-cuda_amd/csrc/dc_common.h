@@ -1,0 +1,31 @@
+// dc_common.h — small device helpers shared by the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dc_internal.h"
+
+namespace dccommon {
+
+// Largest b in [0, n) with bounds[b] <= x (bounds ascending, bounds[0] = 0): the grid of a batch
+// that owns item x.
+__device__ __forceinline__ int find_bucket(const int64_t* __restrict__ bounds, int n, int64_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bounds[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Workgroup b runs on XCD b % 8. Remap so each XCD gets a contiguous range of tiles, keeping
+// neighbouring tiles (which share halo rows) in one L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
+  const int64_t q = nb >> 3, r = nb & 7, x = b & 7, s = b >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+}  // namespace dccommon

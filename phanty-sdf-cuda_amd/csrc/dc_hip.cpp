@@ -41,7 +41,7 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, ft, pcount, pbase, wprefix, alist, v64, v32, faces, status, tilectr,
-      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm, ubar;
+      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm, ubar, fblist;
   DcCounters* h_counters = nullptr;
   hipEvent_t ev[kNumEvents] = {};
   // last forward
@@ -50,7 +50,7 @@ struct dc_hip_context {
   int64_t nv = 0, nf = 0;
   uint64_t capA = 0, capF = 0;
   std::vector<DcGridDev> hgrids;
-  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, dil_b, hbounds;
+  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, dil_b, ft_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_dil = 0;
 };
 
@@ -105,6 +105,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->pen_b.assign(n + 1, 0);
   c->samp_b.assign(n + 1, 0);
   c->dil_b.assign(n + 1, 0);
+  c->ft_b.assign(n + 1, 0);
   int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, dil = 0;
   for (int q = 0; q < n; ++q) {
     const dc_hip_grid& s = grids[q];
@@ -141,6 +142,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     c->pen_b[q] = pen;
     c->samp_b[q] = samp;
     c->dil_b[q] = dil;
+    c->ft_b[q] = ftw;
     words += (int64_t)g.W * g.cy * g.cz;
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
@@ -153,6 +155,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->pen_b[n] = pen;
   c->samp_b[n] = samp;
   c->dil_b[n] = dil;
+  c->ft_b[n] = ftw;
   if ((uint64_t)words * 64ull >= (1ull << 32) * 4ull)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "batch has too many cells for 32-bit vertex ids");
   c->n_words = words;
@@ -176,7 +179,7 @@ int run_once(dc_hip_context* c, int precision) {
   const int n = (int)c->hgrids.size();
   int rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
-  if ((rc = ensure(c, c->bounds, 5 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
+  if ((rc = ensure(c, c->bounds, 6 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
   if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
   if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
@@ -190,6 +193,7 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->v64, c->capA * 24, "fp64 vertices"))) return rc;
   if ((rc = ensure(c, c->v32, c->capA * 12, "vertices"))) return rc;
   if ((rc = ensure(c, c->flags, c->capA * 4, "vertex flags"))) return rc;
+  if ((rc = ensure(c, c->fblist, c->capA * 4, "fallback list"))) return rc;
   if ((rc = ensure(c, c->faces, c->capF * 16, "faces"))) return rc;
   if ((rc = ensure(c, c->pw, c->n_ft * 4, "face word prefix"))) return rc;
   if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
@@ -202,6 +206,7 @@ int run_once(dc_hip_context* c, int precision) {
   bounds.insert(bounds.end(), c->pen_b.begin(), c->pen_b.end());
   bounds.insert(bounds.end(), c->samp_b.begin(), c->samp_b.end());
   bounds.insert(bounds.end(), c->dil_b.begin(), c->dil_b.end());
+  bounds.insert(bounds.end(), c->ft_b.begin(), c->ft_b.end());
   DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
                            hipMemcpyHostToDevice, s), "upload descriptors");
   DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
@@ -209,6 +214,7 @@ int run_once(dc_hip_context* c, int precision) {
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   const int64_t* d_act_b = d_tile_b + (n + 1);
   const int64_t* d_pen_b = d_act_b + (n + 1);
+  const int64_t* d_ft_b = d_tile_b + 5 * (n + 1);
   const DcGridDev* d_grids = as<DcGridDev>(c->grids);
   DcCounters* d_ctr = as<DcCounters>(c->counters);
   uint64_t* st = as<uint64_t>(c->status);
@@ -248,6 +254,8 @@ int run_once(dc_hip_context* c, int precision) {
   fa.n_grids = n;
   fa.pen_bounds = d_pen_b;
   fa.n_pen = c->n_pen;
+  fa.ft_bounds = d_ft_b;
+  fa.n_ft = c->n_ft;
   fa.ft = as<uint64_t>(c->ft);
   fa.pbase = as<uint32_t>(c->pbase);
   fa.pw = as<uint32_t>(c->pw);
@@ -271,6 +279,7 @@ int run_once(dc_hip_context* c, int precision) {
   qa.v64 = as<double>(c->v64);
   qa.v32 = as<float>(c->v32);
   qa.flags = as<uint32_t>(c->flags);
+  qa.fblist = as<uint32_t>(c->fblist);
   qa.ctr_rw = d_ctr;
   qa.ft = as<uint64_t>(c->ft);
   qa.pw = as<uint32_t>(c->pw);
@@ -281,12 +290,12 @@ int run_once(dc_hip_context* c, int precision) {
   DcEmitArgs ea;
   ea.grids = d_grids;
   ea.n_grids = n;
-  ea.pen_bounds = d_pen_b;
-  ea.n_pen = c->n_pen;
+  ea.ft_bounds = d_ft_b;
+  ea.n_ft = c->n_ft;
   ea.act = as<uint64_t>(c->act);
   ea.wprefix = as<uint32_t>(c->wprefix);
   ea.ft = as<uint64_t>(c->ft);
-  ea.pbase = as<uint32_t>(c->pbase);
+  ea.pw = as<uint32_t>(c->pw);
   ea.v64 = as<double>(c->v64);
   ea.v32 = as<float>(c->v32);
   ea.precision = precision;
@@ -423,7 +432,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->status, &c->tilectr,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad, &c->pw,
-                    &c->fnrm, &c->ubar};
+                    &c->fnrm, &c->ubar, &c->fblist};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -585,6 +594,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.wprefix = as<uint32_t>(c->wprefix);
   a.alist = as<uint64_t>(c->alist);
   a.flags = as<uint32_t>(c->flags);
+  a.fblist = as<uint32_t>(c->fblist);
   a.precision = c->precision;
   a.ctr = as<DcCounters>(c->counters);
   a.capA = c->capA;

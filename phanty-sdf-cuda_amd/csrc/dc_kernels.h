@@ -31,7 +31,8 @@ struct DcQefArgs {
   int precision;
   double* v64;
   float* v32;
-  uint32_t* flags;  // per vertex, FP32 mode: 1 when the cell was solved in FP64
+  uint32_t* flags;   // per vertex, FP32 mode: 1 when the cell was solved in FP64
+  uint32_t* fblist;  // FP32 mode: vertices to solve in FP64 (count in ctr->fallback)
   DcCounters* ctr_rw;
   const uint64_t* ft;
   const uint32_t* pw;
@@ -44,6 +45,8 @@ struct DcFaceNormalArgs {
   int n_grids;
   const int64_t* pen_bounds;
   int64_t n_pen;
+  const int64_t* ft_bounds;  // n_grids + 1, in face words
+  int64_t n_ft;
   const uint64_t* ft;
   const uint32_t* pbase;
   uint32_t* pw;
@@ -56,12 +59,12 @@ struct DcFaceNormalArgs {
 struct DcEmitArgs {
   const DcGridDev* grids;
   int n_grids;
-  const int64_t* pen_bounds;
-  int64_t n_pen;
+  const int64_t* ft_bounds;
+  int64_t n_ft;
   const uint64_t* act;
   const uint32_t* wprefix;
   const uint64_t* ft;
-  const uint32_t* pbase;
+  const uint32_t* pw;
   const double* v64;
   const float* v32;
   int precision;
@@ -83,6 +86,7 @@ struct DcBwdArgs {
   const uint32_t* wprefix;
   const uint64_t* alist;
   const uint32_t* flags;
+  const uint32_t* fblist;
   int precision;
   const DcCounters* ctr;
   uint64_t capA;
