@@ -71,7 +71,7 @@ def main():
 
     import torch
 
-    from contouring import HipDualContouringBackend, _lib, synth
+    from contouring import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
 
     n = args.size
     t = time.time()
@@ -83,6 +83,12 @@ def main():
     lib = be._lib
     cg = be._cgrid(g, phi.data_ptr())
     prec = _lib.DC_HIP_PLAIN_FP64
+
+    # mark tile shape: timed on this grid (the sweep is not part of the timed region)
+    gdev = DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, d_values=phi)
+    tile, table = be.autotune(gdev, repeats=3, use_cache=False)
+    for row in table[:6] + table[-2:]:
+        log(f"[tune] layers {row['layers']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
 
     run_steps(be, lib, cg, prec, args.warmup)
     torch.cuda.synchronize()
@@ -166,6 +172,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"csg8 sharp-feature CSG {n}^3 FP32 grid, plain FP64 solve",
                    "grid": n, "seed": args.seed, "precision": "plain-fp64",
+                   "mark_tile": {k: tile[k] for k in ("layers", "waves")},
                    "parallelism": "single GPU"},
         "active_cells": int(st.vertex_count),
         "faces": int(st.face_count),

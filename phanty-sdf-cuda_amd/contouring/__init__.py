@@ -12,3 +12,4 @@ from .backend import (  # noqa: F401
     PRECISIONS,
 )
 from . import synth  # noqa: F401
+from .autotune import autotune_mark  # noqa: F401

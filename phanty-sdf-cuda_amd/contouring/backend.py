@@ -103,7 +103,7 @@ class HipDualContouringBackend:
     "fp32-fallback" (FP32 assembly, FP64 only for ill-conditioned cells).
     """
 
-    def __init__(self, precision="plain-fp64", device=0, mark_rows=None, mark_waves=None):
+    def __init__(self, precision="plain-fp64", device=0, mark_layers=None, mark_waves=None):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
@@ -115,8 +115,8 @@ class HipDualContouringBackend:
             raise _lib.DcHipError(rc, self._lib.dc_hip_status_string(rc).decode())
         self._ctx = ctx
         self.generation = 0
-        if mark_rows is not None:
-            self.set_option("mark_rows", mark_rows)
+        if mark_layers is not None:
+            self.set_option("mark_layers", mark_layers)
         if mark_waves is not None:
             self.set_option("mark_waves", mark_waves)
 
@@ -134,6 +134,12 @@ class HipDualContouringBackend:
 
     def set_option(self, key, value):
         self._check(self._lib.dc_hip_set_option(self._ctx, key.encode(), int(value)))
+
+    def autotune(self, grids, **kw):
+        """Time every mark tile shape on `grids` (device grids) and keep the fastest; see
+        contouring.autotune.autotune_mark."""
+        from .autotune import autotune_mark
+        return autotune_mark(self, grids, **kw)
 
     def set_stream(self, stream_handle):
         self._check(self._lib.dc_hip_set_stream(self._ctx, ctypes.c_void_p(stream_handle or 0)))

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dc_common.h"
 #include "dc_internal.h"
 
 namespace dccell {
@@ -135,7 +136,7 @@ template <typename T>
 struct Cell {
   T S[8];
   T X0, X1, Y0, Y1, Z0, Z1;
-  const float* base;
+  gfloat base;
   int64_t sy, sz;
   __device__ __forceinline__ T s(int q) const {
     return (T)base[cbx(q) + sy * cby(q) + sz * cbz(q)];
@@ -145,13 +146,13 @@ struct Cell {
   __device__ __forceinline__ T cz(int q) const { return cbz(q) ? Z1 : Z0; }
 };
 
-__device__ __forceinline__ const float* sample_ptr(const DcGridDev& G, int i, int j, int k) {
-  return G.val + ((int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k));
+__device__ __forceinline__ gfloat sample_ptr(const DcGridDev& G, int i, int j, int k) {
+  return dccommon::to_global(G.val) + ((int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k));
 }
 
 template <typename T>
 __device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, int k, Cell<T>& c) {
-  const float* b = sample_ptr(G, i, j, k);
+  const gfloat b = sample_ptr(G, i, j, k);
   const int64_t sy = G.nx, sz = (int64_t)G.nx * G.ny;
   c.base = b;
   c.sy = sy;
@@ -702,15 +703,6 @@ __device__ __forceinline__ void hermite_vjp(const Cell<T>& c, int e, const T nb[
   lerp_vjp(c, e, pb, Sb);
 }
 
-// Adds the corner adjoints of the cell at offset (oi,oj,ok) from the active cell into its 4x4x4
-// stencil (sample offsets -1..2 per axis).
-template <typename T>
-__device__ __forceinline__ void add_corners(float* lc, int oi, int oj, int ok, const T Sb[8]) {
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-    lc[(ok + cbz(q) + 1) * 16 + (oj + cby(q) + 1) * 4 + (oi + cbx(q) + 1)] += (float)Sb[q];
-}
-
 // Assembly in T precision for the adjoint (cell frame for float, global frame for double).
 template <typename T>
 __device__ __forceinline__ void assemble_any(const DcGridDev& G, const uint64_t* __restrict__ act,
@@ -755,14 +747,34 @@ __device__ __forceinline__ void assemble_any(const DcGridDev& G, const uint64_t*
   }
 }
 
-// Adjoint phase 1 for one active cell. For each row: the adjoint u-bar of its final hermite
-// normal and p-bar of its hermite point. Averaged strict rows store u-bar in the face's slot for
-// this cell (ubar[4*face + member]); their normal's dependence on the 4 cells is applied in
-// phase 2. Point adjoints and all other rows go straight to the cell's stencil lc[64].
+// Global vertex id of active cell (i,j,k): rank in the batch's vertex order.
+__device__ __forceinline__ uint64_t global_vertex(const DcGridDev& G,
+                                                  const uint64_t* __restrict__ act,
+                                                  const uint32_t* __restrict__ wprefix, int i,
+                                                  int j, int k) {
+  const int64_t w = G.act_off + ((int64_t)k * G.cy + j) * G.W + (i >> 6);
+  const uint64_t below = (1ull << (i & 63)) - 1ull;
+  return (uint64_t)wprefix[w] + (uint64_t)__popcll(act[w] & below);
+}
+
+// Member index of cell (i,j,k) in the group of grid edge (axis, x0,y0,z0): bit 0 = moved along
+// dir1, bit 1 = along dir2.
+__device__ __forceinline__ int member_of(int axis, int i, int j, int k, int x0, int y0, int z0) {
+  const int d1moved = (axis == 0) ? (j != y0) : (i != x0);
+  const int d2moved = (axis == 2) ? (j != y0) : (k != z0);
+  return d1moved + 2 * d2moved;
+}
+
+// Adjoint phase 1 for vertex v = active cell (i,j,k). For each row: the adjoint u-bar of its
+// final hermite normal and p-bar of its hermite point. A row whose normal is a 4-cell average
+// stores u-bar for phase 2 (in the face's slot for this member, ubar[4*face + member], or, for
+// an averaged edge without a face, in the cell's per-edge slot ubar_e[12*v + e]); the point
+// adjoints and the other rows' normals go to the cell's own 8 corners, Sb.
 template <typename T>
 __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
-                              const FaceNormals<T>& fn, int i, int j, int k, const float xb_in[3],
-                              float* lc, float* __restrict__ ubar) {
+                              const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
+                              const float xb_in[3], float* __restrict__ ubar,
+                              float* __restrict__ ubar_e, T Sb[8]) {
   Cell<T> own;
   load_cell(G, i, j, k, own);
   Qef<T> Q;
@@ -824,8 +836,8 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
 #pragma unroll
   for (int a = 0; a < 3; ++a)
     cbm[a] = (xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2])) / T(Q.m);
-  T Sown[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-  const T zero3[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) Sb[q] = T(0);
   for (int e = 0; e < 12; ++e) {
     if (!edge_crosses(own, e)) continue;
     T p[3], nf[3];
@@ -847,78 +859,74 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
     T ub[3];
     normalize_vjp(nf, nb, ub);  // the solve re-normalizes each row
     if (!avg) {
-      hermite_vjp(own, e, ub, pb, Sown);
+      hermite_vjp(own, e, ub, pb, Sb);
       continue;
     }
-    lerp_vjp(own, e, pb, Sown);  // the point enters the solve directly
-    int axis, x0, y0, z0;
-    const bool strict = strict_change(own.s(edge_a(e)), own.s(edge_b(e)));
-    edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0);
+    lerp_vjp(own, e, pb, Sb);  // the point enters the solve directly
+    float* slot;
     if (face != ~0ull) {
-      // member index of this cell in the group: bit 0 = moved along dir1, bit 1 = along dir2
-      const int d1moved = (axis == 0) ? (j != y0) : (i != x0);
-      const int d2moved = (axis == 2) ? (j != y0) : (k != z0);
-      float* slot = ubar + 3 * (4 * face + (d1moved + 2 * d2moved));
-      slot[0] = (float)ub[0];
-      slot[1] = (float)ub[1];
-      slot[2] = (float)ub[2];
-      continue;
+      int axis, x0, y0, z0;
+      edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+      slot = ubar + 3 * (4 * face + member_of(axis, i, j, k, x0, y0, z0));
+    } else {
+      slot = ubar_e + 3 * (12 * v + e);
     }
-    // averaged edge without a face-table entry (a zero endpoint): the whole group here
-    T acc[3];
-    group_sum(G, axis, x0, y0, z0, acc);
-    T accb[3];
-    normalize_vjp(acc, ub, accb);
-    for (int m = 0; m < 4; ++m) {
-      int x, y, z;
-      group_member(axis, m, x0, y0, z0, x, y, z);
-      Cell<T> nc;
-      load_cell(G, x, y, z, nc);
-      T n2[3];
-      raw_normal(nc, group_edge(axis, m), n2);
-      if (!(sq3(n2[0], n2[1], n2[2]) > T(0))) continue;
-      T Sb[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-      hermite_vjp(nc, group_edge(axis, m), accb, zero3, Sb);
-      add_corners(lc, x - i, y - j, z - k, Sb);
-    }
+    slot[0] = (float)ub[0];
+    slot[1] = (float)ub[1];
+    slot[2] = (float)ub[2];
   }
-  add_corners(lc, 0, 0, 0, Sown);
 }
 
-// Adjoint phase 2 for one active cell: for each averaged strict row, the summed u-bar of the
-// face's 4 cells through the average's normalization, then this cell's own raw normal's VJP.
+// Adjoint phase 2 for one active cell: for each of its averaged rows, the u-bar of the group's
+// 4 cells summed, taken through the average's normalization, then through this cell's own raw
+// normal (its share of the average) into Sb.
 template <typename T>
 __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_t* __restrict__ act,
-                               const FaceNormals<T>& fn, int i, int j, int k,
-                               const float* __restrict__ ubar, float* lc) {
+                               const uint32_t* __restrict__ wprefix, const FaceNormals<T>& fn,
+                               int i, int j, int k, const float* __restrict__ ubar,
+                               const float* __restrict__ ubar_e, T Sb[8]) {
   Cell<T> own;
   load_cell(G, i, j, k, own);
-  T Sown[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   const T zero3[3] = {T(0), T(0), T(0)};
-  if (!fn.cap) return;
   for (int e = 0; e < 12; ++e) {
-    if (!strict_change(own.s(edge_a(e)), own.s(edge_b(e)))) continue;
+    const T fa = own.s(edge_a(e)), fb = own.s(edge_b(e));
+    if (skip_edge(fa, fb)) continue;
+    const bool strict = strict_change(fa, fb);
     int axis, x0, y0, z0;
-    if (!edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0)) continue;
-    const uint64_t f = face_id(G, fn.ft, fn.pw, axis, x0, y0, z0);
-    if (f >= fn.cap) continue;
+    if (!edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0)) continue;
     T n[3];
     raw_normal(own, e, n);
     if (!(sq3(n[0], n[1], n[2]) > T(0))) continue;  // excluded from the average
     T us[3] = {T(0), T(0), T(0)};
-    for (int m = 0; m < 4; ++m) {
-      const float* slot = ubar + 3 * (4 * f + m);
-      us[0] += (T)slot[0];
-      us[1] += (T)slot[1];
-      us[2] += (T)slot[2];
+    T acc[3];
+    const uint64_t f = (strict && fn.cap) ? face_id(G, fn.ft, fn.pw, axis, x0, y0, z0) : ~0ull;
+    if (f < fn.cap) {
+      for (int m = 0; m < 4; ++m) {
+        const float* slot = ubar + 3 * (4 * f + m);
+        us[0] += (T)slot[0];
+        us[1] += (T)slot[1];
+        us[2] += (T)slot[2];
+      }
+      const T len = fn.at(4 * f + 3);
+      acc[0] = fn.at(4 * f + 0) * len;
+      acc[1] = fn.at(4 * f + 1) * len;
+      acc[2] = fn.at(4 * f + 2) * len;
+    } else {
+      for (int m = 0; m < 4; ++m) {
+        int x, y, z;
+        group_member(axis, m, x0, y0, z0, x, y, z);
+        const uint64_t vm = global_vertex(G, act, wprefix, x, y, z);
+        const float* slot = ubar_e + 3 * (12 * vm + group_edge(axis, m));
+        us[0] += (T)slot[0];
+        us[1] += (T)slot[1];
+        us[2] += (T)slot[2];
+      }
+      group_sum(G, axis, x0, y0, z0, acc);
     }
-    const T len = fn.at(4 * f + 3);
-    const T acc[3] = {fn.at(4 * f + 0) * len, fn.at(4 * f + 1) * len, fn.at(4 * f + 2) * len};
     T accb[3];
     normalize_vjp(acc, us, accb);
-    hermite_vjp(own, e, accb, zero3, Sown);
+    hermite_vjp(own, e, accb, zero3, Sb);
   }
-  add_corners(lc, 0, 0, 0, Sown);
 }
 
 }  // namespace dccell

@@ -1,17 +1,16 @@
 // dc_cells.hip — per-face and per-active-cell kernels of the dense dual-contouring path (gfx950).
 //
-//   k_pencil_words  per face word: index of its first face (pw), from the pencil bases
+//   k_pencil_words  per pencil: index of the first face of each face word (pw) and the face list
 //   k_face_normals  per face: the 4-cell averaged hermite normal, once per grid edge
 //   k_qef           per active cell: QEF vertex (FP64 plain, or FP32 with a list of cells to
 //                   solve again in FP64, k_qef_fallback)
 //   k_emit          per face: the oriented quad in the CPU backend's order
 //   k_bwd_cell      adjoint per active cell, two phases (rows, then each cell's share of the
-//                   averaged normals' adjoints)
-//   k_bwd_dilate    candidate-sample plane of the adjoint
-//   k_bwd_gather    dL/dphi per sample: fixed-order sum of the stencil contributions
+//                   averaged normals' adjoints), into the cell's own 8 corners
+//   k_bwd_gather    dL/dphi per sample: fixed-order sum over the (up to) 8 cells containing it
 //
-// Faces are enumerated from the face planes one 64-bit word (64 k values of a pencil) per thread;
-// a word's faces are consecutive in the CPU backend's face order (contouring.cpp:607-729).
+// Faces are numbered in the CPU backend's face order (contouring.cpp:607-729) by the pencil scan;
+// the per-face kernels run one thread per face-list entry.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,7 +38,8 @@ __device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, 
 // ---------------------------------------------------------------------------------------------
 // Face enumeration
 // ---------------------------------------------------------------------------------------------
-// First face index of every face word: pw = pbase(pencil) + faces in the pencil's earlier words.
+// Per pencil (a,i,j), taken in (a, j, i) order so ft and pw accesses coalesce: the first face
+// index of each of its face words (pw).
 __global__ void __launch_bounds__(256)
     k_pencil_words(const DcGridDev* __restrict__ grids, int n_grids,
                    const int64_t* __restrict__ pen_bounds, int64_t n_pen,
@@ -52,8 +52,9 @@ __global__ void __launch_bounds__(256)
   const int64_t local = p - G.pen_off;
   const int64_t plane = (int64_t)G.cy * G.cx;
   const int a = (int)(local / plane);
-  const int64_t rem = local - (int64_t)a * plane;
-  uint32_t pos = pbase[p];
+  const int64_t rem = local - (int64_t)a * plane;  // j * cx + i
+  const int j = (int)(rem / G.cx), i = (int)(rem - (int64_t)j * G.cx);
+  uint32_t pos = pbase[G.pen_off + ((int64_t)a * G.cx + i) * G.cy + j];  // face order
   for (int kw = 0; kw < G.Wk; ++kw) {
     const int64_t idx = G.ft_off + ((int64_t)a * G.Wk + kw) * plane + rem;
     pw[idx] = pos;
@@ -61,32 +62,38 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// fn(G, a, i, j, k, face) for every face of face word `idx` (one thread per word).
-template <typename Fn>
-__device__ __forceinline__ void word_faces(const DcGridDev* __restrict__ grids, int n_grids,
-                                           const int64_t* __restrict__ ft_bounds, int64_t n_ft,
-                                           const uint64_t* __restrict__ ft,
-                                           const uint32_t* __restrict__ pw, Fn&& fn) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n_ft) return;
-  uint64_t w = ft[idx];
-  if (!w) return;
-  const int g = find_bucket(ft_bounds, n_grids, idx);
-  const DcGridDev& G = grids[g];
+__device__ __forceinline__ void decode_face(const DcGridDev& G, uint64_t entry, int& a, int& i,
+                                            int& j, int& k) {
   const int64_t plane = (int64_t)G.cy * G.cx;
-  const int64_t local = idx - G.ft_off;
+  const int64_t local = (int64_t)(entry >> 6) - G.ft_off;
   const int64_t aw = local / plane;  // a * Wk + kw
   const int64_t rem = local - aw * plane;
-  const int a = (int)(aw / G.Wk);
+  a = (int)(aw / G.Wk);
   const int kw = (int)(aw - (int64_t)a * G.Wk);
-  const int j = (int)(rem / G.cx);
-  const int i = (int)(rem - (int64_t)j * G.cx);
-  uint64_t face = pw[idx];
-  while (w) {
-    const int b = __ffsll((long long)w) - 1;
-    w &= w - 1;
-    fn(G, a, i, j, 64 * kw + b, face);
-    ++face;
+  j = (int)(rem / G.cx);
+  i = (int)(rem - (int64_t)j * G.cx);
+  k = 64 * kw + (int)(entry & 63);
+}
+
+// fn(G, a, i, j, k, face id) for every face, one thread per face-list entry (grid-stride over
+// the count).
+template <typename Fn>
+__device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, int n_grids,
+                                          const int64_t* __restrict__ ft_bounds,
+                                          const uint64_t* __restrict__ flist,
+                                          const uint64_t* __restrict__ ft,
+                                          const uint32_t* __restrict__ pw, uint64_t n, Fn&& fn) {
+  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n;
+       f += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t entry = flist[f];
+    const int64_t word = (int64_t)(entry >> 6);
+    const uint64_t below = (1ull << (entry & 63)) - 1ull;
+    const uint64_t face = (uint64_t)pw[word] + (uint64_t)__popcll(ft[word] & below);
+    const int g = find_bucket(ft_bounds, n_grids, word);
+    const DcGridDev G = grids[g];
+    int a, i, j, k;
+    decode_face(G, entry, a, i, j, k);
+    fn(G, a, i, j, k, face);
   }
 }
 
@@ -95,21 +102,21 @@ __device__ __forceinline__ void word_faces(const DcGridDev* __restrict__ grids, 
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_face_normals(const DcGridDev* __restrict__ grids, int n_grids,
-                   const int64_t* __restrict__ ft_bounds, int64_t n_ft,
+                   const int64_t* __restrict__ ft_bounds, const uint64_t* __restrict__ flist,
                    const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-                   uint64_t capF, T* __restrict__ nrm) {
-  word_faces(grids, n_grids, ft_bounds, n_ft, ft, pw,
-             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
-               if (face >= capF) return;
-               T n[3];
-               group_sum(G, a, i, j, k, n);
-               const T len = tsqrt(sq3(n[0], n[1], n[2]));
-               normalize3(n);
-               nrm[4 * face + 0] = n[0];
-               nrm[4 * face + 1] = n[1];
-               nrm[4 * face + 2] = n[2];
-               nrm[4 * face + 3] = len;
-             });
+                   const DcCounters* __restrict__ ctr, uint64_t capF, T* __restrict__ nrm) {
+  each_face(grids, n_grids, ft_bounds, flist, ft, pw, min(ctr->n_faces, capF),
+            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
+              if (face >= capF) return;
+              T n[3];
+              group_sum(G, a, i, j, k, n);
+              const T len = tsqrt(sq3(n[0], n[1], n[2]));
+              normalize3(n);
+              nrm[4 * face + 0] = n[0];
+              nrm[4 * face + 1] = n[1];
+              nrm[4 * face + 2] = n[2];
+              nrm[4 * face + 3] = len;
+            });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -223,15 +230,15 @@ __device__ __forceinline__ int32_t vertex_id(const DcGridDev& G, const uint64_t*
 
 __global__ void __launch_bounds__(256)
     k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ ft_bounds,
-           int64_t n_ft, const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
-           const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-           const double* __restrict__ v64, const float* __restrict__ v32, int precision,
-           const DcCounters* __restrict__ ctr, uint64_t capA, uint64_t capF,
-           int32_t* __restrict__ faces) {
+           const uint64_t* __restrict__ flist, const uint64_t* __restrict__ ft,
+           const uint32_t* __restrict__ pw, const uint64_t* __restrict__ act,
+           const uint32_t* __restrict__ wprefix, const double* __restrict__ v64,
+           const float* __restrict__ v32, int precision, const DcCounters* __restrict__ ctr,
+           uint64_t capA, uint64_t capF, int32_t* __restrict__ faces) {
   // an overflowed run is repeated by the host; never index past the buffers
   if (ctr->n_active > capA || ctr->n_faces > capF) return;
-  word_faces(grids, n_grids, ft_bounds, n_ft, ft, pw,
-             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
+  each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces,
+            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
     const uint32_t vbase = wprefix[G.act_off];
     const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
     int c0, c1, c2, c3, e1[3];
@@ -291,23 +298,20 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------------------------
 // Adjoint
 // ---------------------------------------------------------------------------------------------
-constexpr int kBwdThreads = 64;
-constexpr int kStencil = 64;  // 4x4x4 samples around a cell: offsets -1..2 per axis
-
 // Cells processed: every active cell (list == nullptr; with skip_flagged the FP32 kernel leaves
-// the FP64 cells to the list launch) or the listed ones (the FP64 cells of FP32 mode).
+// the FP64 cells to the list launch) or the listed ones (the FP64 cells of FP32 mode). The
+// cell's 8 corner adjoints go to contrib[8v..8v+7] (phase 1 writes, phase 2 adds).
 template <int PHASE, typename T>
-__global__ void __launch_bounds__(kBwdThreads)
+__global__ void __launch_bounds__(256)
     k_bwd_cell(const DcGridDev* __restrict__ grids, int n_grids,
                const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
-               const uint64_t* __restrict__ alist, const uint32_t* __restrict__ flags,
-               int skip_flagged, const uint32_t* __restrict__ list,
-               const DcCounters* __restrict__ ctr, uint64_t cap, const float* __restrict__ gv,
-               const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-               const double* __restrict__ nrm64, const float* __restrict__ nrm32, uint64_t capF,
-               float* __restrict__ ubar, float* __restrict__ contrib) {
-  __shared__ float s_lc[kBwdThreads][kStencil + 1];
-  float* lc = s_lc[threadIdx.x];
+               const uint32_t* __restrict__ wprefix, const uint64_t* __restrict__ alist,
+               const uint32_t* __restrict__ flags, int skip_flagged,
+               const uint32_t* __restrict__ list, const DcCounters* __restrict__ ctr, uint64_t cap,
+               const float* __restrict__ gv, const uint64_t* __restrict__ ft,
+               const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
+               const float* __restrict__ nrm32, uint64_t capF, float* __restrict__ ubar,
+               float* __restrict__ ubar_e, float* __restrict__ contrib) {
   const uint64_t n = list ? min(ctr->fallback, cap) : min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<T> fn{ft, pw, nrm64, nrm32, nf};
@@ -315,102 +319,150 @@ __global__ void __launch_bounds__(kBwdThreads)
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t v = list ? (uint64_t)list[t] : t;
     if (!list && skip_flagged && flags[v]) continue;
-    float* out = contrib + v * kStencil;
-    if (PHASE == 1) {
-      for (int q = 0; q < kStencil; ++q) lc[q] = 0.f;
-    } else {
-      for (int q = 0; q < kStencil; ++q) lc[q] = out[q];
-    }
     const uint64_t entry = alist[v];
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
     decode_cell(G, entry, i, j, k);
+    T Sb[8];
+    float4* out = reinterpret_cast<float4*>(contrib + 8 * v);
     if (PHASE == 1) {
       const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
-      backward_rows<T>(G, act, fn, i, j, k, xb, lc, ubar);
+      backward_rows<T>(G, act, fn, i, j, k, v, xb, ubar, ubar_e, Sb);
+      out[0] = make_float4((float)Sb[0], (float)Sb[1], (float)Sb[2], (float)Sb[3]);
+      out[1] = make_float4((float)Sb[4], (float)Sb[5], (float)Sb[6], (float)Sb[7]);
     } else {
-      backward_faces<T>(G, act, fn, i, j, k, ubar, lc);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Sb[q] = T(0);
+      backward_faces<T>(G, act, wprefix, fn, i, j, k, ubar, ubar_e, Sb);
+      const float4 o0 = out[0], o1 = out[1];
+      out[0] = make_float4(o0.x + (float)Sb[0], o0.y + (float)Sb[1], o0.z + (float)Sb[2],
+                           o0.w + (float)Sb[3]);
+      out[1] = make_float4(o1.x + (float)Sb[4], o1.y + (float)Sb[5], o1.z + (float)Sb[6],
+                           o1.w + (float)Sb[7]);
     }
-    for (int q = 0; q < kStencil; ++q) out[q] = lc[q];
   }
 }
 
-// Candidate samples: bit x of word (z*ny + y)*Ws + x/64 is set when some active cell lies in
-// [x-2, x+1] x [y-2, y+1] x [z-2, z+1] (the cells whose 4x4x4 stencil contains the sample).
-__global__ void __launch_bounds__(256)
-    k_bwd_dilate(const DcGridDev* __restrict__ grids, int n_grids,
-                 const int64_t* __restrict__ dil_bounds, int64_t n_dil,
-                 const uint64_t* __restrict__ act, uint64_t* __restrict__ dil) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_dil) return;
-  const int g = find_bucket(dil_bounds, n_grids, t);
-  const DcGridDev G = grids[g];
-  const int64_t local = t - dil_bounds[g];
-  const int64_t row = local / G.Ws;
-  const int c = (int)(local - row * G.Ws);
-  const int y = (int)(row % G.ny), z = (int)(row / G.ny);
-  uint64_t am = 0, a0 = 0, ap = 0;  // OR of cell chunks c-1, c, c+1 over the 16 cell rows
-  for (int kz = max(z - 2, 0); kz <= min(z + 1, G.cz - 1); ++kz)
-    for (int jy = max(y - 2, 0); jy <= min(y + 1, G.cy - 1); ++jy) {
-      const uint64_t* w = act + G.act_off + ((int64_t)kz * G.cy + jy) * G.W;
-      if (c - 1 >= 0 && c - 1 < G.W) am |= w[c - 1];
-      if (c < G.W) a0 |= w[c];
-      if (c + 1 < G.W) ap |= w[c + 1];
-    }
-  const uint64_t d = a0 | ((a0 >> 1) | (ap << 63)) | ((a0 << 1) | (am >> 63)) |
-                     ((a0 << 2) | (am >> 62));
-  dil[G.dil_off + local] = d;
+// Corner index (contouring.cpp:232-241 numbering) at offset (dx,dy,dz) in {0,1}^3.
+__device__ __forceinline__ int corner_at(int dx, int dy, int dz) {
+  return 4 * dz + (dy ? (dx ? 2 : 3) : dx);
 }
 
-// One wave per candidate-plane word (64 samples of a row): the 16 cell rows' words are loaded once
-// per wave; each candidate lane sums its stencil contributions over cells in (k, j, i) order.
-__global__ void __launch_bounds__(256)
-    k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
-                 const int64_t* __restrict__ dil_bounds, int64_t n_dil,
-                 const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
-                 const uint64_t* __restrict__ dil, const float* __restrict__ contrib,
-                 float* __restrict__ grad) {
+// dL/dphi of sample row (y, z): sample x sums the corner contributions of the active cells
+// (x-1..x, y-1..y, z-1..z) in (k, j, i) order. Per chunk of 64 samples and cell row, the lane of
+// cell x loads the cell's two corner values on this sample row (dx = 0 and 1 are adjacent in the
+// corner numbering) with one float2 load; the dx = 1 value moves one lane up (to sample x+1) and
+// lane 63's into the next chunk. Chunk words with no active cell are skipped on a uniform branch.
+constexpr int kGatherChunks = 8;
+
+__device__ __forceinline__ void gather_row(const DcGridDev& G, int y, int z,
+                                           const uint64_t* __restrict__ act,
+                                           const uint32_t* __restrict__ wprefix,
+                                           const float* __restrict__ contrib,
+                                           float* __restrict__ out) {
   const int lane = threadIdx.x & 63;
-  const int64_t wv = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wv >= n_dil) return;  // uniform per wave
-  const int g = find_bucket(dil_bounds, n_grids, wv);
-  const DcGridDev G = grids[g];
-  const int64_t local = wv - dil_bounds[g];
-  const int64_t row = local / G.Ws;
-  const int c = (int)(local - row * G.Ws);
-  const int y = (int)(row % G.ny), z = (int)(row / G.ny);
-  const int x = 64 * c + lane;
-  const uint64_t d = dil[wv];
-  float acc = 0.f;
-  if (d) {
-    const bool cand = ((d >> lane) & 1ull) && x < G.nx;
-    for (int kz = max(z - 2, 0); kz <= min(z + 1, G.cz - 1); ++kz)
-      for (int jy = max(y - 2, 0); jy <= min(y + 1, G.cy - 1); ++jy) {
-        const int64_t rw = G.act_off + ((int64_t)kz * G.cy + jy) * G.W;
-        const uint64_t wm = (c >= 1 && c - 1 < G.W) ? act[rw + c - 1] : 0ull;
-        const uint64_t w0 = (c < G.W) ? act[rw + c] : 0ull;
-        const uint64_t wp = (c + 1 < G.W) ? act[rw + c + 1] : 0ull;
-        if (!(wm | w0 | wp)) continue;
-        const uint32_t pm = (c >= 1 && c - 1 < G.W) ? wprefix[rw + c - 1] : 0u;
-        const uint32_t p0 = (c < G.W) ? wprefix[rw + c] : 0u;
-        const uint32_t pp = (c + 1 < G.W) ? wprefix[rw + c + 1] : 0u;
-        if (!cand) continue;
-        for (int dd = -2; dd <= 1; ++dd) {
-          const int ix = x + dd;
-          if (ix < 0 || ix >= G.cx) continue;
-          const int sel = (ix >> 6) - c + 1;  // 0: chunk c-1, 1: c, 2: c+1
-          const uint64_t word = sel == 0 ? wm : (sel == 1 ? w0 : wp);
-          const uint32_t pre = sel == 0 ? pm : (sel == 1 ? p0 : pp);
-          const uint64_t bit = 1ull << (ix & 63);
-          if (!(word & bit)) continue;
-          const uint64_t vid = (uint64_t)pre + (uint64_t)__popcll(word & (bit - 1ull));
-          const int slot = (z - kz + 1) * 16 + (y - jy + 1) * 4 + (x - ix + 1);
-          acc += contrib[vid * kStencil + slot];
+  // lane l < 32 holds cell row q = l/8 (bit 0: j = y, bit 1: k = z; (k, j) order), chunk
+  // c0 + l%8
+  const int lq = lane >> 3;
+  const int jy = y - 1 + (lq & 1), kz = z - 1 + (lq >> 1);
+  const bool row_ok = lane < 32 && jy >= 0 && jy < G.cy && kz >= 0 && kz < G.cz;
+  // clamped row so that every lane's (unconditional) prefix load stays inside the grid
+  const int64_t rw = G.act_off + ((int64_t)min(max(kz, 0), G.cz - 1) * G.cy +
+                                  min(max(jy, 0), G.cy - 1)) * G.W;
+  const uint64_t below = (1ull << lane) - 1ull;
+  float carry[4] = {0.f, 0.f, 0.f, 0.f};  // cell 64c-1's dx = 1 value for sample 64c, per row
+  for (int c0 = 0; c0 < G.Ws; c0 += kGatherChunks) {
+    const int cl = c0 + (lane & 7);
+    const bool ok = row_ok && cl < G.W;
+    const int64_t widx = rw + min(cl, G.W - 1);
+    const uint64_t wl = ok ? act[widx] : 0ull;
+    const uint32_t pl = wprefix[widx];
+    float own[kGatherChunks][4], nxt[kGatherChunks][4];
+#pragma unroll
+    for (int cc = 0; cc < kGatherChunks; ++cc) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int src = 8 * q + cc;
+        const uint64_t w = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(wl >> 32), src)
+                            << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wl, src);
+        own[cc][q] = 0.f;
+        nxt[cc][q] = 0.f;
+        if (w) {  // uniform
+          const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pl, src);
+          const bool a = (w >> lane) & 1ull;
+          const uint64_t vid = a ? (uint64_t)p + (uint64_t)__popcll(w & below) : 0ull;
+          const int dy = 1 - (q & 1), dz = 1 - (q >> 1);
+          // corners (dx=0, dx=1) on this sample row: dy = 0 -> (0, 1); dy = 1 -> (3, 2)
+          const float2 v = *reinterpret_cast<const float2*>(contrib + 8 * vid + 4 * dz + 2 * dy);
+          own[cc][q] = a ? (dy ? v.y : v.x) : 0.f;
+          nxt[cc][q] = a ? (dy ? v.x : v.y) : 0.f;
         }
       }
+    }
+#pragma unroll
+    for (int cc = 0; cc < kGatherChunks; ++cc) {
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float prev = __shfl_up(nxt[cc][q], 1);
+        if (lane == 0) prev = carry[q];
+        carry[q] = __shfl(nxt[cc][q], 63);
+        acc += prev;          // cell x-1
+        acc += own[cc][q];    // cell x
+      }
+      const int x = 64 * (c0 + cc) + lane;
+      if (x < G.nx) out[x] = acc;
+    }
   }
-  if (x < G.nx) grad[G.samp_off + row * G.nx + x] = acc;
+}
+
+// One wave per group of kGatherRows sample rows (y0..y0+15, z) of a grid; every sample of the
+// grid is written. A cell row (j, k) holds an active cell iff its vertex prefix grows across it:
+// one load per lane gives the flags of the 17 x 2 cell rows the group touches, empty sample rows
+// are zero-filled and the rest gathered row by row.
+constexpr int kGatherRows = 16;
+
+__global__ void __launch_bounds__(256)
+    k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
+                 const int64_t* __restrict__ rg_bounds, int64_t n_rg,
+                 const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
+                 int64_t n_words, const DcCounters* __restrict__ ctr,
+                 const float* __restrict__ contrib, float* __restrict__ grad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (r >= n_rg) return;  // uniform per wave
+  const int g = find_bucket(rg_bounds, n_grids, r);
+  const DcGridDev G = grids[g];
+  const int64_t local = r - rg_bounds[g];
+  const int gpl = (G.ny + kGatherRows - 1) / kGatherRows;  // groups per sample layer
+  const int z = (int)(local / gpl);
+  const int y0 = (int)(local - (int64_t)z * gpl) * kGatherRows;
+  // lane l < 34: cell row jy = y0 - 1 + l/2, kz = z - 1 + l%2
+  const int jy = y0 - 1 + (lane >> 1), kz = z - 1 + (lane & 1);
+  const bool cell_row = lane < 2 * (kGatherRows + 1) && jy >= 0 && jy < G.cy && kz >= 0 &&
+                        kz < G.cz;
+  const int64_t w0 = G.act_off + ((int64_t)max(kz, 0) * G.cy + max(jy, 0)) * G.W;
+  const int64_t w1 = w0 + G.W;
+  const uint32_t p0 = wprefix[min(w0, n_words - 1)];
+  const uint32_t p1 = w1 < n_words ? wprefix[w1] : (uint32_t)ctr->n_active;
+  const bool ne = cell_row && p1 != p0;
+  const uint64_t nem = __ballot(ne);
+  // sample row y0 + t touches cell rows y0 + t - 1 and y0 + t: lanes 2t..2t+3
+  uint32_t rows = 0;
+#pragma unroll
+  for (int t = 0; t < kGatherRows; ++t) rows |= (uint32_t)(((nem >> (2 * t)) & 15ull) != 0) << t;
+  const int yn = min(kGatherRows, G.ny - y0);
+  for (int t = 0; t < yn; ++t) {
+    float* out = grad + G.samp_off + ((int64_t)z * G.ny + y0 + t) * G.nx;
+    if ((rows >> t) & 1u) {
+      gather_row(G, y0 + t, z, act, wprefix, contrib, out);
+    } else {
+      for (int x = lane; x < G.nx; x += 64) out[x] = 0.f;
+    }
+  }
 }
 
 }  // namespace
@@ -419,18 +471,22 @@ __global__ void __launch_bounds__(256)
 // Host launchers
 // ---------------------------------------------------------------------------------------------
 hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s) {
-  if (a.n_pen > 0)
-    hipLaunchKernelGGL(k_pencil_words, dim3((unsigned)((a.n_pen + 255) / 256)), dim3(256), 0, s,
-                       a.grids, a.n_grids, a.pen_bounds, a.n_pen, a.ft, a.pbase, a.pw);
+  if (a.n_pen <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pencil_words, dim3((unsigned)((a.n_pen + 255) / 256)), dim3(256), 0, s,
+                     a.grids, a.n_grids, a.pen_bounds, a.n_pen, a.ft, a.pbase, a.pw);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || a.n_ft <= 0) return e;
-  const dim3 grid((unsigned)((a.n_ft + 255) / 256)), block(256);
+  if (e != hipSuccess) return e;
+  e = dc_launch_face_list(a.ft, a.n_ft, a.tile_sum, a.tile_base, a.flist, a.capF,
+                          &a.ctr->n_list, s);
+  if (e != hipSuccess) return e;
+  const dim3 grid((unsigned)a.face_blocks), block(256);
   if (a.precision == DC_PREC_FP64)
     hipLaunchKernelGGL(k_face_normals<double>, grid, block, 0, s, a.grids, a.n_grids,
-                       a.ft_bounds, a.n_ft, a.ft, a.pw, a.capF, reinterpret_cast<double*>(a.nrm));
+                       a.ft_bounds, a.flist, a.ft, a.pw, a.ctr, a.capF,
+                       reinterpret_cast<double*>(a.nrm));
   else
     hipLaunchKernelGGL(k_face_normals<float>, grid, block, 0, s, a.grids, a.n_grids, a.ft_bounds,
-                       a.n_ft, a.ft, a.pw, a.capF, reinterpret_cast<float*>(a.nrm));
+                       a.flist, a.ft, a.pw, a.ctr, a.capF, reinterpret_cast<float*>(a.nrm));
   return hipGetLastError();
 }
 
@@ -454,9 +510,8 @@ hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
 }
 
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
-  if (a.n_ft <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit, dim3((unsigned)((a.n_ft + 255) / 256)), dim3(256), 0, s, a.grids,
-                     a.n_grids, a.ft_bounds, a.n_ft, a.act, a.wprefix, a.ft, a.pw, a.v64, a.v32,
+  hipLaunchKernelGGL(k_emit, dim3((unsigned)a.face_blocks), dim3(256), 0, s, a.grids, a.n_grids,
+                     a.ft_bounds, a.flist, a.ft, a.pw, a.act, a.wprefix, a.v64, a.v32,
                      a.precision, a.ctr, a.capA, a.capF, a.faces);
   return hipGetLastError();
 }
@@ -464,37 +519,35 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
 hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
   const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
   const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
-  const dim3 grid((unsigned)a.cell_blocks), block(kBwdThreads);
-  const dim3 fgrid((unsigned)std::max<int64_t>(1, a.cell_blocks / 8));
+  const dim3 grid((unsigned)a.cell_blocks), block(256);
+  const dim3 fgrid((unsigned)a.fallback_blocks);
   hipError_t e;
 #define DC_BWD(PHASE)                                                                              \
   if (a.precision == DC_PREC_FP64) {                                                               \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), grid, block, 0, s, a.grids, a.n_grids,         \
-                       a.act_bounds, a.act, a.alist, a.flags, 0, nullptr, a.ctr, a.capA,           \
-                       a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.contrib);          \
+                       a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, nullptr, a.ctr,        \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
+                       a.contrib);                                                                 \
   } else {                                                                                         \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, float>), grid, block, 0, s, a.grids, a.n_grids,          \
-                       a.act_bounds, a.act, a.alist, a.flags, 1, nullptr, a.ctr, a.capA,           \
-                       a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.contrib);          \
+                       a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 1, nullptr, a.ctr,        \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
+                       a.contrib);                                                                 \
     e = hipGetLastError();                                                                         \
     if (e != hipSuccess) return e;                                                                 \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), fgrid, block, 0, s, a.grids, a.n_grids,        \
-                       a.act_bounds, a.act, a.alist, a.flags, 0, a.fblist, a.ctr, a.capA,          \
-                       a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.contrib);          \
+                       a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, a.fblist, a.ctr,       \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
+                       a.contrib);                                                                 \
   }                                                                                                \
   e = hipGetLastError();                                                                           \
   if (e != hipSuccess) return e;
   DC_BWD(1)
   DC_BWD(2)
 #undef DC_BWD
-  if (a.n_dil > 0) {
-    hipLaunchKernelGGL(k_bwd_dilate, dim3((unsigned)((a.n_dil + 255) / 256)), dim3(256), 0, s,
-                       a.grids, a.n_grids, a.dil_bounds, a.n_dil, a.act, a.dil);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bwd_gather, dim3((unsigned)((a.n_dil + 3) / 4)), dim3(256), 0, s,
-                       a.grids, a.n_grids, a.dil_bounds, a.n_dil, a.act, a.wprefix, a.dil,
+  if (a.n_rg > 0)
+    hipLaunchKernelGGL(k_bwd_gather, dim3((unsigned)((a.n_rg + 3) / 4)), dim3(256), 0, s, a.grids,
+                       a.n_grids, a.rg_bounds, a.n_rg, a.act, a.wprefix, a.n_words, a.ctr,
                        a.contrib, a.grad_values);
-  }
   return hipGetLastError();
 }

@@ -5,7 +5,14 @@
 
 #include "dc_internal.h"
 
+// Pointers in the global address space. The sample pointer reaches the kernels inside the grid
+// descriptor (loaded from memory), where the compiler cannot tell that it is global and would
+// issue flat loads (slower, and counted against the LDS/scalar wait counter as well).
+typedef const __attribute__((address_space(1))) float* gfloat;
+
 namespace dccommon {
+
+__device__ __forceinline__ gfloat to_global(const float* p) { return (gfloat)p; }
 
 // Largest b in [0, n) with bounds[b] <= x (bounds ascending, bounds[0] = 0): the grid of a batch
 // that owns item x.

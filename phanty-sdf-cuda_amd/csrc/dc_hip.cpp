@@ -35,13 +35,14 @@ struct dc_hip_context {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_rows = 4;
+  int mark_layers = 16;
+  int eff_layers = 16;  // the batch's value (mark_layers or fewer for huge x-y layers)
   int mark_waves = 4;
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
-  DevBuf grids, bounds, act, ft, pcount, pbase, wprefix, alist, v64, v32, faces, status, tilectr,
-      counters, upload, flags, bwd_contrib, bwd_grad, pw, fnrm, ubar, fblist;
+  DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
+      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist;
   DcCounters* h_counters = nullptr;
   hipEvent_t ev[kNumEvents] = {};
   // last forward
@@ -50,8 +51,9 @@ struct dc_hip_context {
   int64_t nv = 0, nf = 0;
   uint64_t capA = 0, capF = 0;
   std::vector<DcGridDev> hgrids;
-  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, dil_b, ft_b, hbounds;
-  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_dil = 0;
+  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, tw_b, at_b, hbounds;
+  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_tw = 0,
+          n_at = 0;
 };
 
 namespace {
@@ -97,16 +99,35 @@ T* as(DevBuf& b) {
   return reinterpret_cast<T*>(b.p);
 }
 
+// Cell layers per marking tile for this batch: the configured value, halved while a tile's
+// samples (layers + 2 sample layers, with row slack) would not fit a 32-bit buffer range.
+int mark_layers_for(const dc_hip_context* c, const dc_hip_grid* grids, int n) {
+  int layers = c->mark_layers;
+  for (;;) {
+    bool fits = true;
+    for (int q = 0; q < n; ++q)
+      if ((int64_t)(layers + 2) * grids[q].nx * grids[q].ny * 4 >= (1ll << 32)) fits = false;
+    if (fits) return layers;
+    if (layers <= 8) return 0;
+    layers /= 2;
+  }
+}
+
 // Layout of a batch; returns an error for degenerate or oversized grids.
 int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
+  c->eff_layers = mark_layers_for(c, grids, n);
+  if (c->eff_layers == 0)
+    return fail(c, DC_HIP_ERR_INVALID_GRID, "grid layer too large (nx * ny above 100M samples)");
   c->hgrids.resize(n);
   c->tile_b.assign(n + 1, 0);
   c->act_b.assign(n + 1, 0);
   c->pen_b.assign(n + 1, 0);
   c->samp_b.assign(n + 1, 0);
-  c->dil_b.assign(n + 1, 0);
+  c->sr_b.assign(n + 1, 0);
   c->ft_b.assign(n + 1, 0);
-  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, dil = 0;
+  c->tw_b.assign(n + 1, 0);
+  c->at_b.assign(n + 1, 0);
+  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, sr = 0, tw = 0, at = 0;
   for (int q = 0; q < n; ++q) {
     const dc_hip_grid& s = grids[q];
     if (s.nx < 2 || s.ny < 2 || s.nz < 2)
@@ -136,26 +157,32 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     g.ft_off = ftw;
     g.pen_off = pen;
     g.samp_off = samp;
-    g.dil_off = dil;
+    g.sr_off = sr;
     c->tile_b[q] = tiles;
     c->act_b[q] = words;
     c->pen_b[q] = pen;
     c->samp_b[q] = samp;
-    c->dil_b[q] = dil;
+    c->sr_b[q] = sr;
     c->ft_b[q] = ftw;
+    c->tw_b[q] = tw;
+    c->at_b[q] = at;
     words += (int64_t)g.W * g.cy * g.cz;
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
-    tiles += dc_mark_tiles(g, c->mark_rows);
+    tiles += dc_mark_tiles(g, c->eff_layers);
+    tw += 3ll * g.W * ((g.cy + 7) / 8);
+    at += (int64_t)g.cz * ((g.cy + 63) / 64) * ((g.W + 7) / 8);
     samp += (int64_t)g.nx * g.ny * g.nz;
-    dil += (int64_t)g.Ws * g.ny * g.nz;
+    sr += (int64_t)((g.ny + 15) / 16) * g.nz;  // groups of 16 sample rows (adjoint gather)
   }
   c->tile_b[n] = tiles;
   c->act_b[n] = words;
   c->pen_b[n] = pen;
   c->samp_b[n] = samp;
-  c->dil_b[n] = dil;
+  c->sr_b[n] = sr;
   c->ft_b[n] = ftw;
+  c->tw_b[n] = tw;
+  c->at_b[n] = at;
   if ((uint64_t)words * 64ull >= (1ull << 32) * 4ull)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "batch has too many cells for 32-bit vertex ids");
   c->n_words = words;
@@ -163,7 +190,9 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->n_pen = pen;
   c->n_tiles = tiles;
   c->n_samples = samp;
-  c->n_dil = dil;
+  c->n_sr = sr;
+  c->n_tw = tw;
+  c->n_at = at;
   return DC_HIP_OK;
 }
 
@@ -179,15 +208,18 @@ int run_once(dc_hip_context* c, int precision) {
   const int n = (int)c->hgrids.size();
   int rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
-  if ((rc = ensure(c, c->bounds, 6 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
+  if ((rc = ensure(c, c->bounds, 8 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
+  if ((rc = ensure(c, c->act_kcj, c->n_words * 8, "active plane (marking order)"))) return rc;
+  if ((rc = ensure(c, c->ftx, c->n_words * 8 * 3, "face words over x"))) return rc;
   if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
   if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
   if ((rc = ensure(c, c->pcount, c->n_pen * 4, "pencil counts"))) return rc;
   if ((rc = ensure(c, c->pbase, c->n_pen * 4, "pencil bases"))) return rc;
-  const int64_t st_a = dc_scan_tiles(c->n_words), st_p = dc_scan_tiles(c->n_pen);
-  if ((rc = ensure(c, c->status, (st_a + st_p) * 8, "scan status"))) return rc;
-  if ((rc = ensure(c, c->tilectr, 64, "scan tile counters"))) return rc;
+  const int64_t t_a = dc_scan_tiles(c->n_words), t_p = dc_scan_tiles(c->n_pen),
+                t_f = dc_scan_tiles(c->n_ft);
+  if ((rc = ensure(c, c->scan_sum, (t_a + t_p + t_f) * 4, "scan tile sums"))) return rc;
+  if ((rc = ensure(c, c->scan_base, (t_a + t_p + t_f) * 8, "scan tile bases"))) return rc;
   if ((rc = ensure(c, c->counters, sizeof(DcCounters), "counters"))) return rc;
   if ((rc = ensure(c, c->alist, c->capA * 8, "active list"))) return rc;
   if ((rc = ensure(c, c->v64, c->capA * 24, "fp64 vertices"))) return rc;
@@ -197,6 +229,10 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->faces, c->capF * 16, "faces"))) return rc;
   if ((rc = ensure(c, c->pw, c->n_ft * 4, "face word prefix"))) return rc;
   if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
+  if ((rc = ensure(c, c->flist, c->capF * 8, "face list"))) return rc;
+  const int64_t face_blocks =
+      std::max<int64_t>(1, std::min<int64_t>(((int64_t)c->capF + 255) / 256,
+                                             (int64_t)c->cu_count * 16));
 
   // host staging stays alive in the context until the copies have run
   std::vector<int64_t>& bounds = c->hbounds;
@@ -205,8 +241,10 @@ int run_once(dc_hip_context* c, int precision) {
   bounds.insert(bounds.end(), c->act_b.begin(), c->act_b.end());
   bounds.insert(bounds.end(), c->pen_b.begin(), c->pen_b.end());
   bounds.insert(bounds.end(), c->samp_b.begin(), c->samp_b.end());
-  bounds.insert(bounds.end(), c->dil_b.begin(), c->dil_b.end());
+  bounds.insert(bounds.end(), c->sr_b.begin(), c->sr_b.end());
   bounds.insert(bounds.end(), c->ft_b.begin(), c->ft_b.end());
+  bounds.insert(bounds.end(), c->tw_b.begin(), c->tw_b.end());
+  bounds.insert(bounds.end(), c->at_b.begin(), c->at_b.end());
   DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
                            hipMemcpyHostToDevice, s), "upload descriptors");
   DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
@@ -217,13 +255,10 @@ int run_once(dc_hip_context* c, int precision) {
   const int64_t* d_ft_b = d_tile_b + 5 * (n + 1);
   const DcGridDev* d_grids = as<DcGridDev>(c->grids);
   DcCounters* d_ctr = as<DcCounters>(c->counters);
-  uint64_t* st = as<uint64_t>(c->status);
-  uint32_t* tc = as<uint32_t>(c->tilectr);
+  uint32_t* tsum = as<uint32_t>(c->scan_sum);
+  uint64_t* tbase = as<uint64_t>(c->scan_base);
 
   // clears: kernels rather than memset nodes, so the sequence can be captured in a graph
-  DC_TRY(c, dc_launch_fill_u32(as<uint32_t>(c->pcount), c->n_pen, 0u, s), "clear pencils");
-  DC_TRY(c, dc_launch_fill_u64(st, st_a + st_p, 0ull, s), "clear scan status");
-  DC_TRY(c, dc_launch_fill_u32(tc, 16, 0u, s), "clear tile counters");
   DC_TRY(c, dc_launch_fill_u32(reinterpret_cast<uint32_t*>(d_ctr),
                                (int64_t)(sizeof(DcCounters) / 4), 0u, s), "clear counters");
 
@@ -233,20 +268,28 @@ int run_once(dc_hip_context* c, int precision) {
   ma.n_grids = n;
   ma.tile_bounds = d_tile_b;
   ma.n_tiles = c->n_tiles;
-  ma.rows = c->mark_rows;
+  ma.layers = c->eff_layers;
   ma.waves_per_block = c->mark_waves;
   ma.act = as<uint64_t>(c->act);
+  ma.act_kcj = as<uint64_t>(c->act_kcj);
+  ma.ftx = as<uint64_t>(c->ftx);
+  ma.plane_words = c->n_words;
+  ma.tw_bounds = d_tile_b + 6 * (n + 1);
+  ma.n_tw = c->n_tw;
+  ma.at_bounds = d_tile_b + 7 * (n + 1);
+  ma.n_at = c->n_at;
   ma.ft = as<uint64_t>(c->ft);
   ma.pcount = as<uint32_t>(c->pcount);
   ma.ctr = d_ctr;
   DC_TRY(c, dc_launch_mark(ma, s), "mark");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
-  DC_TRY(c, dc_launch_scan_act(as<uint64_t>(c->act), c->n_words, as<uint32_t>(c->wprefix),
-                               as<uint64_t>(c->alist), c->capA, st, tc, d_ctr, s),
+  DC_TRY(c, dc_launch_scan_act(as<uint64_t>(c->act), c->n_words, tsum, tbase,
+                               as<uint32_t>(c->wprefix), as<uint64_t>(c->alist), c->capA,
+                               &d_ctr->n_active, s),
          "active scan");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[2], s), "event");
-  DC_TRY(c, dc_launch_scan_pen(d_grids, n, d_pen_b, c->n_pen, as<uint32_t>(c->pcount),
-                               as<uint32_t>(c->pbase), st + st_a, tc + 1, d_ctr, s),
+  DC_TRY(c, dc_launch_scan_pen(as<uint32_t>(c->pcount), c->n_pen, tsum + t_a, tbase + t_a,
+                               as<uint32_t>(c->pbase), &d_ctr->n_faces, s),
          "pencil scan");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[3], s), "event");
   DcFaceNormalArgs fa;
@@ -259,8 +302,12 @@ int run_once(dc_hip_context* c, int precision) {
   fa.ft = as<uint64_t>(c->ft);
   fa.pbase = as<uint32_t>(c->pbase);
   fa.pw = as<uint32_t>(c->pw);
+  fa.flist = as<uint64_t>(c->flist);
+  fa.tile_sum = tsum + t_a + t_p;
+  fa.tile_base = tbase + t_a + t_p;
   fa.ctr = d_ctr;
   fa.capF = c->capF;
+  fa.face_blocks = face_blocks;
   fa.precision = precision;
   fa.nrm = c->fnrm.p;
   DC_TRY(c, dc_launch_face_normals(fa, s), "face normals");
@@ -291,11 +338,12 @@ int run_once(dc_hip_context* c, int precision) {
   ea.grids = d_grids;
   ea.n_grids = n;
   ea.ft_bounds = d_ft_b;
-  ea.n_ft = c->n_ft;
-  ea.act = as<uint64_t>(c->act);
-  ea.wprefix = as<uint32_t>(c->wprefix);
+  ea.flist = as<uint64_t>(c->flist);
+  ea.face_blocks = face_blocks;
   ea.ft = as<uint64_t>(c->ft);
   ea.pw = as<uint32_t>(c->pw);
+  ea.act = as<uint64_t>(c->act);
+  ea.wprefix = as<uint32_t>(c->wprefix);
   ea.v64 = as<double>(c->v64);
   ea.v32 = as<float>(c->v32);
   ea.precision = precision;
@@ -329,7 +377,8 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
     rc = run_once(c, precision);
     if (rc) return rc;
     const DcCounters& h = *c->h_counters;
-    if (h.error) return fail(c, DC_HIP_ERR_INTERNAL, "device scan wait limit exceeded");
+    if (h.error) return fail(c, DC_HIP_ERR_INTERNAL, "device error flag set");
+    if (h.n_list != h.n_faces) return fail(c, DC_HIP_ERR_INTERNAL, "face list count mismatch");
     if (h.nonfinite) return fail(c, DC_HIP_ERR_NONFINITE, "grid contains NaN or Inf samples");
     if (h.n_active >= (1ull << 31) || h.n_faces >= (1ull << 31))
       return fail(c, DC_HIP_ERR_INVALID_GRID, "mesh exceeds 32-bit vertex or face ids");
@@ -429,10 +478,10 @@ void dc_hip_destroy(dc_hip_context* c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
-                    &c->alist, &c->v64, &c->v32, &c->faces, &c->status, &c->tilectr,
-                    &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->bwd_grad, &c->pw,
-                    &c->fnrm, &c->ubar, &c->fblist};
+  DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
+                    &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
+                    &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
+                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -451,10 +500,10 @@ int dc_hip_set_stream(dc_hip_context* c, void* stream) {
 int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
   if (!c || !key) return DC_HIP_ERR_INVALID_ARG;
   const std::string k(key);
-  if (k == "mark_rows") {
-    if (value != 2 && value != 4 && value != 8)
-      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_rows must be 2, 4 or 8");
-    c->mark_rows = (int)value;
+  if (k == "mark_layers") {
+    if (value != 8 && value != 16 && value != 32 && value != 64)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_layers must be 8, 16, 32 or 64");
+    c->mark_layers = (int)value;
   } else if (k == "mark_waves") {
     if (value != 1 && value != 2 && value != 4)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_waves must be 1, 2 or 4");
@@ -574,22 +623,20 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   const auto t0 = std::chrono::steady_clock::now();
   const int n = (int)c->hgrids.size();
   int rc;
-  if ((rc = ensure(c, c->bwd_contrib, (size_t)std::max<int64_t>(c->nv, 1) * 64 * 4,
-                   "adjoint contributions")))
-    return rc;
-  if ((rc = ensure(c, c->bwd_grad, c->n_dil * 8, "candidate plane"))) return rc;
+  const size_t nv1 = (size_t)std::max<int64_t>(c->nv, 1);
+  if ((rc = ensure(c, c->bwd_contrib, nv1 * 8 * 4, "adjoint contributions"))) return rc;
   if ((rc = ensure(c, c->ubar, (size_t)std::max<int64_t>(c->nf, 1) * 48, "face adjoint slots")))
     return rc;
+  if ((rc = ensure(c, c->ubar_e, nv1 * 12 * 12, "edge adjoint slots"))) return rc;
   hipStream_t s = c->stream;
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   DcBwdArgs a;
   a.grids = as<DcGridDev>(c->grids);
   a.n_grids = n;
   a.act_bounds = d_tile_b + (n + 1);
-  a.samp_bounds = d_tile_b + 3 * (n + 1);
-  a.dil_bounds = d_tile_b + 4 * (n + 1);
-  a.n_dil = c->n_dil;
-  a.n_samples = c->n_samples;
+  a.rg_bounds = d_tile_b + 4 * (n + 1);
+  a.n_rg = c->n_sr;
+  a.n_words = c->n_words;
   a.act = as<uint64_t>(c->act);
   a.wprefix = as<uint32_t>(c->wprefix);
   a.alist = as<uint64_t>(c->alist);
@@ -598,16 +645,17 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.precision = c->precision;
   a.ctr = as<DcCounters>(c->counters);
   a.capA = c->capA;
-  a.cell_blocks = std::max<int64_t>(1, std::min<int64_t>((c->nv + 63) / 64,
-                                                          (int64_t)c->cu_count * 32));
+  a.cell_blocks = std::max<int64_t>(1, (c->nv + 255) / 256);
+  a.fallback_blocks =
+      std::max<int64_t>(1, ((int64_t)c->h_counters->fallback + 255) / 256);
   a.grad_vertices = grad_vertices;
   a.ft = as<uint64_t>(c->ft);
   a.pw = as<uint32_t>(c->pw);
   a.nrm = c->fnrm.p;
   a.capF = c->capF;
   a.ubar = as<float>(c->ubar);
+  a.ubar_e = as<float>(c->ubar_e);
   a.contrib = as<float>(c->bwd_contrib);
-  a.dil = as<uint64_t>(c->bwd_grad);
   a.grad_values = grad_values;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], s), "event");
   DC_TRY(c, dc_launch_backward(a, s), "backward");

@@ -13,14 +13,23 @@
 //             ((a*Wk + k/64)*cy + j)*cx + i says "quad on the axis-a edge at cell (i,j,k)";
 //             Wk = ceil(cz/64). The k-bits of a pencil (a,i,j) are the face order the CPU
 //             backend emits (contouring.cpp:609-611, 653-655, 693-695: k innermost).
-//   pcount    uint32 faces per pencil, stored (a*cy + j)*cx + i
-//   pbase     uint32 exclusive scan of pcount in (grid, a, i, j) order = first face of the pencil
+//   pcount    uint32 faces per pencil (a,i,j), stored in face order (a*cx + i)*cy + j
+//   pbase     uint32 exclusive scan of pcount over the batch = first face of the pencil
 //   alist     uint64 active cells in vertex order: global act word * 64 + bit
 //   v64/v32   vertices (FP64 for the plain solve, FP32 output), 3 per active cell
+//   pw        uint32 per ft word: index of its first face (pbase + faces in earlier words)
+//   flist     uint64 every face once as ft word * 64 + bit, in face-word (memory) order; its
+//             face id is pw[word] + popc(ft[word] & below(bit))
+//   scan      per scan (act words, pencils, face words): uint32 sum and uint64 base per tile of
+//             4096 items
+//   fnrm      per face, 4 of T (double plain, float FP32 mode): the normalized 4-cell averaged
+//             hermite normal of the face edge and the length of the unnormalized sum
 //   faces     int32 quads, 4 per face, grid-local vertex ids
-//   contrib   float[64] per active cell: its dL/dphi contributions to the 4x4x4 samples around it
-//   dil       uint64 candidate-sample plane: bit x of word (z*ny + y)*Ws + x/64 set when an
-//             active cell's 4x4x4 stencil contains sample (x,y,z)
+// Adjoint:
+//   ubar      float[4][3] per face: each member cell's adjoint of the averaged normal
+//   ubar_e    float[12][3] per active cell: adjoint of an averaged normal that has no face (an
+//             edge with a zero endpoint), per local edge
+//   contrib   float[8] per active cell: its dL/dphi contributions to its own 8 corner samples
 #pragma once
 #include <stdint.h>
 
@@ -31,28 +40,23 @@ struct DcGridDev {
   int32_t nx, ny, nz;
   int32_t cx, cy, cz;
   int32_t W, Wk;
-  int32_t Ws;        // 64-bit words per sample row = ceil(nx/64) (candidate plane of the adjoint)
+  int32_t Ws;        // 64-sample words per sample row = ceil(nx/64)
   int32_t pad_;
   float vx, vy, vz, ox, oy, oz;
   int64_t act_off;   // first act word
   int64_t ft_off;    // first ft word
   int64_t pen_off;   // first pencil (pcount / pbase index)
   int64_t samp_off;  // first sample in a concatenated gradient buffer (backward)
-  int64_t dil_off;   // first word of the adjoint's candidate-sample plane
+  int64_t sr_off;    // first 16-row sample group of the grid in the batch (adjoint gather)
 };
 
 struct DcCounters {
   uint64_t n_active;      // total active cells (vertices), all grids
   uint64_t n_faces;       // total quads, all grids
   uint64_t fallback;      // FP32 mode: cells solved again in FP64
+  uint64_t n_list;        // face list entries (equals n_faces)
   uint32_t nonfinite;     // a NaN or Inf sample was seen
-  uint32_t error;         // internal failure bits (scan wait limit)
+  uint32_t error;         // internal failure bits (none defined at present)
 };
-
-// decoupled look-back tile status: [63:62] flag, [61:0] value
-#define DC_SCAN_FLAG_AGG 1ull
-#define DC_SCAN_FLAG_PREFIX 2ull
-#define DC_SCAN_VALUE_MASK ((1ull << 62) - 1)
-#define DC_ERR_SCAN_WAIT 1u
 
 enum DcPrecision { DC_PREC_FP64 = 0, DC_PREC_FP32 = 1 };

@@ -10,9 +10,16 @@ struct DcMarkArgs {
   int n_grids;
   const int64_t* tile_bounds;  // n_grids + 1
   int64_t n_tiles;
-  int rows;             // cell rows per wave tile: 2, 4 or 8
+  int layers;           // cell layers per wave tile: 8, 16, 32 or 64
   int waves_per_block;  // 1, 2 or 4
-  uint64_t* act;
+  uint64_t* act;        // active words in vertex order [k][j][c]
+  uint64_t* act_kcj;    // the same words as the marking pass writes them, [k][c][j]
+  uint64_t* ftx;        // face words over x: 3 planes of plane_words, [k][c][j] each
+  int64_t plane_words;  // act words of the batch
+  const int64_t* tw_bounds;  // n_grids + 1, face transpose waves (3 * W * ceil(cy/8) per grid)
+  int64_t n_tw;
+  const int64_t* at_bounds;  // n_grids + 1, active transpose waves (cz * ceil(cy/64) * ceil(W/8))
+  int64_t n_at;
   uint64_t* ft;
   uint32_t* pcount;
   DcCounters* ctr;
@@ -50,8 +57,12 @@ struct DcFaceNormalArgs {
   const uint64_t* ft;
   const uint32_t* pbase;
   uint32_t* pw;
-  const DcCounters* ctr;
+  uint64_t* flist;
+  uint32_t* tile_sum;   // face-list scan scratch, dc_scan_tiles(n_ft) entries each
+  uint64_t* tile_base;
+  DcCounters* ctr;
   uint64_t capF;
+  int64_t face_blocks;  // grid-stride blocks of the per-face kernels
   int precision;
   void* nrm;
 };
@@ -60,11 +71,12 @@ struct DcEmitArgs {
   const DcGridDev* grids;
   int n_grids;
   const int64_t* ft_bounds;
-  int64_t n_ft;
-  const uint64_t* act;
-  const uint32_t* wprefix;
+  const uint64_t* flist;
+  int64_t face_blocks;
   const uint64_t* ft;
   const uint32_t* pw;
+  const uint64_t* act;
+  const uint32_t* wprefix;
   const double* v64;
   const float* v32;
   int precision;
@@ -78,10 +90,9 @@ struct DcBwdArgs {
   const DcGridDev* grids;
   int n_grids;
   const int64_t* act_bounds;
-  const int64_t* dil_bounds;
-  const int64_t* samp_bounds;
-  int64_t n_dil;
-  int64_t n_samples;
+  const int64_t* rg_bounds;  // n_grids + 1, in groups of 16 sample rows (z, y/16)
+  int64_t n_rg;
+  int64_t n_words;           // act words of the batch
   const uint64_t* act;
   const uint32_t* wprefix;
   const uint64_t* alist;
@@ -90,30 +101,35 @@ struct DcBwdArgs {
   int precision;
   const DcCounters* ctr;
   uint64_t capA;
-  int64_t cell_blocks;
+  int64_t cell_blocks;      // blocks of 256 over all active cells
+  int64_t fallback_blocks;  // blocks of 256 over the FP32 mode's FP64 cells
   const float* grad_vertices;
   const uint64_t* ft;
   const uint32_t* pw;
   const void* nrm;
   uint64_t capF;
-  float* ubar;  // 4 slots x 3 floats per face
-  float* contrib;
-  uint64_t* dil;
+  float* ubar;    // 4 slots x 3 floats per face
+  float* ubar_e;  // 12 slots x 3 floats per active cell
+  float* contrib;  // 8 floats per active cell
   float* grad_values;
 };
 
 hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
-int64_t dc_mark_tiles(const DcGridDev& g, int rows);
+int64_t dc_mark_tiles(const DcGridDev& g, int layers);
 hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s);
+// Scans over n items in tiles of 4096: tile_sum (uint32) and tile_base (uint64) hold
+// dc_scan_tiles(n) entries; *total receives the sum.
 int64_t dc_scan_tiles(int64_t n);
-hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* wprefix,
-                              uint64_t* alist, uint64_t cap, uint64_t* status, uint32_t* tile_ctr,
-                              DcCounters* ctr, hipStream_t s);
-hipError_t dc_launch_scan_pen(const DcGridDev* grids, int n_grids, const int64_t* pen_bounds,
-                              int64_t n_pen, const uint32_t* pcount, uint32_t* pbase,
-                              uint64_t* status, uint32_t* tile_ctr, DcCounters* ctr,
+hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* tile_sum,
+                              uint64_t* tile_base, uint32_t* wprefix, uint64_t* alist,
+                              uint64_t cap, uint64_t* total, hipStream_t s);
+hipError_t dc_launch_scan_pen(const uint32_t* pcount, int64_t n_pen, uint32_t* tile_sum,
+                              uint64_t* tile_base, uint32_t* pbase, uint64_t* total,
                               hipStream_t s);
+hipError_t dc_launch_face_list(const uint64_t* ft, int64_t n_ft, uint32_t* tile_sum,
+                               uint64_t* tile_base, uint64_t* flist, uint64_t cap,
+                               uint64_t* total, hipStream_t s);
 hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
