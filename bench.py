@@ -5,10 +5,12 @@ Haar-rotated boxes minus a sphere, sampled as FP32 on a 512^3 unit-cube grid (se
 FP64 mode through the C ABI (dc_hip_extract_device). The mesh stays in device memory; each step
 ends with the pipeline's one host synchronization (counts readback).
 
-Prints one JSON line. `value` is ms per extraction (lower is better). `roofline` is the mark pass
-(the dominant HBM pass): algorithmic bytes 4*N_s + N_c/8 per launch over its hipEvent-measured
-duration. `cpu_baseline` is the oracle (the C++ restatement of the reference CPU backend) on the
-same grid, single thread, on this host.
+Prints one JSON line. `value` is ms per extraction (lower is better). `roofline` is the marking
+kernel k_mark (the pass that reads every sample): algorithmic bytes 4*N_s + N_c/8 per launch over
+its hipEvent-measured duration; `traffic` is its measured HBM bytes per launch from the committed
+rocprofv3 counter summary (profiles/k_mark_csg8_512_traffic.json). `cpu_baseline` is the oracle
+(the C++ restatement of the reference CPU backend) on the same grid, single thread, on this host.
+The mark tile shape is the library default; --tune sweeps the shapes on this grid first.
 """
 import argparse
 import ctypes
@@ -44,8 +46,8 @@ def run_steps(be, lib, cg, prec, steps):
     import ctypes
     from contouring import _lib
     st = _lib.DcHipStats()
-    acc = {"marking_ms": 0.0, "compaction_ms": 0.0, "face_prefix_sum_ms": 0.0, "normals_ms": 0.0,
-           "qef_ms": 0.0, "face_fill_ms": 0.0}
+    acc = {"marking_ms": 0.0, "mark_kernel_ms": 0.0, "compaction_ms": 0.0,
+           "face_prefix_sum_ms": 0.0, "normals_ms": 0.0, "qef_ms": 0.0, "face_fill_ms": 0.0}
     for _ in range(steps):
         rc = lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st))
         be._check(rc)
@@ -62,6 +64,11 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--tune", action="store_true",
+                    help="sweep the mark tile shapes on this grid first and use the fastest")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one elementwise kernel over the grid (known bytes: reads and writes "
+                         "4*N_s) for scaling the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the dual-contouring "
@@ -86,7 +93,16 @@ def main():
 
     # mark tile shape: timed on this grid (the sweep is not part of the timed region)
     gdev = DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, d_values=phi)
-    tile, table = be.autotune(gdev, repeats=3, use_cache=False)
+    if args.tune:
+        tile, table = be.autotune(gdev, repeats=3, use_cache=False)
+    else:
+        tile, table = {"layers": 32, "waves": 1}, []  # the library's defaults
+    if args.calibrate:
+        # a kernel of known traffic: its FETCH_SIZE / WRITE_SIZE scale the marking counters
+        dst = torch.empty_like(phi)
+        torch.mul(phi, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del dst
     for row in table[:6] + table[-2:]:
         log(f"[tune] layers {row['layers']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
 
@@ -140,7 +156,7 @@ def main():
     Ns = n ** 3
     Nc = (n - 1) ** 3
     mark_bytes = 4 * Ns + Nc / 8
-    mark_s = passes["marking_ms"] * 1e-3
+    mark_s = passes["mark_kernel_ms"] * 1e-3
     achieved = mark_bytes / mark_s / 1e9 if mark_s > 0 else None
     traffic = parse_traffic(n)
 

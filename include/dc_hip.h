@@ -57,7 +57,7 @@ typedef struct dc_hip_stats {
   int64_t total_cells;
   int64_t active_cells;
   float upload_ms;
-  float marking_ms;        /* k_mark: active + face bit planes */
+  float marking_ms;        /* marking pass: active plane + face planes (mark + transposes) */
   float compaction_ms;     /* active-cell scan */
   float face_prefix_sum_ms;/* pencil scan */
   float qef_ms;            /* vertex solve */
@@ -71,6 +71,7 @@ typedef struct dc_hip_stats {
   int64_t qef_fallback_count; /* FP32 mode: cells solved again in FP64 */
   int64_t invalid_count;
   float normals_ms;        /* averaged hermite normals, once per face */
+  float mark_kernel_ms;    /* the sample-reading kernel of the marking pass alone */
 } dc_hip_stats;
 
 typedef struct dc_hip_context dc_hip_context;

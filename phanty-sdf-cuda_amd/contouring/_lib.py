@@ -47,7 +47,7 @@ class DcHipStats(ctypes.Structure):
                 ("total_ms", ctypes.c_float), ("backward_ms", ctypes.c_float),
                 ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
                 ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64),
-                ("normals_ms", ctypes.c_float)]
+                ("normals_ms", ctypes.c_float), ("mark_kernel_ms", ctypes.c_float)]
 
 
 _hip = None

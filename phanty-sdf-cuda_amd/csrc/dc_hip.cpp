@@ -26,7 +26,7 @@ struct DevBuf {
   size_t cap = 0;
 };
 
-constexpr int kNumEvents = 10;  // 0..6 forward passes, 8..9 upload / backward
+constexpr int kNumEvents = 10;  // 0..7 forward passes, 8..9 upload / backward
 
 }  // namespace
 
@@ -35,9 +35,9 @@ struct dc_hip_context {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_layers = 16;
-  int eff_layers = 16;  // the batch's value (mark_layers or fewer for huge x-y layers)
-  int mark_waves = 4;
+  int mark_layers = 32;
+  int eff_layers = 32;  // the batch's value (mark_layers or fewer for huge x-y layers)
+  int mark_waves = 1;
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
@@ -282,6 +282,8 @@ int run_once(dc_hip_context* c, int precision) {
   ma.pcount = as<uint32_t>(c->pcount);
   ma.ctr = d_ctr;
   DC_TRY(c, dc_launch_mark(ma, s), "mark");
+  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
+  DC_TRY(c, dc_launch_mark_finish(ma, s), "mark transposes");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
   DC_TRY(c, dc_launch_scan_act(as<uint64_t>(c->act), c->n_words, tsum, tbase,
                                as<uint32_t>(c->wprefix), as<uint64_t>(c->alist), c->capA,
@@ -411,6 +413,7 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
       st.compaction_ms = elapsed(c, 1, 2);
       st.face_prefix_sum_ms = elapsed(c, 2, 3);
       st.normals_ms = elapsed(c, 3, 4);
+      st.mark_kernel_ms = elapsed(c, 0, 7);
       st.qef_ms = elapsed(c, 4, 5);
       st.face_fill_ms = elapsed(c, 5, 6);
       st.face_emission_ms = st.face_prefix_sum_ms + st.face_fill_ms;

@@ -117,7 +117,10 @@ struct DcBwdArgs {
 hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int64_t dc_mark_tiles(const DcGridDev& g, int layers);
+// k_mark (active and face words over x, [k][c][j]); then dc_launch_mark_finish: the active plane
+// in vertex order, the face planes over k and the pencil face counts.
 hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s);
+hipError_t dc_launch_mark_finish(const DcMarkArgs& a, hipStream_t s);
 // Scans over n items in tiles of 4096: tile_sum (uint32) and tile_base (uint64) hold
 // dc_scan_tiles(n) entries; *total receives the sum.
 int64_t dc_scan_tiles(int64_t n);

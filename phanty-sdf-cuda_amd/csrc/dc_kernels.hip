@@ -547,8 +547,10 @@ hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
     default: return hipErrorInvalidValue;
   }
 #undef DC_MARK
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_mark_finish(const DcMarkArgs& a, hipStream_t s) {
   if (a.n_at > 0)
     hipLaunchKernelGGL(k_act_transpose, dim3((unsigned)((a.n_at + 3) / 4)), dim3(256), 0, s,
                        a.grids, a.n_grids, a.at_bounds, a.n_at, a.act_kcj, a.act);
