@@ -66,6 +66,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
+    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4", "c5"),
+                    help="c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
+                         "FP32+fallback; c4: 256 uneven grids, one batch vs sequential; c5: "
+                         "forward+adjoint at 128^3 and 256^3")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one elementwise kernel over the grid (known bytes: reads and writes "
                          "4*N_s) for scaling the HBM byte counters")
@@ -79,6 +83,10 @@ def main():
     import torch
 
     from contouring import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
+
+    if args.config != "c2":
+        {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        return
 
     n = args.size
     t = time.time()
@@ -205,6 +213,175 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def _device_grid(g):
+    import torch
+    from contouring import DenseSdfGridDevice
+    phi = torch.from_numpy(g.values).to("cuda:0")
+    return phi, DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz,
+                                   d_values=phi)
+
+
+def _fwd_bwd_ms(n, steps, warmup):
+    """Plain forward ms and FP32+fallback forward+adjoint ms on the csg8 grid of size n."""
+    import torch
+    from contouring import HipDualContouringBackend, _lib, synth
+    g = synth.csg8(n, seed=1234)
+    phi, _ = _device_grid(g)
+    be = HipDualContouringBackend("plain-fp64", device=0)
+    be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    cg = be._cgrid(g, phi.data_ptr())
+    st = _lib.DcHipStats()
+    lib = be._lib
+
+    def fwd(prec):
+        be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st)))
+
+    plain = _timed(lambda: fwd(_lib.DC_HIP_PLAIN_FP64), steps, warmup)
+    fwd(_lib.DC_HIP_FP32_FALLBACK)
+    nv = int(st.vertex_count)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(0)
+    gv = torch.randn((nv, 3), dtype=torch.float32, device="cuda:0", generator=gen)
+    gphi = torch.empty_like(phi)
+
+    def fb():
+        fwd(_lib.DC_HIP_FP32_FALLBACK)
+        be._check(lib.dc_hip_backward(be._ctx, ctypes.c_void_p(gv.data_ptr()),
+                                      ctypes.c_void_p(gphi.data_ptr()), None))
+
+    both = _timed(fb, steps, warmup)
+    return plain, both, nv
+
+
+def bench_c5(args):
+    """Forward + adjoint on 8-box CSG grids at 128^3 and 256^3 (loss = sum w_i v_i, w ~ N(0,1))."""
+    rows = {}
+    for n in (128, 256):
+        plain, both, nv = _fwd_bwd_ms(n, args.steps, args.warmup)
+        rows[n] = {"plain_fwd_ms": plain, "fp32_fwd_bwd_ms": both, "ratio": both / plain,
+                   "vertices": nv}
+        log(f"[c5] {n}^3: plain {plain:.3f} ms, fp32 fwd+bwd {both:.3f} ms ({both / plain:.2f}x)")
+    print(json.dumps({"metric": "fwd+adjoint ms over plain fwd ms (csg8)", "config": "c5",
+                      "unit": "ms", "results": rows, "higher_is_better": False}), flush=True)
+
+
+def bench_c3(args):
+    """1024^3 narrow band (64-sphere smooth-min, clamped to +-4h): plain vs FP32+fallback time and
+    the FP32 vertex error against the plain FP64 vertices."""
+    import torch
+    from contouring import HipDualContouringBackend, _lib, synth
+    n = 1024
+    t = time.time()
+    g = synth.blend64(n)
+    log(f"[c3] generated blend64 {n}^3 in {time.time() - t:.1f}s")
+    phi, _ = _device_grid(g)
+    del g.values
+    be = HipDualContouringBackend("plain-fp64", device=0)
+    be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    cg = be._cgrid(g, phi.data_ptr())
+    st = _lib.DcHipStats()
+    lib = be._lib
+    res = {}
+    for name, prec in (("plain-fp64", _lib.DC_HIP_PLAIN_FP64),
+                       ("fp32-fallback", _lib.DC_HIP_FP32_FALLBACK)):
+        ms = _timed(lambda: be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec,
+                                                                ctypes.byref(st))),
+                    max(3, args.steps // 4), 1)
+        nv = int(st.vertex_count)
+        res[name] = {"ms": ms, "vertices": nv, "faces": int(st.face_count),
+                     "fallback_cells": int(st.qef_fallback_count),
+                     "active_fraction": nv / float((n - 1) ** 3)}
+        if name == "plain-fp64":
+            v64 = torch.empty((nv, 3), dtype=torch.float64, device="cuda:0")
+            be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(v64.data_ptr()),
+                                             None))
+        else:
+            v32 = torch.empty((nv, 3), dtype=torch.float32, device="cuda:0")
+            be._check(lib.dc_hip_copy_result(be._ctx, ctypes.c_void_p(v32.data_ptr()), None,
+                                             None))
+        log(f"[c3] {name}: {ms:.3f} ms, {nv} vertices")
+    err = (v32.double() - v64).abs() / g.h
+    res["fp32_vertex_err_h"] = {"max": float(err.max()), "rms": float(err.pow(2).mean().sqrt())}
+    log(f"[c3] fp32 vertex error: max {res['fp32_vertex_err_h']['max']:.2e} h, "
+        f"rms {res['fp32_vertex_err_h']['rms']:.2e} h")
+    print(json.dumps({"metric": "1024^3 narrow-band DC ms, plain vs FP32+fallback", "config": "c3",
+                      "unit": "ms", "results": res, "higher_is_better": False}), flush=True)
+
+
+def bench_c4(args):
+    """256 uneven CSG grids (32^3..256^3): one extract_batch launch sequence vs 256 sequential
+    extract_device calls on the same device-resident grids; per-grid spot parity."""
+    import torch
+    from contouring import HipDualContouringBackend, _lib, synth
+    t = time.time()
+    grids = synth.batch_grids(256)
+    log(f"[c4] generated 256 grids ({sum(g.values.size for g in grids) / 1e9:.2f} G samples) in "
+        f"{time.time() - t:.1f}s")
+    dev = [_device_grid(g) for g in grids]
+    for g in grids:
+        g.values = None
+    be = HipDualContouringBackend("plain-fp64", device=0)
+    be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    lib = be._lib
+    arr = (_lib.DcHipGrid * len(grids))(*[be._cgrid(g, d[0].data_ptr())
+                                           for g, d in zip(grids, dev)])
+    voff = (ctypes.c_int64 * (len(grids) + 1))()
+    foff = (ctypes.c_int64 * (len(grids) + 1))()
+    st = _lib.DcHipStats()
+    prec = _lib.DC_HIP_PLAIN_FP64
+
+    def batch():
+        be._check(lib.dc_hip_extract_batch(be._ctx, arr, len(grids), prec, voff, foff,
+                                           ctypes.byref(st)))
+
+    steps = max(3, args.steps // 4)
+    batch_ms = _timed(batch, steps, 1)
+    tot_v, tot_f = int(st.vertex_count), int(st.face_count)
+    spot = [0, 17, 101, 255]
+    batch_v = torch.empty((tot_v, 3), dtype=torch.float64, device="cuda:0")
+    batch_f = torch.empty((tot_f, 4), dtype=torch.int32, device="cuda:0")
+    be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(batch_v.data_ptr()),
+                                     ctypes.c_void_p(batch_f.data_ptr())))
+    offs = [(voff[q], voff[q + 1], foff[q], foff[q + 1]) for q in range(len(grids))]
+
+    def sequential():
+        for q in range(len(grids)):
+            be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(arr[q]), prec,
+                                                ctypes.byref(st)))
+
+    seq_ms = _timed(sequential, steps, 1)
+    ok = True
+    for q in spot:
+        be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(arr[q]), prec,
+                                            ctypes.byref(st)))
+        nv, nf = int(st.vertex_count), int(st.face_count)
+        v = torch.empty((nv, 3), dtype=torch.float64, device="cuda:0")
+        f = torch.empty((nf, 4), dtype=torch.int32, device="cuda:0")
+        be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(v.data_ptr()),
+                                         ctypes.c_void_p(f.data_ptr())))
+        a, b, c, d = offs[q]
+        ok &= (b - a == nv and d - c == nf and torch.equal(batch_v[a:b], v)
+               and torch.equal(batch_f[c:d], f))
+    log(f"[c4] batch {batch_ms:.3f} ms vs sequential {seq_ms:.3f} ms "
+        f"({seq_ms / batch_ms:.2f}x), {tot_v} vertices, spot parity {ok}")
+    print(json.dumps({"metric": "256 uneven grids: one batch vs 256 sequential extractions, ms",
+                      "config": "c4", "unit": "ms", "batch_ms": batch_ms, "sequential_ms": seq_ms,
+                      "speedup": seq_ms / batch_ms, "vertices": tot_v, "faces": tot_f,
+                      "spot_parity": bool(ok), "higher_is_better": False}), flush=True)
 
 
 if __name__ == "__main__":
