@@ -89,8 +89,16 @@ __device__ __forceinline__ double tsqrt<double>(double x) {
 }
 template <>
 __device__ __forceinline__ float tsqrt<float>(float x) {
-  return __fsqrt_rn(x);
+  return __builtin_amdgcn_sqrtf(x);
 }
+
+// Division, reciprocal and reciprocal square root: exact IEEE operations in FP64 (the plain mode
+// reproduces the CPU backend's vertices bit for bit), the hardware approximations (about 1 ulp)
+// in FP32, whose results are held to a tolerance.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rc(double b) { return 1.0 / b; }
+__device__ __forceinline__ float rc(float b) { return __builtin_amdgcn_rcpf(b); }
 
 template <typename T>
 __device__ __forceinline__ T sq3(T x, T y, T z) {
@@ -198,35 +206,42 @@ __device__ __forceinline__ void trilinear_grad(const Cell<T>& c, const T p[3], T
   const T dx = c.X1 - c.X0, dy = c.Y1 - c.Y0, dz = c.Z1 - c.Z0;
   const T eps = T(1e-12);
   T u = T(0.5), v = T(0.5), w = T(0.5);
-  if (fabs(dx) > eps) u = (p[0] - c.X0) / dx;
-  if (fabs(dy) > eps) v = (p[1] - c.Y0) / dy;
-  if (fabs(dz) > eps) w = (p[2] - c.Z0) / dz;
+  if (fabs(dx) > eps) u = dv(p[0] - c.X0, dx);
+  if (fabs(dy) > eps) v = dv(p[1] - c.Y0, dy);
+  if (fabs(dz) > eps) w = dv(p[2] - c.Z0, dz);
   u = clamp01(u);
   v = clamp01(v);
   w = clamp01(w);
   const T omu = T(1) - u, omv = T(1) - v, omw = T(1) - w;
   const T du = (S100 - S000) * omv * omw + (S110 - S010) * v * omw + (S101 - S001) * omv * w +
                (S111 - S011) * v * w;
-  const T dv = (S010 - S000) * omu * omw + (S110 - S100) * u * omw + (S011 - S001) * omu * w +
-               (S111 - S101) * u * w;
+  const T dvv = (S010 - S000) * omu * omw + (S110 - S100) * u * omw + (S011 - S001) * omu * w +
+                (S111 - S101) * u * w;
   const T dw = (S001 - S000) * omu * omv + (S101 - S100) * u * omv + (S011 - S010) * omu * v +
                (S111 - S110) * u * v;
   g[0] = T(0);
   g[1] = T(0);
   g[2] = T(0);
-  if (fabs(dx) > eps) g[0] = du / dx;
-  if (fabs(dy) > eps) g[1] = dv / dy;
-  if (fabs(dz) > eps) g[2] = dw / dz;
+  if (fabs(dx) > eps) g[0] = dv(du, dx);
+  if (fabs(dy) > eps) g[1] = dv(dvv, dy);
+  if (fabs(dz) > eps) g[2] = dv(dw, dz);
 }
 
 template <typename T>
 __device__ __forceinline__ void normalize3(T v[3]) {
   const T z = sq3(v[0], v[1], v[2]);
   if (z > T(0)) {
-    const T s = tsqrt(z);
-    v[0] = v[0] / s;
-    v[1] = v[1] / s;
-    v[2] = v[2] / s;
+    if constexpr (sizeof(T) == 8) {
+      const T s = tsqrt(z);
+      v[0] = v[0] / s;
+      v[1] = v[1] / s;
+      v[2] = v[2] / s;
+    } else {
+      const float r = __builtin_amdgcn_rsqf(z);
+      v[0] = v[0] * r;
+      v[1] = v[1] * r;
+      v[2] = v[2] * r;
+    }
   }
 }
 
@@ -234,7 +249,7 @@ template <typename T>
 __device__ __forceinline__ void hermite_point(const Cell<T>& c, int e, T p[3]) {
   const int a = edge_a(e), b = edge_b(e);
   const T fa = c.s(a), fb = c.s(b);
-  const T t = fa / (fa - fb + T(1e-16));
+  const T t = dv(fa, fa - fb + T(1e-16));
   p[0] = (T(1) - t) * c.cx(a) + t * c.cx(b);
   p[1] = (T(1) - t) * c.cy(a) + t * c.cy(b);
   p[2] = (T(1) - t) * c.cz(a) + t * c.cz(b);
@@ -243,6 +258,21 @@ __device__ __forceinline__ void hermite_point(const Cell<T>& c, int e, T p[3]) {
 template <typename T>
 __device__ __forceinline__ bool edge_crosses(const Cell<T>& c, int e) {
   return !skip_edge(c.s(edge_a(e)), c.s(edge_b(e)));
+}
+
+// Bit e set for each edge e that carries a hermite sample (fa*fb <= 0), from the corner values in
+// registers. Loops over the set bits (ascending e, the row order of the solve) give every lane its
+// own trip count, rather than all 12 edges for the whole wave.
+template <typename T>
+__device__ __forceinline__ uint32_t crossing_mask(const Cell<T>& c) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const int a = (e < 8) ? e : e - 8;
+    const int b = (e < 8) ? (((e & 3) == 3) ? e - 3 : e + 1) : e - 4;
+    m |= (uint32_t)(!skip_edge(c.S[a], c.S[b])) << e;
+  }
+  return m;
 }
 
 // Cell.cpp:180-209: the normalized trilinear normal at the hermite point of edge e (the caller
@@ -406,15 +436,15 @@ __device__ __forceinline__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3])
       const int r = 3 - p - q;
       const T apq = a[p][q];
       if (apq == T(0)) continue;
-      const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+      const T theta = dv(a[q][q] - a[p][p], T(2) * apq);
       const T at = fabs(theta);
       T t;
       if (at > eig_big<T>())
-        t = T(0.5) / theta;
+        t = dv(T(0.5), theta);
       else
-        t = T(1) / (at + tsqrt(theta * theta + T(1)));
+        t = rc(at + tsqrt(theta * theta + T(1)));
       if (theta < T(0)) t = -t;
-      const T c = T(1) / tsqrt(t * t + T(1));
+      const T c = rc(tsqrt(t * t + T(1)));
       const T s = t * c;
       const T arp = a[r][p], arq = a[r][q];
       a[p][p] = a[p][p] - t * apq;
@@ -451,8 +481,8 @@ __device__ __forceinline__ void solve_cell_fp64(const DcGridDev& G, const uint64
   double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
   double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
   int m = 0;
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     double p[3];
     hermite_point(own, e, p);
     double nf[3];
@@ -529,8 +559,8 @@ __device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t
                               const Cell<float>& own, Qef<float>& Q) {
   NSum s0, s1, s2;
   int m = 0;
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     float p[3];
     hermite_point(own, e, p);
     s0.add(p[0]);
@@ -540,13 +570,13 @@ __device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t
   }
   const float fm = (float)m;
   Q.m = m;
-  Q.c[0] = s0.val() / fm;
-  Q.c[1] = s1.val() / fm;
-  Q.c[2] = s2.val() / fm;
+  Q.c[0] = dv(s0.val(), fm);
+  Q.c[1] = dv(s1.val(), fm);
+  Q.c[2] = dv(s2.val(), fm);
   NSum M00, M01, M02, M11, M12, M22, g0, g1, g2;
   bool ok = true;
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     float p[3];
     hermite_point(own, e, p);
     float nf[3];
@@ -596,7 +626,7 @@ __device__ __forceinline__ bool solve_cell_fp32(const DcGridDev& G, const uint64
     if (fabsf(fabsf(l) - (float)kSvdThreshold) < kFbBand) ok = false;
     if (!(fabsf(l) > (float)kSvdThreshold)) continue;
     if (l < kFbKept) ok = false;
-    const float coef = (V[0][q] * Q.g[0] + V[1][q] * Q.g[1] + V[2][q] * Q.g[2]) / l;
+    const float coef = dv(V[0][q] * Q.g[0] + V[1][q] * Q.g[1] + V[2][q] * Q.g[2], l);
     y[0] += coef * V[0][q];
     y[1] += coef * V[1][q];
     y[2] += coef * V[2][q];
@@ -619,9 +649,9 @@ __device__ __forceinline__ void trilinear_vjp(const Cell<T>& c, const T p[3], co
   const T eps = T(1e-12);
   const bool hx = fabs(dx) > eps, hy = fabs(dy) > eps, hz = fabs(dz) > eps;
   T u = T(0.5), v = T(0.5), w = T(0.5);
-  if (hx) u = (p[0] - c.X0) / dx;
-  if (hy) v = (p[1] - c.Y0) / dy;
-  if (hz) w = (p[2] - c.Z0) / dz;
+  if (hx) u = dv(p[0] - c.X0, dx);
+  if (hy) v = dv(p[1] - c.Y0, dy);
+  if (hz) w = dv(p[2] - c.Z0, dz);
   const bool iu = hx && u >= T(0) && u <= T(1);
   const bool iv = hy && v >= T(0) && v <= T(1);
   const bool iw = hz && w >= T(0) && w <= T(1);
@@ -629,9 +659,9 @@ __device__ __forceinline__ void trilinear_vjp(const Cell<T>& c, const T p[3], co
   v = clamp01(v);
   w = clamp01(w);
   const T omu = T(1) - u, omv = T(1) - v, omw = T(1) - w;
-  const T a = hx ? Gb[0] / dx : T(0);
-  const T bb = hy ? Gb[1] / dy : T(0);
-  const T cc = hz ? Gb[2] / dz : T(0);
+  const T a = hx ? dv(Gb[0], dx) : T(0);
+  const T bb = hy ? dv(Gb[1], dy) : T(0);
+  const T cc = hz ? dv(Gb[2], dz) : T(0);
   Sb[1] += a * omv * omw; Sb[0] -= a * omv * omw;
   Sb[2] += a * v * omw;   Sb[3] -= a * v * omw;
   Sb[5] += a * omv * w;   Sb[4] -= a * omv * w;
@@ -653,9 +683,9 @@ __device__ __forceinline__ void trilinear_vjp(const Cell<T>& c, const T p[3], co
   const T ub = bb * ddv_du + cc * ddw_du;
   const T vb = a * ddu_dv + cc * ddw_dv;
   const T wb = a * ddu_dw + bb * ddv_dw;
-  if (iu) pb[0] += ub / dx;
-  if (iv) pb[1] += vb / dy;
-  if (iw) pb[2] += wb / dz;
+  if (iu) pb[0] += dv(ub, dx);
+  if (iv) pb[1] += dv(vb, dy);
+  if (iw) pb[2] += dv(wb, dz);
 }
 
 // VJP of n = v/|v| (identity when |v| = 0, as Eigen's normalize leaves v unchanged).
@@ -668,12 +698,12 @@ __device__ __forceinline__ void normalize_vjp(const T v[3], const T nb[3], T vb[
     vb[2] = nb[2];
     return;
   }
-  const T s = tsqrt(z);
-  const T n[3] = {v[0] / s, v[1] / s, v[2] / s};
+  const T rs = rc(tsqrt(z));
+  const T n[3] = {v[0] * rs, v[1] * rs, v[2] * rs};
   const T d = dot3(n, nb);
-  vb[0] = (nb[0] - n[0] * d) / s;
-  vb[1] = (nb[1] - n[1] * d) / s;
-  vb[2] = (nb[2] - n[2] * d) / s;
+  vb[0] = (nb[0] - n[0] * d) * rs;
+  vb[1] = (nb[1] - n[1] * d) * rs;
+  vb[2] = (nb[2] - n[2] * d) * rs;
 }
 
 // Adjoint of t = fa/(fa-fb+1e-16) for a point adjoint pb (p = (1-t) pa + t pb).
@@ -685,8 +715,9 @@ __device__ __forceinline__ void lerp_vjp(const Cell<T>& c, int e, const T pb[3],
   const T fa = c.s(a), fb = c.s(b);
   const T D = fa - fb + T(1e-16);
   const T D2 = D * D;
-  add8(Sb, a, tb * (T(1e-16) - fb) / D2);
-  add8(Sb, b, tb * fa / D2);
+  const T rD2 = rc(D2);
+  add8(Sb, a, tb * (T(1e-16) - fb) * rD2);
+  add8(Sb, b, tb * fa * rD2);
 }
 
 // Hermite sample (cell, e): adjoint nb of its normalized normal plus point adjoint pb_in -> the
@@ -710,8 +741,8 @@ __device__ __forceinline__ void assemble_any(const DcGridDev& G, const uint64_t*
                              Qef<T>& Q) {
   T c[3] = {T(0), T(0), T(0)};
   int m = 0;
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     T p[3];
     hermite_point(own, e, p);
     c[0] += p[0];
@@ -721,15 +752,15 @@ __device__ __forceinline__ void assemble_any(const DcGridDev& G, const uint64_t*
   }
   Q.m = m;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) Q.c[d] = c[d] / T(m);
+  for (int d = 0; d < 3; ++d) Q.c[d] = dv(c[d], T(m));
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     Q.g[r] = T(0);
 #pragma unroll
     for (int s = 0; s < 3; ++s) Q.M[r][s] = T(0);
   }
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     T p[3], nf[3];
     hermite_point(own, e, p);
     uint64_t face;
@@ -790,7 +821,7 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
     if (!kept[q]) continue;
     const T qx = (V[0][q] * xb[0] + V[1][q] * xb[1]) + V[2][q] * xb[2];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) gb[d] += V[d][q] * qx / lam[q];
+    for (int d = 0; d < 3; ++d) gb[d] += V[d][q] * dv(qx, lam[q]);
   }
   // Daleckii-Krein: M-bar = V (F o (V^T sym(xb g^T) V)) V^T, F_ab = (f(l_a)-f(l_b))/(l_a-l_b),
   // f(l) = 1/l above the threshold, 0 below.
@@ -811,12 +842,11 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
         for (int w = 0; w < 3; ++w) s += V[u][a] * S[u][w] * V[w][b];
       T F;
       if (kept[a] && kept[b])
-        F = T(-1) / (lam[a] * lam[b]);
+        F = -rc(lam[a] * lam[b]);
       else if (!kept[a] && !kept[b])
         F = T(0);
       else
-        F = ((kept[a] ? T(1) / lam[a] : T(0)) - (kept[b] ? T(1) / lam[b] : T(0))) /
-            (lam[a] - lam[b]);
+        F = dv((kept[a] ? rc(lam[a]) : T(0)) - (kept[b] ? rc(lam[b]) : T(0)), lam[a] - lam[b]);
       B[a][b] = F * s;
     }
   T Mb[3][3];
@@ -835,11 +865,11 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
   T cbm[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
-    cbm[a] = (xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2])) / T(Q.m);
+    cbm[a] = dv(xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2]), T(Q.m));
 #pragma unroll
   for (int q = 0; q < 8; ++q) Sb[q] = T(0);
-  for (int e = 0; e < 12; ++e) {
-    if (!edge_crosses(own, e)) continue;
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     T p[3], nf[3];
     hermite_point(own, e, p);
     uint64_t face;
@@ -888,9 +918,9 @@ __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_
   Cell<T> own;
   load_cell(G, i, j, k, own);
   const T zero3[3] = {T(0), T(0), T(0)};
-  for (int e = 0; e < 12; ++e) {
+  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+    const int e = __builtin_ctz(em);
     const T fa = own.s(edge_a(e)), fb = own.s(edge_b(e));
-    if (skip_edge(fa, fb)) continue;
     const bool strict = strict_change(fa, fb);
     int axis, x0, y0, z0;
     if (!edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0)) continue;

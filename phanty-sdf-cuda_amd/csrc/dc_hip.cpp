@@ -173,7 +173,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     tw += 3ll * g.W * ((g.cy + 7) / 8);
     at += (int64_t)g.cz * ((g.cy + 63) / 64) * ((g.W + 7) / 8);
     samp += (int64_t)g.nx * g.ny * g.nz;
-    sr += (int64_t)((g.ny + 15) / 16) * g.nz;  // groups of 16 sample rows (adjoint gather)
+    sr += (int64_t)g.ny * g.nz;  // sample rows (adjoint gather)
   }
   c->tile_b[n] = tiles;
   c->act_b[n] = words;
@@ -637,9 +637,8 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.grids = as<DcGridDev>(c->grids);
   a.n_grids = n;
   a.act_bounds = d_tile_b + (n + 1);
-  a.rg_bounds = d_tile_b + 4 * (n + 1);
-  a.n_rg = c->n_sr;
-  a.n_words = c->n_words;
+  a.sr_bounds = d_tile_b + 4 * (n + 1);
+  a.n_sr = c->n_sr;
   a.act = as<uint64_t>(c->act);
   a.wprefix = as<uint32_t>(c->wprefix);
   a.alist = as<uint64_t>(c->alist);

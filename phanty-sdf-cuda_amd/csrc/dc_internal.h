@@ -47,7 +47,7 @@ struct DcGridDev {
   int64_t ft_off;    // first ft word
   int64_t pen_off;   // first pencil (pcount / pbase index)
   int64_t samp_off;  // first sample in a concatenated gradient buffer (backward)
-  int64_t sr_off;    // first 16-row sample group of the grid in the batch (adjoint gather)
+  int64_t sr_off;    // first sample row (y, z) of the grid in the batch (adjoint gather)
 };
 
 struct DcCounters {

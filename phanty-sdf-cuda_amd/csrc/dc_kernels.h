@@ -90,9 +90,8 @@ struct DcBwdArgs {
   const DcGridDev* grids;
   int n_grids;
   const int64_t* act_bounds;
-  const int64_t* rg_bounds;  // n_grids + 1, in groups of 16 sample rows (z, y/16)
-  int64_t n_rg;
-  int64_t n_words;           // act words of the batch
+  const int64_t* sr_bounds;  // n_grids + 1, in sample rows (y, z)
+  int64_t n_sr;
   const uint64_t* act;
   const uint32_t* wprefix;
   const uint64_t* alist;
@@ -137,3 +136,7 @@ hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
 hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s);
+// the contraction-enabled launchers (dc_cells_fast.hip)
+hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s);
+hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s);
+hipError_t dc_launch_bwd_cells(const DcBwdArgs& a, hipStream_t s);
