@@ -82,8 +82,9 @@ void dc_hip_destroy(dc_hip_context* ctx);
 /* Runs all later work on `stream` (a hipStream_t; NULL restores the context's own stream). */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_layers" (8, 16, 32, 64: cell layers per marking wave tile), "mark_waves"
- * (1, 2, 4: waves per marking block), "timing" (0/1, per-pass events), "qef_blocks"
- * (persistent blocks of the solve). */
+ * (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
+ * adjoint's gradient gather; applies from the next forward), "timing" (0/1, per-pass events),
+ * "qef_blocks" (persistent blocks of the solve). */
 int dc_hip_set_option(dc_hip_context* ctx, const char* key, int64_t value);
 
 /* Replaces CudaDualContouringBackend::extract (src/cuda/dc_cuda.cu:614-649): host values are

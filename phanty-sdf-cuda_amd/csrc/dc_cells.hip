@@ -46,9 +46,19 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
 hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
   hipError_t e = dc_launch_bwd_cells(a, s);
   if (e != hipSuccess) return e;
-  if (a.n_sr > 0)
-    hipLaunchKernelGGL(k_bwd_gather, dim3((unsigned)((a.n_sr + 3) / 4)), dim3(256), 0, s, a.grids,
-                       a.n_grids, a.sr_bounds, a.n_sr, a.act, a.wprefix, a.contrib,
-                       a.grad_values);
+  if (a.n_sr <= 0) return hipGetLastError();
+  const dim3 grid((unsigned)((a.n_sr + 3) / 4));
+  const size_t lds = 4 * ((size_t)a.gather_lds_row + 2 * kGatherCap) * 4;
+#define DC_GATHER(R)                                                                              \
+  hipLaunchKernelGGL(k_bwd_gather<R>, grid, dim3(256), lds, s, a.grids, a.n_grids, a.sr_bounds, \
+                     a.n_sr, a.gather_lds_row, a.act, a.wprefix, a.contrib, a.grad_values)
+  switch (a.gather_rows) {
+    case 1: DC_GATHER(1); break;
+    case 2: DC_GATHER(2); break;
+    case 4: DC_GATHER(4); break;
+    case 8: DC_GATHER(8); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef DC_GATHER
   return hipGetLastError();
 }

@@ -355,95 +355,241 @@ __device__ __forceinline__ int corner_at(int dx, int dy, int dz) {
   return 4 * dz + (dy ? (dx ? 2 : 3) : dx);
 }
 
-// dL/dphi, one wave per sample row (y, z): sample x sums the corner contributions of the active
-// cells (x-1..x, y-1..y, z-1..z) in (k, j, i) order, and every sample is written.
-// The row's 4 cell rows ((k, j) order; q bit 0: j = y, bit 1: k = z) are fetched with one vector
-// load of their act words and vertex prefixes (lane = 16q + chunk, for up to 16 chunks per
-// step). A row with no active cell is zero-filled. Otherwise, per chunk of 64 samples and cell
-// row, the lane of cell x loads the cell's two corner values on this sample row with one float2
-// load (dx = 0 and 1 are adjacent in the corner numbering); the dx = 1 value moves one lane up (to
-// sample x+1) and lane 63's into the next chunk. Chunk words with no active cell are skipped on a
-// uniform branch, and a step's contribution loads are all issued before any sum.
-constexpr int kGatherChunks = 8;   // chunks whose loads are in flight together
-constexpr int kGatherStep = 16;    // chunks per act-word load
-
-__global__ void __launch_bounds__(256)
-    k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
-                 const int64_t* __restrict__ sr_bounds, int64_t n_sr,
-                 const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
-                 const float* __restrict__ contrib, float* __restrict__ grad) {
-  const int lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) +
-                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (r >= n_sr) return;  // uniform per wave
-  const int g = find_bucket(sr_bounds, n_grids, r);
-  const DcGridDev G = grids[g];
-  const int64_t local = r - G.sr_off;
-  const int y = (int)(local % G.ny), z = (int)(local / G.ny);
-  float* out = grad + G.samp_off + local * G.nx;
+// Act word and vertex prefix for lane 16q + c of the gather: chunk s0 + c of cell row
+// (y - 1 + (q & 1), z - 1 + (q >> 1)); the word is 0 outside the grid. The index is clamped so that
+// the loads are unconditional.
+__device__ __forceinline__ void gather_words(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                             const uint32_t* __restrict__ wprefix, int y, int z,
+                                             int s0, int lane, uint64_t& wl, uint32_t& pl) {
   const int lq = lane >> 4;
   const int jy = y - 1 + (lq & 1), kz = z - 1 + (lq >> 1);
-  const bool row_ok = jy >= 0 && jy < G.cy && kz >= 0 && kz < G.cz;
-  // clamped row so that every lane's (unconditional) prefix load stays inside the grid
-  const int64_t rw = G.act_off + ((int64_t)min(max(kz, 0), G.cz - 1) * G.cy +
-                                  min(max(jy, 0), G.cy - 1)) * G.W;
-  const uint64_t below = (1ull << lane) - 1ull;
-  float carry[4] = {0.f, 0.f, 0.f, 0.f};  // cell 64c-1's dx = 1 value for sample 64c, per row
-  for (int s0 = 0; s0 < G.Ws; s0 += kGatherStep) {
-    const int cl = s0 + (lane & 15);
-    const bool ok = row_ok && cl < G.W;
-    const int64_t widx = rw + min(cl, G.W - 1);
-    const uint64_t wl = ok ? act[widx] : 0ull;
-    const uint32_t pl = wprefix[widx];
-    const int nstep = min(kGatherStep, G.Ws - s0);
-    const bool carried = carry[0] != 0.f || carry[1] != 0.f || carry[2] != 0.f || carry[3] != 0.f;
-    if (__ballot(wl != 0ull) == 0ull && !carried) {  // no active cell touches these samples
-      for (int x = 64 * s0 + lane; x < min(G.nx, 64 * (s0 + nstep)); x += 64) out[x] = 0.f;
+  const int cl = s0 + (lane & 15);
+  const bool ok = jy >= 0 && jy < G.cy && kz >= 0 && kz < G.cz && cl < G.W;
+  const int64_t widx = G.act_off +
+                       ((int64_t)min(max(kz, 0), G.cz - 1) * G.cy + min(max(jy, 0), G.cy - 1)) *
+                           G.W +
+                       min(cl, G.W - 1);
+  const uint64_t w = act[widx];
+  pl = wprefix[widx];
+  wl = ok ? w : 0ull;
+}
+
+// dL/dphi. One wave per group of R consecutive sample rows (y fastest, then z). Sample x of a row
+// is the sum of the corner contributions of the active cells (x-1..x, y-1..y, z-1..z), and every
+// sample is written. Per step of up to 16 chunks of 64 samples:
+//  - the act words and vertex prefixes of each row's 4 cell rows (q bit 0: j = y, bit 1: k = z)
+//    are loaded together for all R rows (lane = 16q + chunk);
+//  - the active cells of all R rows are listed in LDS (row, then q, then x), and one float2 load
+//    per 64 listed cells fetches each cell's two corner values on its sample row (dx = 0 and 1
+//    are adjacent in the corner numbering); all of them are issued before the first sum;
+//  - row by row, the values are added into the wave's LDS row buffer at samples x and x+1, one q
+//    at a time, so no two lanes of one add touch the same sample; a wave's LDS operations
+//    complete in issue order, so every sum is formed in the same order on every run. The row is
+//    then stored; a row with no active cell (and nothing carried in) is zero-filled directly. No
+//    global load follows a store, so the stores drain only at the end of the wave;
+//  - the sample one past the step carries into the next step.
+// When a step lists more than kGatherCap cells, the rows are instead summed one at a time, in
+// passes of kGatherCap cells.
+// LDS per wave: a row buffer of lds_row floats (lds_row >= 64 * chunks per step + 1), then
+// kGatherCap cell entries (vertex id, (row << 14) | (q << 12) | sample).
+constexpr int kGatherStep = 16;                 // chunks per step
+constexpr int kGatherLoads = kGatherCap / 64;   // contribution loads in flight per lane
+
+__device__ __forceinline__ void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive prefix sum over the 64 lanes: DPP row shifts within each row of 16 lanes, then the
+// totals of the lower rows.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v, int lane) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const int row = lane >> 4;
+  return v + (row >= 1 ? t0 : 0u) + (row >= 2 ? t1 : 0u) + (row >= 3 ? t2 : 0u);
+}
+
+// Lists this lane's active cells whose list index k (from `first`) falls in [lo, hi).
+__device__ __forceinline__ void list_cells(uint2* __restrict__ cells, uint64_t w, uint32_t p,
+                                           uint32_t first, uint32_t lo, uint32_t hi,
+                                           uint32_t meta) {
+  uint32_t k = first, n = 0;
+  while (w && k < hi) {
+    const int bit = __builtin_ctzll(w);
+    w &= w - 1ull;
+    if (k >= lo) cells[k - lo] = make_uint2(p + n, meta + (uint32_t)bit);
+    ++k;
+    ++n;
+  }
+}
+
+// One float2 of corner values for listed cell d: (value at sample x, value at sample x+1).
+__device__ __forceinline__ float2 cell_pair(const float* __restrict__ contrib, uint2 d) {
+  const int q = (int)((d.y >> 12) & 3u);
+  const int dy = 1 - (q & 1), dz = 1 - (q >> 1);
+  // corners (dx=0, dx=1) on this sample row: dy = 0 -> (0, 1); dy = 1 -> (3, 2)
+  const float2 v =
+      *reinterpret_cast<const float2*>(contrib + 8 * (uint64_t)d.x + 4 * dz + 2 * dy);
+  return dy ? make_float2(v.y, v.x) : v;
+}
+
+// Adds the listed cells of row r (lanes with `on`) into the row buffer, one q at a time.
+__device__ __forceinline__ void add_cells(float* __restrict__ buf, uint2 d, float2 v, bool on,
+                                          int r) {
+  const int q = (int)((d.y >> 12) & 3u);
+  const bool mine = on && (int)(d.y >> 14) == r;
+  float* b = buf + (int)(d.y & 4095u);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) carry[q] = 0.f;
-      continue;
+  for (int qq = 0; qq < 4; ++qq) {
+    if (mine && q == qq) {
+      lds_add(b, v.x);
+      lds_add(b + 1, v.y);
     }
-    for (int b0 = 0; b0 < nstep; b0 += kGatherChunks) {
-      float own[kGatherChunks][4], nxt[kGatherChunks][4];
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256)
+    k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
+                 const int64_t* __restrict__ grp_bounds, int64_t n_grp, int lds_row,
+                 const uint64_t* __restrict__ act, const uint32_t* __restrict__ wprefix,
+                 const float* __restrict__ contrib, float* __restrict__ grad) {
+  extern __shared__ float s_gather[];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t grp = (int64_t)blockIdx.x * 4 + wv;
+  if (grp >= n_grp) return;  // uniform per wave
+  float* buf = s_gather + (size_t)wv * (lds_row + 2 * kGatherCap);
+  uint2* cells = reinterpret_cast<uint2*>(buf + lds_row);
+  const int g = find_bucket(grp_bounds, n_grids, grp);
+  const DcGridDev G = grids[g];
+  const int64_t lr0 = (grp - G.sr_off) * R;
+  const int nrow = (int)min<int64_t>(R, (int64_t)G.ny * G.nz - lr0);
+  int ys[R], zs[R];
+  {
+    // lr0 < ny * nz <= 2^32 (dimensions are at most 65536): 32-bit division suffices
+    const uint32_t l32 = (uint32_t)lr0, ny32 = (uint32_t)G.ny;
+    int y = (int)(l32 % ny32), z = (int)(l32 / ny32);
 #pragma unroll
-      for (int cc = 0; cc < kGatherChunks; ++cc) {
+    for (int r = 0; r < R; ++r) {
+      ys[r] = y;
+      zs[r] = z;
+      if (++y == G.ny) {
+        y = 0;
+        ++z;
+      }
+    }
+  }
+  const uint32_t lane_meta = ((uint32_t)(lane >> 4) << 12) | (uint32_t)(64 * (lane & 15));
+  // rows start on 16-byte boundaries: row buffers, stores and zero fills move float4s
+  const bool vec4 = (G.nx & 3) == 0 && (G.samp_off & 3) == 0 && ((uintptr_t)grad & 15) == 0;
+  float carry[R];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int src = 16 * q + ((b0 + cc) & 15);
-          const uint64_t w =
-              (b0 + cc < nstep)
-                  ? (((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(wl >> 32), src) << 32) |
-                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wl, src))
-                  : 0ull;
-          own[cc][q] = 0.f;
-          nxt[cc][q] = 0.f;
-          if (w) {  // uniform
-            const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pl, src);
-            const bool a = (w >> lane) & 1ull;
-            const uint64_t vid = a ? (uint64_t)p + (uint64_t)__popcll(w & below) : 0ull;
-            const int dy = 1 - (q & 1), dz = 1 - (q >> 1);
-            // corners (dx=0, dx=1) on this sample row: dy = 0 -> (0, 1); dy = 1 -> (3, 2)
-            const float2 v =
-                *reinterpret_cast<const float2*>(contrib + 8 * vid + 4 * dz + 2 * dy);
-            own[cc][q] = a ? (dy ? v.y : v.x) : 0.f;
-            nxt[cc][q] = a ? (dy ? v.x : v.y) : 0.f;
-          }
+  for (int r = 0; r < R; ++r) carry[r] = 0.f;
+  for (int s0 = 0; s0 < G.Ws; s0 += kGatherStep) {
+    const int nstep = min(kGatherStep, G.Ws - s0);
+    const int nsamp = min(G.nx - 64 * s0, 64 * nstep);
+    uint64_t wl[R];
+    uint32_t pl[R], off[R], cnt[R], base[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      wl[r] = 0ull;
+      pl[r] = 0u;
+      if (r < nrow) gather_words(G, act, wprefix, ys[r], zs[r], s0, lane, wl[r], pl[r]);
+    }
+    uint32_t total = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t c = (uint32_t)__popcll(wl[r]);
+      const uint32_t inc = wave_scan_incl(c, lane);
+      cnt[r] = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+      base[r] = total;
+      off[r] = total + inc - c;
+      total += cnt[r];
+    }
+    auto init_row = [&](int r) {
+      float4* b4 = reinterpret_cast<float4*>(buf);
+      for (int x = lane; x <= 16 * nstep; x += 64) b4[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane == 0) buf[0] = carry[r];
+    };
+    auto store_row = [&](int r, bool summed) {
+      float* out = grad + G.samp_off + ((int64_t)zs[r] * G.ny + ys[r]) * G.nx + 64 * s0;
+      if (vec4) {
+        float4* o4 = reinterpret_cast<float4*>(out);
+        const float4* b4 = reinterpret_cast<const float4*>(buf);
+        if (summed)
+          for (int x = lane; x < (nsamp >> 2); x += 64) o4[x] = b4[x];
+        else
+          for (int x = lane; x < (nsamp >> 2); x += 64) o4[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+      } else {
+        if (summed)
+          for (int x = lane; x < nsamp; x += 64) out[x] = buf[x];
+        else
+          for (int x = lane; x < nsamp; x += 64) out[x] = 0.f;
+      }
+      if (summed) carry[r] = buf[64 * nstep];
+    };
+    if (total <= (uint32_t)kGatherCap) {  // uniform: every listed cell loaded at once
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        list_cells(cells, wl[r], pl[r], off[r], 0u, total, lane_meta | ((uint32_t)r << 14));
+      wave_lds_sync();
+      uint2 d[kGatherLoads];
+      float2 v[kGatherLoads];
+#pragma unroll
+      for (int u = 0; u < kGatherLoads; ++u) {
+        d[u] = make_uint2(0u, 0u);
+        v[u] = make_float2(0.f, 0.f);
+        if (64u * u < total) {  // uniform
+          d[u] = cells[min(64u * u + lane, total - 1u)];
+          v[u] = cell_pair(contrib, d[u]);
         }
       }
 #pragma unroll
-      for (int cc = 0; cc < kGatherChunks; ++cc) {
-        if (b0 + cc >= nstep) break;  // uniform
-        float acc = 0.f;
+      for (int r = 0; r < R; ++r) {
+        if (r >= nrow) break;  // uniform
+        const bool summed = cnt[r] != 0u || carry[r] != 0.f;
+        if (summed) {
+          wave_lds_sync();
+          init_row(r);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          float prev = __shfl_up(nxt[cc][q], 1);
-          if (lane == 0) prev = carry[q];
-          carry[q] = __shfl(nxt[cc][q], 63);
-          acc += prev;          // cell x-1
-          acc += own[cc][q];    // cell x
+          for (int u = 0; u < kGatherLoads; ++u)
+            if (64u * u < total) add_cells(buf, d[u], v[u], 64u * u + lane < total, r);
+          wave_lds_sync();
         }
-        const int x = 64 * (s0 + b0 + cc) + lane;
-        if (x < G.nx) out[x] = acc;
+        store_row(r, summed);
+      }
+    } else {  // a dense step: one row at a time, in passes of kGatherCap cells
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r >= nrow) break;  // uniform
+        const bool summed = cnt[r] != 0u || carry[r] != 0.f;
+        if (summed) {
+          wave_lds_sync();
+          init_row(r);
+          for (uint32_t lo = base[r]; lo < base[r] + cnt[r]; lo += kGatherCap) {
+            const uint32_t hi = min(base[r] + cnt[r], lo + (uint32_t)kGatherCap);
+            wave_lds_sync();
+            list_cells(cells, wl[r], pl[r], off[r], lo, hi, lane_meta | ((uint32_t)r << 14));
+            wave_lds_sync();
+            for (uint32_t k0 = 0; k0 < hi - lo; k0 += 64) {
+              const uint2 d = cells[min(k0 + lane, hi - lo - 1u)];
+              add_cells(buf, d, cell_pair(contrib, d), k0 + lane < hi - lo, r);
+            }
+          }
+          wave_lds_sync();
+        }
+        store_row(r, summed);
       }
     }
   }

@@ -38,6 +38,9 @@ struct dc_hip_context {
   int mark_layers = 32;
   int eff_layers = 32;  // the batch's value (mark_layers or fewer for huge x-y layers)
   int mark_waves = 1;
+  int gather_rows = 1;
+  int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
+  int gather_lds_row = 128;
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
@@ -115,6 +118,13 @@ int mark_layers_for(const dc_hip_context* c, const dc_hip_grid* grids, int n) {
 
 // Layout of a batch; returns an error for degenerate or oversized grids.
 int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
+  {
+    // adjoint gather: LDS row length for the widest grid (at most 16 chunks per step)
+    int max_ws = 1;
+    for (int q = 0; q < n; ++q) max_ws = std::max(max_ws, std::min((grids[q].nx + 63) / 64, 16));
+    c->gather_lds_row = 64 * max_ws + 64;
+    c->eff_gather_rows = c->gather_rows;
+  }
   c->eff_layers = mark_layers_for(c, grids, n);
   if (c->eff_layers == 0)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "grid layer too large (nx * ny above 100M samples)");
@@ -173,7 +183,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     tw += 3ll * g.W * ((g.cy + 7) / 8);
     at += (int64_t)g.cz * ((g.cy + 63) / 64) * ((g.W + 7) / 8);
     samp += (int64_t)g.nx * g.ny * g.nz;
-    sr += (int64_t)g.ny * g.nz;  // sample rows (adjoint gather)
+    sr += ((int64_t)g.ny * g.nz + c->eff_gather_rows - 1) / c->eff_gather_rows;  // gather groups
   }
   c->tile_b[n] = tiles;
   c->act_b[n] = words;
@@ -511,6 +521,10 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (value != 1 && value != 2 && value != 4)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_waves must be 1, 2 or 4");
     c->mark_waves = (int)value;
+  } else if (k == "gather_rows") {
+    if (value != 1 && value != 2 && value != 4 && value != 8)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "gather_rows must be 1, 2, 4 or 8");
+    c->gather_rows = (int)value;
   } else if (k == "timing") {
     c->timing = value != 0;
   } else if (k == "qef_blocks") {
@@ -639,6 +653,8 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.act_bounds = d_tile_b + (n + 1);
   a.sr_bounds = d_tile_b + 4 * (n + 1);
   a.n_sr = c->n_sr;
+  a.gather_rows = c->eff_gather_rows;
+  a.gather_lds_row = c->gather_lds_row;
   a.act = as<uint64_t>(c->act);
   a.wprefix = as<uint32_t>(c->wprefix);
   a.alist = as<uint64_t>(c->alist);

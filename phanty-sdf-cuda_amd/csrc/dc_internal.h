@@ -47,7 +47,8 @@ struct DcGridDev {
   int64_t ft_off;    // first ft word
   int64_t pen_off;   // first pencil (pcount / pbase index)
   int64_t samp_off;  // first sample in a concatenated gradient buffer (backward)
-  int64_t sr_off;    // first sample row (y, z) of the grid in the batch (adjoint gather)
+  int64_t sr_off;    // first row group of the grid in the batch (adjoint gather; a group is
+                     // gather_rows consecutive sample rows, y fastest, then z)
 };
 
 struct DcCounters {

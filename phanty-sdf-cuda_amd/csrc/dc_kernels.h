@@ -86,12 +86,17 @@ struct DcEmitArgs {
   int32_t* faces;
 };
 
+// Adjoint gather: cell entries per pass in each wave's LDS list (8 bytes each; a multiple of 64).
+constexpr int kGatherCap = 256;
+
 struct DcBwdArgs {
   const DcGridDev* grids;
   int n_grids;
   const int64_t* act_bounds;
-  const int64_t* sr_bounds;  // n_grids + 1, in sample rows (y, z)
+  const int64_t* sr_bounds;  // n_grids + 1, in gather row groups
   int64_t n_sr;
+  int gather_rows;           // sample rows per gather group (1, 2, 4 or 8)
+  int gather_lds_row;        // LDS floats per gathered row: 64 * min(16, max Ws) + 64
   const uint64_t* act;
   const uint32_t* wprefix;
   const uint64_t* alist;
