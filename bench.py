@@ -114,14 +114,20 @@ def main():
     for row in table[:6] + table[-2:]:
         log(f"[tune] layers {row['layers']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
 
-    run_steps(be, lib, cg, prec, args.warmup)
+    # per-pass device times (events between the kernels: eager launches)
+    passes, st = run_steps(be, lib, cg, prec, max(3, args.steps // 2))
+    # the timed region: no per-pass events, so the launch sequence is captured once and replayed
+    be.set_option("timing", 0)
+    run_steps(be, lib, cg, prec, max(args.warmup, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    passes, st = run_steps(be, lib, cg, prec, args.steps)
+    _, st = run_steps(be, lib, cg, prec, args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
-    log(f"[bench] plain fp64: {ms:.3f} ms/extraction, passes {passes}")
+    graph_mode = int(st.graph_mode)
+    log(f"[bench] plain fp64: {ms:.3f} ms/extraction (launch sequence "
+        f"{'replayed' if graph_mode == 2 else 'eager'}), passes {passes}")
 
     # plain-mode FP64 vertices (bit-identical to the CPU backend's) for the FP32 error below
     nv = int(st.vertex_count)
@@ -146,12 +152,14 @@ def main():
             be._check(rc)
         return acc
 
-    fwd_bwd(args.warmup)
+    fwd_bwd(max(args.warmup, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fwd_bwd(args.steps)
     torch.cuda.synchronize()
     fb_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    be.set_option("timing", 1)  # one more step with events for the adjoint's own time
+    fwd_bwd(1)
     fallback = int(st.qef_fallback_count)
     v32 = torch.empty((nv, 3), dtype=torch.float32, device="cuda:0")
     be._check(lib.dc_hip_copy_result(be._ctx, ctypes.c_void_p(v32.data_ptr()), None, None))
@@ -202,6 +210,7 @@ def main():
         "faces": int(st.face_count),
         "active_cells_per_s": st.vertex_count / (ms * 1e-3),
         "pass_ms": passes,
+        "launch": "hipGraph replay" if graph_mode == 2 else "eager",
         "fp32_bwd_ms": fb_ms,
         "fp32_bwd_over_plain": fb_ms / ms,
         "fp32_vertex_err_h": {"max": err_max, "rms": err_rms},
@@ -243,6 +252,7 @@ def _fwd_bwd_ms(n, steps, warmup):
     phi, _ = _device_grid(g)
     be = HipDualContouringBackend("plain-fp64", device=0)
     be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    be.set_option("timing", 0)  # replayed launch sequences
     cg = be._cgrid(g, phi.data_ptr())
     st = _lib.DcHipStats()
     lib = be._lib
@@ -250,7 +260,7 @@ def _fwd_bwd_ms(n, steps, warmup):
     def fwd(prec):
         be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st)))
 
-    plain = _timed(lambda: fwd(_lib.DC_HIP_PLAIN_FP64), steps, warmup)
+    plain = _timed(lambda: fwd(_lib.DC_HIP_PLAIN_FP64), steps, max(warmup, 3))
     fwd(_lib.DC_HIP_FP32_FALLBACK)
     nv = int(st.vertex_count)
     gen = torch.Generator(device="cuda:0")
@@ -263,7 +273,7 @@ def _fwd_bwd_ms(n, steps, warmup):
         be._check(lib.dc_hip_backward(be._ctx, ctypes.c_void_p(gv.data_ptr()),
                                       ctypes.c_void_p(gphi.data_ptr()), None))
 
-    both = _timed(fb, steps, warmup)
+    both = _timed(fb, steps, max(warmup, 3))
     return plain, both, nv
 
 
@@ -336,6 +346,7 @@ def bench_c4(args):
         g.values = None
     be = HipDualContouringBackend("plain-fp64", device=0)
     be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    be.set_option("timing", 0)  # the batch's launch sequence is captured once and replayed
     lib = be._lib
     arr = (_lib.DcHipGrid * len(grids))(*[be._cgrid(g, d[0].data_ptr())
                                            for g, d in zip(grids, dev)])
@@ -349,7 +360,7 @@ def bench_c4(args):
                                            ctypes.byref(st)))
 
     steps = max(3, args.steps // 4)
-    batch_ms = _timed(batch, steps, 1)
+    batch_ms = _timed(batch, steps, 2)
     tot_v, tot_f = int(st.vertex_count), int(st.face_count)
     spot = [0, 17, 101, 255]
     batch_v = torch.empty((tot_v, 3), dtype=torch.float64, device="cuda:0")

@@ -72,6 +72,7 @@ typedef struct dc_hip_stats {
   int64_t invalid_count;
   float normals_ms;        /* averaged hermite normals, once per face */
   float mark_kernel_ms;    /* the sample-reading kernel of the marking pass alone */
+  int32_t graph_mode;      /* forward launch sequence: 0 eager, 1 captured now, 2 replayed */
 } dc_hip_stats;
 
 typedef struct dc_hip_context dc_hip_context;
@@ -83,8 +84,10 @@ void dc_hip_destroy(dc_hip_context* ctx);
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_layers" (8, 16, 32, 64: cell layers per marking wave tile), "mark_waves"
  * (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
- * adjoint's gradient gather; applies from the next forward), "timing" (0/1, per-pass events),
- * "qef_blocks" (persistent blocks of the solve). */
+ * adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
+ * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
+ * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
+ * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve). */
 int dc_hip_set_option(dc_hip_context* ctx, const char* key, int64_t value);
 
 /* Replaces CudaDualContouringBackend::extract (src/cuda/dc_cuda.cu:614-649): host values are
@@ -120,6 +123,16 @@ int dc_hip_extract_batch(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t 
  * fixed order, no floating-point atomics. */
 int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad_values,
                     dc_hip_stats* stats);
+
+/* Tile-shape sweep of the marking pass (no counterpart in the reference, whose marking kernel has
+ * one fixed launch shape, src/cuda/dc_cuda.cu:241-281 launched at :687-692). Every
+ * (mark_layers, mark_waves) in {8, 16, 32, 64} x {1, 2, 4} is timed on these device-resident grids
+ * with the per-pass events (best of `repeats` runs after one warm-up run); the fastest is kept on
+ * the context and returned. table_ms (12 floats, layers-major; may be NULL) receives every
+ * candidate's marking-pass time. The context's "timing" option is restored afterwards; the last
+ * extraction's result stays in the context like any other extraction's. */
+int dc_hip_autotune(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t n_grids, int precision,
+                    int32_t repeats, int32_t* best_layers, int32_t* best_waves, float* table_ms);
 
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);

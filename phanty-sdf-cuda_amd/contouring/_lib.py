@@ -25,8 +25,8 @@ DC_HIP_FP32_FALLBACK = 1
 EXPORTS = (
     "dc_hip_create", "dc_hip_destroy", "dc_hip_set_stream", "dc_hip_set_option",
     "dc_hip_extract", "dc_hip_extract_device", "dc_hip_result_counts", "dc_hip_copy_result",
-    "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_last_error",
-    "dc_hip_status_string",
+    "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
+    "dc_hip_last_error", "dc_hip_status_string",
 )
 
 
@@ -47,7 +47,8 @@ class DcHipStats(ctypes.Structure):
                 ("total_ms", ctypes.c_float), ("backward_ms", ctypes.c_float),
                 ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
                 ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64),
-                ("normals_ms", ctypes.c_float), ("mark_kernel_ms", ctypes.c_float)]
+                ("normals_ms", ctypes.c_float), ("mark_kernel_ms", ctypes.c_float),
+                ("graph_mode", ctypes.c_int32)]
 
 
 _hip = None
@@ -82,6 +83,9 @@ def load_hip():
         "dc_hip_extract_batch": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, P, P,
                                      ctypes.POINTER(DcHipStats)]),
         "dc_hip_backward": (I, [P, P, P, ctypes.POINTER(DcHipStats)]),
+        "dc_hip_autotune": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, ctypes.c_int32,
+                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_float)]),
         "dc_hip_last_error": (ctypes.c_char_p, [P]),
         "dc_hip_status_string": (ctypes.c_char_p, [I]),
     }

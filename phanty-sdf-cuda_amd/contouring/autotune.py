@@ -1,9 +1,10 @@
 """Autotuner for the mark pass's tile shape (cell layers per wave tile, waves per block).
 
 The reference CUDA backend launches its marking kernel with one fixed block shape
-(src/cuda/dc_cuda.cu:687-692). Here the shape is picked per grid size by timing every candidate
-on the caller's own grid with the pipeline's hipEvent pass timers, and the winner is cached on
-disk keyed by device name and grid shape, so later processes skip the sweep.
+(src/cuda/dc_cuda.cu:687-692). Here the shape is picked per grid size by the library's own sweep
+(dc_hip_autotune: every candidate timed on the caller's grids with the pipeline's hipEvent pass
+timers), and the winner is cached on disk keyed by device name and grid shape, so later processes
+skip the sweep.
 """
 import ctypes
 import itertools
@@ -82,22 +83,16 @@ def autotune_mark(backend, grids, repeats=3, cache_path=None, use_cache=True):
     prec = PRECISIONS[backend.precision]
     if not all(isinstance(g.d_values, int) for g in grids):
         backend._use_torch_stream()
-    backend.set_option("timing", 1)
-    st = _lib.DcHipStats()
-    table = []
-    for layers, waves in itertools.product(LAYERS, WAVES):
-        cfg = {"layers": layers, "waves": waves}
-        _apply(backend, cfg)
-        times = []
-        for _ in range(repeats + 1):  # the first run warms up (and sizes buffers)
-            backend._check(backend._lib.dc_hip_extract_batch(backend._ctx, arr, len(grids), prec,
-                                                            None, None, ctypes.byref(st)))
-            times.append(st.marking_ms)
-        cfg["mark_ms"] = float(min(times[1:]))
-        table.append(cfg)
+    ms = (ctypes.c_float * (len(LAYERS) * len(WAVES)))()
+    bl, bw = ctypes.c_int32(), ctypes.c_int32()
+    backend._check(backend._lib.dc_hip_autotune(backend._ctx, arr, len(grids), prec, repeats,
+                                                ctypes.byref(bl), ctypes.byref(bw), ms))
+    table = [{"layers": layers, "waves": waves, "mark_ms": float(ms[q])}
+             for q, (layers, waves) in enumerate(itertools.product(LAYERS, WAVES))]
     table.sort(key=lambda c: c["mark_ms"])
-    winner = dict(table[0])
-    _apply(backend, winner)
+    winner = {"layers": int(bl.value), "waves": int(bw.value),
+              "mark_ms": next(c["mark_ms"] for c in table
+                              if (c["layers"], c["waves"]) == (bl.value, bw.value))}
     if use_cache:
         t = _load(cache_path)
         t[key] = winner

@@ -81,6 +81,7 @@ class DualContouringStats:
         st.active_cell_marking_ms = s.marking_ms
         st.active_cell_compaction_ms = s.compaction_ms
         st.normals_ms = s.normals_ms
+        st.graph_mode = s.graph_mode
         return st
 
     def as_dict(self):

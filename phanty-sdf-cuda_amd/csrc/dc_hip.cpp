@@ -41,13 +41,25 @@ struct dc_hip_context {
   int gather_rows = 1;
   int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
   int gather_lds_row = 128;
+  bool use_graph = true;
+  uint64_t alloc_gen = 0;
+  // captured forward launch sequence and the plan it was captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
+  std::string last_key;  // plan of the previous forward
+  std::string dev_key;   // plan whose descriptors and bounds are in the device buffers
+  int graph_mode = 0;    // last forward: 0 eager, 1 captured, 2 replayed
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
-      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist;
+      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff;
   DcCounters* h_counters = nullptr;
+  uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
+  int h_goff_cap = 0;
   hipEvent_t ev[kNumEvents] = {};
+  hipEvent_t ev_join = nullptr;  // orders the context's stream after the null stream
   // last forward
   bool have = false;
   int precision = 0;
@@ -78,9 +90,12 @@ int hip_fail(dc_hip_context* c, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
   } while (0)
 
-int ensure(dc_hip_context* c, DevBuf& b, size_t bytes, const char* name) {
+// Grows a device buffer. forward: the buffer is used by the forward launch sequence, so moving it
+// makes a captured sequence stale.
+int ensure(dc_hip_context* c, DevBuf& b, size_t bytes, const char* name, bool forward = true) {
   if (bytes == 0) bytes = 256;
   if (b.cap >= bytes) return DC_HIP_OK;
+  if (forward) ++c->alloc_gen;
   if (b.p) {
     hipError_t e = hipFree(b.p);
     b.p = nullptr;
@@ -212,11 +227,48 @@ float elapsed(dc_hip_context* c, int a, int b) {
   return ms;
 }
 
-// The launch sequence. Returns DC_HIP_OK after the counters have been read back.
+// The launch sequence after the buffers and descriptors are in place: kernels from the counter
+// clear to the counters' (and batch offsets') read-back into pinned memory. No host
+// synchronisation and no memset nodes, so it can be captured in a graph and replayed.
+int enqueue(dc_hip_context* c, int precision, hipStream_t s);
+
+void drop_graph(dc_hip_context* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->gkey.clear();
+}
+
+// Everything a captured sequence depends on: descriptors (sizes, spacing, value pointers), the
+// per-grid bounds, precision, launch options, capacities, the stream and the buffer addresses.
+std::string plan_key(dc_hip_context* c, int precision, hipStream_t s) {
+  std::string k;
+  auto put = [&](const void* p, size_t n) { k.append(reinterpret_cast<const char*>(p), n); };
+  const int n = (int)c->hgrids.size();
+  put(&n, sizeof(n));
+  put(c->hgrids.data(), n * sizeof(DcGridDev));
+  put(c->hbounds.data(), c->hbounds.size() * sizeof(int64_t));
+  const int64_t v[] = {precision, c->eff_layers, c->mark_waves, c->qef_blocks, c->timing,
+                       (int64_t)c->capA, (int64_t)c->capF, (int64_t)c->alloc_gen,
+                       (int64_t)(uintptr_t)s, (int64_t)(uintptr_t)c->h_goff};
+  put(v, sizeof(v));
+  return k;
+}
+
+// One pass: buffers, descriptor upload, then the launch sequence (replayed from the captured
+// graph when the plan is unchanged). Returns DC_HIP_OK after the counters have been read back.
 int run_once(dc_hip_context* c, int precision) {
   hipStream_t s = c->stream;
   const int n = (int)c->hgrids.size();
   int rc;
+  if (s == nullptr) {
+    // The null stream cannot be captured: run on the context's stream, after the work already
+    // queued on the null stream (the call returns only when its own work is complete).
+    DC_TRY(c, hipEventRecord(c->ev_join, nullptr), "join null stream");
+    DC_TRY(c, hipStreamWaitEvent(c->own, c->ev_join, 0), "join null stream");
+    s = c->own;
+  }
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
   if ((rc = ensure(c, c->bounds, 8 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
@@ -240,9 +292,15 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->pw, c->n_ft * 4, "face word prefix"))) return rc;
   if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
   if ((rc = ensure(c, c->flist, c->capF * 8, "face list"))) return rc;
-  const int64_t face_blocks =
-      std::max<int64_t>(1, std::min<int64_t>(((int64_t)c->capF + 255) / 256,
-                                             (int64_t)c->cu_count * 16));
+  if ((rc = ensure(c, c->goff, 8 * (size_t)n, "grid offsets"))) return rc;
+  if (n > c->h_goff_cap) {
+    if (c->h_goff) hipHostFree(c->h_goff);
+    c->h_goff = nullptr;
+    c->h_goff_cap = 0;
+    if (hipHostMalloc((void**)&c->h_goff, 8 * (size_t)n, hipHostMallocDefault) != hipSuccess)
+      return fail(c, DC_HIP_ERR_OUT_OF_MEMORY, "pinned grid offsets");
+    c->h_goff_cap = n;
+  }
 
   // host staging stays alive in the context until the copies have run
   std::vector<int64_t>& bounds = c->hbounds;
@@ -255,10 +313,54 @@ int run_once(dc_hip_context* c, int precision) {
   bounds.insert(bounds.end(), c->ft_b.begin(), c->ft_b.end());
   bounds.insert(bounds.end(), c->tw_b.begin(), c->tw_b.end());
   bounds.insert(bounds.end(), c->at_b.begin(), c->at_b.end());
-  DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
-                           hipMemcpyHostToDevice, s), "upload descriptors");
-  DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
-                           hipMemcpyHostToDevice, s), "upload bounds");
+  // A plan is captured the second time in a row it runs (one-off plans stay eager) and replayed
+  // while it stays unchanged. Per-pass timing needs events between the kernels, which a replayed
+  // graph does not order reliably: with timing on, every run is eager.
+  const bool graphs = c->use_graph && !c->timing;
+  std::string key = plan_key(c, precision, s);
+  const bool replay = graphs && c->gexec && key == c->gkey;
+  const bool capture = graphs && !replay && key == c->last_key;
+  if (key != c->dev_key) {
+    // the device copies of the descriptors and bounds (read by every kernel, not part of a
+    // captured sequence) are rewritten only when they belong to another plan
+    DC_TRY(c, hipMemcpyAsync(c->grids.p, c->hgrids.data(), n * sizeof(DcGridDev),
+                             hipMemcpyHostToDevice, s), "upload descriptors");
+    DC_TRY(c, hipMemcpyAsync(c->bounds.p, bounds.data(), bounds.size() * sizeof(int64_t),
+                             hipMemcpyHostToDevice, s), "upload bounds");
+    c->dev_key = key;
+  }
+  c->last_key = std::move(key);
+  c->graph_mode = replay ? 2 : capture ? 1 : 0;
+  if (!replay && !capture) {
+    if ((rc = enqueue(c, precision, s))) return rc;
+  } else {
+    if (capture) {
+      drop_graph(c);
+      DC_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      rc = enqueue(c, precision, s);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return hip_fail(c, e, "end capture");
+      c->graph = g;
+      DC_TRY(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0), "instantiate graph");
+      c->gkey = c->last_key;
+    }
+    DC_TRY(c, hipGraphLaunch(c->gexec, s), "graph launch");
+  }
+  DC_TRY(c, hipStreamSynchronize(s), "pipeline");
+  return DC_HIP_OK;
+}
+
+int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
+  const int n = (int)c->hgrids.size();
+  const int64_t t_a = dc_scan_tiles(c->n_words), t_p = dc_scan_tiles(c->n_pen);
+  const int64_t face_blocks =
+      std::max<int64_t>(1, std::min<int64_t>(((int64_t)c->capF + 255) / 256,
+                                             (int64_t)c->cu_count * 16));
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   const int64_t* d_act_b = d_tile_b + (n + 1);
   const int64_t* d_pen_b = d_act_b + (n + 1);
@@ -365,9 +467,15 @@ int run_once(dc_hip_context* c, int precision) {
   ea.capA = c->capA;
   DC_TRY(c, dc_launch_emit(ea, s), "emit");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
+  if (n > 1) {
+    DC_TRY(c, dc_launch_grid_offsets(d_act_b, d_pen_b, n, as<uint32_t>(c->wprefix),
+                                     as<uint32_t>(c->pbase), as<uint32_t>(c->goff), s),
+           "grid offsets");
+    DC_TRY(c, hipMemcpyAsync(c->h_goff, c->goff.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s),
+           "read grid offsets");
+  }
   DC_TRY(c, hipMemcpyAsync(c->h_counters, d_ctr, sizeof(DcCounters), hipMemcpyDeviceToHost, s),
          "read counters");
-  DC_TRY(c, hipStreamSynchronize(s), "pipeline");
   return DC_HIP_OK;
 }
 
@@ -418,6 +526,7 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
     st.vertex_count = c->nv;
     st.face_count = c->nf;
     st.qef_fallback_count = (int64_t)c->h_counters->fallback;
+    st.graph_mode = c->graph_mode;
     if (c->timing) {
       st.marking_ms = elapsed(c, 0, 1);
       st.compaction_ms = elapsed(c, 1, 2);
@@ -477,6 +586,10 @@ int dc_hip_create(int device, dc_hip_context** out) {
       return DC_HIP_ERR_HIP;
     }
   }
+  if (hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    dc_hip_destroy(c);
+    return DC_HIP_ERR_HIP;
+  }
   if (hipHostMalloc((void**)&c->h_counters, sizeof(DcCounters), hipHostMallocDefault) !=
       hipSuccess) {
     dc_hip_destroy(c);
@@ -494,12 +607,15 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
-                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist};
+                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  drop_graph(c);
   if (c->h_counters) hipHostFree(c->h_counters);
+  if (c->h_goff) hipHostFree(c->h_goff);
   if (c->own) hipStreamDestroy(c->own);
   delete c;
 }
@@ -525,6 +641,9 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (value != 1 && value != 2 && value != 4 && value != 8)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "gather_rows must be 1, 2, 4 or 8");
     c->gather_rows = (int)value;
+  } else if (k == "graph") {
+    c->use_graph = value != 0;
+    if (!c->use_graph) drop_graph(c);
   } else if (k == "timing") {
     c->timing = value != 0;
   } else if (k == "qef_blocks") {
@@ -551,7 +670,7 @@ int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
     return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimensions must be >= 2");
   DC_TRY(c, hipSetDevice(c->device), "set device");
   const size_t bytes = (size_t)grid->nx * grid->ny * grid->nz * sizeof(float);
-  int rc = ensure(c, c->upload, bytes, "upload buffer");
+  int rc = ensure(c, c->upload, bytes, "upload buffer", false);  // in the plan key as values
   if (rc) return rc;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], c->stream), "event");
   DC_TRY(c, hipMemcpyAsync(c->upload.p, grid->values, bytes, hipMemcpyHostToDevice, c->stream),
@@ -573,22 +692,58 @@ int dc_hip_extract_batch(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_
   int rc = run_pipeline(c, grids, n_grids, precision, stats);
   if (rc) return rc;
   if (vertex_offsets || face_offsets) {
-    // per-grid offsets: wprefix at each grid's first act word, pbase at its first pencil
-    std::vector<uint32_t> vw(n_grids), fw(n_grids);
+    // per-grid offsets, read back with the counters (one grid: zero)
     for (int q = 0; q < n_grids; ++q) {
-      DC_TRY(c, hipMemcpyAsync(&vw[q], as<uint32_t>(c->wprefix) + c->act_b[q], 4,
-                               hipMemcpyDeviceToHost, c->stream), "offsets");
-      DC_TRY(c, hipMemcpyAsync(&fw[q], as<uint32_t>(c->pbase) + c->pen_b[q], 4,
-                               hipMemcpyDeviceToHost, c->stream), "offsets");
-    }
-    DC_TRY(c, hipStreamSynchronize(c->stream), "offsets");
-    for (int q = 0; q < n_grids; ++q) {
-      if (vertex_offsets) vertex_offsets[q] = vw[q];
-      if (face_offsets) face_offsets[q] = fw[q];
+      if (vertex_offsets) vertex_offsets[q] = n_grids > 1 ? c->h_goff[2 * q] : 0;
+      if (face_offsets) face_offsets[q] = n_grids > 1 ? c->h_goff[2 * q + 1] : 0;
     }
     if (vertex_offsets) vertex_offsets[n_grids] = c->nv;
     if (face_offsets) face_offsets[n_grids] = c->nf;
   }
+  return DC_HIP_OK;
+}
+
+int dc_hip_autotune(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_grids, int precision,
+                    int32_t repeats, int32_t* best_layers, int32_t* best_waves, float* table_ms) {
+  if (!c || !grids || n_grids <= 0 || repeats < 1)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "bad autotune arguments");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  static const int kLayers[4] = {8, 16, 32, 64};
+  static const int kWaves[3] = {1, 2, 4};
+  const bool timing = c->timing;
+  const int layers0 = c->mark_layers, waves0 = c->mark_waves;
+  c->timing = true;
+  float best = 0.f;
+  int bl = layers0, bw = waves0, rc = DC_HIP_OK;
+  for (int a = 0; a < 4 && rc == DC_HIP_OK; ++a) {
+    for (int b = 0; b < 3 && rc == DC_HIP_OK; ++b) {
+      c->mark_layers = kLayers[a];
+      c->mark_waves = kWaves[b];
+      float t = 0.f;
+      for (int r = 0; r <= repeats && rc == DC_HIP_OK; ++r) {  // run 0 warms up
+        dc_hip_stats st{};
+        rc = run_pipeline(c, grids, n_grids, precision, &st);
+        if (r == 1 || (r > 1 && st.marking_ms < t)) t = st.marking_ms;
+      }
+      if (rc != DC_HIP_OK) break;
+      if (table_ms) table_ms[3 * a + b] = t;
+      if ((a == 0 && b == 0) || t < best) {
+        best = t;
+        bl = kLayers[a];
+        bw = kWaves[b];
+      }
+    }
+  }
+  c->timing = timing;
+  if (rc != DC_HIP_OK) {
+    c->mark_layers = layers0;
+    c->mark_waves = waves0;
+    return rc;
+  }
+  c->mark_layers = bl;
+  c->mark_waves = bw;
+  if (best_layers) *best_layers = bl;
+  if (best_waves) *best_waves = bw;
   return DC_HIP_OK;
 }
 
@@ -641,10 +796,11 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   const int n = (int)c->hgrids.size();
   int rc;
   const size_t nv1 = (size_t)std::max<int64_t>(c->nv, 1);
-  if ((rc = ensure(c, c->bwd_contrib, nv1 * 8 * 4, "adjoint contributions"))) return rc;
-  if ((rc = ensure(c, c->ubar, (size_t)std::max<int64_t>(c->nf, 1) * 48, "face adjoint slots")))
+  if ((rc = ensure(c, c->bwd_contrib, nv1 * 8 * 4, "adjoint contributions", false))) return rc;
+  if ((rc = ensure(c, c->ubar, (size_t)std::max<int64_t>(c->nf, 1) * 48, "face adjoint slots",
+                   false)))
     return rc;
-  if ((rc = ensure(c, c->ubar_e, nv1 * 12 * 12, "edge adjoint slots"))) return rc;
+  if ((rc = ensure(c, c->ubar_e, nv1 * 12 * 12, "edge adjoint slots", false))) return rc;
   hipStream_t s = c->stream;
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   DcBwdArgs a;

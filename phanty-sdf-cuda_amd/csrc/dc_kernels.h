@@ -120,6 +120,10 @@ struct DcBwdArgs {
 
 hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+// Per-grid first vertex and first face of a batch (2 * n uint32, grid-major).
+hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, int n,
+                                  const uint32_t* wprefix, const uint32_t* pbase, uint32_t* out,
+                                  hipStream_t s);
 int64_t dc_mark_tiles(const DcGridDev& g, int layers);
 // k_mark (active and face words over x, [k][c][j]); then dc_launch_mark_finish: the active plane
 // in vertex order, the face planes over k and the pencil face counts.

@@ -500,6 +500,17 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
     p[q] = v;
 }
 
+// First vertex and first face of each grid of a batch: the vertex prefix at its first act word
+// and the pencil base at its first pencil (out[2q], out[2q+1]).
+__global__ void k_grid_offsets(const int64_t* __restrict__ act_b, const int64_t* __restrict__ pen_b,
+                               int n, const uint32_t* __restrict__ wprefix,
+                               const uint32_t* __restrict__ pbase, uint32_t* __restrict__ out) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= n) return;
+  out[2 * q] = wprefix[act_b[q]];
+  out[2 * q + 1] = pbase[pen_b[q]];
+}
+
 int64_t fill_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b > 4096) b = 4096;
@@ -521,6 +532,15 @@ hipError_t dc_launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s)
 hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)fill_blocks(n)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, int n,
+                                  const uint32_t* wprefix, const uint32_t* pbase, uint32_t* out,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grid_offsets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, act_b,
+                     pen_b, n, wprefix, pbase, out);
   return hipGetLastError();
 }
 

@@ -68,3 +68,33 @@ def test_backend_raises_without_device():
     from contouring import HipDualContouringBackend
     with pytest.raises(_lib.DcHipError):
         HipDualContouringBackend()
+
+
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of a dc_hip.h struct, from a C program built against the header."""
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dc_hip.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({struct}));']
+    lines += [f'  printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)],
+                   check=True)
+    return [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+
+
+@pytest.mark.parametrize("struct,mirror", [("dc_hip_stats", _lib.DcHipStats),
+                                           ("dc_hip_grid", _lib.DcHipGrid)])
+def test_ctypes_mirrors_match_the_header(struct, mirror, tmp_path):
+    names = [f[0] for f in mirror._fields_]
+    got = _c_layout(struct, names, tmp_path)
+    assert got[0] == ctypes.sizeof(mirror)
+    for name, off in zip(names, got[1:]):
+        assert getattr(mirror, name).offset == off, name
+
+
+def test_autotune_rejects_bad_arguments():
+    lib = _lib.load_hip()
+    assert lib.dc_hip_autotune(None, None, 0, 0, 1, None, None, None) == _lib.DC_HIP_ERR_INVALID_ARG

@@ -1,0 +1,36 @@
+"""The marking pass's tile-shape sweep (dc_hip_autotune through contouring.autotune_mark).
+
+The tile shape changes only the launch geometry: the tuned context must give the same mesh as the
+default one, bit for bit. The sweep times all 12 candidates, keeps the fastest on the context and
+restores the caller's timing option.
+"""
+import numpy as np
+import pytest
+
+from contouring import DenseSdfGridDevice, HipDualContouringBackend, autotune_mark, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_results_and_picks_fastest(tmp_path):
+    import torch
+    g = synth.csg8(96, seed=21)
+    phi = torch.from_numpy(np.ascontiguousarray(g.values, np.float32)).cuda()
+    dev = DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, d_values=phi)
+    ref = HipDualContouringBackend("plain-fp64")
+    (V0, F0), _ = ref.extract_tensor(phi.reshape(g.nz, g.ny, g.nx), (g.vx, g.vy, g.vz),
+                                     (g.ox, g.oy, g.oz))
+    be = HipDualContouringBackend("plain-fp64")
+    be.set_option("timing", 0)
+    # a fresh cache file: the first call sweeps and stores its choice
+    best, table = autotune_mark(be, dev, repeats=2, cache_path=str(tmp_path / "tune.json"))
+    print(f"[autotune] best {best}", flush=True)
+    assert len(table) == 12 and all(row["mark_ms"] > 0 for row in table)
+    assert best["mark_ms"] == min(row["mark_ms"] for row in table)
+    (V1, F1), st = be.extract_tensor(phi.reshape(g.nz, g.ny, g.nx), (g.vx, g.vy, g.vz),
+                                     (g.ox, g.oy, g.oz))
+    assert torch.equal(V0, V1) and torch.equal(F0, F1)
+    assert st.marking_ms == 0  # timing stayed off
+    # the disk cache gives the same choice without a sweep
+    best2, table2 = autotune_mark(be, dev, cache_path=str(tmp_path / "tune.json"), use_cache=True)
+    assert table2 == [] and (best2["layers"], best2["waves"]) == (best["layers"], best["waves"])
