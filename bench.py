@@ -114,7 +114,9 @@ def main():
     for row in table[:6] + table[-2:]:
         log(f"[tune] layers {row['layers']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
 
-    # per-pass device times (events between the kernels: eager launches)
+    # per-pass device times (events between the kernels: eager launches), after a warm-up that
+    # sizes the buffers
+    run_steps(be, lib, cg, prec, max(args.warmup, 1))
     passes, st = run_steps(be, lib, cg, prec, max(3, args.steps // 2))
     # the timed region: no per-pass events, so the launch sequence is captured once and replayed
     be.set_option("timing", 0)

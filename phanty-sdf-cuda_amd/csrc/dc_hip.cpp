@@ -28,6 +28,23 @@ struct DevBuf {
 
 constexpr int kNumEvents = 10;  // 0..7 forward passes, 8..9 upload / backward
 
+// A launch sequence that is captured as a hipGraph the second time in a row it runs with the same
+// key (one-off sequences stay eager) and replayed while the key stays the same.
+struct Captured {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string key;       // what `exec` was captured for
+  std::string last_key;  // the previous run's key
+  int mode = 0;          // the last run: 0 eager, 1 captured, 2 replayed
+  void drop() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+};
+
 }  // namespace
 
 struct dc_hip_context {
@@ -43,13 +60,8 @@ struct dc_hip_context {
   int gather_lds_row = 128;
   bool use_graph = true;
   uint64_t alloc_gen = 0;
-  // captured forward launch sequence and the plan it was captured for
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::string gkey;
-  std::string last_key;  // plan of the previous forward
+  Captured fwd, bwd;     // forward and adjoint launch sequences
   std::string dev_key;   // plan whose descriptors and bounds are in the device buffers
-  int graph_mode = 0;    // last forward: 0 eager, 1 captured, 2 replayed
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
@@ -233,11 +245,49 @@ float elapsed(dc_hip_context* c, int a, int b) {
 int enqueue(dc_hip_context* c, int precision, hipStream_t s);
 
 void drop_graph(dc_hip_context* c) {
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
-  c->gkey.clear();
+  c->fwd.drop();
+  c->bwd.drop();
+}
+
+// Runs `enqueue(s)` (a launch sequence without host synchronisation) on stream s: eagerly, or
+// through cap's graph (captured now when `key` repeats the previous run's, replayed while it
+// matches the captured one). graphs = false forces an eager run.
+template <class Enqueue>
+int run_captured(dc_hip_context* c, Captured& cap, std::string key, bool graphs, hipStream_t s,
+                 Enqueue&& enqueue) {
+  const bool replay = graphs && cap.exec && key == cap.key;
+  const bool capture = graphs && !replay && key == cap.last_key;
+  cap.last_key = std::move(key);
+  cap.mode = replay ? 2 : capture ? 1 : 0;
+  if (!replay && !capture) return enqueue(s);
+  if (capture) {
+    cap.drop();
+    DC_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    const int rc = enqueue(s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "end capture");
+    cap.graph = g;
+    DC_TRY(c, hipGraphInstantiate(&cap.exec, g, nullptr, nullptr, 0), "instantiate graph");
+    cap.key = cap.last_key;
+  }
+  DC_TRY(c, hipGraphLaunch(cap.exec, s), "graph launch");
+  return DC_HIP_OK;
+}
+
+// The null stream cannot be captured: work for it runs on the context's stream, after the work
+// already queued on the null stream (the calls return only when their own work is complete).
+int work_stream(dc_hip_context* c, hipStream_t* s) {
+  *s = c->stream;
+  if (*s != nullptr) return DC_HIP_OK;
+  DC_TRY(c, hipEventRecord(c->ev_join, nullptr), "join null stream");
+  DC_TRY(c, hipStreamWaitEvent(c->own, c->ev_join, 0), "join null stream");
+  *s = c->own;
+  return DC_HIP_OK;
 }
 
 // Everything a captured sequence depends on: descriptors (sizes, spacing, value pointers), the
@@ -259,16 +309,10 @@ std::string plan_key(dc_hip_context* c, int precision, hipStream_t s) {
 // One pass: buffers, descriptor upload, then the launch sequence (replayed from the captured
 // graph when the plan is unchanged). Returns DC_HIP_OK after the counters have been read back.
 int run_once(dc_hip_context* c, int precision) {
-  hipStream_t s = c->stream;
   const int n = (int)c->hgrids.size();
   int rc;
-  if (s == nullptr) {
-    // The null stream cannot be captured: run on the context's stream, after the work already
-    // queued on the null stream (the call returns only when its own work is complete).
-    DC_TRY(c, hipEventRecord(c->ev_join, nullptr), "join null stream");
-    DC_TRY(c, hipStreamWaitEvent(c->own, c->ev_join, 0), "join null stream");
-    s = c->own;
-  }
+  hipStream_t s;
+  if ((rc = work_stream(c, &s))) return rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
   if ((rc = ensure(c, c->bounds, 8 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
@@ -316,10 +360,7 @@ int run_once(dc_hip_context* c, int precision) {
   // A plan is captured the second time in a row it runs (one-off plans stay eager) and replayed
   // while it stays unchanged. Per-pass timing needs events between the kernels, which a replayed
   // graph does not order reliably: with timing on, every run is eager.
-  const bool graphs = c->use_graph && !c->timing;
   std::string key = plan_key(c, precision, s);
-  const bool replay = graphs && c->gexec && key == c->gkey;
-  const bool capture = graphs && !replay && key == c->last_key;
   if (key != c->dev_key) {
     // the device copies of the descriptors and bounds (read by every kernel, not part of a
     // captured sequence) are rewritten only when they belong to another plan
@@ -329,28 +370,9 @@ int run_once(dc_hip_context* c, int precision) {
                              hipMemcpyHostToDevice, s), "upload bounds");
     c->dev_key = key;
   }
-  c->last_key = std::move(key);
-  c->graph_mode = replay ? 2 : capture ? 1 : 0;
-  if (!replay && !capture) {
-    if ((rc = enqueue(c, precision, s))) return rc;
-  } else {
-    if (capture) {
-      drop_graph(c);
-      DC_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-      rc = enqueue(c, precision, s);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(s, &g);
-      if (rc) {
-        if (g) hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return hip_fail(c, e, "end capture");
-      c->graph = g;
-      DC_TRY(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0), "instantiate graph");
-      c->gkey = c->last_key;
-    }
-    DC_TRY(c, hipGraphLaunch(c->gexec, s), "graph launch");
-  }
+  rc = run_captured(c, c->fwd, std::move(key), c->use_graph && !c->timing, s,
+                    [&](hipStream_t st) { return enqueue(c, precision, st); });
+  if (rc) return rc;
   DC_TRY(c, hipStreamSynchronize(s), "pipeline");
   return DC_HIP_OK;
 }
@@ -526,7 +548,7 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
     st.vertex_count = c->nv;
     st.face_count = c->nf;
     st.qef_fallback_count = (int64_t)c->h_counters->fallback;
-    st.graph_mode = c->graph_mode;
+    st.graph_mode = c->fwd.mode;
     if (c->timing) {
       st.marking_ms = elapsed(c, 0, 1);
       st.compaction_ms = elapsed(c, 1, 2);
@@ -801,9 +823,11 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                    false)))
     return rc;
   if ((rc = ensure(c, c->ubar_e, nv1 * 12 * 12, "edge adjoint slots", false))) return rc;
-  hipStream_t s = c->stream;
+  hipStream_t s;
+  if ((rc = work_stream(c, &s))) return rc;
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
   DcBwdArgs a;
+  std::memset(&a, 0, sizeof(a));  // its bytes are part of the capture key
   a.grids = as<DcGridDev>(c->grids);
   a.n_grids = n;
   a.act_bounds = d_tile_b + (n + 1);
@@ -832,7 +856,17 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.contrib = as<float>(c->bwd_contrib);
   a.grad_values = grad_values;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], s), "event");
-  DC_TRY(c, dc_launch_backward(a, s), "backward");
+  // capture key: every launch argument (pointers, sizes, block counts), the stream and the plan
+  // whose descriptors are on the device
+  std::string key(reinterpret_cast<const char*>(&a), sizeof(a));
+  key.append(reinterpret_cast<const char*>(&s), sizeof(s));
+  key += c->dev_key;
+  rc = run_captured(c, c->bwd, std::move(key), c->use_graph && !c->timing, s,
+                    [&](hipStream_t st) -> int {
+                      DC_TRY(c, dc_launch_backward(a, st), "backward");
+                      return DC_HIP_OK;
+                    });
+  if (rc) return rc;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], s), "event");
   DC_TRY(c, hipStreamSynchronize(s), "backward");
   if (stats) {
@@ -840,6 +874,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
     stats->vertex_count = c->nv;
     stats->face_count = c->nf;
     stats->backward_ms = c->timing ? elapsed(c, 8, 9) : 0.f;
+    stats->graph_mode = c->bwd.mode;
     stats->total_ms =
         std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

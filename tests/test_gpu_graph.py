@@ -86,14 +86,17 @@ def test_adjoint_after_replay_matches_eager():
     w = torch.randn(V.shape, device="cuda", generator=gen)
     want = torch.empty_like(phi)
     eager.backward(w, want)
-    for rep in range(4):  # eager, capture, replay, replay
+    got = torch.empty_like(phi)
+    for rep in range(4):  # forward and adjoint each: eager, capture, replay, replay
         print(f"[graph] adjoint rep {rep}", flush=True)
         (V2, F2), st = be.extract_tensor(phi, (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
         assert st.graph_mode == [0, 1, 2, 2][rep]
         assert torch.equal(V2, V) and torch.equal(F2, F)
-        got = torch.empty_like(phi)
-        be.backward(w, got)
+        got.fill_(float("nan"))
+        bst = be.backward(w, got)
+        assert bst.graph_mode == [0, 1, 2, 2][rep]
         assert torch.equal(got, want), rep
+        torch.cuda.synchronize()
 
 
 def test_batch_replays_match_eager():
