@@ -609,9 +609,56 @@ __device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t
   return ok;
 }
 
-// FP32 solve in the cell frame. Returns false when the cell must be solved in FP64.
+// The FP32 solve's state, kept for the adjoint (kQefState floats per vertex, six float4s): mass
+// point c, M (upper triangle), g, the eigenvalues and the eigenvectors (V[r][q], row-major).
+constexpr int kQefState = 24;
+
+__device__ __forceinline__ void store_state(float* __restrict__ dst, const Qef<float>& Q,
+                                            const float lam[3], const float V[3][3]) {
+  float4* d = reinterpret_cast<float4*>(dst);
+  d[0] = make_float4(Q.c[0], Q.c[1], Q.c[2], Q.M[0][0]);
+  d[1] = make_float4(Q.M[0][1], Q.M[0][2], Q.M[1][1], Q.M[1][2]);
+  d[2] = make_float4(Q.M[2][2], Q.g[0], Q.g[1], Q.g[2]);
+  d[3] = make_float4(lam[0], lam[1], lam[2], V[0][0]);
+  d[4] = make_float4(V[0][1], V[0][2], V[1][0], V[1][1]);
+  d[5] = make_float4(V[1][2], V[2][0], V[2][1], V[2][2]);
+}
+
+__device__ __forceinline__ void load_state(const float* __restrict__ src, Qef<float>& Q,
+                                           float lam[3], float V[3][3]) {
+  const float4* s = reinterpret_cast<const float4*>(src);
+  const float4 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5];
+  Q.c[0] = a.x;
+  Q.c[1] = a.y;
+  Q.c[2] = a.z;
+  Q.M[0][0] = a.w;
+  Q.M[0][1] = Q.M[1][0] = b.x;
+  Q.M[0][2] = Q.M[2][0] = b.y;
+  Q.M[1][1] = b.z;
+  Q.M[1][2] = Q.M[2][1] = b.w;
+  Q.M[2][2] = c.x;
+  Q.g[0] = c.y;
+  Q.g[1] = c.z;
+  Q.g[2] = c.w;
+  lam[0] = d.x;
+  lam[1] = d.y;
+  lam[2] = d.z;
+  V[0][0] = d.w;
+  V[0][1] = e.x;
+  V[0][2] = e.y;
+  V[1][0] = e.z;
+  V[1][1] = e.w;
+  V[1][2] = f.x;
+  V[2][0] = f.y;
+  V[2][1] = f.z;
+  V[2][2] = f.w;
+}
+
+// FP32 solve in the cell frame. Returns false when the cell must be solved in FP64; otherwise
+// stores the solve's state for the adjoint at `state` (kQefState floats) unless it is null.
 __device__ __forceinline__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                const FaceNormals<float>& fn, int i, int j, int k, float xl[3]) {
+                                const FaceNormals<float>& fn, int i, int j, int k, float xl[3],
+                                float* __restrict__ state) {
   Cell<float> own;
   load_cell(G, i, j, k, own);
   Qef<float> Q;
@@ -634,6 +681,7 @@ __device__ __forceinline__ bool solve_cell_fp32(const DcGridDev& G, const uint64
   xl[0] = Q.c[0] + y[0];
   xl[1] = Q.c[1] + y[1];
   xl[2] = Q.c[2] + y[2];
+  if (ok && state) store_state(state, Q, lam, V);
   return ok;
 }
 
@@ -801,17 +849,29 @@ __device__ __forceinline__ int member_of(int axis, int i, int j, int k, int x0, 
 // stores u-bar for phase 2 (in the face's slot for this member, ubar[4*face + member], or, for
 // an averaged edge without a face, in the cell's per-edge slot ubar_e[12*v + e]); the point
 // adjoints and the other rows' normals go to the cell's own 8 corners, Sb.
+// state: the forward's FP32 solve state of this cell (T = float), or null to assemble and solve
+// again here.
 template <typename T>
 __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
                               const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
-                              const float xb_in[3], float* __restrict__ ubar,
-                              float* __restrict__ ubar_e, T Sb[8]) {
+                              const float xb_in[3], const float* __restrict__ state,
+                              float* __restrict__ ubar, float* __restrict__ ubar_e, T Sb[8]) {
   Cell<T> own;
   load_cell(G, i, j, k, own);
   Qef<T> Q;
-  assemble_any(G, act, fn, i, j, k, own, Q);
   T lam[3], V[3][3];
-  sym3_eig(Q.M, lam, V);
+  if constexpr (sizeof(T) == 4) {
+    if (state) {
+      load_state(state, Q, lam, V);
+      Q.m = __popc(crossing_mask(own));
+    } else {
+      assemble_any(G, act, fn, i, j, k, own, Q);
+      sym3_eig(Q.M, lam, V);
+    }
+  } else {
+    assemble_any(G, act, fn, i, j, k, own, Q);
+    sym3_eig(Q.M, lam, V);
+  }
   const T xb[3] = {(T)xb_in[0], (T)xb_in[1], (T)xb_in[2]};
   bool kept[3];
   T gb[3] = {T(0), T(0), T(0)};

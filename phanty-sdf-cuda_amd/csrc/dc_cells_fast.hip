@@ -15,7 +15,7 @@ hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.blocks), block(256);
   hipLaunchKernelGGL(k_qef<DC_PREC_FP32>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
                      a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags, a.fblist, a.ft, a.pw,
-                     a.nrm, a.capF);
+                     a.nrm, a.capF, a.qstate);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_qef_fallback, dim3((unsigned)std::max<int64_t>(1, a.blocks / 4)), block,
@@ -34,19 +34,19 @@ hipError_t dc_launch_bwd_cells(const DcBwdArgs& a, hipStream_t s) {
   if (a.precision == DC_PREC_FP64) {                                                               \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), grid, block, 0, s, a.grids, a.n_grids,         \
                        a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, nullptr, a.ctr,        \
-                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
-                       a.contrib);                                                                 \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr, a.ubar,     \
+                       a.ubar_e, a.contrib);                                                       \
   } else {                                                                                         \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, float>), grid, block, 0, s, a.grids, a.n_grids,          \
                        a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 1, nullptr, a.ctr,        \
-                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
-                       a.contrib);                                                                 \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.qstate, a.ubar,    \
+                       a.ubar_e, a.contrib);                                                       \
     e = hipGetLastError();                                                                         \
     if (e != hipSuccess) return e;                                                                 \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), fgrid, block, 0, s, a.grids, a.n_grids,        \
                        a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, a.fblist, a.ctr,       \
-                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.ubar, a.ubar_e,    \
-                       a.contrib);                                                                 \
+                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr, a.ubar,     \
+                       a.ubar_e, a.contrib);                                                       \
   }                                                                                                \
   e = hipGetLastError();                                                                           \
   if (e != hipSuccess) return e;

@@ -130,13 +130,13 @@ __global__ void __launch_bounds__(256)
 // One thread per active cell (persistent grid-stride over the device-side count). In FP32 mode a
 // cell that needs FP64 is appended to fblist (order irrelevant: results are per vertex).
 template <int PREC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
     k_qef(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
           const uint64_t* __restrict__ act, const uint64_t* __restrict__ alist,
           DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
           float* __restrict__ v32, uint32_t* __restrict__ flags, uint32_t* __restrict__ fblist,
           const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-          const void* __restrict__ nrm, uint64_t capF) {
+          const void* __restrict__ nrm, uint64_t capF, float* __restrict__ qstate) {
   const uint64_t n = min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm), nullptr, nf};
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256)
       v32[3 * v + 2] = (float)x[2];
     } else {
       float xl[3];
-      if (solve_cell_fp32(G, act, fn32, i, j, k, xl)) {
+      if (solve_cell_fp32(G, act, fn32, i, j, k, xl, qstate ? qstate + kQefState * v : nullptr)) {
         v32[3 * v + 0] = (float)(coordd(G.ox, i, G.vx) + (double)xl[0]);
         v32[3 * v + 1] = (float)(coordd(G.oy, j, G.vy) + (double)xl[1]);
         v32[3 * v + 2] = (float)(coordd(G.oz, k, G.vz) + (double)xl[2]);
@@ -316,8 +316,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
                const uint32_t* __restrict__ list, const DcCounters* __restrict__ ctr, uint64_t cap,
                const float* __restrict__ gv, const uint64_t* __restrict__ ft,
                const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
-               const float* __restrict__ nrm32, uint64_t capF, float* __restrict__ ubar,
-               float* __restrict__ ubar_e, float* __restrict__ contrib) {
+               const float* __restrict__ nrm32, uint64_t capF, const float* __restrict__ qstate,
+               float* __restrict__ ubar, float* __restrict__ ubar_e, float* __restrict__ contrib) {
   const uint64_t n = list ? min(ctr->fallback, cap) : min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<T> fn{ft, pw, nrm64, nrm32, nf};
@@ -334,7 +334,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     float4* out = reinterpret_cast<float4*>(contrib + 8 * v);
     if (PHASE == 1) {
       const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
-      backward_rows<T>(G, act, fn, i, j, k, v, xb, ubar, ubar_e, Sb);
+      backward_rows<T>(G, act, fn, i, j, k, v, xb, qstate ? qstate + kQefState * v : nullptr,
+                       ubar, ubar_e, Sb);
       out[0] = make_float4((float)Sb[0], (float)Sb[1], (float)Sb[2], (float)Sb[3]);
       out[1] = make_float4((float)Sb[4], (float)Sb[5], (float)Sb[6], (float)Sb[7]);
     } else {

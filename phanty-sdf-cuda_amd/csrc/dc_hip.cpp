@@ -66,7 +66,7 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
-      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff;
+      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff, qstate;
   DcCounters* h_counters = nullptr;
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
@@ -337,6 +337,9 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
   if ((rc = ensure(c, c->flist, c->capF * 8, "face list"))) return rc;
   if ((rc = ensure(c, c->goff, 8 * (size_t)n, "grid offsets"))) return rc;
+  if (precision == DC_HIP_FP32_FALLBACK &&
+      (rc = ensure(c, c->qstate, c->capA * 4 * 24, "solve state")))
+    return rc;
   if (n > c->h_goff_cap) {
     if (c->h_goff) hipHostFree(c->h_goff);
     c->h_goff = nullptr;
@@ -468,6 +471,7 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   qa.pw = as<uint32_t>(c->pw);
   qa.nrm = c->fnrm.p;
   qa.capF = c->capF;
+  qa.qstate = precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
   DC_TRY(c, dc_launch_qef(qa, s), "qef");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
   DcEmitArgs ea;
@@ -629,7 +633,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
-                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff};
+                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff, &c->qstate};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -854,6 +858,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.ubar = as<float>(c->ubar);
   a.ubar_e = as<float>(c->ubar_e);
   a.contrib = as<float>(c->bwd_contrib);
+  a.qstate = c->precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
   a.grad_values = grad_values;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], s), "event");
   // capture key: every launch argument (pointers, sizes, block counts), the stream and the plan

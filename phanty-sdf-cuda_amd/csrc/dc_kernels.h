@@ -45,6 +45,7 @@ struct DcQefArgs {
   const uint32_t* pw;
   const void* nrm;  // per-face averaged normals (double in FP64 mode, float in FP32 mode)
   uint64_t capF;
+  float* qstate;    // FP32 mode: per-vertex solve state for the adjoint (24 floats), or null
 };
 
 struct DcFaceNormalArgs {
@@ -115,6 +116,7 @@ struct DcBwdArgs {
   float* ubar;    // 4 slots x 3 floats per face
   float* ubar_e;  // 12 slots x 3 floats per active cell
   float* contrib;  // 8 floats per active cell
+  const float* qstate;  // FP32 mode: the forward's per-vertex solve state (24 floats), or null
   float* grad_values;
 };
 
