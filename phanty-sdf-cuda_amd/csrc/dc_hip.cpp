@@ -52,9 +52,9 @@ struct dc_hip_context {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_layers = 32;
-  int eff_layers = 32;  // the batch's value (mark_layers or fewer for huge x-y layers)
-  int mark_waves = 1;
+  int mark_layers = 8;   // default tile: 8 cell layers x 4 waves per block (fastest at 512^3)
+  int eff_layers = 8;  // the batch's value (mark_layers or fewer for huge x-y layers)
+  int mark_waves = 4;
   int gather_rows = 1;
   int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
   int gather_lds_row = 128;

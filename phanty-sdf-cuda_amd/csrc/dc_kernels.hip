@@ -123,22 +123,25 @@ __global__ void __launch_bounds__(256)
   };
   float badacc = 0.f;  // becomes NaN once a NaN or Inf sample has been read
   // Masks of one sample layer for sample row j0 + lane over x: negative (N) and positive (P)
-  // samples from v; each row of layer sl_next is loaded into w right after the same row of v is
-  // taken, so a layer's loads stay in flight. v and w swap roles every layer (no copies).
+  // samples from v. All rows of layer sl_next are first loaded into w in one burst, so the whole
+  // layer is in flight while v is taken apart. v and w swap roles every layer (no copies).
   auto masks = [&](const float (&v)[kMarkRows + 1], float (&w)[kMarkRows + 1], float ev,
                    float& ew, int sl_next, uint64_t& N, uint64_t& P) {
     const int lsoff = (int)((int64_t)(sl_next - k0) * lbytes);
+    static_for<kMarkRows + 1>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      w[r] = ld(vx, lsoff + (j0 + r) * G.nx * 4);
+    });
+    ew = ld(ve, lsoff);
     uint32_t nlo = 0, nhi = 0, plo = 0, phi = 0;
     static_for<kMarkRows + 1>([&](auto rc) {
       constexpr int r = decltype(rc)::value;
       const uint64_t mn = __ballot(v[r] < 0.f);
       const uint64_t mp = __ballot(v[r] > 0.f);
       badacc = __builtin_fmaf(v[r], 0.f, badacc);
-      w[r] = ld(vx, lsoff + (j0 + r) * G.nx * 4);
       write_lane4<r>(nlo, nhi, plo, phi, (uint32_t)mn, (uint32_t)(mn >> 32), (uint32_t)mp,
                      (uint32_t)(mp >> 32));
     });
-    ew = ld(ve, lsoff);
     N = ((uint64_t)nhi << 32) | nlo;
     P = ((uint64_t)phi << 32) | plo;
     (void)ev;
