@@ -474,22 +474,17 @@ __device__ __forceinline__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3])
 // FP64 plain solve: Cell::update (Cell.cpp:284-361) with rows streamed in ascending edge order,
 // matching the CPU restatement's sums term by term.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act,
-                                const FaceNormals<double>& fn, int i, int j, int k, double x[3]) {
-  Cell<double> own;
-  load_cell(G, i, j, k, own);
+// The FP64 solve from the rows (hermite point p, final normal nf) of the cell's crossing edges in
+// ascending order; row(e, p, nf) provides edge e's row.
+template <class Row>
+__device__ __forceinline__ void solve_rows_fp64(uint32_t cross, Row&& row, double x[3]) {
   double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
   double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
   int m = 0;
-  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+  for (uint32_t em = cross; em; em &= em - 1) {
     const int e = __builtin_ctz(em);
-    double p[3];
-    hermite_point(own, e, p);
-    double nf[3];
-    uint64_t face;
-    bool avg;
-    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
-    if (tsqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
+    double p[3], nf[3];
+    row(e, p, nf);
     const double b = (nf[0] * p[0] + nf[1] * p[1]) + nf[2] * p[2];
     M00 = M00 + nf[0] * nf[0];
     M01 = M01 + nf[0] * nf[1];
@@ -528,6 +523,27 @@ __device__ __forceinline__ void solve_cell_fp64(const DcGridDev& G, const uint64
   x[0] = c0 + y0;
   x[1] = c1 + y1;
   x[2] = c2 + y2;
+}
+
+// The row of crossing edge e: hermite point and final (unit) normal.
+__device__ __forceinline__ void fp64_row(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                         const FaceNormals<double>& fn, int i, int j, int k,
+                                         const Cell<double>& own, int e, double p[3],
+                                         double nf[3]) {
+  hermite_point(own, e, p);
+  uint64_t face;
+  bool avg;
+  final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
+  if (tsqrt(sq3(nf[0], nf[1], nf[2])) > 0) normalize3(nf);
+}
+
+__device__ __forceinline__ void solve_cell_fp64(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                const FaceNormals<double>& fn, int i, int j, int k, double x[3]) {
+  Cell<double> own;
+  load_cell(G, i, j, k, own);
+  solve_rows_fp64(crossing_mask(own),
+                  [&](int e, double p[3], double nf[3]) { fp64_row(G, act, fn, i, j, k, own, e, p, nf); },
+                  x);
 }
 
 // Neumaier compensated sum.

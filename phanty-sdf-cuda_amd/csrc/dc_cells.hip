@@ -43,9 +43,7 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s) {
-  hipError_t e = dc_launch_bwd_cells(a, s);
-  if (e != hipSuccess) return e;
+hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s) {
   if (a.n_sr <= 0) return hipGetLastError();
   const dim3 grid((unsigned)((a.n_sr + 3) / 4));
   const size_t lds = 4 * ((size_t)a.gather_lds_row + 2 * kGatherCap) * 4;

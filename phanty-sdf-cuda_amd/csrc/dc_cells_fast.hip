@@ -18,40 +18,43 @@ hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
                      a.nrm, a.capF, a.qstate);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_qef_fallback, dim3((unsigned)std::max<int64_t>(1, a.blocks / 4)), block,
+  hipLaunchKernelGGL(k_qef_fallback, dim3((unsigned)std::max<int64_t>(1, a.blocks * 2)), block,
                      0, s, a.grids, a.n_grids, a.act_bounds, a.act, a.alist, a.ctr_rw, a.capA,
                      a.fblist, a.v32);
   return hipGetLastError();
 }
 
-hipError_t dc_launch_bwd_cells(const DcBwdArgs& a, hipStream_t s) {
+hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64) {
   const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
   const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
   const dim3 grid((unsigned)a.cell_blocks), block(256);
   const dim3 fgrid((unsigned)a.fallback_blocks);
-  hipError_t e;
 #define DC_BWD(PHASE)                                                                              \
   if (a.precision == DC_PREC_FP64) {                                                               \
     hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), grid, block, 0, s, a.grids, a.n_grids,         \
                        a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, nullptr, a.ctr,        \
                        a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr, a.ubar,     \
                        a.ubar_e, a.contrib);                                                       \
-  } else {                                                                                         \
-    hipLaunchKernelGGL((k_bwd_cell<PHASE, float>), grid, block, 0, s, a.grids, a.n_grids,          \
-                       a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 1, nullptr, a.ctr,        \
-                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.qstate, a.ubar,    \
-                       a.ubar_e, a.contrib);                                                       \
-    e = hipGetLastError();                                                                         \
-    if (e != hipSuccess) return e;                                                                 \
-    hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), fgrid, block, 0, s, a.grids, a.n_grids,        \
-                       a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, a.fblist, a.ctr,       \
-                       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr, a.ubar,     \
-                       a.ubar_e, a.contrib);                                                       \
+    return hipGetLastError();                                                                      \
   }                                                                                                \
-  e = hipGetLastError();                                                                           \
-  if (e != hipSuccess) return e;
-  DC_BWD(1)
+  /* the few FP64 cells first: their long per-thread chains start before the FP32 cells' */     \
+  /* blocks fill the device */                                                                     \
+  hipLaunchKernelGGL((k_bwd_cell<PHASE, double>), fgrid, block, 0, s64, a.grids, a.n_grids,        \
+                     a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, a.fblist, a.ctr, a.capA, \
+                     a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr, a.ubar, a.ubar_e,     \
+                     a.contrib);                                                                   \
+  {                                                                                                \
+    const hipError_t e = hipGetLastError();                                                        \
+    if (e != hipSuccess) return e;                                                                 \
+  }                                                                                                \
+  hipLaunchKernelGGL((k_bwd_cell<PHASE, float>), grid, block, 0, s, a.grids, a.n_grids,            \
+                     a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 1, nullptr, a.ctr, a.capA,  \
+                     a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.qstate, a.ubar, a.ubar_e,    \
+                     a.contrib);                                                                   \
+  return hipGetLastError();
+  if (phase == 1) {
+    DC_BWD(1)
+  }
   DC_BWD(2)
 #undef DC_BWD
-  return hipSuccess;
 }

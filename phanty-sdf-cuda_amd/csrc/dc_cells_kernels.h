@@ -174,6 +174,13 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
 }
 
 // FP32 mode: the listed cells solved in FP64 (inline averaged normals, the plain arithmetic).
+// 16 lanes per cell: lane e computes the row of edge e (an averaged normal takes up to 4 raw
+// normals), then every lane of the group sums the rows in ascending edge order through shuffles
+// (the serial solve's operation order) and solves; the group's first lane writes the vertex.
+// The lanes of a group share their cell and so their loop trip counts: the shuffles only read
+// lanes of their own, active, group.
+constexpr int kFbLanes = 16;
+
 __global__ void __launch_bounds__(256)
     k_qef_fallback(const DcGridDev* __restrict__ grids, int n_grids,
                    const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
@@ -181,19 +188,40 @@ __global__ void __launch_bounds__(256)
                    uint64_t cap, const uint32_t* __restrict__ fblist, float* __restrict__ v32) {
   const uint64_t n = min(ctr->fallback, cap);
   const FaceNormals<double> inline64{nullptr, nullptr, nullptr, nullptr, 0ull};
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (uint64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int sub = lane & (kFbLanes - 1);
+  const int base = lane - sub;
+  const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kFbLanes;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kFbLanes; t < n;
+       t += groups) {
     const uint32_t v = fblist[t];
     const uint64_t entry = alist[v];
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
     decode_cell(G, entry, i, j, k);
+    Cell<double> own;
+    load_cell(G, i, j, k, own);
+    const uint32_t cross = crossing_mask(own);
+    double p[3] = {0.0, 0.0, 0.0}, nf[3] = {0.0, 0.0, 0.0};
+    if (sub < 12 && ((cross >> sub) & 1u)) fp64_row(G, act, inline64, i, j, k, own, sub, p, nf);
     double x[3];
-    solve_cell_fp64(G, act, inline64, i, j, k, x);
-    v32[3 * (uint64_t)v + 0] = (float)x[0];
-    v32[3 * (uint64_t)v + 1] = (float)x[1];
-    v32[3 * (uint64_t)v + 2] = (float)x[2];
+    solve_rows_fp64(
+        cross,
+        [&](int e, double pe[3], double ne[3]) {
+          const int src = base + e;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            pe[d] = __shfl(p[d], src);
+            ne[d] = __shfl(nf[d], src);
+          }
+        },
+        x);
+    if (sub == 0) {
+      v32[3 * (uint64_t)v + 0] = (float)x[0];
+      v32[3 * (uint64_t)v + 1] = (float)x[1];
+      v32[3 * (uint64_t)v + 2] = (float)x[2];
+    }
   }
 }
 

@@ -72,6 +72,8 @@ struct dc_hip_context {
   int h_goff_cap = 0;
   hipEvent_t ev[kNumEvents] = {};
   hipEvent_t ev_join = nullptr;  // orders the context's stream after the null stream
+  hipStream_t side = nullptr;    // second stream for independent work beside the main one
+  hipEvent_t ev_fork = nullptr, ev_side = nullptr;
   // last forward
   bool have = false;
   int precision = 0;
@@ -612,6 +614,12 @@ int dc_hip_create(int device, dc_hip_context** out) {
       return DC_HIP_ERR_HIP;
     }
   }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming) != hipSuccess) {
+    dc_hip_destroy(c);
+    return DC_HIP_ERR_HIP;
+  }
   if (hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     dc_hip_destroy(c);
     return DC_HIP_ERR_HIP;
@@ -639,9 +647,15 @@ void dc_hip_destroy(dc_hip_context* c) {
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_side) hipEventDestroy(c->ev_side);
   drop_graph(c);
   if (c->h_counters) hipHostFree(c->h_counters);
   if (c->h_goff) hipHostFree(c->h_goff);
+  if (c->side) {
+    hipStreamSynchronize(c->side);
+    hipStreamDestroy(c->side);
+  }
   if (c->own) hipStreamDestroy(c->own);
   delete c;
 }
@@ -868,7 +882,22 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   key += c->dev_key;
   rc = run_captured(c, c->bwd, std::move(key), c->use_graph && !c->timing, s,
                     [&](hipStream_t st) -> int {
-                      DC_TRY(c, dc_launch_backward(a, st), "backward");
+                      // FP32 mode: the FP64 cells' kernels run beside the FP32 cells' on the side
+                      // stream (disjoint cells and slots), joined before the next phase
+                      const bool fork = a.precision != DC_PREC_FP64;
+                      for (int phase = 1; phase <= 2; ++phase) {
+                        if (fork) {
+                          DC_TRY(c, hipEventRecord(c->ev_fork, st), "fork");
+                          DC_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "fork");
+                        }
+                        DC_TRY(c, dc_launch_bwd_phase(a, phase, st, fork ? c->side : st),
+                               "backward cells");
+                        if (fork) {
+                          DC_TRY(c, hipEventRecord(c->ev_side, c->side), "join");
+                          DC_TRY(c, hipStreamWaitEvent(st, c->ev_side, 0), "join");
+                        }
+                      }
+                      DC_TRY(c, dc_launch_gather(a, st), "backward gather");
                       return DC_HIP_OK;
                     });
   if (rc) return rc;

@@ -146,8 +146,11 @@ hipError_t dc_launch_face_list(const uint64_t* ft, int64_t n_ft, uint32_t* tile_
 hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
-hipError_t dc_launch_backward(const DcBwdArgs& a, hipStream_t s);
+// Adjoint: per-cell phase 1 or 2 (FP32 mode: the FP32-solved cells on s, the FP64 cells on s64,
+// which may be another stream: the two sets are disjoint), then the gradient gather.
+hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64);
+hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s);
 // the contraction-enabled launchers (dc_cells_fast.hip)
 hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s);
-hipError_t dc_launch_bwd_cells(const DcBwdArgs& a, hipStream_t s);
+
