@@ -105,6 +105,13 @@ int dc_hip_result_counts(dc_hip_context* ctx, int64_t* n_vertices, int64_t* n_fa
  * 4*n_faces int32. Returns when the copies are complete. */
 int dc_hip_copy_result(dc_hip_context* ctx, float* vertices, double* vertices_f64,
                        int32_t* faces);
+/* The last result's quads split into triangles by the reference's rule, triangulate_v1
+ * (src/cpp/contouring.cpp:88-113, orientation :40-86): quad (a,b,c,d) -> triangles 2f = (a,b,c)
+ * and 2f+1 = (a,c,d), each with its last two vertices swapped when its normal points against the
+ * quad normal (b-a)x(d-a) (or (c-a)x(d-a) when that is degenerate), using the FP64 vertices in
+ * plain mode and the FP32 ones otherwise. triangles: 6*n_faces int32, host or device memory;
+ * vertex ids are grid-local as in the quads. Returns when the copy is complete. */
+int dc_hip_triangulate(dc_hip_context* ctx, int32_t* triangles);
 /* Device pointers of the last result, owned by the context and valid until its next call. */
 int dc_hip_result_device(dc_hip_context* ctx, const float** vertices,
                          const double** vertices_f64, const int32_t** faces);

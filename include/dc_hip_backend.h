@@ -31,6 +31,9 @@ class HipDualContouringBackend : public IDualContouringBackend {
   DualContouringMesh extract_device(const DenseSdfGridDevice& grid, DualContouringStats& stats);
   // FP64 vertices of the last extraction (plain FP64 mode).
   std::vector<double> last_vertices_f64();
+  // The last extraction's quads as triangles (3 ints each), split and oriented like
+  // triangulate_v1 (src/cpp/contouring.cpp:88-113).
+  std::vector<int> last_triangles();
   void set_option(const std::string& key, int64_t value);
   int precision() const { return precision_; }
 

@@ -37,6 +37,7 @@ def load():
     lib.oracle_dc_backward.argtypes = [P, P, P]
     lib.oracle_sym3_eig.argtypes = [P, P, P]
     lib.oracle_qef_solve.argtypes = [I, P, P, P]
+    lib.oracle_triangulate.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64, P]
     _lib = lib
     return lib
 
@@ -117,3 +118,13 @@ def qef_solve(normals, points):
     x = np.empty(3, np.float64)
     load().oracle_qef_solve(n.shape[0], _p(n), _p(p), _p(x))
     return x
+
+
+def triangulate(F, V):
+    """triangulate_v1 of the reference (contouring.cpp:88-113): quads (F x 4 int32) and vertices
+    (A x 3 doubles) -> 2F x 3 int32 triangles, each oriented with its quad's normal."""
+    F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 4)
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    T = np.empty((2 * F.shape[0], 3), np.int32)
+    load().oracle_triangulate(_p(F), F.shape[0], _p(V), V.shape[0], _p(T))
+    return T

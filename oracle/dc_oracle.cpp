@@ -823,6 +823,59 @@ void oracle_dc_backward(const OracleResult* R, const double* gV, double* gphi) {
 
 void oracle_dc_free(OracleResult* R) { delete R; }
 
+// triangulate_v1 (contouring.cpp:88-113) with orient_triangles_for_quad (contouring.cpp:40-86):
+// quad f = (a, b, c, d) -> triangles 2f = (a, b, c) and 2f+1 = (a, c, d). The quad normal is
+// (pb - pa) x (pd - pa), or (pc - pa) x (pd - pa) when the first has squared length < 1e-24; a
+// quad whose normal is still that short keeps both triangles as they are. Otherwise a triangle
+// whose normal (p2 - p1) x (p3 - p1) has a negative dot with the quad normal has its second and
+// third vertices swapped. V: nv x 3 doubles; T: 2 nf x 3 int32.
+void oracle_triangulate(const int32_t* F, int64_t nf, const double* V, int64_t nv, int32_t* T) {
+  (void)nv;
+  auto P = [&](int32_t q, double out[3]) {
+    out[0] = V[3 * (int64_t)q];
+    out[1] = V[3 * (int64_t)q + 1];
+    out[2] = V[3 * (int64_t)q + 2];
+  };
+  auto cross = [](const double u[3], const double w[3], double o[3]) {
+    o[0] = u[1] * w[2] - u[2] * w[1];
+    o[1] = u[2] * w[0] - u[0] * w[2];
+    o[2] = u[0] * w[1] - u[1] * w[0];
+  };
+  auto sub = [](const double u[3], const double w[3], double o[3]) {
+    o[0] = u[0] - w[0];
+    o[1] = u[1] - w[1];
+    o[2] = u[2] - w[2];
+  };
+  auto dot = [](const double u[3], const double w[3]) {
+    return (u[0] * w[0] + u[1] * w[1]) + u[2] * w[2];
+  };
+  for (int64_t f = 0; f < nf; ++f) {
+    const int32_t a = F[4 * f], b = F[4 * f + 1], c = F[4 * f + 2], d = F[4 * f + 3];
+    int32_t* t0 = T + 6 * f;
+    int32_t* t1 = t0 + 3;
+    t0[0] = a; t0[1] = b; t0[2] = c;
+    t1[0] = a; t1[1] = c; t1[2] = d;
+    double pa[3], pb[3], pc[3], pd[3], u[3], w[3], qn[3];
+    P(a, pa); P(b, pb); P(c, pc); P(d, pd);
+    sub(pb, pa, u);
+    sub(pd, pa, w);
+    cross(u, w, qn);
+    if (dot(qn, qn) < 1e-24) {
+      sub(pc, pa, u);
+      cross(u, w, qn);
+    }
+    if (dot(qn, qn) < 1e-24) continue;
+    for (int32_t* t : {t0, t1}) {
+      double p1[3], p2[3], p3[3], tn[3];
+      P(t[0], p1); P(t[1], p2); P(t[2], p3);
+      sub(p2, p1, u);
+      sub(p3, p1, w);
+      cross(u, w, tn);
+      if (dot(tn, qn) < 0.0) std::swap(t[1], t[2]);
+    }
+  }
+}
+
 void oracle_dc_counts(const OracleResult* R, int64_t* n_vertices, int64_t* n_faces,
                       int64_t* averaged_edges) {
   *n_vertices = (int64_t)R->active.size();

@@ -26,6 +26,7 @@ EXPORTS = (
     "dc_hip_create", "dc_hip_destroy", "dc_hip_set_stream", "dc_hip_set_option",
     "dc_hip_extract", "dc_hip_extract_device", "dc_hip_result_counts", "dc_hip_copy_result",
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
+    "dc_hip_triangulate",
     "dc_hip_last_error", "dc_hip_status_string",
 )
 
@@ -83,6 +84,7 @@ def load_hip():
         "dc_hip_extract_batch": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, P, P,
                                      ctypes.POINTER(DcHipStats)]),
         "dc_hip_backward": (I, [P, P, P, ctypes.POINTER(DcHipStats)]),
+        "dc_hip_triangulate": (I, [P, P]),
         "dc_hip_autotune": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, ctypes.c_int32,
                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                 ctypes.POINTER(ctypes.c_float)]),

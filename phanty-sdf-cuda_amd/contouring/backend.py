@@ -270,6 +270,17 @@ class HipDualContouringBackend:
             out.append(item)
         return out, DualContouringStats.from_c(st)
 
+    def triangles(self):
+        """The last extraction's quads as 2F x 3 int32 triangles on the device, split and oriented
+        like the reference's triangulate_v1 (src/cpp/contouring.cpp:88-113)."""
+        import torch
+        _, nf = self.counts()
+        T = torch.empty((2 * nf, 3), dtype=torch.int32, device=torch.device("cuda", self.device))
+        if nf:
+            self._use_torch_stream()
+            self._check(self._lib.dc_hip_triangulate(self._ctx, ctypes.c_void_p(T.data_ptr())))
+        return T
+
     def backward(self, grad_vertices, grad_phi):
         """dL/dphi (written into grad_phi, float32 CUDA) for dL/dV of the last forward."""
         import torch

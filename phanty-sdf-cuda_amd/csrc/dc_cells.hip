@@ -43,6 +43,16 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t dc_launch_triangulate(const int32_t* faces, uint64_t nf, int n_grids,
+                                 const uint32_t* goff, const double* v64, const float* v32,
+                                 int use64, int32_t* tris, int64_t blocks, hipStream_t s) {
+  if (nf == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_triangulate, dim3((unsigned)blocks), dim3(256), 0, s,
+                     reinterpret_cast<const int4*>(faces), nf, n_grids, goff, v64, v32, use64,
+                     tris);
+  return hipGetLastError();
+}
+
 hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s) {
   if (a.n_sr <= 0) return hipGetLastError();
   const dim3 grid((unsigned)((a.n_sr + 3) / 4));

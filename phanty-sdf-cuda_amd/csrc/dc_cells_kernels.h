@@ -329,6 +329,101 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Triangles
+// ---------------------------------------------------------------------------------------------
+// triangulate_v1 (contouring.cpp:88-113) with orient_triangles_for_quad (contouring.cpp:40-86),
+// one thread per quad: (a, b, c, d) -> (a, b, c), (a, c, d); the quad normal is
+// (pb - pa) x (pd - pa), or (pc - pa) x (pd - pa) when that has squared length < 1e-24; a quad
+// whose normal is still that short keeps both triangles; otherwise a triangle whose normal points
+// against the quad normal has its last two vertices swapped. Positions are the FP64 vertices
+// (plain mode) or the FP32 ones widened. In a batch, quad f's vertex ids are local to its grid:
+// goff holds each grid's (first vertex, first face).
+__device__ __forceinline__ void tri_sub(const double u[3], const double w[3], double o[3]) {
+  o[0] = u[0] - w[0];
+  o[1] = u[1] - w[1];
+  o[2] = u[2] - w[2];
+}
+__device__ __forceinline__ void tri_cross(const double u[3], const double w[3], double o[3]) {
+  o[0] = u[1] * w[2] - u[2] * w[1];
+  o[1] = u[2] * w[0] - u[0] * w[2];
+  o[2] = u[0] * w[1] - u[1] * w[0];
+}
+__device__ __forceinline__ double tri_dot(const double u[3], const double w[3]) {
+  return (u[0] * w[0] + u[1] * w[1]) + u[2] * w[2];
+}
+
+__global__ void __launch_bounds__(256)
+    k_triangulate(const int4* __restrict__ faces, uint64_t nf, int n_grids,
+                  const uint32_t* __restrict__ goff, const double* __restrict__ v64,
+                  const float* __restrict__ v32, int use64, int32_t* __restrict__ tris) {
+  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+       f += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t vb = 0;
+    if (n_grids > 1) {  // last grid whose first face is <= f
+      int lo = 0, hi = n_grids - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)goff[2 * mid + 1] <= f)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      vb = goff[2 * lo];
+    }
+    const int4 q = faces[f];
+    auto P = [&](int32_t id, double o[3]) {
+      const uint64_t b = 3 * (vb + (uint64_t)id);
+      if (use64) {
+        o[0] = v64[b];
+        o[1] = v64[b + 1];
+        o[2] = v64[b + 2];
+      } else {
+        o[0] = (double)v32[b];
+        o[1] = (double)v32[b + 1];
+        o[2] = (double)v32[b + 2];
+      }
+    };
+    int32_t t[2][3] = {{q.x, q.y, q.z}, {q.x, q.z, q.w}};
+    double pa[3], pb[3], pc[3], pd[3], u[3], w[3], qn[3];
+    P(q.x, pa);
+    P(q.y, pb);
+    P(q.z, pc);
+    P(q.w, pd);
+    tri_sub(pb, pa, u);
+    tri_sub(pd, pa, w);
+    tri_cross(u, w, qn);
+    if (tri_dot(qn, qn) < 1e-24) {
+      tri_sub(pc, pa, u);
+      tri_cross(u, w, qn);
+    }
+    if (tri_dot(qn, qn) >= 1e-24) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        double p1[3], p2[3], p3[3], tn[3];
+        P(t[r][0], p1);
+        P(t[r][1], p2);
+        P(t[r][2], p3);
+        tri_sub(p2, p1, u);
+        tri_sub(p3, p1, w);
+        tri_cross(u, w, tn);
+        if (tri_dot(tn, qn) < 0.0) {
+          const int32_t x = t[r][1];
+          t[r][1] = t[r][2];
+          t[r][2] = x;
+        }
+      }
+    }
+    int32_t* o = tris + 6 * f;
+    o[0] = t[0][0];
+    o[1] = t[0][1];
+    o[2] = t[0][2];
+    o[3] = t[1][0];
+    o[4] = t[1][1];
+    o[5] = t[1][2];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Adjoint
 // ---------------------------------------------------------------------------------------------
 // Cells processed: every active cell (list == nullptr; with skip_flagged the FP32 kernel leaves

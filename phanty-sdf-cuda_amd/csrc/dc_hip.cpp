@@ -66,7 +66,8 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
-      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff, qstate;
+      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff, qstate,
+      tris;
   DcCounters* h_counters = nullptr;
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
@@ -641,7 +642,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
-                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff, &c->qstate};
+                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff, &c->qstate, &c->tris};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -784,6 +785,30 @@ int dc_hip_autotune(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_grids
   c->mark_waves = bw;
   if (best_layers) *best_layers = bl;
   if (best_waves) *best_waves = bw;
+  return DC_HIP_OK;
+}
+
+int dc_hip_triangulate(dc_hip_context* c, int32_t* triangles) {
+  if (!c || !triangles) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (c->nf == 0) return DC_HIP_OK;
+  int rc;
+  if ((rc = ensure(c, c->tris, (size_t)c->nf * 24, "triangles", false))) return rc;
+  hipStream_t s;
+  if ((rc = work_stream(c, &s))) return rc;
+  const int n = (int)c->hgrids.size();
+  const int64_t blocks =
+      std::max<int64_t>(1, std::min<int64_t>((c->nf + 255) / 256, (int64_t)c->cu_count * 16));
+  DC_TRY(c, dc_launch_triangulate(as<int32_t>(c->faces), (uint64_t)c->nf, n,
+                                  as<uint32_t>(c->goff), as<double>(c->v64), as<float>(c->v32),
+                                  c->precision == DC_HIP_PLAIN_FP64 ? 1 : 0, as<int32_t>(c->tris),
+                                  blocks, s),
+         "triangulate");
+  // hipMemcpyDefault: the destination may be host or device memory (unified addressing)
+  DC_TRY(c, hipMemcpyAsync(triangles, c->tris.p, (size_t)c->nf * 24, hipMemcpyDefault, s),
+         "copy triangles");
+  DC_TRY(c, hipStreamSynchronize(s), "triangulate");
   return DC_HIP_OK;
 }
 

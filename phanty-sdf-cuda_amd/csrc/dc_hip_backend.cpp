@@ -109,3 +109,11 @@ std::vector<double> HipDualContouringBackend::last_vertices_f64() {
   if (nv) check(dc_hip_copy_result(ctx_, nullptr, v.data(), nullptr));
   return v;
 }
+
+std::vector<int> HipDualContouringBackend::last_triangles() {
+  int64_t nv = 0, nf = 0;
+  check(dc_hip_result_counts(ctx_, &nv, &nf));
+  std::vector<int> t(6 * (size_t)nf);
+  if (nf) check(dc_hip_triangulate(ctx_, t.data()));
+  return t;
+}

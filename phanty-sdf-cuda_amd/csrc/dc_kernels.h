@@ -150,6 +150,10 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
 // which may be another stream: the two sets are disjoint), then the gradient gather.
 hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64);
 hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s);
+// triangulate_v1 of the last mesh: 2 nf x 3 int32 (goff: per-grid first vertex and face, n > 1).
+hipError_t dc_launch_triangulate(const int32_t* faces, uint64_t nf, int n_grids,
+                                 const uint32_t* goff, const double* v64, const float* v32,
+                                 int use64, int32_t* tris, int64_t blocks, hipStream_t s);
 // the contraction-enabled launchers (dc_cells_fast.hip)
 hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s);

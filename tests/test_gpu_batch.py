@@ -40,12 +40,17 @@ def test_batch_slices_equal_single_extractions(precision):
     assert len(pairs) == len(grids)
     assert st.vertex_count == sum(int(p[0].shape[0]) for p in pairs)
     assert st.face_count == sum(int(p[1].shape[0]) for p in pairs)
+    T_all = be.triangles()  # grid-local ids, 2 triangles per quad, grid-major
     single = HipDualContouringBackend(precision)
+    f0 = 0
     for g, d, item in zip(grids, dev, pairs):
         (V, F), _ = single.extract_tensor(d.d_values.reshape(g.nz, g.ny, g.nx),
                                           (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
         assert torch.equal(item[0], V), g.name
         assert torch.equal(item[1], F), g.name
+        nf = int(F.shape[0])
+        assert torch.equal(T_all[2 * f0:2 * (f0 + nf)], single.triangles()), g.name
+        f0 += nf
 
 
 def test_batch_plain_matches_oracle():

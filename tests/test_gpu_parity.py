@@ -59,3 +59,16 @@ def test_repeat_is_deterministic(hip_backend):
     m2, _ = hip_backend.extract(grid)
     np.testing.assert_array_equal(m1.faces, m2.faces)
     np.testing.assert_array_equal(m1.vertices_f64, m2.vertices_f64)
+
+
+@pytest.mark.parametrize("idx", [0, 1, 3, 6, 10, 13])
+def test_triangles_match_oracle(hip_backend, idx):
+    """dc_hip_triangulate vs the oracle's triangulate_v1 on the same quads and FP64 vertices."""
+    from contouring import DenseSdfGrid
+    g = small_suite()[idx]
+    print(f"[triangles] {g.name}", flush=True)
+    ref = oracle.run_grid(g)
+    grid = DenseSdfGrid(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, g.values)
+    mesh, _ = hip_backend.extract(grid)
+    T = hip_backend.triangles().cpu().numpy()
+    np.testing.assert_array_equal(T, oracle.triangulate(ref.F, ref.V64), err_msg=g.name)
