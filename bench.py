@@ -66,8 +66,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
-    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4", "c5"),
-                    help="c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
+    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"),
+                    help="c1: 64^3 spheres with oracle parity; "
+                         "c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
                          "FP32+fallback; c4: 256 uneven grids, one batch vs sequential; c5: "
                          "forward+adjoint at 128^3 and 256^3")
     ap.add_argument("--calibrate", action="store_true",
@@ -85,7 +86,7 @@ def main():
     from contouring import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
 
     if args.config != "c2":
-        {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
         return
 
     n = args.size
@@ -289,6 +290,48 @@ def bench_c5(args):
         log(f"[c5] {n}^3: plain {plain:.3f} ms, fp32 fwd+bwd {both:.3f} ms ({both / plain:.2f}x)")
     print(json.dumps({"metric": "fwd+adjoint ms over plain fwd ms (csg8)", "config": "c5",
                       "unit": "ms", "results": rows, "higher_is_better": False}), flush=True)
+
+
+def bench_c1(args):
+    """64^3 spheres: the unit-cube sphere (r 0.35) and dc_cli --generate sphere -g 64 ([-1,1]^3,
+    r 0.5). Plain FP64 ms per extraction (replayed launch sequence), active cells/s, and parity
+    with the oracle: quads and FP32 vertices exact, FP64 vertices within 1e-6 h."""
+    import numpy as np
+    import torch
+    from contouring import HipDualContouringBackend, _lib, synth
+    import oracle
+    rows = {}
+    for g in (synth.sphere_unit(64), synth.cli_generate("sphere", 64)):
+        phi, _ = _device_grid(g)
+        be = HipDualContouringBackend("plain-fp64", device=0)
+        be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+        be.set_option("timing", 0)
+        cg = be._cgrid(g, phi.data_ptr())
+        st = _lib.DcHipStats()
+        lib = be._lib
+        ms = _timed(lambda: be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg),
+                                                                _lib.DC_HIP_PLAIN_FP64,
+                                                                ctypes.byref(st))),
+                    args.steps, max(args.warmup, 3))
+        nv, nf = int(st.vertex_count), int(st.face_count)
+        V32 = np.empty((nv, 3), np.float32)
+        V64 = np.empty((nv, 3), np.float64)
+        F = np.empty((nf, 4), np.int32)
+        be._check(lib.dc_hip_copy_result(be._ctx, V32.ctypes.data_as(ctypes.c_void_p),
+                                         V64.ctypes.data_as(ctypes.c_void_p),
+                                         F.ctypes.data_as(ctypes.c_void_p)))
+        ref = oracle.run_grid(g)
+        h = min(g.vx, g.vy, g.vz)
+        ok = (nv == ref.n_vertices and nf == ref.n_faces and np.array_equal(F, ref.F)
+              and np.array_equal(V32, ref.V32) and float(np.abs(V64 - ref.V64).max()) <= 1e-6 * h)
+        rows[g.name] = {"ms": ms, "vertices": nv, "faces": nf,
+                        "active_cells_per_s": nv / (ms * 1e-3), "parity": bool(ok),
+                        "launch": "hipGraph replay" if st.graph_mode == 2 else "eager"}
+        log(f"[c1] {g.name}: {ms:.3f} ms, {nv} vertices, {nv / (ms * 1e-3) / 1e6:.1f} M active "
+            f"cells/s, parity {ok}")
+    print(json.dumps({"metric": "64^3 sphere DC ms (plain FP64), parity with the oracle",
+                      "config": "c1", "unit": "ms", "results": rows, "higher_is_better": False}),
+          flush=True)
 
 
 def bench_c3(args):
