@@ -707,8 +707,13 @@ int dc_hip_extract_device(dc_hip_context* c, const dc_hip_grid* grid, int precis
 int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
                    dc_hip_stats* stats) {
   if (!c || !grid || !grid->values) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  // everything the upload's size depends on is checked before the host buffer is read
   if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimensions must be >= 2");
+  if (grid->nx > 65536 || grid->ny > 65536 || grid->nz > 65536)
+    return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimension above 65536");
+  if (precision != DC_HIP_PLAIN_FP64 && precision != DC_HIP_FP32_FALLBACK)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "unknown precision");
   DC_TRY(c, hipSetDevice(c->device), "set device");
   const size_t bytes = (size_t)grid->nx * grid->ny * grid->nz * sizeof(float);
   int rc = ensure(c, c->upload, bytes, "upload buffer", false);  // in the plan key as values
