@@ -229,21 +229,34 @@ __global__ void __launch_bounds__(256)
 // Quads (contouring.cpp:607-729)
 // ---------------------------------------------------------------------------------------------
 // contouring.cpp:141-207 gradientAt: central differences with clamped neighbours, normalized.
-__device__ __forceinline__ void grid_gradient(const DcGridDev& G, int i, int j, int k,
-                                              double out[3]) {
+// The six central-difference samples of grid point (i,j,k) (clamped at the border): x+, x-, y+,
+// y-, z+, z-. Loaded unconditionally, so a caller can have them in flight with other loads.
+__device__ __forceinline__ void gradient_samples(const DcGridDev& G, int i, int j, int k,
+                                                 float f[6]) {
   const int im = max(i - 1, 0), ip = min(i + 1, G.nx - 1);
   const int jm = max(j - 1, 0), jp = min(j + 1, G.ny - 1);
   const int km = max(k - 1, 0), kp = min(k + 1, G.nz - 1);
-  out[0] = out[1] = out[2] = 0.0;
+  f[0] = *sample_ptr(G, ip, j, k);
+  f[1] = *sample_ptr(G, im, j, k);
+  f[2] = *sample_ptr(G, i, jp, k);
+  f[3] = *sample_ptr(G, i, jm, k);
+  f[4] = *sample_ptr(G, i, j, kp);
+  f[5] = *sample_ptr(G, i, j, km);
+}
+
+// contouring.cpp:141-207 (gradientAt): the normalized central-difference gradient at grid point
+// (i,j,k) from its samples f (gradient_samples); a component whose spacing is below 1e-12 is 0.
+__device__ __forceinline__ void grid_gradient_from(const DcGridDev& G, int i, int j, int k,
+                                                   const float f[6], double out[3]) {
+  const int im = max(i - 1, 0), ip = min(i + 1, G.nx - 1);
+  const int jm = max(j - 1, 0), jp = min(j + 1, G.ny - 1);
+  const int km = max(k - 1, 0), kp = min(k + 1, G.nz - 1);
   const double dx = coordd(G.ox, ip, G.vx) - coordd(G.ox, im, G.vx);
-  if (fabs(dx) > 1e-12)
-    out[0] = ((double)*sample_ptr(G, ip, j, k) - (double)*sample_ptr(G, im, j, k)) / dx;
   const double dy = coordd(G.oy, jp, G.vy) - coordd(G.oy, jm, G.vy);
-  if (fabs(dy) > 1e-12)
-    out[1] = ((double)*sample_ptr(G, i, jp, k) - (double)*sample_ptr(G, i, jm, k)) / dy;
   const double dz = coordd(G.oz, kp, G.vz) - coordd(G.oz, km, G.vz);
-  if (fabs(dz) > 1e-12)
-    out[2] = ((double)*sample_ptr(G, i, j, kp) - (double)*sample_ptr(G, i, j, km)) / dz;
+  out[0] = fabs(dx) > 1e-12 ? ((double)f[0] - (double)f[1]) / dx : 0.0;
+  out[1] = fabs(dy) > 1e-12 ? ((double)f[2] - (double)f[3]) / dy : 0.0;
+  out[2] = fabs(dz) > 1e-12 ? ((double)f[4] - (double)f[5]) / dz : 0.0;
   const double nrm = sqrt(sq3(out[0], out[1], out[2]));
   if (nrm > 0.0) {
     out[0] = out[0] / nrm;
@@ -272,31 +285,25 @@ __global__ void __launch_bounds__(256)
   if (ctr->n_active > capA || ctr->n_faces > capF) return;
   each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces,
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
+    // the edge's far endpoint; the quad's cells: c0 = (u-1, v-1), c1 = (u, v-1), c2 = (u, v),
+    // c3 = (u-1, v) over the two axes (u, v) across the edge (x: (y, z); y: (x, z); z: (x, y))
+    const int e1x = i + (a == 0), e1y = j + (a == 1), e1z = k + (a == 2);
+    const int ux = (a != 0), uy = (a == 0);  // unit step of u
+    const int vy = (a == 2), vz = (a != 2);  // unit step of v
+    // all loads of the quad are issued before its arithmetic: the stencil samples of both
+    // endpoints, then the four cells' act words and prefixes
+    float f0[6], f1[6];
+    gradient_samples(G, i, j, k, f0);
+    gradient_samples(G, e1x, e1y, e1z, f1);
     const uint32_t vbase = wprefix[G.act_off];
     const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
-    int c0, c1, c2, c3, e1[3];
-    if (a == 0) {
-      c0 = vertex_id(G, act, wprefix, vbase, i, j - 1, k - 1);
-      c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
-      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-      c3 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
-      e1[0] = i + 1; e1[1] = j; e1[2] = k;
-    } else if (a == 1) {
-      c0 = vertex_id(G, act, wprefix, vbase, i - 1, j, k - 1);
-      c1 = vertex_id(G, act, wprefix, vbase, i, j, k - 1);
-      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-      c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
-      e1[0] = i; e1[1] = j + 1; e1[2] = k;
-    } else {
-      c0 = vertex_id(G, act, wprefix, vbase, i - 1, j - 1, k);
-      c1 = vertex_id(G, act, wprefix, vbase, i, j - 1, k);
-      c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-      c3 = vertex_id(G, act, wprefix, vbase, i - 1, j, k);
-      e1[0] = i; e1[1] = j; e1[2] = k + 1;
-    }
+    int c0 = vertex_id(G, act, wprefix, vbase, i - ux, j - uy - vy, k - vz);
+    int c1 = vertex_id(G, act, wprefix, vbase, i, j - vy, k - vz);
+    const int c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
+    int c3 = vertex_id(G, act, wprefix, vbase, i - ux, j - uy, k);
     double g0[3], g1[3];
-    grid_gradient(G, i, j, k, g0);
-    grid_gradient(G, e1[0], e1[1], e1[2], g1);
+    grid_gradient_from(G, i, j, k, f0, g0);
+    grid_gradient_from(G, e1x, e1y, e1z, f1, g1);
     const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
     double pa[3], pb[3], pc[3], pd[3];
     const int64_t vv[4] = {vglob + c0, vglob + c1, vglob + c2, vglob + c3};
