@@ -138,8 +138,9 @@ __device__ __forceinline__ bool strict_change(float fa, float fb) {
   return (fa < 0.f && fb > 0.f) || (fa > 0.f && fb < 0.f);
 }
 
-// S[] holds the corner values for constant-index use; s(q) for a run-time corner index re-reads
-// the sample through the cache (indexing the register array would move it to scratch memory).
+// S[] holds the corner values; s(q) for a run-time corner index selects from them with a chain
+// of compares (indexing the register array would move it to scratch memory, and re-reading the
+// sample would wait on memory again).
 template <typename T>
 struct Cell {
   T S[8];
@@ -318,21 +319,43 @@ __device__ __forceinline__ bool edge_group(const DcGridDev& G, const uint64_t* _
 }
 
 // Sum of the group's nonzero raw normals in process_edge order.
-template <typename T>
+template <typename T, bool kPreload = false>
 __device__ __forceinline__ void group_sum(const DcGridDev& G, int axis, int x0, int y0, int z0,
                                           T acc[3]) {
   acc[0] = acc[1] = acc[2] = T(0);
-  for (int q = 0; q < 4; ++q) {
-    int x, y, z;
-    group_member(axis, q, x0, y0, z0, x, y, z);
-    Cell<T> nc;
-    load_cell(G, x, y, z, nc);
-    T n[3];
-    raw_normal(nc, group_edge(axis, q), n);
-    if (sq3(n[0], n[1], n[2]) > T(0)) {
-      acc[0] = acc[0] + n[0];
-      acc[1] = acc[1] + n[1];
-      acc[2] = acc[2] + n[2];
+  if constexpr (kPreload) {
+    // all four members' corners in flight before any normal is formed (more registers: used
+    // where the average is all the thread does)
+    Cell<T> nc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int x, y, z;
+      group_member(axis, q, x0, y0, z0, x, y, z);
+      load_cell(G, x, y, z, nc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T n[3];
+      raw_normal(nc[q], group_edge(axis, q), n);
+      if (sq3(n[0], n[1], n[2]) > T(0)) {
+        acc[0] = acc[0] + n[0];
+        acc[1] = acc[1] + n[1];
+        acc[2] = acc[2] + n[2];
+      }
+    }
+  } else {
+    for (int q = 0; q < 4; ++q) {
+      int x, y, z;
+      group_member(axis, q, x0, y0, z0, x, y, z);
+      Cell<T> nc;
+      load_cell(G, x, y, z, nc);
+      T n[3];
+      raw_normal(nc, group_edge(axis, q), n);
+      if (sq3(n[0], n[1], n[2]) > T(0)) {
+        acc[0] = acc[0] + n[0];
+        acc[1] = acc[1] + n[1];
+        acc[2] = acc[2] + n[2];
+      }
     }
   }
 }

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256)
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
               if (face >= capF) return;
               T n[3];
-              group_sum(G, a, i, j, k, n);
+              group_sum<T, true>(G, a, i, j, k, n);
               const T len = tsqrt(sq3(n[0], n[1], n[2]));
               normalize3(n);
               nrm[4 * face + 0] = n[0];
