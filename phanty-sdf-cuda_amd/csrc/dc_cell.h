@@ -591,11 +591,21 @@ struct Qef {
   int m;
 };
 
+// Kinds of a cell's rows, recorded for the adjoint: a face id (the row's normal is that face's
+// average), or one of these.
+constexpr uint32_t kRowOwn = 0xffffffffu;       // not averaged: the cell's own normal
+constexpr uint32_t kRowEdgeSlot = 0xfffffffeu;  // averaged, without a face
+// Threads of the FP32 solve's blocks: stride of a thread's kinds column in LDS.
+constexpr int kKindStride = 256;
+
 // Mass-point-centred assembly in the cell frame: centroid c, M = sum n n^T, g = sum n (n.(p-c)),
 // Neumaier-compensated. Returns false when a final normal is degenerate (then solve in FP64).
+// kinds (or null): each crossing row's kind for the adjoint (the face id, kRowEdgeSlot or kRowOwn)
+// at kinds[e * kKindStride] (the caller's LDS column).
 __device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
                               const FaceNormals<float>& fn, int i, int j, int k,
-                              const Cell<float>& own, Qef<float>& Q) {
+                              const Cell<float>& own, Qef<float>& Q,
+                              uint32_t* __restrict__ kinds) {
   NSum s0, s1, s2;
   int m = 0;
   for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
@@ -622,6 +632,7 @@ __device__ __forceinline__ bool assemble_fp32(const DcGridDev& G, const uint64_t
     uint64_t face;
     bool avg;
     final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
+    if (kinds) kinds[e * kKindStride] = !avg ? kRowOwn : (face != ~0ull ? (uint32_t)face : kRowEdgeSlot);
     if (!(sq3(nf[0], nf[1], nf[2]) > 0.5f)) ok = false;  // zero or degenerate normal
     normalize3(nf);
     const float q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
@@ -694,14 +705,15 @@ __device__ __forceinline__ void load_state(const float* __restrict__ src, Qef<fl
 }
 
 // FP32 solve in the cell frame. Returns false when the cell must be solved in FP64; otherwise
-// stores the solve's state for the adjoint at `state` (kQefState floats) unless it is null.
+// stores the solve's state for the adjoint at `state` (kQefState floats) unless it is null. The
+// row kinds go to `kinds` (unless null) either way.
 __device__ __forceinline__ bool solve_cell_fp32(const DcGridDev& G, const uint64_t* __restrict__ act,
                                 const FaceNormals<float>& fn, int i, int j, int k, float xl[3],
-                                float* __restrict__ state) {
+                                float* __restrict__ state, uint32_t* __restrict__ kinds) {
   Cell<float> own;
   load_cell(G, i, j, k, own);
   Qef<float> Q;
-  if (!assemble_fp32(G, act, fn, i, j, k, own, Q)) return false;
+  if (!assemble_fp32(G, act, fn, i, j, k, own, Q, kinds)) return false;
   float lam[3], V[3][3];
   sym3_eig(Q.M, lam, V);
   float y[3] = {0.f, 0.f, 0.f};
@@ -888,25 +900,55 @@ __device__ __forceinline__ int member_of(int axis, int i, int j, int k, int x0, 
 // stores u-bar for phase 2 (in the face's slot for this member, ubar[4*face + member], or, for
 // an averaged edge without a face, in the cell's per-edge slot ubar_e[12*v + e]); the point
 // adjoints and the other rows' normals go to the cell's own 8 corners, Sb.
-// state: the forward's FP32 solve state of this cell (T = float), or null to assemble and solve
-// again here.
+// A row's kind: the face id, kRowEdgeSlot (averaged, no face) or kRowOwn (not averaged). T = float
+// (FP32 mode): state is the forward's solve state of this cell and rows[12v + e] the kinds the
+// forward recorded; the first kPre face rows' normals are loaded together before the first row is
+// processed. T = double: assembled and solved again here; the kinds go to kinds[e * kKindStride]
+// (the caller's LDS column) unless it is null.
+
+// Entry e of a cell's 12 row kinds held as three uint4.
+__device__ __forceinline__ uint32_t row_kind(const uint4 r[3], int e) {
+  const uint4 q = e < 4 ? r[0] : (e < 8 ? r[1] : r[2]);
+  const int c = e & 3;
+  return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
+}
+
+// Crossing rows with a face (fmask) and with an edge slot (smask), from the 12 kinds.
+__device__ __forceinline__ void kind_masks(const uint4 kinds[3], uint32_t cross, uint32_t& fmask,
+                                           uint32_t& smask) {
+  fmask = smask = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const uint32_t kind = row_kind(kinds, e);
+    fmask |= (uint32_t)(kind < kRowEdgeSlot) << e;
+    smask |= (uint32_t)(kind == kRowEdgeSlot) << e;
+  }
+  fmask &= cross;
+  smask &= cross;
+}
+
 template <typename T>
 __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
                               const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
                               const float xb_in[3], const float* __restrict__ state,
-                              float* __restrict__ ubar, float* __restrict__ ubar_e, T Sb[8]) {
+                              float* __restrict__ ubar, float* __restrict__ ubar_e,
+                              const uint32_t* __restrict__ rows, uint32_t* __restrict__ kinds_out,
+                              T Sb[8]) {
+  constexpr bool kGiven = sizeof(T) == 4;  // solve state and row kinds from the forward
+  uint4 kinds[3];
+  if constexpr (kGiven) {
+    const uint4* r4 = reinterpret_cast<const uint4*>(rows + 12 * v);
+    kinds[0] = r4[0];
+    kinds[1] = r4[1];
+    kinds[2] = r4[2];
+  }
   Cell<T> own;
   load_cell(G, i, j, k, own);
   Qef<T> Q;
   T lam[3], V[3][3];
-  if constexpr (sizeof(T) == 4) {
-    if (state) {
-      load_state(state, Q, lam, V);
-      Q.m = __popc(crossing_mask(own));
-    } else {
-      assemble_any(G, act, fn, i, j, k, own, Q);
-      sym3_eig(Q.M, lam, V);
-    }
+  if constexpr (kGiven) {
+    load_state(state, Q, lam, V);
+    Q.m = __popc(crossing_mask(own));
   } else {
     assemble_any(G, act, fn, i, j, k, own, Q);
     sym3_eig(Q.M, lam, V);
@@ -967,13 +1009,10 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
     cbm[a] = dv(xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2]), T(Q.m));
 #pragma unroll
   for (int q = 0; q < 8; ++q) Sb[q] = T(0);
-  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
-    const int e = __builtin_ctz(em);
-    T p[3], nf[3];
+  // one row: final normal nf; slot: where an averaged row's u-bar goes (null: not averaged)
+  auto row = [&](int e, const T nf[3], float* slot) {
+    T p[3];
     hermite_point(own, e, p);
-    uint64_t face;
-    bool avg;
-    final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
     T n[3] = {nf[0], nf[1], nf[2]};
     normalize3(n);
     const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
@@ -987,60 +1026,172 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
     }
     T ub[3];
     normalize_vjp(nf, nb, ub);  // the solve re-normalizes each row
-    if (!avg) {
+    if (!slot) {
       hermite_vjp(own, e, ub, pb, Sb);
-      continue;
+      return;
     }
     lerp_vjp(own, e, pb, Sb);  // the point enters the solve directly
-    float* slot;
-    if (face != ~0ull) {
-      int axis, x0, y0, z0;
-      edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
-      slot = ubar + 3 * (4 * face + member_of(axis, i, j, k, x0, y0, z0));
-    } else {
-      slot = ubar_e + 3 * (12 * v + e);
-    }
     slot[0] = (float)ub[0];
     slot[1] = (float)ub[1];
     slot[2] = (float)ub[2];
+  };
+  auto face_slot = [&](int e, uint32_t face) {
+    int axis, x0, y0, z0;  // the group's geometry (the row is averaged)
+    edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+    return ubar + 3 * (4 * (uint64_t)face + member_of(axis, i, j, k, x0, y0, z0));
+  };
+  const uint32_t cross = crossing_mask(own);
+  if constexpr (kGiven) {
+    constexpr int kPre = 3;
+    uint32_t fmask, smask;
+    kind_masks(kinds, cross, fmask, smask);
+    T pre[kPre][3];
+    uint32_t pf[kPre];
+    {
+      uint32_t pm = fmask;
+#pragma unroll
+      for (int q = 0; q < kPre; ++q) {
+        pre[q][0] = pre[q][1] = pre[q][2] = T(0);
+        pf[q] = 0;
+        if (pm) {
+          pf[q] = row_kind(kinds, __builtin_ctz(pm));
+          pre[q][0] = fn.at(4 * (uint64_t)pf[q] + 0);
+          pre[q][1] = fn.at(4 * (uint64_t)pf[q] + 1);
+          pre[q][2] = fn.at(4 * (uint64_t)pf[q] + 2);
+          pm &= pm - 1;
+        }
+      }
+    }
+    int q = 0;  // face rows seen
+    for (uint32_t em = cross; em; em &= em - 1) {
+      const int e = __builtin_ctz(em);
+      T nf[3];
+      float* slot = nullptr;
+      if ((fmask >> e) & 1u) {
+        uint32_t face;
+        if (q < kPre) {
+          face = q == 0 ? pf[0] : (q == 1 ? pf[1] : pf[2]);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            nf[d] = q == 0 ? pre[0][d] : (q == 1 ? pre[1][d] : pre[2][d]);
+        } else {
+          face = rows[12 * v + e];
+          nf[0] = fn.at(4 * (uint64_t)face + 0);
+          nf[1] = fn.at(4 * (uint64_t)face + 1);
+          nf[2] = fn.at(4 * (uint64_t)face + 2);
+        }
+        ++q;
+        slot = face_slot(e, face);
+      } else if ((smask >> e) & 1u) {
+        int axis, x0, y0, z0;
+        edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+        group_sum(G, axis, x0, y0, z0, nf);
+        normalize3(nf);
+        slot = ubar_e + 3 * (12 * v + e);
+      } else {
+        raw_normal(own, e, nf);
+      }
+      row(e, nf, slot);
+    }
+  } else {
+    for (uint32_t em = cross; em; em &= em - 1) {
+      const int e = __builtin_ctz(em);
+      T nf[3];
+      uint64_t face;
+      bool avg;
+      final_normal(G, act, fn, i, j, k, e, own, nf, &face, &avg);
+      if (kinds_out)
+        kinds_out[e * kKindStride] =
+            !avg ? kRowOwn : (face != ~0ull ? (uint32_t)face : kRowEdgeSlot);
+      float* slot = nullptr;
+      if (avg) slot = face != ~0ull ? face_slot(e, (uint32_t)face) : ubar_e + 3 * (12 * v + e);
+      row(e, nf, slot);
+    }
   }
 }
 
-// Adjoint phase 2 for one active cell: for each of its averaged rows, the u-bar of the group's
-// 4 cells summed, taken through the average's normalization, then through this cell's own raw
-// normal (its share of the average) into Sb.
-template <typename T>
+// Between the phases, per face f: the adjoint of the face's averaged (normalized) normal taken
+// back to the 4-cell sum, from the members' u-bar summed in member order, computed in FP64 and
+// written as S (double: plain mode, float: FP32 mode) over the face's first slot floats.
+template <typename S>
+__device__ __forceinline__ void face_adjoint(const FaceNormals<double>& fn, uint64_t f,
+                                             float* __restrict__ ubar) {
+  const float4* u = reinterpret_cast<const float4*>(ubar + 12 * f);
+  const float4 a = u[0], b = u[1], c = u[2];
+  // slots m = 0..3: (a.xyz), (a.w, b.xy), (b.zw, c.x), (c.yzw)
+  const double us[3] = {(((double)a.x + (double)a.w) + (double)b.z) + (double)c.y,
+                        (((double)a.y + (double)b.x) + (double)b.w) + (double)c.z,
+                        (((double)a.z + (double)b.y) + (double)c.x) + (double)c.w};
+  const double len = fn.at(4 * f + 3);
+  const double acc[3] = {fn.at(4 * f + 0) * len, fn.at(4 * f + 1) * len, fn.at(4 * f + 2) * len};
+  double accb[3];
+  normalize_vjp(acc, us, accb);
+  S* o = reinterpret_cast<S*>(ubar + 12 * f);
+  o[0] = (S)accb[0];
+  o[1] = (S)accb[1];
+  o[2] = (S)accb[2];
+}
+
+// Adjoint phase 2 for one active cell: for each of its averaged rows, the adjoint of the group's
+// 4-cell normal sum (per face: computed once by face_adjoint, stored as S; for an averaged edge
+// without a face: the u-bar of the group's 4 cells summed and taken through the average's
+// normalization here), then through this cell's own raw normal (its share of the average) into
+// Sb. rows: the cell's 12 row kinds written by phase 1. The first kPre face rows' adjoints are
+// loaded together before the first row is processed.
+template <typename T, typename S>
 __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_t* __restrict__ act,
-                               const uint32_t* __restrict__ wprefix, const FaceNormals<T>& fn,
-                               int i, int j, int k, const float* __restrict__ ubar,
+                               const uint32_t* __restrict__ wprefix, int i, int j, int k,
+                               const uint32_t* __restrict__ rows, const float* __restrict__ ubar,
                                const float* __restrict__ ubar_e, T Sb[8]) {
+  constexpr int kPre = 3;
+  const uint4* r4 = reinterpret_cast<const uint4*>(rows);
+  const uint4 kinds[3] = {r4[0], r4[1], r4[2]};
   Cell<T> own;
   load_cell(G, i, j, k, own);
+  uint32_t fmask, smask;  // crossing rows with a face, with an edge slot
+  kind_masks(kinds, crossing_mask(own), fmask, smask);
+  T pre[kPre][3];
+  {
+    uint32_t pm = fmask;
+#pragma unroll
+    for (int p = 0; p < kPre; ++p) {
+      pre[p][0] = pre[p][1] = pre[p][2] = T(0);
+      if (pm) {
+        const S* a =
+            reinterpret_cast<const S*>(ubar + 12 * (uint64_t)row_kind(kinds, __builtin_ctz(pm)));
+        pre[p][0] = (T)a[0];
+        pre[p][1] = (T)a[1];
+        pre[p][2] = (T)a[2];
+        pm &= pm - 1;
+      }
+    }
+  }
   const T zero3[3] = {T(0), T(0), T(0)};
-  for (uint32_t em = crossing_mask(own); em; em &= em - 1) {
+  int p = 0;  // face rows seen
+  for (uint32_t em = fmask | smask; em; em &= em - 1) {
     const int e = __builtin_ctz(em);
-    const T fa = own.s(edge_a(e)), fb = own.s(edge_b(e));
-    const bool strict = strict_change(fa, fb);
-    int axis, x0, y0, z0;
-    if (!edge_group(G, act, i, j, k, e, strict, axis, x0, y0, z0)) continue;
+    const bool has_face = (fmask >> e) & 1u;
+    T accb[3];
+    if (has_face) {
+      if (p < kPre) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          accb[d] = p == 0 ? pre[0][d] : (p == 1 ? pre[1][d] : pre[2][d]);
+      } else {
+        const S* a = reinterpret_cast<const S*>(ubar + 12 * (uint64_t)rows[e]);
+        accb[0] = (T)a[0];
+        accb[1] = (T)a[1];
+        accb[2] = (T)a[2];
+      }
+      ++p;
+    }
     T n[3];
     raw_normal(own, e, n);
     if (!(sq3(n[0], n[1], n[2]) > T(0))) continue;  // excluded from the average
-    T us[3] = {T(0), T(0), T(0)};
-    T acc[3];
-    const uint64_t f = (strict && fn.cap) ? face_id(G, fn.ft, fn.pw, axis, x0, y0, z0) : ~0ull;
-    if (f < fn.cap) {
-      for (int m = 0; m < 4; ++m) {
-        const float* slot = ubar + 3 * (4 * f + m);
-        us[0] += (T)slot[0];
-        us[1] += (T)slot[1];
-        us[2] += (T)slot[2];
-      }
-      const T len = fn.at(4 * f + 3);
-      acc[0] = fn.at(4 * f + 0) * len;
-      acc[1] = fn.at(4 * f + 1) * len;
-      acc[2] = fn.at(4 * f + 2) * len;
-    } else {
+    if (!has_face) {
+      int axis, x0, y0, z0;  // the group's geometry only (phase 1 found it averaged)
+      edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+      T us[3] = {T(0), T(0), T(0)};
       for (int m = 0; m < 4; ++m) {
         int x, y, z;
         group_member(axis, m, x0, y0, z0, x, y, z);
@@ -1050,10 +1201,10 @@ __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_
         us[1] += (T)slot[1];
         us[2] += (T)slot[2];
       }
+      T acc[3];
       group_sum(G, axis, x0, y0, z0, acc);
+      normalize_vjp(acc, us, accb);
     }
-    T accb[3];
-    normalize_vjp(acc, us, accb);
     hermite_vjp(own, e, accb, zero3, Sb);
   }
 }

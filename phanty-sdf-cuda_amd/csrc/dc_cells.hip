@@ -30,7 +30,7 @@ hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
   if (a.precision == DC_PREC_FP64) {
     hipLaunchKernelGGL(k_qef<DC_PREC_FP64>, grid, block, 0, s, a.grids, a.n_grids, a.act_bounds,
                        a.act, a.alist, a.ctr_rw, a.capA, a.v64, a.v32, a.flags, a.fblist, a.ft,
-                       a.pw, a.nrm, a.capF, nullptr);
+                       a.pw, a.nrm, a.capF, nullptr, nullptr);
     return hipGetLastError();
   }
   return dc_launch_qef_fp32(a, s);

@@ -128,7 +128,8 @@ __global__ void __launch_bounds__(256)
 // Vertices
 // ---------------------------------------------------------------------------------------------
 // One thread per active cell (persistent grid-stride over the device-side count). In FP32 mode a
-// cell that needs FP64 is appended to fblist (order irrelevant: results are per vertex).
+// cell that needs FP64 is appended to fblist (order irrelevant: results are per vertex), and the
+// adjoint's per-cell row kinds are recorded in rows (12 per vertex) when it is not null.
 template <int PREC>
 __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
     k_qef(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ act_bounds,
@@ -136,11 +137,15 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
           DcCounters* __restrict__ ctr, uint64_t cap, double* __restrict__ v64,
           float* __restrict__ v32, uint32_t* __restrict__ flags, uint32_t* __restrict__ fblist,
           const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-          const void* __restrict__ nrm, uint64_t capF, float* __restrict__ qstate) {
+          const void* __restrict__ nrm, uint64_t capF, float* __restrict__ qstate,
+          uint32_t* __restrict__ rows) {
   const uint64_t n = min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm), nullptr, nf};
   const FaceNormals<float> fn32{ft, pw, nullptr, reinterpret_cast<const float*>(nrm), nf};
+  // FP32 mode: each thread's row kinds gathered in its LDS column (e * 256 + thread), then written
+  // as three 16-byte stores, so a wave writes whole lines
+  __shared__ uint32_t s_kinds[PREC == DC_PREC_FP32 ? 12 * kKindStride : 1];
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t entry = alist[v];
@@ -159,7 +164,18 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
       v32[3 * v + 2] = (float)x[2];
     } else {
       float xl[3];
-      if (solve_cell_fp32(G, act, fn32, i, j, k, xl, qstate ? qstate + kQefState * v : nullptr)) {
+      uint32_t* kinds = rows ? s_kinds + threadIdx.x : nullptr;
+      const bool ok = solve_cell_fp32(G, act, fn32, i, j, k, xl,
+                                      qstate ? qstate + kQefState * v : nullptr, kinds);
+      if (kinds) {  // a non-crossing edge's entry is stale LDS content, never read (the readers
+                    // mask with the cell's crossing edges)
+        uint4* o = reinterpret_cast<uint4*>(rows + 12 * v);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          o[q] = make_uint4(kinds[(4 * q) * kKindStride], kinds[(4 * q + 1) * kKindStride],
+                            kinds[(4 * q + 2) * kKindStride], kinds[(4 * q + 3) * kKindStride]);
+      }
+      if (ok) {
         v32[3 * v + 0] = (float)(coordd(G.ox, i, G.vx) + (double)xl[0]);
         v32[3 * v + 1] = (float)(coordd(G.oy, j, G.vy) + (double)xl[1]);
         v32[3 * v + 2] = (float)(coordd(G.oz, k, G.vz) + (double)xl[2]);
@@ -435,9 +451,12 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------------------------
 // Cells processed: every active cell (list == nullptr; with skip_flagged the FP32 kernel leaves
 // the FP64 cells to the list launch) or the listed ones (the FP64 cells of FP32 mode). The
-// cell's 8 corner adjoints go to contrib[8v..8v+7] (phase 1 writes, phase 2 adds).
+// cell's 8 corner adjoints go to contrib[8v..8v+7] (phase 1 writes, phase 2 adds). The kinds of the
+// cell's rows, rows[12v..12v+11], come from the FP32 forward in FP32 mode; in plain mode phase 1
+// records them (gathered in LDS, then written as three 16-byte stores).
 // FP32 instances fit 4 waves per SIMD without spilling; FP64 ones keep the default bound.
-template <int PHASE, typename T>
+// S: the type face_adjoint stored the faces' adjoints in (phase 2).
+template <int PHASE, typename T, typename S>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     k_bwd_cell(const DcGridDev* __restrict__ grids, int n_grids,
                const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
@@ -447,10 +466,13 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
                const float* __restrict__ gv, const uint64_t* __restrict__ ft,
                const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
                const float* __restrict__ nrm32, uint64_t capF, const float* __restrict__ qstate,
-               float* __restrict__ ubar, float* __restrict__ ubar_e, float* __restrict__ contrib) {
+               float* __restrict__ ubar, float* __restrict__ ubar_e, uint32_t* __restrict__ rows,
+               float* __restrict__ contrib) {
   const uint64_t n = list ? min(ctr->fallback, cap) : min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<T> fn{ft, pw, nrm64, nrm32, nf};
+  constexpr bool kRecord = PHASE == 1 && sizeof(T) == 8;
+  __shared__ uint32_t s_kinds[kRecord ? 12 * kKindStride : 1];
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t v = list ? (uint64_t)list[t] : t;
@@ -464,14 +486,23 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     float4* out = reinterpret_cast<float4*>(contrib + 8 * v);
     if (PHASE == 1) {
       const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
+      // plain mode records the kinds (FP32 mode's forward did, for every cell)
+      uint32_t* kinds = kRecord && !list ? s_kinds + threadIdx.x : nullptr;
       backward_rows<T>(G, act, fn, i, j, k, v, xb, qstate ? qstate + kQefState * v : nullptr,
-                       ubar, ubar_e, Sb);
+                       ubar, ubar_e, rows, kinds, Sb);
+      if (kinds) {  // a non-crossing edge's entry is stale LDS content, never read
+        uint4* o = reinterpret_cast<uint4*>(rows + 12 * v);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          o[q] = make_uint4(kinds[(4 * q) * kKindStride], kinds[(4 * q + 1) * kKindStride],
+                            kinds[(4 * q + 2) * kKindStride], kinds[(4 * q + 3) * kKindStride]);
+      }
       out[0] = make_float4((float)Sb[0], (float)Sb[1], (float)Sb[2], (float)Sb[3]);
       out[1] = make_float4((float)Sb[4], (float)Sb[5], (float)Sb[6], (float)Sb[7]);
     } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q) Sb[q] = T(0);
-      backward_faces<T>(G, act, wprefix, fn, i, j, k, ubar, ubar_e, Sb);
+      backward_faces<T, S>(G, act, wprefix, i, j, k, rows + 12 * v, ubar, ubar_e, Sb);
       const float4 o0 = out[0], o1 = out[1];
       out[0] = make_float4(o0.x + (float)Sb[0], o0.y + (float)Sb[1], o0.z + (float)Sb[2],
                            o0.w + (float)Sb[3]);
@@ -479,6 +510,20 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
                            o1.w + (float)Sb[7]);
     }
   }
+}
+
+// Between the adjoint phases: each face's averaged-normal adjoint, once per face (face_adjoint),
+// stored as double (plain mode) or float (FP32 mode).
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_bwd_face_adjoint(const DcCounters* __restrict__ ctr, uint64_t capF,
+                       const double* __restrict__ nrm64, const float* __restrict__ nrm32,
+                       float* __restrict__ ubar) {
+  const uint64_t nf = min(ctr->n_faces, capF);
+  const FaceNormals<double> fn{nullptr, nullptr, nrm64, nrm32, nf};
+  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+       f += (uint64_t)gridDim.x * blockDim.x)
+    face_adjoint<S>(fn, f, ubar);
 }
 
 // Corner index (contouring.cpp:232-241 numbering) at offset (dx,dy,dz) in {0,1}^3.

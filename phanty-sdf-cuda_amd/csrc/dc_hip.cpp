@@ -66,8 +66,8 @@ struct dc_hip_context {
   int64_t qef_blocks = 0;
   int cu_count = 256;
   DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
-      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, fblist, flist, goff, qstate,
-      tris;
+      counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, bwd_rows, fblist, flist, goff,
+      qstate, tris;
   DcCounters* h_counters = nullptr;
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
@@ -341,7 +341,8 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->flist, c->capF * 8, "face list"))) return rc;
   if ((rc = ensure(c, c->goff, 8 * (size_t)n, "grid offsets"))) return rc;
   if (precision == DC_HIP_FP32_FALLBACK &&
-      (rc = ensure(c, c->qstate, c->capA * 4 * 24, "solve state")))
+      ((rc = ensure(c, c->qstate, c->capA * 4 * 24, "solve state")) ||
+       (rc = ensure(c, c->bwd_rows, c->capA * 4 * 12, "adjoint row kinds"))))
     return rc;
   if (n > c->h_goff_cap) {
     if (c->h_goff) hipHostFree(c->h_goff);
@@ -475,6 +476,7 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   qa.nrm = c->fnrm.p;
   qa.capF = c->capF;
   qa.qstate = precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
+  qa.rows = precision == DC_HIP_FP32_FALLBACK ? as<uint32_t>(c->bwd_rows) : nullptr;
   DC_TRY(c, dc_launch_qef(qa, s), "qef");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
   DcEmitArgs ea;
@@ -642,7 +644,8 @@ void dc_hip_destroy(dc_hip_context* c) {
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
-                    &c->ubar, &c->ubar_e, &c->fblist, &c->flist, &c->goff, &c->qstate, &c->tris};
+                    &c->ubar, &c->ubar_e, &c->bwd_rows, &c->fblist, &c->flist, &c->goff,
+                    &c->qstate, &c->tris};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -871,6 +874,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                    false)))
     return rc;
   if ((rc = ensure(c, c->ubar_e, nv1 * 12 * 12, "edge adjoint slots", false))) return rc;
+  if ((rc = ensure(c, c->bwd_rows, nv1 * 12 * 4, "adjoint row kinds", false))) return rc;
   hipStream_t s;
   if ((rc = work_stream(c, &s))) return rc;
   const int64_t* d_tile_b = as<int64_t>(c->bounds);
@@ -901,6 +905,8 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.capF = c->capF;
   a.ubar = as<float>(c->ubar);
   a.ubar_e = as<float>(c->ubar_e);
+  a.rows = as<uint32_t>(c->bwd_rows);
+  a.face_blocks = std::max<int64_t>(1, (c->nf + 255) / 256);
   a.contrib = as<float>(c->bwd_contrib);
   a.qstate = c->precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
   a.grad_values = grad_values;
@@ -926,6 +932,8 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                           DC_TRY(c, hipEventRecord(c->ev_side, c->side), "join");
                           DC_TRY(c, hipStreamWaitEvent(st, c->ev_side, 0), "join");
                         }
+                        if (phase == 1)
+                          DC_TRY(c, dc_launch_bwd_face_adjoint(a, st), "backward faces");
                       }
                       DC_TRY(c, dc_launch_gather(a, st), "backward gather");
                       return DC_HIP_OK;

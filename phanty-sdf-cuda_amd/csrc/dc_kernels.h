@@ -46,6 +46,7 @@ struct DcQefArgs {
   const void* nrm;  // per-face averaged normals (double in FP64 mode, float in FP32 mode)
   uint64_t capF;
   float* qstate;    // FP32 mode: per-vertex solve state for the adjoint (24 floats), or null
+  uint32_t* rows;   // FP32 mode: per-vertex row kinds for the adjoint (12), or null
 };
 
 struct DcFaceNormalArgs {
@@ -113,8 +114,10 @@ struct DcBwdArgs {
   const uint32_t* pw;
   const void* nrm;
   uint64_t capF;
-  float* ubar;    // 4 slots x 3 floats per face
+  float* ubar;    // 4 slots x 3 floats per face (after the face pass: 3 doubles per face)
   float* ubar_e;  // 12 slots x 3 floats per active cell
+  uint32_t* rows;  // 12 row kinds per active cell (FP32 mode: from the forward; else phase 1's)
+  int64_t face_blocks;  // blocks of 256 over the faces
   float* contrib;  // 8 floats per active cell
   const float* qstate;  // FP32 mode: the forward's per-vertex solve state (24 floats), or null
   float* grad_values;
@@ -149,6 +152,8 @@ hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
 // Adjoint: per-cell phase 1 or 2 (FP32 mode: the FP32-solved cells on s, the FP64 cells on s64,
 // which may be another stream: the two sets are disjoint), then the gradient gather.
 hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64);
+// Between the phases: every face's averaged-normal adjoint, once per face.
+hipError_t dc_launch_bwd_face_adjoint(const DcBwdArgs& a, hipStream_t s);
 hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s);
 // triangulate_v1 of the last mesh: 2 nf x 3 int32 (goff: per-grid first vertex and face, n > 1).
 hipError_t dc_launch_triangulate(const int32_t* faces, uint64_t nf, int n_grids,
