@@ -87,7 +87,10 @@ int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
  * adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
  * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
  * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
- * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve). */
+ * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve),
+ * "fallback_state" (FP32 mode: up to how many FP64-solved cells keep their solve states for the
+ * adjoint; with more, the adjoint solves them all again; -1, the default: a quarter of the
+ * active-cell capacity). */
 int dc_hip_set_option(dc_hip_context* ctx, const char* key, int64_t value);
 
 /* Replaces CudaDualContouringBackend::extract (src/cuda/dc_cuda.cu:614-649): host values are

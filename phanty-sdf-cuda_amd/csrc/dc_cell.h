@@ -498,9 +498,11 @@ __device__ __forceinline__ void sym3_eig(const T Min[3][3], T lam[3], T Q[3][3])
 // matching the CPU restatement's sums term by term.
 // ---------------------------------------------------------------------------------------------
 // The FP64 solve from the rows (hermite point p, final normal nf) of the cell's crossing edges in
-// ascending order; row(e, p, nf) provides edge e's row.
+// ascending order; row(e, p, nf) provides edge e's row. state (unless null) receives the solve's
+// state for the adjoint (kQefState doubles, the FP32 state's layout; g = r - M c = sum n n.(p-c)).
 template <class Row>
-__device__ __forceinline__ void solve_rows_fp64(uint32_t cross, Row&& row, double x[3]) {
+__device__ __forceinline__ void solve_rows_fp64(uint32_t cross, Row&& row, double x[3],
+                                                double* __restrict__ state = nullptr) {
   double M00 = 0, M01 = 0, M02 = 0, M11 = 0, M12 = 0, M22 = 0;
   double r0 = 0, r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0;
   int m = 0;
@@ -533,6 +535,21 @@ __device__ __forceinline__ void solve_rows_fp64(uint32_t cross, Row&& row, doubl
   const double g0 = r0 - ((M00 * c0 + M01 * c1) + M02 * c2);
   const double g1 = r1 - ((M01 * c0 + M11 * c1) + M12 * c2);
   const double g2 = r2 - ((M02 * c0 + M12 * c1) + M22 * c2);
+  if (state) {
+    double2* d = reinterpret_cast<double2*>(state);
+    d[0] = make_double2(c0, c1);
+    d[1] = make_double2(c2, M00);
+    d[2] = make_double2(M01, M02);
+    d[3] = make_double2(M11, M12);
+    d[4] = make_double2(M22, g0);
+    d[5] = make_double2(g1, g2);
+    d[6] = make_double2(lam[0], lam[1]);
+    d[7] = make_double2(lam[2], Q[0][0]);
+    d[8] = make_double2(Q[0][1], Q[0][2]);
+    d[9] = make_double2(Q[1][0], Q[1][1]);
+    d[10] = make_double2(Q[1][2], Q[2][0]);
+    d[11] = make_double2(Q[2][1], Q[2][2]);
+  }
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -702,6 +719,38 @@ __device__ __forceinline__ void load_state(const float* __restrict__ src, Qef<fl
   V[2][0] = f.y;
   V[2][1] = f.z;
   V[2][2] = f.w;
+}
+
+// The FP64 fallback solve's state (solve_rows_fp64), same layout.
+__device__ __forceinline__ void load_state(const double* __restrict__ src, Qef<double>& Q,
+                                           double lam[3], double V[3][3]) {
+  const double2* s = reinterpret_cast<const double2*>(src);
+  double t[24];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const double2 u = s[q];
+    t[2 * q] = u.x;
+    t[2 * q + 1] = u.y;
+  }
+  Q.c[0] = t[0];
+  Q.c[1] = t[1];
+  Q.c[2] = t[2];
+  Q.M[0][0] = t[3];
+  Q.M[0][1] = Q.M[1][0] = t[4];
+  Q.M[0][2] = Q.M[2][0] = t[5];
+  Q.M[1][1] = t[6];
+  Q.M[1][2] = Q.M[2][1] = t[7];
+  Q.M[2][2] = t[8];
+  Q.g[0] = t[9];
+  Q.g[1] = t[10];
+  Q.g[2] = t[11];
+  lam[0] = t[12];
+  lam[1] = t[13];
+  lam[2] = t[14];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) V[r][c] = t[15 + 3 * r + c];
 }
 
 // FP32 solve in the cell frame. Returns false when the cell must be solved in FP64; otherwise
@@ -900,11 +949,12 @@ __device__ __forceinline__ int member_of(int axis, int i, int j, int k, int x0, 
 // stores u-bar for phase 2 (in the face's slot for this member, ubar[4*face + member], or, for
 // an averaged edge without a face, in the cell's per-edge slot ubar_e[12*v + e]); the point
 // adjoints and the other rows' normals go to the cell's own 8 corners, Sb.
-// A row's kind: the face id, kRowEdgeSlot (averaged, no face) or kRowOwn (not averaged). T = float
-// (FP32 mode): state is the forward's solve state of this cell and rows[12v + e] the kinds the
-// forward recorded; the first kPre face rows' normals are loaded together before the first row is
-// processed. T = double: assembled and solved again here; the kinds go to kinds[e * kKindStride]
-// (the caller's LDS column) unless it is null.
+// A row's kind: the face id, kRowEdgeSlot (averaged, no face) or kRowOwn (not averaged).
+// kGiven (FP32 mode): state is the forward's solve state of this cell (FP32, or FP64 for a cell
+// the forward solved in FP64) and rows[12v + e] the kinds the forward recorded; the first kPre
+// face rows' normals are loaded together before the first row is processed. Otherwise the cell is
+// assembled and solved again here, and the kinds go to kinds_out[e * kKindStride] (the caller's
+// LDS column) unless it is null.
 
 // Entry e of a cell's 12 row kinds held as three uint4.
 __device__ __forceinline__ uint32_t row_kind(const uint4 r[3], int e) {
@@ -927,14 +977,13 @@ __device__ __forceinline__ void kind_masks(const uint4 kinds[3], uint32_t cross,
   smask &= cross;
 }
 
-template <typename T>
+template <typename T, bool kGiven>
 __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
                               const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
-                              const float xb_in[3], const float* __restrict__ state,
+                              const float xb_in[3], const T* __restrict__ state,
                               float* __restrict__ ubar, float* __restrict__ ubar_e,
                               const uint32_t* __restrict__ rows, uint32_t* __restrict__ kinds_out,
                               T Sb[8]) {
-  constexpr bool kGiven = sizeof(T) == 4;  // solve state and row kinds from the forward
   uint4 kinds[3];
   if constexpr (kGiven) {
     const uint4* r4 = reinterpret_cast<const uint4*>(rows + 12 * v);

@@ -2,8 +2,6 @@
 // with floating-point contraction (FMA) allowed: the FP32 mode's face normals and solve, its FP64
 // fallback cells, and the adjoint's per-cell kernels (both precisions).
 
-#include <type_traits>
-
 #include "dc_cells_kernels.h"
 
 hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s) {
@@ -22,7 +20,7 @@ hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_qef_fallback, dim3((unsigned)std::max<int64_t>(1, a.blocks * 2)), block,
                      0, s, a.grids, a.n_grids, a.act_bounds, a.act, a.alist, a.ctr_rw, a.capA,
-                     a.fblist, a.v32);
+                     a.fblist, a.v32, a.qstate64, a.cap64);
   return hipGetLastError();
 }
 
@@ -30,37 +28,48 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
   const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
   const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
   const dim3 grid((unsigned)a.cell_blocks), block(256);
-  const dim3 fgrid((unsigned)a.fallback_blocks);
-// Phase 2 reads the faces' adjoints as double in plain mode and as float in FP32 mode (S); phase 1
-// does not read them, so its FP64 instance serves both modes.
-#define DC_BWD(PHASE)                                                                              \
-  if (a.precision == DC_PREC_FP64) {                                                               \
-    hipLaunchKernelGGL((k_bwd_cell<PHASE, double, double>), grid, block, 0, s, a.grids,            \
-                       a.n_grids, a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 0, nullptr,    \
-                       a.ctr, a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, nullptr,      \
-                       a.ubar, a.ubar_e, a.rows, a.contrib);                                       \
-    return hipGetLastError();                                                                      \
-  }                                                                                                \
-  /* the few FP64 cells first: their long per-thread chains start before the FP32 cells' */     \
-  /* blocks fill the device */                                                                     \
-  hipLaunchKernelGGL((k_bwd_cell<PHASE, double, std::conditional_t<PHASE == 1, double, float>>),   \
-                     fgrid, block, 0, s64, a.grids, a.n_grids, a.act_bounds, a.act, a.wprefix,     \
-                     a.alist, a.flags, 0, a.fblist, a.ctr, a.capA, a.grad_vertices, a.ft, a.pw,    \
-                     n64, n32, a.capF, nullptr, a.ubar, a.ubar_e, a.rows, a.contrib);              \
-  {                                                                                                \
-    const hipError_t e = hipGetLastError();                                                        \
-    if (e != hipSuccess) return e;                                                                 \
-  }                                                                                                \
-  hipLaunchKernelGGL((k_bwd_cell<PHASE, float, float>), grid, block, 0, s, a.grids, a.n_grids,     \
-                     a.act_bounds, a.act, a.wprefix, a.alist, a.flags, 1, nullptr, a.ctr, a.capA,  \
-                     a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, a.qstate, a.ubar, a.ubar_e,    \
-                     a.rows, a.contrib);                                                           \
-  return hipGetLastError();
-  if (phase == 1) {
-    DC_BWD(1)
+  // every launch takes the same arguments apart from the cell selection and the state
+#define DC_BWD_ARGS(FLAGGED, LIST, STATE)                                                         \
+  a.grids, a.n_grids, a.act_bounds, a.act, a.wprefix, a.alist, a.flags, FLAGGED, LIST, a.ctr,     \
+      a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, STATE, a.ubar, a.ubar_e, a.rows,     \
+      a.contrib
+  if (a.precision == DC_PREC_FP64) {
+    if (phase == 1)
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), grid, block, 0, s,
+                         DC_BWD_ARGS(0, nullptr, nullptr));
+    else
+      hipLaunchKernelGGL((k_bwd_cell<2, double, double, false>), grid, block, 0, s,
+                         DC_BWD_ARGS(0, nullptr, nullptr));
+    return hipGetLastError();
   }
-  DC_BWD(2)
-#undef DC_BWD
+  // FP32 mode. The few FP64 cells first (on s64): their long per-thread chains start before the
+  // FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback solves'
+  // states when all of them were kept, else solves every one again (which cells got a state
+  // depends on the order of the fallback list, so a mix would not be reproducible).
+  const dim3 fgrid((unsigned)a.fallback_blocks);
+  if (phase == 1) {
+    if (a.fb_state)
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, true>), fgrid, block, 0, s64,
+                         DC_BWD_ARGS(0, a.fblist, a.qstate64));
+    else
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), fgrid, block, 0, s64,
+                         DC_BWD_ARGS(0, a.fblist, nullptr));
+  } else {
+    hipLaunchKernelGGL((k_bwd_cell<2, double, float, false>), fgrid, block, 0, s64,
+                       DC_BWD_ARGS(0, a.fblist, nullptr));
+  }
+  {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (phase == 1)
+    hipLaunchKernelGGL((k_bwd_cell<1, float, float, true>), grid, block, 0, s,
+                       DC_BWD_ARGS(1, nullptr, a.qstate));
+  else
+    hipLaunchKernelGGL((k_bwd_cell<2, float, float, false>), grid, block, 0, s,
+                       DC_BWD_ARGS(1, nullptr, nullptr));
+#undef DC_BWD_ARGS
+  return hipGetLastError();
 }
 
 hipError_t dc_launch_bwd_face_adjoint(const DcBwdArgs& a, hipStream_t s) {

@@ -29,6 +29,7 @@ namespace {
 
 using namespace dccell;
 using dccommon::find_bucket;
+using dccommon::wave_scan_incl;
 
 __device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, int& i, int& j,
                                             int& k) {
@@ -189,7 +190,8 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 1)
   }
 }
 
-// FP32 mode: the listed cells solved in FP64 (inline averaged normals, the plain arithmetic).
+// FP32 mode: the listed cells solved in FP64 (inline averaged normals, the plain arithmetic). The
+// solve's state of list entry t < cap_state goes to state + kQefState * t for the adjoint.
 // 16 lanes per cell: lane e computes the row of edge e (an averaged normal takes up to 4 raw
 // normals), then every lane of the group sums the rows in ascending edge order through shuffles
 // (the serial solve's operation order) and solves; the group's first lane writes the vertex.
@@ -201,7 +203,8 @@ __global__ void __launch_bounds__(256)
     k_qef_fallback(const DcGridDev* __restrict__ grids, int n_grids,
                    const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
                    const uint64_t* __restrict__ alist, const DcCounters* __restrict__ ctr,
-                   uint64_t cap, const uint32_t* __restrict__ fblist, float* __restrict__ v32) {
+                   uint64_t cap, const uint32_t* __restrict__ fblist, float* __restrict__ v32,
+                   double* __restrict__ state, uint64_t cap_state) {
   const uint64_t n = min(ctr->fallback, cap);
   const FaceNormals<double> inline64{nullptr, nullptr, nullptr, nullptr, 0ull};
   const int lane = threadIdx.x & 63;
@@ -232,7 +235,7 @@ __global__ void __launch_bounds__(256)
             ne[d] = __shfl(nf[d], src);
           }
         },
-        x);
+        x, (sub == 0 && state && t < cap_state) ? state + kQefState * t : nullptr);
     if (sub == 0) {
       v32[3 * (uint64_t)v + 0] = (float)x[0];
       v32[3 * (uint64_t)v + 1] = (float)x[1];
@@ -450,28 +453,30 @@ __global__ void __launch_bounds__(256)
 // Adjoint
 // ---------------------------------------------------------------------------------------------
 // Cells processed: every active cell (list == nullptr; with skip_flagged the FP32 kernel leaves
-// the FP64 cells to the list launch) or the listed ones (the FP64 cells of FP32 mode). The
-// cell's 8 corner adjoints go to contrib[8v..8v+7] (phase 1 writes, phase 2 adds). The kinds of the
-// cell's rows, rows[12v..12v+11], come from the FP32 forward in FP32 mode; in plain mode phase 1
-// records them (gathered in LDS, then written as three 16-byte stores).
+// the FP64 cells to the list launch) or the listed ones (the FP64 cells of FP32 mode). The cell's 8 corner adjoints go to contrib[8v..8v+7] (phase 1 writes,
+// phase 2 adds). The kinds of the cell's rows, rows[12v..12v+11], come from the FP32 forward in
+// FP32 mode; in plain mode phase 1 records them (gathered in LDS, then written as three 16-byte
+// stores).
+// kGiven (phase 1, FP32 mode): the forward's solve state is used, at state + kQefState * v (FP32
+// cells) or state + kQefState * t (FP64 cells: the fallback solve's, by list index).
 // FP32 instances fit 4 waves per SIMD without spilling; FP64 ones keep the default bound.
 // S: the type face_adjoint stored the faces' adjoints in (phase 2).
-template <int PHASE, typename T, typename S>
+template <int PHASE, typename T, typename S, bool kGiven>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     k_bwd_cell(const DcGridDev* __restrict__ grids, int n_grids,
                const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
                const uint32_t* __restrict__ wprefix, const uint64_t* __restrict__ alist,
                const uint32_t* __restrict__ flags, int skip_flagged,
-               const uint32_t* __restrict__ list, const DcCounters* __restrict__ ctr, uint64_t cap,
-               const float* __restrict__ gv, const uint64_t* __restrict__ ft,
-               const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
-               const float* __restrict__ nrm32, uint64_t capF, const float* __restrict__ qstate,
-               float* __restrict__ ubar, float* __restrict__ ubar_e, uint32_t* __restrict__ rows,
+               const uint32_t* __restrict__ list, const DcCounters* __restrict__ ctr, uint64_t cap, const float* __restrict__ gv,
+               const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
+               const double* __restrict__ nrm64, const float* __restrict__ nrm32, uint64_t capF,
+               const T* __restrict__ state, float* __restrict__ ubar,
+               float* __restrict__ ubar_e, uint32_t* __restrict__ rows,
                float* __restrict__ contrib) {
   const uint64_t n = list ? min(ctr->fallback, cap) : min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
   const FaceNormals<T> fn{ft, pw, nrm64, nrm32, nf};
-  constexpr bool kRecord = PHASE == 1 && sizeof(T) == 8;
+  constexpr bool kRecord = PHASE == 1 && !kGiven;
   __shared__ uint32_t s_kinds[kRecord ? 12 * kKindStride : 1];
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (uint64_t)gridDim.x * blockDim.x) {
@@ -488,8 +493,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
       const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
       // plain mode records the kinds (FP32 mode's forward did, for every cell)
       uint32_t* kinds = kRecord && !list ? s_kinds + threadIdx.x : nullptr;
-      backward_rows<T>(G, act, fn, i, j, k, v, xb, qstate ? qstate + kQefState * v : nullptr,
-                       ubar, ubar_e, rows, kinds, Sb);
+      const T* st = kGiven ? state + kQefState * (sizeof(T) == 4 ? v : t) : nullptr;
+      backward_rows<T, kGiven>(G, act, fn, i, j, k, v, xb, st, ubar, ubar_e, rows, kinds, Sb);
       if (kinds) {  // a non-crossing edge's entry is stale LDS content, never read
         uint4* o = reinterpret_cast<uint4*>(rows + 12 * v);
 #pragma unroll
@@ -579,20 +584,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Inclusive prefix sum over the 64 lanes: DPP row shifts within each row of 16 lanes, then the
-// totals of the lower rows.
-__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v, int lane) {
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
-  const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
-  const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
-  const int row = lane >> 4;
-  return v + (row >= 1 ? t0 : 0u) + (row >= 2 ? t1 : 0u) + (row >= 3 ? t2 : 0u);
 }
 
 // Lists this lane's active cells whose list index k (from `first`) falls in [lo, hi).

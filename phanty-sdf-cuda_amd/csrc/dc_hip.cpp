@@ -67,7 +67,7 @@ struct dc_hip_context {
   int cu_count = 256;
   DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
       counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, bwd_rows, fblist, flist, goff,
-      qstate, tris;
+      qstate, qstate64, tris;
   DcCounters* h_counters = nullptr;
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
@@ -80,6 +80,8 @@ struct dc_hip_context {
   int precision = 0;
   int64_t nv = 0, nf = 0;
   uint64_t capA = 0, capF = 0;
+  uint64_t cap64 = 0;  // FP32 mode: FP64 cells whose solve state is kept for the adjoint
+  int64_t cap64_opt = -1;  // option "fallback_state" (-1: capA / 4 + 64)
   std::vector<DcGridDev> hgrids;
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, tw_b, at_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_tw = 0,
@@ -303,7 +305,7 @@ std::string plan_key(dc_hip_context* c, int precision, hipStream_t s) {
   put(c->hgrids.data(), n * sizeof(DcGridDev));
   put(c->hbounds.data(), c->hbounds.size() * sizeof(int64_t));
   const int64_t v[] = {precision, c->eff_layers, c->mark_waves, c->qef_blocks, c->timing,
-                       (int64_t)c->capA, (int64_t)c->capF, (int64_t)c->alloc_gen,
+                       (int64_t)c->capA, (int64_t)c->capF, c->cap64_opt, (int64_t)c->alloc_gen,
                        (int64_t)(uintptr_t)s, (int64_t)(uintptr_t)c->h_goff};
   put(v, sizeof(v));
   return k;
@@ -331,7 +333,9 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->scan_base, (t_a + t_p + t_f) * 8, "scan tile bases"))) return rc;
   if ((rc = ensure(c, c->counters, sizeof(DcCounters), "counters"))) return rc;
   if ((rc = ensure(c, c->alist, c->capA * 8, "active list"))) return rc;
-  if ((rc = ensure(c, c->v64, c->capA * 24, "fp64 vertices"))) return rc;
+  if (precision == DC_HIP_PLAIN_FP64 &&
+      (rc = ensure(c, c->v64, c->capA * 24, "fp64 vertices")))
+    return rc;
   if ((rc = ensure(c, c->v32, c->capA * 12, "vertices"))) return rc;
   if ((rc = ensure(c, c->flags, c->capA * 4, "vertex flags"))) return rc;
   if ((rc = ensure(c, c->fblist, c->capA * 4, "fallback list"))) return rc;
@@ -340,9 +344,14 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = ensure(c, c->fnrm, c->capF * 32, "face normals"))) return rc;
   if ((rc = ensure(c, c->flist, c->capF * 8, "face list"))) return rc;
   if ((rc = ensure(c, c->goff, 8 * (size_t)n, "grid offsets"))) return rc;
+  // FP32 mode: the adjoint's inputs from the forward. The FP64 cells' solve states are kept for
+  // up to cap64 of them (a few percent of the cells are typical on fine grids; with more, the
+  // adjoint solves every FP64 cell again).
+  c->cap64 = c->cap64_opt >= 0 ? (uint64_t)c->cap64_opt : c->capA / 4 + 64;
   if (precision == DC_HIP_FP32_FALLBACK &&
       ((rc = ensure(c, c->qstate, c->capA * 4 * 24, "solve state")) ||
-       (rc = ensure(c, c->bwd_rows, c->capA * 4 * 12, "adjoint row kinds"))))
+       (rc = ensure(c, c->bwd_rows, c->capA * 4 * 12, "adjoint row kinds")) ||
+       (rc = ensure(c, c->qstate64, c->cap64 * 8 * 24, "fallback solve state"))))
     return rc;
   if (n > c->h_goff_cap) {
     if (c->h_goff) hipHostFree(c->h_goff);
@@ -477,6 +486,8 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   qa.capF = c->capF;
   qa.qstate = precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
   qa.rows = precision == DC_HIP_FP32_FALLBACK ? as<uint32_t>(c->bwd_rows) : nullptr;
+  qa.qstate64 = precision == DC_HIP_FP32_FALLBACK ? as<double>(c->qstate64) : nullptr;
+  qa.cap64 = c->cap64;
   DC_TRY(c, dc_launch_qef(qa, s), "qef");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
   DcEmitArgs ea;
@@ -645,7 +656,7 @@ void dc_hip_destroy(dc_hip_context* c) {
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
                     &c->ubar, &c->ubar_e, &c->bwd_rows, &c->fblist, &c->flist, &c->goff,
-                    &c->qstate, &c->tris};
+                    &c->qstate, &c->qstate64, &c->tris};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
@@ -690,6 +701,9 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (!c->use_graph) drop_graph(c);
   } else if (k == "timing") {
     c->timing = value != 0;
+  } else if (k == "fallback_state") {
+    if (value < -1) return fail(c, DC_HIP_ERR_INVALID_ARG, "fallback_state must be >= -1");
+    c->cap64_opt = value;
   } else if (k == "qef_blocks") {
     if (value < 0) return fail(c, DC_HIP_ERR_INVALID_ARG, "qef_blocks must be >= 0");
     c->qef_blocks = value;
@@ -896,8 +910,9 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.ctr = as<DcCounters>(c->counters);
   a.capA = c->capA;
   a.cell_blocks = std::max<int64_t>(1, (c->nv + 255) / 256);
-  a.fallback_blocks =
-      std::max<int64_t>(1, ((int64_t)c->h_counters->fallback + 255) / 256);
+  const int64_t n_fb = (int64_t)c->h_counters->fallback;
+  a.fallback_blocks = std::max<int64_t>(1, (n_fb + 255) / 256);
+  a.fb_state = n_fb <= (int64_t)c->cap64 ? 1 : 0;
   a.grad_vertices = grad_vertices;
   a.ft = as<uint64_t>(c->ft);
   a.pw = as<uint32_t>(c->pw);
@@ -909,6 +924,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.face_blocks = std::max<int64_t>(1, (c->nf + 255) / 256);
   a.contrib = as<float>(c->bwd_contrib);
   a.qstate = c->precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
+  a.qstate64 = c->precision == DC_HIP_FP32_FALLBACK ? as<double>(c->qstate64) : nullptr;
   a.grad_values = grad_values;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], s), "event");
   // capture key: every launch argument (pointers, sizes, block counts), the stream and the plan

@@ -21,7 +21,7 @@
 //   flist     uint64 every face once as ft word * 64 + bit, in face-word (memory) order; its
 //             face id is pw[word] + popc(ft[word] & below(bit))
 //   scan      per scan (act words, pencils, face words): uint32 sum and uint64 base per tile of
-//             4096 items
+//             1024 items
 //   fnrm      per face, 4 of T (double plain, float FP32 mode): the normalized 4-cell averaged
 //             hermite normal of the face edge and the length of the unnormalized sum
 //   faces     int32 quads, 4 per face, grid-local vertex ids

@@ -47,6 +47,8 @@ struct DcQefArgs {
   uint64_t capF;
   float* qstate;    // FP32 mode: per-vertex solve state for the adjoint (24 floats), or null
   uint32_t* rows;   // FP32 mode: per-vertex row kinds for the adjoint (12), or null
+  double* qstate64;  // FP32 mode: the FP64 cells' solve state by fallback list index (24 doubles)
+  uint64_t cap64;    // entries of qstate64
 };
 
 struct DcFaceNormalArgs {
@@ -120,6 +122,8 @@ struct DcBwdArgs {
   int64_t face_blocks;  // blocks of 256 over the faces
   float* contrib;  // 8 floats per active cell
   const float* qstate;  // FP32 mode: the forward's per-vertex solve state (24 floats), or null
+  const double* qstate64;  // FP32 mode: the FP64 cells' solve state by fallback list index
+  int fb_state;            // 1: qstate64 holds the state of every FP64 cell
   float* grad_values;
 };
 
@@ -134,7 +138,7 @@ int64_t dc_mark_tiles(const DcGridDev& g, int layers);
 // in vertex order, the face planes over k and the pencil face counts.
 hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s);
 hipError_t dc_launch_mark_finish(const DcMarkArgs& a, hipStream_t s);
-// Scans over n items in tiles of 4096: tile_sum (uint32) and tile_base (uint64) hold
+// Scans over n items in tiles of 1024: tile_sum (uint32) and tile_base (uint64) hold
 // dc_scan_tiles(n) entries; *total receives the sum.
 int64_t dc_scan_tiles(int64_t n);
 hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* tile_sum,

@@ -30,6 +30,7 @@ namespace {
 constexpr int kWave = 64;
 
 using dccommon::find_bucket;
+using dccommon::wave_scan_incl;
 using dccommon::xcd_remap;
 
 // f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N-1>), in order.
@@ -306,12 +307,12 @@ __global__ void __launch_bounds__(256)
 // ever waits on another, and no counter is shared by all blocks).
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kTileScanThreads = 1024;
 
-// Wave w of a block owns items [tile*kScanTile + w*1024, +1024), one per lane per round of 64
-// (coalesced); this is item 0 of the calling lane.
+// Wave w of a block owns items [tile*kScanTile + w*64*kScanItems, +64*kScanItems), one per lane
+// per round of 64 (coalesced); this is item 0 of the calling lane.
 __device__ __forceinline__ int64_t seg_base(int64_t tile) {
   return tile * kScanTile + (int64_t)(threadIdx.x >> 6) * (64 * kScanItems) + (threadIdx.x & 63);
 }
@@ -435,11 +436,11 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int q = 0; q < kScanItems; ++q) {
     const int64_t idx = base + 64 * q;
     const uint32_t c = (uint32_t)__popcll(w[q]);
-    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t incl = wave_scan_incl(c, threadIdx.x & 63);
     const uint64_t pos = run + incl - c;
     if (idx < n_words) wprefix[idx] = (uint32_t)pos;
     append_bits(w[q], idx, pos, cap, alist);
-    run += (uint32_t)__shfl(incl, 63);
+    run += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   }
 }
 
@@ -460,9 +461,9 @@ __global__ void __launch_bounds__(kScanThreads)
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const int64_t idx = base + 64 * q;
-    const uint32_t incl = wave_incl_scan(v[q]);
+    const uint32_t incl = wave_scan_incl(v[q], threadIdx.x & 63);
     if (idx < n_pen) pbase[idx] = (uint32_t)(run + incl - v[q]);
-    run += (uint32_t)__shfl(incl, 63);
+    run += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   }
 }
 
@@ -485,9 +486,9 @@ __global__ void __launch_bounds__(kScanThreads)
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t c = (uint32_t)__popcll(w[q]);
-    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t incl = wave_scan_incl(c, threadIdx.x & 63);
     append_bits(w[q], base + 64 * q, run + incl - c, cap, flist);
-    run += (uint32_t)__shfl(incl, 63);
+    run += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   }
 }
 

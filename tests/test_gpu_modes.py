@@ -107,3 +107,33 @@ def test_adjoint_deterministic_and_autograd():
     assert torch.equal(grads[0], grads[1])
     assert torch.isfinite(grads[0]).all()
     del w
+
+
+@pytest.mark.parametrize("kept", [0, 5])
+def test_adjoint_fallback_state_capacity(kept):
+    """FP32 mode keeps the FP64 fallback solves' states for up to `fallback_state` FP64 cells;
+    with more, the adjoint solves them all again. Either way the gradient is the oracle's to the
+    FP32 tolerance, the default capacity's (states used) to 1e-4 (relative, max-norm), and the
+    same on a second context."""
+    import torch
+    g = synth.csg8(40, seed=77)
+    phi = torch.from_numpy(np.asarray(g.values, np.float32)).reshape(g.nz, g.ny, g.nx).cuda()
+    rng = np.random.default_rng(3)
+    grads = []
+    for opt in (-1, kept, kept):
+        be = HipDualContouringBackend("fp32-fallback")
+        be.set_option("fallback_state", opt)
+        (V, F), st = be.extract_tensor(phi, (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
+        assert st.qef_fallback_count > kept  # some FP64 cells are solved again
+        if not grads:
+            w = torch.from_numpy(rng.normal(size=(V.shape[0], 3)).astype(np.float32)).cuda()
+        gphi = torch.empty_like(phi)
+        be.backward(w, gphi)
+        grads.append(gphi.cpu().numpy().ravel().astype(np.float64))
+    print(f"[adjoint] fallback_state {kept}: {st.qef_fallback_count} FP64 cells", flush=True)
+    ref = oracle.run_grid(g)
+    want = ref.backward(w.cpu().numpy().astype(np.float64))
+    scale = np.abs(want).max()
+    assert np.abs(grads[1] - want).max() / scale <= 2e-3
+    assert np.abs(grads[1] - grads[0]).max() / scale <= 1e-4
+    assert np.array_equal(grads[1], grads[2])  # reproducible across contexts
