@@ -247,7 +247,7 @@ def _device_grid(g):
                                    d_values=phi)
 
 
-def _fwd_bwd_ms(n, steps, warmup):
+def _fwd_bwd_ms(n, steps, warmup, options=None):
     """Plain forward ms and FP32+fallback forward+adjoint ms on the csg8 grid of size n."""
     import torch
     from contouring import HipDualContouringBackend, _lib, synth
@@ -256,6 +256,8 @@ def _fwd_bwd_ms(n, steps, warmup):
     be = HipDualContouringBackend("plain-fp64", device=0)
     be.set_stream(torch.cuda.current_stream(0).cuda_stream)
     be.set_option("timing", 0)  # replayed launch sequences
+    for key, value in (options or {}).items():
+        be.set_option(key, value)
     cg = be._cgrid(g, phi.data_ptr())
     st = _lib.DcHipStats()
     lib = be._lib

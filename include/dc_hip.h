@@ -88,6 +88,9 @@ int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
  * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
  * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
  * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve),
+ * "adjoint_lanes", "adjoint_face_lanes" (0, 1, 4, 16: lanes per active cell in the adjoint's
+ * first and second per-cell kernels; 0, the default, picks by mesh size: several lanes where one
+ * thread per cell would leave the device idle),
  * "fallback_state" (FP32 mode: up to how many FP64-solved cells keep their solve states for the
  * adjoint; with more, the adjoint solves them all again; -1, the default: a quarter of the
  * active-cell capacity). */

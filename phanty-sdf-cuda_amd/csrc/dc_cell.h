@@ -977,41 +977,30 @@ __device__ __forceinline__ void kind_masks(const uint4 kinds[3], uint32_t cross,
   smask &= cross;
 }
 
-template <typename T, bool kGiven>
-__device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
-                              const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
-                              const float xb_in[3], const T* __restrict__ state,
-                              float* __restrict__ ubar, float* __restrict__ ubar_e,
-                              const uint32_t* __restrict__ rows, uint32_t* __restrict__ kinds_out,
-                              T Sb[8]) {
-  uint4 kinds[3];
-  if constexpr (kGiven) {
-    const uint4* r4 = reinterpret_cast<const uint4*>(rows + 12 * v);
-    kinds[0] = r4[0];
-    kinds[1] = r4[1];
-    kinds[2] = r4[2];
-  }
-  Cell<T> own;
-  load_cell(G, i, j, k, own);
-  Qef<T> Q;
-  T lam[3], V[3][3];
-  if constexpr (kGiven) {
-    load_state(state, Q, lam, V);
-    Q.m = __popc(crossing_mask(own));
-  } else {
-    assemble_any(G, act, fn, i, j, k, own, Q);
-    sym3_eig(Q.M, lam, V);
-  }
+// The solve's adjoint terms shared by a cell's rows, for a vertex adjoint xb: gb = P xb (P the
+// thresholded pseudo-inverse), M-bar (Daleckii-Krein) and c-bar / m.
+template <typename T>
+struct RowsAdjoint {
+  T c[3], gb[3], Mb[3][3], cbm[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void rows_adjoint(const Qef<T>& Q, const T lam[3], const T V[3][3],
+                                             const float xb_in[3], RowsAdjoint<T>& R) {
   const T xb[3] = {(T)xb_in[0], (T)xb_in[1], (T)xb_in[2]};
   bool kept[3];
-  T gb[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    R.c[d] = Q.c[d];
+    R.gb[d] = T(0);
+  }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     kept[q] = fabs(lam[q]) > T(kSvdThreshold);
     if (!kept[q]) continue;
     const T qx = (V[0][q] * xb[0] + V[1][q] * xb[1]) + V[2][q] * xb[2];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) gb[d] += V[d][q] * dv(qx, lam[q]);
+    for (int d = 0; d < 3; ++d) R.gb[d] += V[d][q] * dv(qx, lam[q]);
   }
   // Daleckii-Krein: M-bar = V (F o (V^T sym(xb g^T) V)) V^T, F_ab = (f(l_a)-f(l_b))/(l_a-l_b),
   // f(l) = 1/l above the threshold, 0 below.
@@ -1039,7 +1028,6 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
         F = dv((kept[a] ? rc(lam[a]) : T(0)) - (kept[b] ? rc(lam[b]) : T(0)), lam[a] - lam[b]);
       B[a][b] = F * s;
     }
-  T Mb[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1049,46 +1037,102 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
       for (int u = 0; u < 3; ++u)
 #pragma unroll
         for (int w = 0; w < 3; ++w) s += V[a][u] * B[u][w] * V[b][w];
-      Mb[a][b] = s;
+      R.Mb[a][b] = s;
     }
   // x = c + P g with g = sum n (n.(p-c)): c-bar = xb - M gb, shared by the rows' points.
-  T cbm[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
-    cbm[a] = dv(xb[a] - ((Q.M[a][0] * gb[0] + Q.M[a][1] * gb[1]) + Q.M[a][2] * gb[2]), T(Q.m));
+    R.cbm[a] =
+        dv(xb[a] - ((Q.M[a][0] * R.gb[0] + Q.M[a][1] * R.gb[1]) + Q.M[a][2] * R.gb[2]), T(Q.m));
+}
+
+// One row of the adjoint: edge e with final normal nf. slot: where an averaged row's u-bar goes
+// (null: not averaged; its normal's adjoint goes to the cell's corners).
+template <typename T>
+__device__ __forceinline__ void row_adjoint(const Cell<T>& own, const RowsAdjoint<T>& R, int e,
+                                            const T nf[3], float* __restrict__ slot, T Sb[8]) {
+  T p[3];
+  hermite_point(own, e, p);
+  T n[3] = {nf[0], nf[1], nf[2]};
+  normalize3(n);
+  const T q[3] = {p[0] - R.c[0], p[1] - R.c[1], p[2] - R.c[2]};
+  const T nq = dot3(n, q), ng = dot3(n, R.gb);
+  T nb[3], pb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T Mn =
+        (T(2) * R.Mb[d][0] * n[0] + T(2) * R.Mb[d][1] * n[1]) + T(2) * R.Mb[d][2] * n[2];
+    nb[d] = Mn + R.gb[d] * nq + q[d] * ng;
+    pb[d] = n[d] * ng + R.cbm[d];
+  }
+  T ub[3];
+  normalize_vjp(nf, nb, ub);  // the solve re-normalizes each row
+  if (!slot) {
+    hermite_vjp(own, e, ub, pb, Sb);
+    return;
+  }
+  lerp_vjp(own, e, pb, Sb);  // the point enters the solve directly
+  slot[0] = (float)ub[0];
+  slot[1] = (float)ub[1];
+  slot[2] = (float)ub[2];
+}
+
+// Where an averaged row's u-bar goes: the face's slot for this member of its group.
+__device__ __forceinline__ float* face_slot(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                            int i, int j, int k, int e, uint32_t face,
+                                            float* __restrict__ ubar) {
+  int axis, x0, y0, z0;  // the group's geometry (the row is averaged)
+  edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+  return ubar + 3 * (4 * (uint64_t)face + member_of(axis, i, j, k, x0, y0, z0));
+}
+
+// Final normal and u-bar slot of a row whose kind is not a face (kRowEdgeSlot: the 4-cell average
+// formed here; kRowOwn: the cell's own normal, no slot).
+template <typename T>
+__device__ __forceinline__ float* row_nonface(const DcGridDev& G, const uint64_t* __restrict__ act,
+                                              int i, int j, int k, const Cell<T>& own, int e,
+                                              uint32_t kind, uint64_t v, float* __restrict__ ubar_e,
+                                              T nf[3]) {
+  if (kind == kRowEdgeSlot) {
+    int axis, x0, y0, z0;
+    edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+    group_sum(G, axis, x0, y0, z0, nf);
+    normalize3(nf);
+    return ubar_e + 3 * (12 * v + e);
+  }
+  raw_normal(own, e, nf);
+  return nullptr;
+}
+
+template <typename T, bool kGiven>
+__device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t* __restrict__ act,
+                              const FaceNormals<T>& fn, int i, int j, int k, uint64_t v,
+                              const float xb_in[3], const T* __restrict__ state,
+                              float* __restrict__ ubar, float* __restrict__ ubar_e,
+                              const uint32_t* __restrict__ rows, uint32_t* __restrict__ kinds_out,
+                              T Sb[8]) {
+  uint4 kinds[3];
+  if constexpr (kGiven) {
+    const uint4* r4 = reinterpret_cast<const uint4*>(rows + 12 * v);
+    kinds[0] = r4[0];
+    kinds[1] = r4[1];
+    kinds[2] = r4[2];
+  }
+  Cell<T> own;
+  load_cell(G, i, j, k, own);
+  Qef<T> Q;
+  T lam[3], V[3][3];
+  if constexpr (kGiven) {
+    load_state(state, Q, lam, V);
+    Q.m = __popc(crossing_mask(own));
+  } else {
+    assemble_any(G, act, fn, i, j, k, own, Q);
+    sym3_eig(Q.M, lam, V);
+  }
+  RowsAdjoint<T> R;
+  rows_adjoint(Q, lam, V, xb_in, R);
 #pragma unroll
   for (int q = 0; q < 8; ++q) Sb[q] = T(0);
-  // one row: final normal nf; slot: where an averaged row's u-bar goes (null: not averaged)
-  auto row = [&](int e, const T nf[3], float* slot) {
-    T p[3];
-    hermite_point(own, e, p);
-    T n[3] = {nf[0], nf[1], nf[2]};
-    normalize3(n);
-    const T q[3] = {p[0] - Q.c[0], p[1] - Q.c[1], p[2] - Q.c[2]};
-    const T nq = dot3(n, q), ng = dot3(n, gb);
-    T nb[3], pb[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const T Mn = (T(2) * Mb[d][0] * n[0] + T(2) * Mb[d][1] * n[1]) + T(2) * Mb[d][2] * n[2];
-      nb[d] = Mn + gb[d] * nq + q[d] * ng;
-      pb[d] = n[d] * ng + cbm[d];
-    }
-    T ub[3];
-    normalize_vjp(nf, nb, ub);  // the solve re-normalizes each row
-    if (!slot) {
-      hermite_vjp(own, e, ub, pb, Sb);
-      return;
-    }
-    lerp_vjp(own, e, pb, Sb);  // the point enters the solve directly
-    slot[0] = (float)ub[0];
-    slot[1] = (float)ub[1];
-    slot[2] = (float)ub[2];
-  };
-  auto face_slot = [&](int e, uint32_t face) {
-    int axis, x0, y0, z0;  // the group's geometry (the row is averaged)
-    edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
-    return ubar + 3 * (4 * (uint64_t)face + member_of(axis, i, j, k, x0, y0, z0));
-  };
   const uint32_t cross = crossing_mask(own);
   if constexpr (kGiven) {
     constexpr int kPre = 3;
@@ -1115,7 +1159,7 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
     for (uint32_t em = cross; em; em &= em - 1) {
       const int e = __builtin_ctz(em);
       T nf[3];
-      float* slot = nullptr;
+      float* slot;
       if ((fmask >> e) & 1u) {
         uint32_t face;
         if (q < kPre) {
@@ -1130,17 +1174,12 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
           nf[2] = fn.at(4 * (uint64_t)face + 2);
         }
         ++q;
-        slot = face_slot(e, face);
-      } else if ((smask >> e) & 1u) {
-        int axis, x0, y0, z0;
-        edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
-        group_sum(G, axis, x0, y0, z0, nf);
-        normalize3(nf);
-        slot = ubar_e + 3 * (12 * v + e);
+        slot = face_slot(G, act, i, j, k, e, face, ubar);
       } else {
-        raw_normal(own, e, nf);
+        slot = row_nonface(G, act, i, j, k, own, e, ((smask >> e) & 1u) ? kRowEdgeSlot : kRowOwn,
+                           v, ubar_e, nf);
       }
-      row(e, nf, slot);
+      row_adjoint(own, R, e, nf, slot, Sb);
     }
   } else {
     for (uint32_t em = cross; em; em &= em - 1) {
@@ -1153,10 +1192,35 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
         kinds_out[e * kKindStride] =
             !avg ? kRowOwn : (face != ~0ull ? (uint32_t)face : kRowEdgeSlot);
       float* slot = nullptr;
-      if (avg) slot = face != ~0ull ? face_slot(e, (uint32_t)face) : ubar_e + 3 * (12 * v + e);
-      row(e, nf, slot);
+      if (avg)
+        slot = face != ~0ull ? face_slot(G, act, i, j, k, e, (uint32_t)face, ubar)
+                             : ubar_e + 3 * (12 * v + e);
+      row_adjoint(own, R, e, nf, slot, Sb);
     }
   }
+}
+
+// The same for one row e of a cell whose kinds the forward recorded (kind: the row's kind), with
+// the cell's RowsAdjoint: for one lane of a cell's lane group.
+template <typename T>
+__device__ __forceinline__ void backward_row_given(const DcGridDev& G,
+                                                   const uint64_t* __restrict__ act,
+                                                   const FaceNormals<T>& fn, int i, int j, int k,
+                                                   uint64_t v, const Cell<T>& own,
+                                                   const RowsAdjoint<T>& R, int e, uint32_t kind,
+                                                   float* __restrict__ ubar,
+                                                   float* __restrict__ ubar_e, T Sb[8]) {
+  T nf[3];
+  float* slot;
+  if (kind < kRowEdgeSlot) {
+    nf[0] = fn.at(4 * (uint64_t)kind + 0);
+    nf[1] = fn.at(4 * (uint64_t)kind + 1);
+    nf[2] = fn.at(4 * (uint64_t)kind + 2);
+    slot = face_slot(G, act, i, j, k, e, kind, ubar);
+  } else {
+    slot = row_nonface(G, act, i, j, k, own, e, kind, v, ubar_e, nf);
+  }
+  row_adjoint(own, R, e, nf, slot, Sb);
 }
 
 // Between the phases, per face f: the adjoint of the face's averaged (normalized) normal taken
@@ -1179,6 +1243,32 @@ __device__ __forceinline__ void face_adjoint(const FaceNormals<double>& fn, uint
   o[0] = (S)accb[0];
   o[1] = (S)accb[1];
   o[2] = (S)accb[2];
+}
+
+// Adjoint of the 4-cell normal sum of averaged edge e of cell (i,j,k) that has no face: the
+// u-bar of the group's 4 cells (their per-edge slots) summed and taken through the average's
+// normalization.
+template <typename T>
+__device__ __forceinline__ void edge_slot_adjoint(const DcGridDev& G,
+                                                  const uint64_t* __restrict__ act,
+                                                  const uint32_t* __restrict__ wprefix, int i,
+                                                  int j, int k, int e,
+                                                  const float* __restrict__ ubar_e, T accb[3]) {
+  int axis, x0, y0, z0;  // the group's geometry only (phase 1 found it averaged)
+  edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
+  T us[3] = {T(0), T(0), T(0)};
+  for (int m = 0; m < 4; ++m) {
+    int x, y, z;
+    group_member(axis, m, x0, y0, z0, x, y, z);
+    const uint64_t vm = global_vertex(G, act, wprefix, x, y, z);
+    const float* slot = ubar_e + 3 * (12 * vm + group_edge(axis, m));
+    us[0] += (T)slot[0];
+    us[1] += (T)slot[1];
+    us[2] += (T)slot[2];
+  }
+  T acc[3];
+  group_sum(G, axis, x0, y0, z0, acc);
+  normalize_vjp(acc, us, accb);
 }
 
 // Adjoint phase 2 for one active cell: for each of its averaged rows, the adjoint of the group's
@@ -1237,25 +1327,33 @@ __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_
     T n[3];
     raw_normal(own, e, n);
     if (!(sq3(n[0], n[1], n[2]) > T(0))) continue;  // excluded from the average
-    if (!has_face) {
-      int axis, x0, y0, z0;  // the group's geometry only (phase 1 found it averaged)
-      edge_group(G, act, i, j, k, e, true, axis, x0, y0, z0);
-      T us[3] = {T(0), T(0), T(0)};
-      for (int m = 0; m < 4; ++m) {
-        int x, y, z;
-        group_member(axis, m, x0, y0, z0, x, y, z);
-        const uint64_t vm = global_vertex(G, act, wprefix, x, y, z);
-        const float* slot = ubar_e + 3 * (12 * vm + group_edge(axis, m));
-        us[0] += (T)slot[0];
-        us[1] += (T)slot[1];
-        us[2] += (T)slot[2];
-      }
-      T acc[3];
-      group_sum(G, axis, x0, y0, z0, acc);
-      normalize_vjp(acc, us, accb);
-    }
+    if (!has_face) edge_slot_adjoint(G, act, wprefix, i, j, k, e, ubar_e, accb);
     hermite_vjp(own, e, accb, zero3, Sb);
   }
+}
+
+// Phase 2 for one row e of a cell (kind: its kind from phase 1), for one lane of a cell's lane
+// group: the same terms as backward_faces adds for that row.
+template <typename T, typename S>
+__device__ __forceinline__ void backward_face_row(const DcGridDev& G,
+                                                  const uint64_t* __restrict__ act,
+                                                  const uint32_t* __restrict__ wprefix, int i,
+                                                  int j, int k, const Cell<T>& own, int e,
+                                                  uint32_t kind, const float* __restrict__ ubar,
+                                                  const float* __restrict__ ubar_e, T Sb[8]) {
+  T accb[3];
+  if (kind < kRowEdgeSlot) {
+    const S* a = reinterpret_cast<const S*>(ubar + 12 * (uint64_t)kind);
+    accb[0] = (T)a[0];
+    accb[1] = (T)a[1];
+    accb[2] = (T)a[2];
+  }
+  T n[3];
+  raw_normal(own, e, n);
+  if (!(sq3(n[0], n[1], n[2]) > T(0))) return;  // excluded from the average
+  if (kind == kRowEdgeSlot) edge_slot_adjoint(G, act, wprefix, i, j, k, e, ubar_e, accb);
+  const T zero3[3] = {T(0), T(0), T(0)};
+  hermite_vjp(own, e, accb, zero3, Sb);
 }
 
 }  // namespace dccell

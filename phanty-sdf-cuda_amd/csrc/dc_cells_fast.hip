@@ -24,50 +24,70 @@ hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+dim3 lane_grid(int64_t cells, int lanes) {
+  return dim3((unsigned)std::max<int64_t>(1, (cells * lanes + 255) / 256));
+}
+}  // namespace
+
 hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64) {
   const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
   const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
-  const dim3 grid((unsigned)a.cell_blocks), block(256);
+  const dim3 block(256);
+  const int lanes = phase == 1 ? a.lanes1 : a.lanes2;  // for the active cells' kernel
+  const dim3 grid = lane_grid(a.n_cells, lanes);
   // every launch takes the same arguments apart from the cell selection and the state
 #define DC_BWD_ARGS(FLAGGED, LIST, STATE)                                                         \
   a.grids, a.n_grids, a.act_bounds, a.act, a.wprefix, a.alist, a.flags, FLAGGED, LIST, a.ctr,     \
       a.capA, a.grad_vertices, a.ft, a.pw, n64, n32, a.capF, STATE, a.ubar, a.ubar_e, a.rows,     \
       a.contrib
+  // the active cells' kernel of phase P at `lanes` lanes per cell
+#define DC_BWD_CELLS(P, T, S, GIVEN, FLAGGED, STATE)                                              \
+  if (lanes == 16)                                                                                \
+    hipLaunchKernelGGL((k_bwd_cell_lanes<P, T, S, 16>), grid, block, 0, s,                        \
+                       DC_BWD_ARGS(FLAGGED, nullptr, STATE));                                     \
+  else if (lanes == 4)                                                                            \
+    hipLaunchKernelGGL((k_bwd_cell_lanes<P, T, S, 4>), grid, block, 0, s,                         \
+                       DC_BWD_ARGS(FLAGGED, nullptr, STATE));                                     \
+  else                                                                                            \
+    hipLaunchKernelGGL((k_bwd_cell<P, T, S, GIVEN>), grid, block, 0, s,                           \
+                       DC_BWD_ARGS(FLAGGED, nullptr, STATE));
   if (a.precision == DC_PREC_FP64) {
-    if (phase == 1)
-      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), grid, block, 0, s,
-                         DC_BWD_ARGS(0, nullptr, nullptr));
-    else
-      hipLaunchKernelGGL((k_bwd_cell<2, double, double, false>), grid, block, 0, s,
-                         DC_BWD_ARGS(0, nullptr, nullptr));
+    if (phase == 1)  // solved again per cell: one thread per cell
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), lane_grid(a.n_cells, 1), block,
+                         0, s, DC_BWD_ARGS(0, nullptr, nullptr));
+    else {
+      DC_BWD_CELLS(2, double, double, false, 0, nullptr)
+    }
     return hipGetLastError();
   }
-  // FP32 mode. The few FP64 cells first (on s64): their long per-thread chains start before the
-  // FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback solves'
-  // states when all of them were kept, else solves every one again (which cells got a state
-  // depends on the order of the fallback list, so a mix would not be reproducible).
-  const dim3 fgrid((unsigned)a.fallback_blocks);
+  // FP32 mode. The few FP64 cells first (on s64), 16 lanes per cell: their long chains start
+  // before the FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback
+  // solves' states when all of them were kept, else solves every one again, one thread per cell
+  // (which cells got a state depends on the order of the fallback list, so a mix would not be
+  // reproducible).
+  const dim3 fgrid16 = lane_grid(a.n_fallback, 16);
   if (phase == 1) {
     if (a.fb_state)
-      hipLaunchKernelGGL((k_bwd_cell<1, double, double, true>), fgrid, block, 0, s64,
+      hipLaunchKernelGGL((k_bwd_cell_lanes<1, double, double, 16>), fgrid16, block, 0, s64,
                          DC_BWD_ARGS(0, a.fblist, a.qstate64));
     else
-      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), fgrid, block, 0, s64,
-                         DC_BWD_ARGS(0, a.fblist, nullptr));
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), lane_grid(a.n_fallback, 1),
+                         block, 0, s64, DC_BWD_ARGS(0, a.fblist, nullptr));
   } else {
-    hipLaunchKernelGGL((k_bwd_cell<2, double, float, false>), fgrid, block, 0, s64,
+    hipLaunchKernelGGL((k_bwd_cell_lanes<2, double, float, 16>), fgrid16, block, 0, s64,
                        DC_BWD_ARGS(0, a.fblist, nullptr));
   }
   {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (phase == 1)
-    hipLaunchKernelGGL((k_bwd_cell<1, float, float, true>), grid, block, 0, s,
-                       DC_BWD_ARGS(1, nullptr, a.qstate));
-  else
-    hipLaunchKernelGGL((k_bwd_cell<2, float, float, false>), grid, block, 0, s,
-                       DC_BWD_ARGS(1, nullptr, nullptr));
+  if (phase == 1) {
+    DC_BWD_CELLS(1, float, float, true, 1, a.qstate)
+  } else {
+    DC_BWD_CELLS(2, float, float, false, 1, nullptr)
+  }
+#undef DC_BWD_CELLS
 #undef DC_BWD_ARGS
   return hipGetLastError();
 }

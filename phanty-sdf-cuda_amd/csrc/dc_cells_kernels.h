@@ -517,6 +517,99 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
   }
 }
 
+// Edge-parallel form of k_bwd_cell for grids too small to fill the device one cell per thread:
+// L lanes per cell (4 or 16, within a row of the wave); lane `sub` takes the cell's crossing rows
+// e = sub, sub + L, ... (< 12). The lanes' corner adjoints are summed over the group by an xor
+// butterfly (a fixed order: results are reproducible) and the group's first lane writes
+// (phase 1) or adds (phase 2) them. Phase 1 uses the forward's solve state and row kinds (FP32
+// mode), at state + kQefState * v (FP32 cells) or state + kQefState * t (FP64 cells, by list
+// index); every lane forms the cell's RowsAdjoint. The lanes of a group share their cell and so
+// their loop trip counts: the shuffles only read lanes of their own, active, group.
+template <int PHASE, typename T, typename S, int L>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
+    k_bwd_cell_lanes(const DcGridDev* __restrict__ grids, int n_grids,
+                     const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
+                     const uint32_t* __restrict__ wprefix, const uint64_t* __restrict__ alist,
+                     const uint32_t* __restrict__ flags, int skip_flagged,
+                     const uint32_t* __restrict__ list, const DcCounters* __restrict__ ctr,
+                     uint64_t cap, const float* __restrict__ gv, const uint64_t* __restrict__ ft,
+                     const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
+                     const float* __restrict__ nrm32, uint64_t capF, const T* __restrict__ state,
+                     float* __restrict__ ubar, float* __restrict__ ubar_e,
+                     uint32_t* __restrict__ rows, float* __restrict__ contrib) {
+  static_assert(L == 4 || L == 16, "4 or 16 lanes per cell");
+  const uint64_t n = list ? min(ctr->fallback, cap) : min(ctr->n_active, cap);
+  const uint64_t nf = min(ctr->n_faces, capF);
+  const FaceNormals<T> fn{ft, pw, nrm64, nrm32, nf};
+  const int sub = (int)(threadIdx.x & (L - 1));
+  const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / L;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; t < n; t += groups) {
+    const uint64_t v = list ? (uint64_t)list[t] : t;
+    if (!list && skip_flagged && flags[v]) continue;
+    const uint64_t entry = alist[v];
+    const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
+    const DcGridDev G = grids[g];
+    int i, j, k;
+    decode_cell(G, entry, i, j, k);
+    // L = 16: the lane's one row, its kind loaded with the cell
+    const uint32_t kind16 = L == 16 ? rows[12 * v + (uint64_t)min(sub, 11)] : 0u;
+    Cell<T> own;
+    load_cell(G, i, j, k, own);
+    const uint32_t cross = crossing_mask(own);
+    T Sb[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Sb[q] = T(0);
+    if (PHASE == 1) {
+      Qef<T> Q;
+      T lam[3], V[3][3];
+      load_state(state + kQefState * (sizeof(T) == 4 ? v : t), Q, lam, V);
+      Q.m = __popc(cross);
+      const float xb[3] = {gv[3 * v], gv[3 * v + 1], gv[3 * v + 2]};
+      RowsAdjoint<T> R;
+      rows_adjoint(Q, lam, V, xb, R);
+      if constexpr (L == 16) {
+        if (sub < 12 && ((cross >> sub) & 1u))
+          backward_row_given(G, act, fn, i, j, k, v, own, R, sub, kind16, ubar, ubar_e, Sb);
+      } else {
+        for (int e = sub; e < 12; e += L)
+          if ((cross >> e) & 1u)
+            backward_row_given(G, act, fn, i, j, k, v, own, R, e, rows[12 * v + e], ubar,
+                               ubar_e, Sb);
+      }
+    } else {
+      if constexpr (L == 16) {
+        if (sub < 12 && ((cross >> sub) & 1u) && kind16 != kRowOwn)
+          backward_face_row<T, S>(G, act, wprefix, i, j, k, own, sub, kind16, ubar, ubar_e, Sb);
+      } else {
+#pragma unroll 1
+        for (int e = sub; e < 12; e += L) {
+          if (!((cross >> e) & 1u)) continue;
+          const uint32_t kind = rows[12 * v + e];
+          if (kind != kRowOwn)
+            backward_face_row<T, S>(G, act, wprefix, i, j, k, own, e, kind, ubar, ubar_e, Sb);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int off = L / 2; off >= 1; off >>= 1) Sb[q] += __shfl_xor(Sb[q], off, L);
+    if (sub == 0) {
+      float4* out = reinterpret_cast<float4*>(contrib + 8 * v);
+      if (PHASE == 1) {
+        out[0] = make_float4((float)Sb[0], (float)Sb[1], (float)Sb[2], (float)Sb[3]);
+        out[1] = make_float4((float)Sb[4], (float)Sb[5], (float)Sb[6], (float)Sb[7]);
+      } else {
+        const float4 o0 = out[0], o1 = out[1];
+        out[0] = make_float4(o0.x + (float)Sb[0], o0.y + (float)Sb[1], o0.z + (float)Sb[2],
+                             o0.w + (float)Sb[3]);
+        out[1] = make_float4(o1.x + (float)Sb[4], o1.y + (float)Sb[5], o1.z + (float)Sb[6],
+                             o1.w + (float)Sb[7]);
+      }
+    }
+  }
+}
+
 // Between the adjoint phases: each face's averaged-normal adjoint, once per face (face_adjoint),
 // stored as double (plain mode) or float (FP32 mode).
 template <typename S>

@@ -27,6 +27,9 @@ struct DevBuf {
 };
 
 constexpr int kNumEvents = 10;  // 0..7 forward passes, 8..9 upload / backward
+// Adjoint: active cells per CU up to which the second per-cell kernel runs 4 lanes per cell
+// (its crossing edges split over the lanes) rather than one thread per cell.
+constexpr int64_t kWideCellsPerCu = 1024;
 
 // A launch sequence that is captured as a hipGraph the second time in a row it runs with the same
 // key (one-off sequences stay eager) and replayed while the key stays the same.
@@ -82,6 +85,7 @@ struct dc_hip_context {
   uint64_t capA = 0, capF = 0;
   uint64_t cap64 = 0;  // FP32 mode: FP64 cells whose solve state is kept for the adjoint
   int64_t cap64_opt = -1;  // option "fallback_state" (-1: capA / 4 + 64)
+  int bwd_lanes1 = 0, bwd_lanes2 = 0;  // options "adjoint_lanes", "adjoint_face_lanes" (0: auto)
   std::vector<DcGridDev> hgrids;
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, tw_b, at_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_tw = 0,
@@ -701,6 +705,10 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (!c->use_graph) drop_graph(c);
   } else if (k == "timing") {
     c->timing = value != 0;
+  } else if (k == "adjoint_lanes" || k == "adjoint_face_lanes") {
+    if (value != 0 && value != 1 && value != 4 && value != 16)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, k + " must be 0, 1, 4 or 16");
+    (k == "adjoint_lanes" ? c->bwd_lanes1 : c->bwd_lanes2) = (int)value;
   } else if (k == "fallback_state") {
     if (value < -1) return fail(c, DC_HIP_ERR_INVALID_ARG, "fallback_state must be >= -1");
     c->cap64_opt = value;
@@ -909,10 +917,15 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.precision = c->precision;
   a.ctr = as<DcCounters>(c->counters);
   a.capA = c->capA;
-  a.cell_blocks = std::max<int64_t>(1, (c->nv + 255) / 256);
   const int64_t n_fb = (int64_t)c->h_counters->fallback;
-  a.fallback_blocks = std::max<int64_t>(1, (n_fb + 255) / 256);
   a.fb_state = n_fb <= (int64_t)c->cap64 ? 1 : 0;
+  a.n_cells = c->nv;
+  a.n_fallback = n_fb;
+  // phase 2 edge-parallel (4 lanes per cell) while one thread per cell would leave the device
+  // idle; phase 1 repeats its per-cell setup on every lane, which costs more than it saves
+  const bool small = c->nv <= (int64_t)c->cu_count * kWideCellsPerCu;
+  a.lanes1 = c->bwd_lanes1 ? c->bwd_lanes1 : 1;
+  a.lanes2 = c->bwd_lanes2 ? c->bwd_lanes2 : (small ? 4 : 1);
   a.grad_vertices = grad_vertices;
   a.ft = as<uint64_t>(c->ft);
   a.pw = as<uint32_t>(c->pw);

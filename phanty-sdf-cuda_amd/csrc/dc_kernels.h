@@ -109,8 +109,9 @@ struct DcBwdArgs {
   int precision;
   const DcCounters* ctr;
   uint64_t capA;
-  int64_t cell_blocks;      // blocks of 256 over all active cells
-  int64_t fallback_blocks;  // blocks of 256 over the FP32 mode's FP64 cells
+  int64_t n_cells;          // active cells (host copy of the forward's count)
+  int64_t n_fallback;       // FP32 mode's FP64 cells (host copy)
+  int lanes1, lanes2;       // lanes per active cell in the phase 1 / phase 2 kernels: 1, 4 or 16
   const float* grad_vertices;
   const uint64_t* ft;
   const uint32_t* pw;

@@ -137,3 +137,33 @@ def test_adjoint_fallback_state_capacity(kept):
     assert np.abs(grads[1] - want).max() / scale <= 2e-3
     assert np.abs(grads[1] - grads[0]).max() / scale <= 1e-4
     assert np.array_equal(grads[1], grads[2])  # reproducible across contexts
+
+
+@pytest.mark.parametrize("precision", ["plain-fp64", "fp32-fallback"])
+def test_adjoint_lanes_agree(precision):
+    """The per-cell adjoint kernels at one thread per cell and at 4 and 16 lanes per cell (edges
+    split over the lanes) sum the same terms in other orders: equal to 1e-5 (relative,
+    max-norm), each reproducible."""
+    import torch
+    g = synth.csg8(36, seed=21)
+    phi = torch.from_numpy(np.asarray(g.values, np.float32)).reshape(g.nz, g.ny, g.nx).cuda()
+    grads = {}
+    for lanes in (1, 4, 16, 16):
+        be = HipDualContouringBackend(precision)
+        be.set_option("adjoint_lanes", lanes)
+        be.set_option("adjoint_face_lanes", lanes)
+        (V, F), _ = be.extract_tensor(phi, (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(5)
+        w = torch.randn(V.shape, device="cuda", generator=gen)
+        gphi = torch.empty_like(phi)
+        be.backward(w, gphi)
+        got = gphi.cpu().numpy().ravel().astype(np.float64)
+        if lanes in grads:
+            assert np.array_equal(got, grads[lanes])
+        grads[lanes] = got
+    scale = np.abs(grads[1]).max()
+    for lanes in (4, 16):
+        rel = np.abs(grads[lanes] - grads[1]).max() / scale
+        print(f"[adjoint] {precision}: lanes 1 vs {lanes} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-5
