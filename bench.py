@@ -254,8 +254,12 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
     g = synth.csg8(n, seed=1234)
     phi, _ = _device_grid(g)
     be = HipDualContouringBackend("plain-fp64", device=0)
-    be.set_stream(torch.cuda.current_stream(0).cuda_stream)
+    # a created stream (default priority): the adjoint then returns once enqueued, like any
+    # stream-ordered library call; the plain forward is timed on the same stream
+    stream = torch.cuda.Stream(device=0)
+    be.set_stream(stream.cuda_stream)
     be.set_option("timing", 0)  # replayed launch sequences
+    torch.cuda.synchronize()  # the grid was uploaded on the default stream
     for key, value in (options or {}).items():
         be.set_option(key, value)
     cg = be._cgrid(g, phi.data_ptr())
@@ -272,6 +276,7 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
     gen.manual_seed(0)
     gv = torch.randn((nv, 3), dtype=torch.float32, device="cuda:0", generator=gen)
     gphi = torch.empty_like(phi)
+    torch.cuda.synchronize()  # made on the default stream, used on `stream`
 
     def fb():
         fwd(_lib.DC_HIP_FP32_FALLBACK)

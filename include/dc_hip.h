@@ -80,7 +80,9 @@ typedef struct dc_hip_context dc_hip_context;
 /* Creates a context on HIP device `device` with its own stream. */
 int dc_hip_create(int device, dc_hip_context** out);
 void dc_hip_destroy(dc_hip_context* ctx);
-/* Runs all later work on `stream` (a hipStream_t; NULL restores the context's own stream). */
+/* Runs all later work on `stream` (a hipStream_t). NULL, the default, means the null stream's
+ * order: the work runs on the context's own stream after the work already queued on the null
+ * stream, and each call returns when its work is complete. */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_layers" (8, 16, 32, 64: cell layers per marking wave tile), "mark_waves"
  * (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
@@ -133,7 +135,9 @@ int dc_hip_extract_batch(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t 
 /* Adjoint of the last extraction at fixed topology (no counterpart in the reference):
  * grad_values[N_s] (device, overwritten) = dL/dvalues for grad_vertices[3*n_vertices]
  * (device) = dL/dvertices. Deterministic: per-cell stencil contributions are gathered in a
- * fixed order, no floating-point atomics. */
+ * fixed order, no floating-point atomics. Returns when the gradient is complete, except with
+ * the "timing" option off on a stream set by dc_hip_set_stream: then it returns once the work is
+ * enqueued on that stream (later work on the stream sees the gradient). */
 int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad_values,
                     dc_hip_stats* stats);
 

@@ -625,7 +625,7 @@ int dc_hip_create(int device, dc_hip_context** out) {
     delete c;
     return DC_HIP_ERR_HIP;
   }
-  c->stream = c->own;
+  c->stream = nullptr;  // the null stream's order (work_stream)
   for (int q = 0; q < kNumEvents; ++q) {
     if (hipEventCreate(&c->ev[q]) != hipSuccess) {
       dc_hip_destroy(c);
@@ -655,6 +655,7 @@ int dc_hip_create(int device, dc_hip_context** out) {
 void dc_hip_destroy(dc_hip_context* c) {
   if (!c) return;
   hipSetDevice(c->device);
+  if (c->own) hipStreamSynchronize(c->own);
   if (c->stream) hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
@@ -681,7 +682,7 @@ void dc_hip_destroy(dc_hip_context* c) {
 
 int dc_hip_set_stream(dc_hip_context* c, void* stream) {
   if (!c) return DC_HIP_ERR_INVALID_ARG;
-  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
+  c->stream = reinterpret_cast<hipStream_t>(stream);
   return DC_HIP_OK;
 }
 
@@ -968,8 +969,11 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                       return DC_HIP_OK;
                     });
   if (rc) return rc;
+  // On a caller's stream without timing the work stays enqueued (stream order covers it); on the
+  // null stream the call returns on completion (an event wait by the null stream after a graph
+  // launch on the context's stream did not order later null-stream work on this runtime).
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], s), "event");
-  DC_TRY(c, hipStreamSynchronize(s), "backward");
+  if (c->timing || c->stream == nullptr) DC_TRY(c, hipStreamSynchronize(s), "backward");
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->vertex_count = c->nv;
