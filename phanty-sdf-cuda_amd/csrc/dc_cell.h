@@ -138,9 +138,9 @@ __device__ __forceinline__ bool strict_change(float fa, float fb) {
   return (fa < 0.f && fb > 0.f) || (fa > 0.f && fb < 0.f);
 }
 
-// S[] holds the corner values; s(q) for a run-time corner index selects from them with a chain
-// of compares (indexing the register array would move it to scratch memory, and re-reading the
-// sample would wait on memory again).
+// S[] holds the corner values; s(q) for a run-time corner index picks one with a tree of three
+// selects on the bits of q (indexing the register array would move it to scratch memory, and
+// re-reading the sample would wait on memory again).
 template <typename T>
 struct Cell {
   T S[8];
@@ -148,7 +148,16 @@ struct Cell {
   gfloat base;
   int64_t sy, sz;
   __device__ __forceinline__ T s(int q) const {
-    return (T)base[cbx(q) + sy * cby(q) + sz * cbz(q)];
+    T v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      v[r] = S[r];
+      asm("" : "+v"(v[r]));  // a register value, so the selects stay selects
+    }
+    const bool b0 = q & 1, b1 = q & 2, b2 = q & 4;
+    const T lo = b1 ? (b0 ? v[3] : v[2]) : (b0 ? v[1] : v[0]);
+    const T hi = b1 ? (b0 ? v[7] : v[6]) : (b0 ? v[5] : v[4]);
+    return b2 ? hi : lo;
   }
   __device__ __forceinline__ T cx(int q) const { return cbx(q) ? X1 : X0; }
   __device__ __forceinline__ T cy(int q) const { return cby(q) ? Y1 : Y0; }
