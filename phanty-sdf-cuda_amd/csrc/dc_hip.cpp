@@ -216,7 +216,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
     tiles += dc_mark_tiles(g, c->eff_layers);
-    tw += 3ll * g.W * ((g.cy + 7) / 8);
+    tw += 3ll * g.W * ((g.cy + kFtRows - 1) / kFtRows);
     at += (int64_t)g.cz * ((g.cy + 63) / 64) * ((g.W + 7) / 8);
     samp += (int64_t)g.nx * g.ny * g.nz;
     sr += ((int64_t)g.ny * g.nz + c->eff_gather_rows - 1) / c->eff_gather_rows;  // gather groups

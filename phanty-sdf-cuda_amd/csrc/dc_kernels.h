@@ -5,6 +5,13 @@
 
 #include "dc_internal.h"
 
+// Cell rows per wave of the face transpose (its wave count: 3 * W * ceil(cy / kFtRows) per grid).
+// 4 rows: 36 us at 512^3, against 46 (8 rows: too few waves to hide the loads) and 38-44 (2, 1).
+#ifndef DC_FT_ROWS
+#define DC_FT_ROWS 4
+#endif
+constexpr int kFtRows = DC_FT_ROWS;
+
 struct DcMarkArgs {
   const DcGridDev* grids;
   int n_grids;
@@ -16,7 +23,7 @@ struct DcMarkArgs {
   uint64_t* act_kcj;    // the same words as the marking pass writes them, [k][c][j]
   uint64_t* ftx;        // face words over x: 3 planes of plane_words, [k][c][j] each
   int64_t plane_words;  // act words of the batch
-  const int64_t* tw_bounds;  // n_grids + 1, face transpose waves (3 * W * ceil(cy/8) per grid)
+  const int64_t* tw_bounds;  // n_grids + 1, face transpose waves (3 * W * ceil(cy/kFtRows))
   int64_t n_tw;
   const int64_t* at_bounds;  // n_grids + 1, active transpose waves (cz * ceil(cy/64) * ceil(W/8))
   int64_t n_at;

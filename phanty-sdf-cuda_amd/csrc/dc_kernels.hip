@@ -213,10 +213,9 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x) {
 }
 
 // Face words over x ([a][k][c][j]) -> face planes over k (word (a, k/64, j, i), bit k%64) and
-// the pencil face counts in face order (a, i, j). One wave per (a, c, 8 rows): per 64 layers,
-// lane = layer loads the 8 rows' words (64 contiguous bytes) and eight wave transposes give
+// the pencil face counts in face order (a, i, j). One wave per (a, c, kFtRows rows): per 64
+// layers, lane = layer loads the rows' words (contiguous) and one wave transpose per row gives
 // lane = cell i.
-constexpr int kFtRows = 8;
 
 __global__ void __launch_bounds__(256)
     k_ft_transpose(const DcGridDev* __restrict__ grids, int n_grids,
