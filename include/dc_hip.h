@@ -92,7 +92,8 @@ int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
  * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve),
  * "adjoint_lanes", "adjoint_face_lanes" (0, 1, 4, 16: lanes per active cell in the adjoint's
  * first and second per-cell kernels; 0, the default, picks by mesh size: several lanes where one
- * thread per cell would leave the device idle),
+ * thread per cell would leave the device idle), "adjoint_fallback_lanes" (0, 1, 16: the same for
+ * FP32 mode's FP64 cells; 0, the default: 16 while they are few),
  * "fallback_state" (FP32 mode: up to how many FP64-solved cells keep their solve states for the
  * adjoint; with more, the adjoint solves them all again; -1, the default: a quarter of the
  * active-cell capacity). */

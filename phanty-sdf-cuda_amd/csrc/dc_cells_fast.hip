@@ -61,21 +61,28 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
     }
     return hipGetLastError();
   }
-  // FP32 mode. The few FP64 cells first (on s64), 16 lanes per cell: their long chains start
-  // before the FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback
+  // FP32 mode. The few FP64 cells first (on s64), at lanes64 lanes per cell: their long chains
+  // start before the FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback
   // solves' states when all of them were kept, else solves every one again, one thread per cell
   // (which cells got a state depends on the order of the fallback list, so a mix would not be
   // reproducible).
-  const dim3 fgrid16 = lane_grid(a.n_fallback, 16);
+  const dim3 fgrid = lane_grid(a.n_fallback, a.lanes64);
+  const bool wide64 = a.lanes64 == 16;
   if (phase == 1) {
-    if (a.fb_state)
-      hipLaunchKernelGGL((k_bwd_cell_lanes<1, double, double, 16>), fgrid16, block, 0, s64,
+    if (a.fb_state && wide64)
+      hipLaunchKernelGGL((k_bwd_cell_lanes<1, double, double, 16>), fgrid, block, 0, s64,
+                         DC_BWD_ARGS(0, a.fblist, a.qstate64));
+    else if (a.fb_state)
+      hipLaunchKernelGGL((k_bwd_cell<1, double, double, true>), fgrid, block, 0, s64,
                          DC_BWD_ARGS(0, a.fblist, a.qstate64));
     else
       hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), lane_grid(a.n_fallback, 1),
                          block, 0, s64, DC_BWD_ARGS(0, a.fblist, nullptr));
+  } else if (wide64) {
+    hipLaunchKernelGGL((k_bwd_cell_lanes<2, double, float, 16>), fgrid, block, 0, s64,
+                       DC_BWD_ARGS(0, a.fblist, nullptr));
   } else {
-    hipLaunchKernelGGL((k_bwd_cell_lanes<2, double, float, 16>), fgrid16, block, 0, s64,
+    hipLaunchKernelGGL((k_bwd_cell<2, double, float, false>), fgrid, block, 0, s64,
                        DC_BWD_ARGS(0, a.fblist, nullptr));
   }
   {

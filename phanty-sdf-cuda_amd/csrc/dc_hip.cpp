@@ -86,6 +86,7 @@ struct dc_hip_context {
   uint64_t cap64 = 0;  // FP32 mode: FP64 cells whose solve state is kept for the adjoint
   int64_t cap64_opt = -1;  // option "fallback_state" (-1: capA / 4 + 64)
   int bwd_lanes1 = 0, bwd_lanes2 = 0;  // options "adjoint_lanes", "adjoint_face_lanes" (0: auto)
+  int bwd_lanes64 = 0;                 // option "adjoint_fallback_lanes" (0: auto)
   std::vector<DcGridDev> hgrids;
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, tw_b, at_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_tw = 0,
@@ -710,6 +711,10 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (value != 0 && value != 1 && value != 4 && value != 16)
       return fail(c, DC_HIP_ERR_INVALID_ARG, k + " must be 0, 1, 4 or 16");
     (k == "adjoint_lanes" ? c->bwd_lanes1 : c->bwd_lanes2) = (int)value;
+  } else if (k == "adjoint_fallback_lanes") {
+    if (value != 0 && value != 1 && value != 16)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "adjoint_fallback_lanes must be 0, 1 or 16");
+    c->bwd_lanes64 = (int)value;
   } else if (k == "fallback_state") {
     if (value < -1) return fail(c, DC_HIP_ERR_INVALID_ARG, "fallback_state must be >= -1");
     c->cap64_opt = value;
@@ -927,6 +932,9 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   const bool small = c->nv <= (int64_t)c->cu_count * kWideCellsPerCu;
   a.lanes1 = c->bwd_lanes1 ? c->bwd_lanes1 : 1;
   a.lanes2 = c->bwd_lanes2 ? c->bwd_lanes2 : (small ? 4 : 1);
+  // the FP64 cells: 16 lanes while few (they finish early on the side stream); when many, the
+  // 16-fold per-cell setup competes with the FP32 cells' kernel for the device
+  a.lanes64 = c->bwd_lanes64 ? c->bwd_lanes64 : (n_fb <= (int64_t)c->cu_count * 32 ? 16 : 1);
   a.grad_vertices = grad_vertices;
   a.ft = as<uint64_t>(c->ft);
   a.pw = as<uint32_t>(c->pw);

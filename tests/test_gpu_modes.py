@@ -153,6 +153,7 @@ def test_adjoint_lanes_agree(precision):
         be = HipDualContouringBackend(precision)
         be.set_option("adjoint_lanes", lanes)
         be.set_option("adjoint_face_lanes", lanes)
+        be.set_option("adjoint_fallback_lanes", 1 if lanes == 1 else 16)
         (V, F), _ = be.extract_tensor(phi, (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
         gen = torch.Generator(device="cuda")
         gen.manual_seed(5)
