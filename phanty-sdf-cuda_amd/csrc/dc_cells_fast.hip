@@ -61,14 +61,14 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
     }
     return hipGetLastError();
   }
-  // FP32 mode. The few FP64 cells first (on s64), at lanes64 lanes per cell: their long chains
-  // start before the FP32 cells' blocks fill the device. Phase 1 of the FP64 cells uses their fallback
-  // solves' states when all of them were kept, else solves every one again, one thread per cell
-  // (which cells got a state depends on the order of the fallback list, so a mix would not be
+  // FP32 mode. Phase 1: the few FP64 cells first (on s64), at lanes64 lanes per cell: their long
+  // chains start before the FP32 cells' blocks fill the device. They use their fallback solves'
+  // states when all of them were kept, else solve every one again, one thread per cell (which
+  // cells got a state depends on the order of the fallback list, so a mix would not be
   // reproducible).
   const dim3 fgrid = lane_grid(a.n_fallback, a.lanes64);
   const bool wide64 = a.lanes64 == 16;
-  if (phase == 1) {
+  if (phase == 1) {  // s64: the FP64 cells (phase 2 runs on every cell below)
     if (a.fb_state && wide64)
       hipLaunchKernelGGL((k_bwd_cell_lanes<1, double, double, 16>), fgrid, block, 0, s64,
                          DC_BWD_ARGS(0, a.fblist, a.qstate64));
@@ -78,12 +78,6 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
     else
       hipLaunchKernelGGL((k_bwd_cell<1, double, double, false>), lane_grid(a.n_fallback, 1),
                          block, 0, s64, DC_BWD_ARGS(0, a.fblist, nullptr));
-  } else if (wide64) {
-    hipLaunchKernelGGL((k_bwd_cell_lanes<2, double, float, 16>), fgrid, block, 0, s64,
-                       DC_BWD_ARGS(0, a.fblist, nullptr));
-  } else {
-    hipLaunchKernelGGL((k_bwd_cell<2, double, float, false>), fgrid, block, 0, s64,
-                       DC_BWD_ARGS(0, a.fblist, nullptr));
   }
   {
     const hipError_t e = hipGetLastError();
@@ -92,7 +86,9 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
   if (phase == 1) {
     DC_BWD_CELLS(1, float, float, true, 1, a.qstate)
   } else {
-    DC_BWD_CELLS(2, float, float, false, 1, nullptr)
+    // every cell: phase 2 takes each averaged row's face adjoint through the cell's own raw
+    // normal, which does not depend on how well the cell's solve was conditioned
+    DC_BWD_CELLS(2, float, float, false, 0, nullptr)
   }
 #undef DC_BWD_CELLS
 #undef DC_BWD_ARGS

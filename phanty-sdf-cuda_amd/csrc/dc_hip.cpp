@@ -956,10 +956,11 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   key += c->dev_key;
   rc = run_captured(c, c->bwd, std::move(key), c->use_graph && !c->timing, s,
                     [&](hipStream_t st) -> int {
-                      // FP32 mode: the FP64 cells' kernels run beside the FP32 cells' on the side
-                      // stream (disjoint cells and slots), joined before the next phase
-                      const bool fork = a.precision != DC_PREC_FP64;
+                      // FP32 mode: phase 1 of the FP64 cells runs beside the FP32 cells' on the
+                      // side stream (disjoint cells and slots), joined before the face pass;
+                      // phase 2 is one kernel over all cells
                       for (int phase = 1; phase <= 2; ++phase) {
+                        const bool fork = a.precision != DC_PREC_FP64 && phase == 1;
                         if (fork) {
                           DC_TRY(c, hipEventRecord(c->ev_fork, st), "fork");
                           DC_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "fork");
