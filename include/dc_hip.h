@@ -80,9 +80,8 @@ typedef struct dc_hip_context dc_hip_context;
 /* Creates a context on HIP device `device` with its own stream. */
 int dc_hip_create(int device, dc_hip_context** out);
 void dc_hip_destroy(dc_hip_context* ctx);
-/* Runs all later work on `stream` (a hipStream_t). NULL, the default, means the null stream's
- * order: the work runs on the context's own stream after the work already queued on the null
- * stream, and each call returns when its work is complete. */
+/* Runs all later work on `stream` (a hipStream_t). NULL, the default, is the null stream: each
+ * call runs there and returns when its work is complete. */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_layers" (8, 16, 32, 64: cell layers per marking wave tile), "mark_waves"
  * (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the

@@ -52,7 +52,7 @@ struct Captured {
 
 struct dc_hip_context {
   int device = 0;
-  hipStream_t own = nullptr;
+  hipStream_t own = nullptr;     // captures sequences meant for the null stream
   hipStream_t stream = nullptr;
   std::string err;
   int mark_layers = 8;   // default tile: 8 cell layers x 4 waves per block (fastest at 512^3)
@@ -75,7 +75,6 @@ struct dc_hip_context {
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
   hipEvent_t ev[kNumEvents] = {};
-  hipEvent_t ev_join = nullptr;  // orders the context's stream after the null stream
   hipStream_t side = nullptr;    // second stream for independent work beside the main one
   hipEvent_t ev_fork = nullptr, ev_side = nullptr;
   // last forward
@@ -261,7 +260,9 @@ void drop_graph(dc_hip_context* c) {
 
 // Runs `enqueue(s)` (a launch sequence without host synchronisation) on stream s: eagerly, or
 // through cap's graph (captured now when `key` repeats the previous run's, replayed while it
-// matches the captured one). graphs = false forces an eager run.
+// matches the captured one). graphs = false forces an eager run. The null stream cannot be
+// captured: its sequence is captured on the context's own stream (capture records, it runs
+// nothing) and the graph is launched on the null stream.
 template <class Enqueue>
 int run_captured(dc_hip_context* c, Captured& cap, std::string key, bool graphs, hipStream_t s,
                  Enqueue&& enqueue) {
@@ -272,10 +273,11 @@ int run_captured(dc_hip_context* c, Captured& cap, std::string key, bool graphs,
   if (!replay && !capture) return enqueue(s);
   if (capture) {
     cap.drop();
-    DC_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-    const int rc = enqueue(s);
+    const hipStream_t cs = s ? s : c->own;
+    DC_TRY(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+    const int rc = enqueue(cs);
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &g);
+    const hipError_t e = hipStreamEndCapture(cs, &g);
     if (rc) {
       if (g) hipGraphDestroy(g);
       return rc;
@@ -289,14 +291,9 @@ int run_captured(dc_hip_context* c, Captured& cap, std::string key, bool graphs,
   return DC_HIP_OK;
 }
 
-// The null stream cannot be captured: work for it runs on the context's stream, after the work
-// already queued on the null stream (the calls return only when their own work is complete).
+// The stream the calls run on: the caller's (set by dc_hip_set_stream), or the null stream.
 int work_stream(dc_hip_context* c, hipStream_t* s) {
   *s = c->stream;
-  if (*s != nullptr) return DC_HIP_OK;
-  DC_TRY(c, hipEventRecord(c->ev_join, nullptr), "join null stream");
-  DC_TRY(c, hipStreamWaitEvent(c->own, c->ev_join, 0), "join null stream");
-  *s = c->own;
   return DC_HIP_OK;
 }
 
@@ -639,10 +636,6 @@ int dc_hip_create(int device, dc_hip_context** out) {
     dc_hip_destroy(c);
     return DC_HIP_ERR_HIP;
   }
-  if (hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
-    dc_hip_destroy(c);
-    return DC_HIP_ERR_HIP;
-  }
   if (hipHostMalloc((void**)&c->h_counters, sizeof(DcCounters), hipHostMallocDefault) !=
       hipSuccess) {
     dc_hip_destroy(c);
@@ -667,7 +660,6 @@ void dc_hip_destroy(dc_hip_context* c) {
     if (b->p) hipFree(b->p);
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
-  if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_side) hipEventDestroy(c->ev_side);
   drop_graph(c);
@@ -979,8 +971,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
                     });
   if (rc) return rc;
   // On a caller's stream without timing the work stays enqueued (stream order covers it); on the
-  // null stream the call returns on completion (an event wait by the null stream after a graph
-  // launch on the context's stream did not order later null-stream work on this runtime).
+  // null stream the call returns on completion, like every call there.
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], s), "event");
   if (c->timing || c->stream == nullptr) DC_TRY(c, hipStreamSynchronize(s), "backward");
   if (stats) {
