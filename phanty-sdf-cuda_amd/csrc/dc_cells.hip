@@ -15,9 +15,11 @@ hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s) {
                      a.grids, a.n_grids, a.pen_bounds, a.n_pen, a.ft, a.pbase, a.pw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = dc_launch_face_list(a.ft, a.n_ft, a.tile_sum, a.tile_base, a.flist, a.capF,
-                          &a.ctr->n_list, s);
-  if (e != hipSuccess) return e;
+  if (!a.list_done) {
+    e = dc_launch_face_list(a.ft, a.n_ft, a.tile_sum, a.tile_base, a.flist, a.capF,
+                            &a.ctr->n_list, s);
+    if (e != hipSuccess) return e;
+  }
   if (a.precision != DC_PREC_FP64) return dc_launch_face_normals_fp32(a, s);
   hipLaunchKernelGGL(k_face_normals<double>, dim3((unsigned)a.face_blocks), dim3(256), 0, s,
                      a.grids, a.n_grids, a.ft_bounds, a.flist, a.ft, a.pw, a.ctr, a.capF,

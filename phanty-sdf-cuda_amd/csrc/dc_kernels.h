@@ -58,6 +58,27 @@ struct DcQefArgs {
   uint64_t cap64;    // entries of qstate64
 };
 
+// The three scans of one pass in shared launches (dc_launch_scan3): the active words (vertex ids
+// and cell list), the pencil face counts (pencil bases) and the face words (face list).
+struct DcScan3Args {
+  const uint64_t* act;
+  int64_t n_words;
+  uint32_t* wprefix;
+  uint64_t* alist;
+  uint64_t capA;
+  const uint32_t* pcount;
+  int64_t n_pen;
+  uint32_t* pbase;
+  const uint64_t* ft;
+  int64_t n_ft;
+  uint64_t* flist;
+  uint64_t capF;
+  int64_t tiles_a, tiles_p, tiles_f;  // dc_scan_tiles of each
+  uint32_t* tile_sum;   // tiles_a + tiles_p + tiles_f entries, in that order
+  uint64_t* tile_base;
+  DcCounters* ctr;      // n_active, n_faces, n_list
+};
+
 struct DcFaceNormalArgs {
   const DcGridDev* grids;
   int n_grids;
@@ -76,6 +97,7 @@ struct DcFaceNormalArgs {
   int64_t face_blocks;  // grid-stride blocks of the per-face kernels
   int precision;
   void* nrm;
+  int list_done;        // the face list is already built (dc_launch_scan3)
 };
 
 struct DcEmitArgs {
@@ -156,6 +178,8 @@ hipError_t dc_launch_scan_act(const uint64_t* act, int64_t n_words, uint32_t* ti
 hipError_t dc_launch_scan_pen(const uint32_t* pcount, int64_t n_pen, uint32_t* tile_sum,
                               uint64_t* tile_base, uint32_t* pbase, uint64_t* total,
                               hipStream_t s);
+// All three scans in three launches (reduce, tile scan, apply), each launch covering all three.
+hipError_t dc_launch_scan3(const DcScan3Args& a, hipStream_t s);
 hipError_t dc_launch_face_list(const uint64_t* ft, int64_t n_ft, uint32_t* tile_sum,
                                uint64_t* tile_base, uint64_t* flist, uint64_t cap,
                                uint64_t* total, hipStream_t s);

@@ -343,11 +343,13 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   return v;
 }
 
+// The scan steps as block bodies over an explicit tile index, so that the three scans' steps can
+// also share one launch (k_scan3_*).
 template <typename T>
-__global__ void __launch_bounds__(kScanThreads)
-    k_tile_reduce(const T* __restrict__ items, int64_t n, uint32_t* __restrict__ tile_sum) {
+__device__ __forceinline__ void tile_reduce(const T* __restrict__ items, int64_t n,
+                                            uint32_t* __restrict__ tile_sum, int64_t tile) {
   __shared__ uint32_t s_w[kScanThreads / 64];
-  const int64_t base = seg_base(blockIdx.x);
+  const int64_t base = seg_base(tile);
   uint32_t cnt = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) cnt += weight(load_clamped(items, base + 64 * q, n));
@@ -358,14 +360,14 @@ __global__ void __launch_bounds__(kScanThreads)
     uint32_t t = 0;
 #pragma unroll
     for (int w = 0; w < kScanThreads / 64; ++w) t += s_w[w];
-    tile_sum[blockIdx.x] = t;
+    tile_sum[tile] = t;
   }
 }
 
 // One block: exclusive scan of the tile sums (each thread takes a run of consecutive tiles).
-__global__ void __launch_bounds__(kTileScanThreads)
-    k_tile_scan(const uint32_t* __restrict__ tile_sum, int64_t n_tiles,
-                uint64_t* __restrict__ tile_base, uint64_t* __restrict__ total) {
+__device__ __forceinline__ void tile_scan(const uint32_t* __restrict__ tile_sum, int64_t n_tiles,
+                                          uint64_t* __restrict__ tile_base,
+                                          uint64_t* __restrict__ total) {
   __shared__ uint64_t s_w[kTileScanThreads / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t per = (n_tiles + kTileScanThreads - 1) / kTileScanThreads;
@@ -418,11 +420,12 @@ __device__ __forceinline__ void append_bits(uint64_t w, int64_t word, uint64_t p
 
 // contouring.cpp:509-524: vertex id = rank of the active cell in (k, j, i) order. Writes the
 // per-word prefix and the active-cell list in vertex order.
-__global__ void __launch_bounds__(kScanThreads)
-    k_scan_act(const uint64_t* __restrict__ act, int64_t n_words,
-               const uint64_t* __restrict__ tile_base, uint32_t* __restrict__ wprefix,
-               uint64_t* __restrict__ alist, uint64_t cap) {
-  const int64_t base = seg_base(blockIdx.x);
+__device__ __forceinline__ void scan_act_tile(const uint64_t* __restrict__ act, int64_t n_words,
+                                              const uint64_t* __restrict__ tile_base,
+                                              uint32_t* __restrict__ wprefix,
+                                              uint64_t* __restrict__ alist, uint64_t cap,
+                                              int64_t tile) {
+  const int64_t base = seg_base(tile);
   uint64_t w[kScanItems];
   uint32_t cnt = 0;
 #pragma unroll
@@ -430,7 +433,7 @@ __global__ void __launch_bounds__(kScanThreads)
     w[q] = load_clamped(act, base + 64 * q, n_words);
     cnt += (uint32_t)__popcll(w[q]);
   }
-  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[blockIdx.x]);
+  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[tile]);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const int64_t idx = base + 64 * q;
@@ -445,10 +448,10 @@ __global__ void __launch_bounds__(kScanThreads)
 
 // First face of every pencil: exclusive scan of the pencil face counts, which are stored in the
 // CPU backend's face order (grid, axis, i, j; k is inside a pencil).
-__global__ void __launch_bounds__(kScanThreads)
-    k_scan_pen(const uint32_t* __restrict__ pcount, int64_t n_pen,
-               const uint64_t* __restrict__ tile_base, uint32_t* __restrict__ pbase) {
-  const int64_t base = seg_base(blockIdx.x);
+__device__ __forceinline__ void scan_pen_tile(const uint32_t* __restrict__ pcount, int64_t n_pen,
+                                              const uint64_t* __restrict__ tile_base,
+                                              uint32_t* __restrict__ pbase, int64_t tile) {
+  const int64_t base = seg_base(tile);
   uint32_t v[kScanItems];
   uint32_t cnt = 0;
 #pragma unroll
@@ -456,7 +459,7 @@ __global__ void __launch_bounds__(kScanThreads)
     v[q] = load_clamped(pcount, base + 64 * q, n_pen);
     cnt += v[q];
   }
-  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[blockIdx.x]);
+  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[tile]);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const int64_t idx = base + 64 * q;
@@ -469,11 +472,11 @@ __global__ void __launch_bounds__(kScanThreads)
 // Face list: every face once, as face word * 64 + bit, in face-word (memory) order, so list
 // neighbours are memory neighbours (consecutive cells along x) rather than the face order's
 // k-runs. A face's id is pw[word] + popc(ft[word] & below(bit)).
-__global__ void __launch_bounds__(kScanThreads)
-    k_face_list(const uint64_t* __restrict__ ft, int64_t n_ft,
-                const uint64_t* __restrict__ tile_base, uint64_t* __restrict__ flist,
-                uint64_t cap) {
-  const int64_t base = seg_base(blockIdx.x);
+__device__ __forceinline__ void face_list_tile(const uint64_t* __restrict__ ft, int64_t n_ft,
+                                               const uint64_t* __restrict__ tile_base,
+                                               uint64_t* __restrict__ flist, uint64_t cap,
+                                               int64_t tile) {
+  const int64_t base = seg_base(tile);
   uint64_t w[kScanItems];
   uint32_t cnt = 0;
 #pragma unroll
@@ -481,7 +484,7 @@ __global__ void __launch_bounds__(kScanThreads)
     w[q] = load_clamped(ft, base + 64 * q, n_ft);
     cnt += (uint32_t)__popcll(w[q]);
   }
-  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[blockIdx.x]);
+  uint64_t run = wave_offset(wave_sum_u32(cnt), tile_base[tile]);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     const uint32_t c = (uint32_t)__popcll(w[q]);
@@ -489,6 +492,72 @@ __global__ void __launch_bounds__(kScanThreads)
     append_bits(w[q], base + 64 * q, run + incl - c, cap, flist);
     run += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads)
+    k_tile_reduce(const T* __restrict__ items, int64_t n, uint32_t* __restrict__ tile_sum) {
+  tile_reduce(items, n, tile_sum, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kTileScanThreads)
+    k_tile_scan(const uint32_t* __restrict__ tile_sum, int64_t n_tiles,
+                uint64_t* __restrict__ tile_base, uint64_t* __restrict__ total) {
+  tile_scan(tile_sum, n_tiles, tile_base, total);
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_act(const uint64_t* __restrict__ act, int64_t n_words,
+               const uint64_t* __restrict__ tile_base, uint32_t* __restrict__ wprefix,
+               uint64_t* __restrict__ alist, uint64_t cap) {
+  scan_act_tile(act, n_words, tile_base, wprefix, alist, cap, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_pen(const uint32_t* __restrict__ pcount, int64_t n_pen,
+               const uint64_t* __restrict__ tile_base, uint32_t* __restrict__ pbase) {
+  scan_pen_tile(pcount, n_pen, tile_base, pbase, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_face_list(const uint64_t* __restrict__ ft, int64_t n_ft,
+                const uint64_t* __restrict__ tile_base, uint64_t* __restrict__ flist,
+                uint64_t cap) {
+  face_list_tile(ft, n_ft, tile_base, flist, cap, blockIdx.x);
+}
+
+// The three scans (active words, pencil counts, face words) step by step, each step of all three
+// in one launch: block b of a step belongs to the active scan (b < ta), the pencil scan (then
+// b < ta + tp) or the face scan. The tile sums and bases hold the three scans' tiles in that
+// order; the tile-scan step is one block per scan.
+__global__ void __launch_bounds__(kScanThreads) k_scan3_reduce(DcScan3Args a) {
+  const int64_t b = blockIdx.x, ta = a.tiles_a, tp = a.tiles_p;
+  if (b < ta)
+    tile_reduce(a.act, a.n_words, a.tile_sum, b);
+  else if (b < ta + tp)
+    tile_reduce(a.pcount, a.n_pen, a.tile_sum + ta, b - ta);
+  else
+    tile_reduce(a.ft, a.n_ft, a.tile_sum + ta + tp, b - ta - tp);
+}
+
+__global__ void __launch_bounds__(kTileScanThreads) k_scan3_tiles(DcScan3Args a) {
+  const int64_t ta = a.tiles_a, tp = a.tiles_p;
+  if (blockIdx.x == 0)
+    tile_scan(a.tile_sum, ta, a.tile_base, &a.ctr->n_active);
+  else if (blockIdx.x == 1)
+    tile_scan(a.tile_sum + ta, tp, a.tile_base + ta, &a.ctr->n_faces);
+  else
+    tile_scan(a.tile_sum + ta + tp, a.tiles_f, a.tile_base + ta + tp, &a.ctr->n_list);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan3_apply(DcScan3Args a) {
+  const int64_t b = blockIdx.x, ta = a.tiles_a, tp = a.tiles_p;
+  if (b < ta)
+    scan_act_tile(a.act, a.n_words, a.tile_base, a.wprefix, a.alist, a.capA, b);
+  else if (b < ta + tp)
+    scan_pen_tile(a.pcount, a.n_pen, a.tile_base + ta, a.pbase, b - ta);
+  else
+    face_list_tile(a.ft, a.n_ft, a.tile_base + ta + tp, a.flist, a.capF, b - ta - tp);
 }
 
 __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
@@ -618,6 +687,15 @@ hipError_t dc_launch_scan_pen(const uint32_t* pcount, int64_t n_pen, uint32_t* t
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_pen, dim3((unsigned)dc_scan_tiles(n_pen)), dim3(kScanThreads), 0, s,
                      pcount, n_pen, tile_base, pbase);
+  return hipGetLastError();
+}
+
+hipError_t dc_launch_scan3(const DcScan3Args& a, hipStream_t s) {
+  const int64_t tiles = a.tiles_a + a.tiles_p + a.tiles_f;
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan3_reduce, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a);
+  hipLaunchKernelGGL(k_scan3_tiles, dim3(3), dim3(kTileScanThreads), 0, s, a);
+  hipLaunchKernelGGL(k_scan3_apply, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a);
   return hipGetLastError();
 }
 
