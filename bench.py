@@ -105,7 +105,7 @@ def main():
     if args.tune:
         tile, table = be.autotune(gdev, repeats=3, use_cache=False)
     else:
-        tile, table = {"layers": 8, "waves": 4}, []  # the library's defaults
+        tile, table = {"rows": 8, "waves": 4}, []  # the library's defaults
     if args.calibrate:
         # a kernel of known traffic: its FETCH_SIZE / WRITE_SIZE scale the marking counters
         dst = torch.empty_like(phi)
@@ -113,7 +113,7 @@ def main():
         torch.cuda.synchronize()
         del dst
     for row in table[:6] + table[-2:]:
-        log(f"[tune] layers {row['layers']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
+        log(f"[tune] rows {row['rows']} waves {row['waves']}: mark {row['mark_ms'] * 1e3:.1f} us")
 
     # per-pass device times (events between the kernels: eager launches), after a warm-up that
     # sizes the buffers
@@ -207,7 +207,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"csg8 sharp-feature CSG {n}^3 FP32 grid, plain FP64 solve",
                    "grid": n, "seed": args.seed, "precision": "plain-fp64",
-                   "mark_tile": {k: tile[k] for k in ("layers", "waves")},
+                   "mark_tile": {k: tile[k] for k in ("rows", "waves")},
                    "parallelism": "single GPU"},
         "active_cells": int(st.vertex_count),
         "faces": int(st.face_count),

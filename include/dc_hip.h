@@ -83,8 +83,8 @@ void dc_hip_destroy(dc_hip_context* ctx);
 /* Runs all later work on `stream` (a hipStream_t). NULL, the default, is the null stream: each
  * call runs there and returns when its work is complete. */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
-/* Options: "mark_layers" (8, 16, 32, 64: cell layers per marking wave tile), "mark_waves"
- * (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
+/* Options: "mark_rows" (4, 8, 16, 32: sample rows per wave tile of the marking sign pass),
+ * "mark_waves" (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
  * adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
  * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
  * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
@@ -143,13 +143,13 @@ int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad
 
 /* Tile-shape sweep of the marking pass (no counterpart in the reference, whose marking kernel has
  * one fixed launch shape, src/cuda/dc_cuda.cu:241-281 launched at :687-692). Every
- * (mark_layers, mark_waves) in {8, 16, 32, 64} x {1, 2, 4} is timed on these device-resident grids
+ * (mark_rows, mark_waves) in {4, 8, 16, 32} x {1, 2, 4} is timed on these device-resident grids
  * with the per-pass events (best of `repeats` runs after one warm-up run); the fastest is kept on
- * the context and returned. table_ms (12 floats, layers-major; may be NULL) receives every
- * candidate's marking-pass time. The context's "timing" option is restored afterwards; the last
+ * the context and returned. table_ms (12 floats, rows-major; may be NULL) receives every
+ * candidate's marking time (sign and cell passes). The context's "timing" option is restored afterwards; the last
  * extraction's result stays in the context like any other extraction's. */
 int dc_hip_autotune(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t n_grids, int precision,
-                    int32_t repeats, int32_t* best_layers, int32_t* best_waves, float* table_ms);
+                    int32_t repeats, int32_t* best_rows, int32_t* best_waves, float* table_ms);
 
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);

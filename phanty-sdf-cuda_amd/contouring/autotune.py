@@ -1,4 +1,4 @@
-"""Autotuner for the mark pass's tile shape (cell layers per wave tile, waves per block).
+"""Autotuner for the marking sign pass's tile shape (sample rows per wave tile, waves per block).
 
 The reference CUDA backend launches its marking kernel with one fixed block shape
 (src/cuda/dc_cuda.cu:687-692). Here the shape is picked per grid size by the library's own sweep
@@ -13,7 +13,7 @@ import os
 
 from . import _lib
 
-LAYERS = (8, 16, 32, 64)
+ROWS = (4, 8, 16, 32)
 WAVES = (1, 2, 4)
 
 
@@ -33,7 +33,7 @@ def _device_name(device):
 def _key(device, grids):
     shapes = ",".join(f"{int(g.nx)}x{int(g.ny)}x{int(g.nz)}" for g in grids[:4])
     total = sum(int(g.nx) * int(g.ny) * int(g.nz) for g in grids)
-    return f"{_device_name(device)}|{len(grids)}|{shapes}|{total}"
+    return f"rows|{_device_name(device)}|{len(grids)}|{shapes}|{total}"
 
 
 def _load(path):
@@ -56,7 +56,7 @@ def _store(path, table):
 
 
 def _apply(backend, cfg):
-    backend.set_option("mark_layers", cfg["layers"])
+    backend.set_option("mark_rows", cfg["rows"])
     backend.set_option("mark_waves", cfg["waves"])
 
 
@@ -64,7 +64,7 @@ def autotune_mark(backend, grids, repeats=3, cache_path=None, use_cache=True):
     """Pick the fastest mark tile shape for `grids` (DenseSdfGridDevice or a list of them, values
     resident on the backend's device) and set it on `backend`.
 
-    Returns (best, table): best = {"layers", "waves", "mark_ms"}; table = every candidate
+    Returns (best, table): best = {"rows", "waves", "mark_ms"}; table = every candidate
     timed (empty when the cached choice was used).
     """
     if not isinstance(grids, (list, tuple)):
@@ -83,16 +83,16 @@ def autotune_mark(backend, grids, repeats=3, cache_path=None, use_cache=True):
     prec = PRECISIONS[backend.precision]
     if not all(isinstance(g.d_values, int) for g in grids):
         backend._use_torch_stream()
-    ms = (ctypes.c_float * (len(LAYERS) * len(WAVES)))()
+    ms = (ctypes.c_float * (len(ROWS) * len(WAVES)))()
     bl, bw = ctypes.c_int32(), ctypes.c_int32()
     backend._check(backend._lib.dc_hip_autotune(backend._ctx, arr, len(grids), prec, repeats,
                                                 ctypes.byref(bl), ctypes.byref(bw), ms))
-    table = [{"layers": layers, "waves": waves, "mark_ms": float(ms[q])}
-             for q, (layers, waves) in enumerate(itertools.product(LAYERS, WAVES))]
+    table = [{"rows": rows, "waves": waves, "mark_ms": float(ms[q])}
+             for q, (rows, waves) in enumerate(itertools.product(ROWS, WAVES))]
     table.sort(key=lambda c: c["mark_ms"])
-    winner = {"layers": int(bl.value), "waves": int(bw.value),
+    winner = {"rows": int(bl.value), "waves": int(bw.value),
               "mark_ms": next(c["mark_ms"] for c in table
-                              if (c["layers"], c["waves"]) == (bl.value, bw.value))}
+                              if (c["rows"], c["waves"]) == (bl.value, bw.value))}
     if use_cache:
         t = _load(cache_path)
         t[key] = winner

@@ -104,7 +104,7 @@ class HipDualContouringBackend:
     "fp32-fallback" (FP32 assembly, FP64 only for ill-conditioned cells).
     """
 
-    def __init__(self, precision="plain-fp64", device=0, mark_layers=None, mark_waves=None):
+    def __init__(self, precision="plain-fp64", device=0, mark_rows=None, mark_waves=None):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
@@ -116,8 +116,8 @@ class HipDualContouringBackend:
             raise _lib.DcHipError(rc, self._lib.dc_hip_status_string(rc).decode())
         self._ctx = ctx
         self.generation = 0
-        if mark_layers is not None:
-            self.set_option("mark_layers", mark_layers)
+        if mark_rows is not None:
+            self.set_option("mark_rows", mark_rows)
         if mark_waves is not None:
             self.set_option("mark_waves", mark_waves)
 

@@ -55,8 +55,7 @@ struct dc_hip_context {
   hipStream_t own = nullptr;     // captures sequences meant for the null stream
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_layers = 8;   // default tile: 8 cell layers x 4 waves per block (fastest at 512^3)
-  int eff_layers = 8;  // the batch's value (mark_layers or fewer for huge x-y layers)
+  int mark_rows = 8;   // default sign-pass tile: 8 sample rows x 4 waves per block
   int mark_waves = 4;
   int gather_rows = 1;
   int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
@@ -68,7 +67,7 @@ struct dc_hip_context {
   bool timing = true;
   int64_t qef_blocks = 0;
   int cu_count = 256;
-  DevBuf grids, bounds, act, act_kcj, ftx, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
+  DevBuf grids, bounds, act, sg, ft, pcount, pbase, wprefix, alist, v64, v32, faces, scan_sum, scan_base,
       counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, bwd_rows, fblist, flist, goff,
       qstate, qstate64, tris;
   DcCounters* h_counters = nullptr;
@@ -87,9 +86,9 @@ struct dc_hip_context {
   int bwd_lanes1 = 0, bwd_lanes2 = 0;  // options "adjoint_lanes", "adjoint_face_lanes" (0: auto)
   int bwd_lanes64 = 0;                 // option "adjoint_fallback_lanes" (0: auto)
   std::vector<DcGridDev> hgrids;
-  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, tw_b, at_b, hbounds;
-  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_tw = 0,
-          n_at = 0;
+  std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, cw_b, hbounds;
+  int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_cw = 0,
+          n_sg = 0;
 };
 
 namespace {
@@ -138,20 +137,6 @@ T* as(DevBuf& b) {
   return reinterpret_cast<T*>(b.p);
 }
 
-// Cell layers per marking tile for this batch: the configured value, halved while a tile's
-// samples (layers + 2 sample layers, with row slack) would not fit a 32-bit buffer range.
-int mark_layers_for(const dc_hip_context* c, const dc_hip_grid* grids, int n) {
-  int layers = c->mark_layers;
-  for (;;) {
-    bool fits = true;
-    for (int q = 0; q < n; ++q)
-      if ((int64_t)(layers + 2) * grids[q].nx * grids[q].ny * 4 >= (1ll << 32)) fits = false;
-    if (fits) return layers;
-    if (layers <= 8) return 0;
-    layers /= 2;
-  }
-}
-
 // Layout of a batch; returns an error for degenerate or oversized grids.
 int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   {
@@ -161,9 +146,6 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     c->gather_lds_row = 64 * max_ws + 64;
     c->eff_gather_rows = c->gather_rows;
   }
-  c->eff_layers = mark_layers_for(c, grids, n);
-  if (c->eff_layers == 0)
-    return fail(c, DC_HIP_ERR_INVALID_GRID, "grid layer too large (nx * ny above 100M samples)");
   c->hgrids.resize(n);
   c->tile_b.assign(n + 1, 0);
   c->act_b.assign(n + 1, 0);
@@ -171,9 +153,8 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->samp_b.assign(n + 1, 0);
   c->sr_b.assign(n + 1, 0);
   c->ft_b.assign(n + 1, 0);
-  c->tw_b.assign(n + 1, 0);
-  c->at_b.assign(n + 1, 0);
-  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, sr = 0, tw = 0, at = 0;
+  c->cw_b.assign(n + 1, 0);
+  int64_t words = 0, ftw = 0, pen = 0, tiles = 0, samp = 0, sr = 0, cw = 0, sgn = 0;
   for (int q = 0; q < n; ++q) {
     const dc_hip_grid& s = grids[q];
     if (s.nx < 2 || s.ny < 2 || s.nz < 2)
@@ -181,6 +162,9 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     if (s.nx > 65536 || s.ny > 65536 || s.nz > 65536)
       return fail(c, DC_HIP_ERR_INVALID_GRID, "grid dimension above 65536");
     if (!s.values) return fail(c, DC_HIP_ERR_INVALID_ARG, "grid values pointer is NULL");
+    // the sign pass addresses one sample layer with 32-bit byte offsets
+    if ((int64_t)s.nx * s.ny > (1ll << 30) - 64)
+      return fail(c, DC_HIP_ERR_INVALID_GRID, "grid layer too large (nx * ny above 2^30 samples)");
     DcGridDev& g = c->hgrids[q];
     g.val = s.values;
     g.nx = s.nx;
@@ -192,7 +176,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     g.W = (g.cx + 63) / 64;
     g.Wk = (g.cz + 63) / 64;
     g.Ws = (g.nx + 63) / 64;
-    g.pad_ = 0;
+    g.Wks = (g.nz + 63) / 64;
     g.vx = s.vx;
     g.vy = s.vy;
     g.vz = s.vz;
@@ -200,6 +184,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     g.oy = s.oy;
     g.oz = s.oz;
     g.act_off = words;
+    g.sg_off = sgn;
     g.ft_off = ftw;
     g.pen_off = pen;
     g.samp_off = samp;
@@ -210,14 +195,13 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     c->samp_b[q] = samp;
     c->sr_b[q] = sr;
     c->ft_b[q] = ftw;
-    c->tw_b[q] = tw;
-    c->at_b[q] = at;
+    c->cw_b[q] = cw;
     words += (int64_t)g.W * g.cy * g.cz;
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
-    tiles += dc_mark_tiles(g, c->eff_layers);
-    tw += 3ll * g.W * ((g.cy + kFtRows - 1) / kFtRows);
-    at += (int64_t)g.cz * ((g.cy + 63) / 64) * ((g.W + 7) / 8);
+    tiles += dc_mark_tiles(g, c->mark_rows);
+    cw += dc_cell_waves(g);
+    sgn += (int64_t)g.Wks * g.ny * 64 * g.Ws;
     samp += (int64_t)g.nx * g.ny * g.nz;
     sr += ((int64_t)g.ny * g.nz + c->eff_gather_rows - 1) / c->eff_gather_rows;  // gather groups
   }
@@ -227,8 +211,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->samp_b[n] = samp;
   c->sr_b[n] = sr;
   c->ft_b[n] = ftw;
-  c->tw_b[n] = tw;
-  c->at_b[n] = at;
+  c->cw_b[n] = cw;
   if ((uint64_t)words * 64ull >= (1ull << 32) * 4ull)
     return fail(c, DC_HIP_ERR_INVALID_GRID, "batch has too many cells for 32-bit vertex ids");
   c->n_words = words;
@@ -237,8 +220,8 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
   c->n_tiles = tiles;
   c->n_samples = samp;
   c->n_sr = sr;
-  c->n_tw = tw;
-  c->n_at = at;
+  c->n_cw = cw;
+  c->n_sg = sgn;
   return DC_HIP_OK;
 }
 
@@ -306,7 +289,7 @@ std::string plan_key(dc_hip_context* c, int precision, hipStream_t s) {
   put(&n, sizeof(n));
   put(c->hgrids.data(), n * sizeof(DcGridDev));
   put(c->hbounds.data(), c->hbounds.size() * sizeof(int64_t));
-  const int64_t v[] = {precision, c->eff_layers, c->mark_waves, c->qef_blocks, c->timing,
+  const int64_t v[] = {precision, c->mark_rows, c->mark_waves, c->qef_blocks, c->timing,
                        (int64_t)c->capA, (int64_t)c->capF, c->cap64_opt, (int64_t)c->alloc_gen,
                        (int64_t)(uintptr_t)s, (int64_t)(uintptr_t)c->h_goff};
   put(v, sizeof(v));
@@ -321,10 +304,9 @@ int run_once(dc_hip_context* c, int precision) {
   hipStream_t s;
   if ((rc = work_stream(c, &s))) return rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
-  if ((rc = ensure(c, c->bounds, 8 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
+  if ((rc = ensure(c, c->bounds, 7 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
   if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
-  if ((rc = ensure(c, c->act_kcj, c->n_words * 8, "active plane (marking order)"))) return rc;
-  if ((rc = ensure(c, c->ftx, c->n_words * 8 * 3, "face words over x"))) return rc;
+  if ((rc = ensure(c, c->sg, c->n_sg * 16, "sign planes"))) return rc;
   if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
   if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
   if ((rc = ensure(c, c->pcount, c->n_pen * 4, "pencil counts"))) return rc;
@@ -373,8 +355,7 @@ int run_once(dc_hip_context* c, int precision) {
   bounds.insert(bounds.end(), c->samp_b.begin(), c->samp_b.end());
   bounds.insert(bounds.end(), c->sr_b.begin(), c->sr_b.end());
   bounds.insert(bounds.end(), c->ft_b.begin(), c->ft_b.end());
-  bounds.insert(bounds.end(), c->tw_b.begin(), c->tw_b.end());
-  bounds.insert(bounds.end(), c->at_b.begin(), c->at_b.end());
+  bounds.insert(bounds.end(), c->cw_b.begin(), c->cw_b.end());
   // A plan is captured the second time in a row it runs (one-off plans stay eager) and replayed
   // while it stays unchanged. Per-pass timing needs events between the kernels, which a replayed
   // graph does not order reliably: with timing on, every run is eager.
@@ -420,22 +401,18 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   ma.n_grids = n;
   ma.tile_bounds = d_tile_b;
   ma.n_tiles = c->n_tiles;
-  ma.layers = c->eff_layers;
+  ma.rows = c->mark_rows;
   ma.waves_per_block = c->mark_waves;
+  ma.sg = as<uint4>(c->sg);
+  ma.cw_bounds = d_tile_b + 6 * (n + 1);
+  ma.n_cw = c->n_cw;
   ma.act = as<uint64_t>(c->act);
-  ma.act_kcj = as<uint64_t>(c->act_kcj);
-  ma.ftx = as<uint64_t>(c->ftx);
-  ma.plane_words = c->n_words;
-  ma.tw_bounds = d_tile_b + 6 * (n + 1);
-  ma.n_tw = c->n_tw;
-  ma.at_bounds = d_tile_b + 7 * (n + 1);
-  ma.n_at = c->n_at;
   ma.ft = as<uint64_t>(c->ft);
   ma.pcount = as<uint32_t>(c->pcount);
   ma.ctr = d_ctr;
   DC_TRY(c, dc_launch_mark(ma, s), "mark");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
-  DC_TRY(c, dc_launch_mark_finish(ma, s), "mark transposes");
+  DC_TRY(c, dc_launch_mark_finish(ma, s), "cell pass");
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
   // With per-pass timing off the three scans (active words, pencil counts, face words) share
   // their launches (reduce, tile scan, apply: 3 launches instead of 9); with it on each runs by
@@ -679,7 +656,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   hipSetDevice(c->device);
   if (c->own) hipStreamSynchronize(c->own);
   if (c->stream) hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->act_kcj, &c->ftx, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
+  DevBuf* bufs[] = {&c->grids, &c->bounds, &c->act, &c->sg, &c->ft, &c->pcount, &c->pbase, &c->wprefix,
                     &c->alist, &c->v64, &c->v32, &c->faces, &c->scan_sum, &c->scan_base,
                     &c->counters, &c->upload, &c->flags, &c->bwd_contrib, &c->pw, &c->fnrm,
                     &c->ubar, &c->ubar_e, &c->bwd_rows, &c->fblist, &c->flist, &c->goff,
@@ -710,10 +687,10 @@ int dc_hip_set_stream(dc_hip_context* c, void* stream) {
 int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
   if (!c || !key) return DC_HIP_ERR_INVALID_ARG;
   const std::string k(key);
-  if (k == "mark_layers") {
-    if (value != 8 && value != 16 && value != 32 && value != 64)
-      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_layers must be 8, 16, 32 or 64");
-    c->mark_layers = (int)value;
+  if (k == "mark_rows") {
+    if (value != 4 && value != 8 && value != 16 && value != 32)
+      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_rows must be 4, 8, 16 or 32");
+    c->mark_rows = (int)value;
   } else if (k == "mark_waves") {
     if (value != 1 && value != 2 && value != 4)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_waves must be 1, 2 or 4");
@@ -801,20 +778,20 @@ int dc_hip_extract_batch(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_
 }
 
 int dc_hip_autotune(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_grids, int precision,
-                    int32_t repeats, int32_t* best_layers, int32_t* best_waves, float* table_ms) {
+                    int32_t repeats, int32_t* best_rows, int32_t* best_waves, float* table_ms) {
   if (!c || !grids || n_grids <= 0 || repeats < 1)
     return fail(c, DC_HIP_ERR_INVALID_ARG, "bad autotune arguments");
   DC_TRY(c, hipSetDevice(c->device), "set device");
-  static const int kLayers[4] = {8, 16, 32, 64};
+  static const int kRows[4] = {4, 8, 16, 32};
   static const int kWaves[3] = {1, 2, 4};
   const bool timing = c->timing;
-  const int layers0 = c->mark_layers, waves0 = c->mark_waves;
+  const int rows0 = c->mark_rows, waves0 = c->mark_waves;
   c->timing = true;
   float best = 0.f;
-  int bl = layers0, bw = waves0, rc = DC_HIP_OK;
+  int bl = rows0, bw = waves0, rc = DC_HIP_OK;
   for (int a = 0; a < 4 && rc == DC_HIP_OK; ++a) {
     for (int b = 0; b < 3 && rc == DC_HIP_OK; ++b) {
-      c->mark_layers = kLayers[a];
+      c->mark_rows = kRows[a];
       c->mark_waves = kWaves[b];
       float t = 0.f;
       for (int r = 0; r <= repeats && rc == DC_HIP_OK; ++r) {  // run 0 warms up
@@ -826,20 +803,20 @@ int dc_hip_autotune(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_grids
       if (table_ms) table_ms[3 * a + b] = t;
       if ((a == 0 && b == 0) || t < best) {
         best = t;
-        bl = kLayers[a];
+        bl = kRows[a];
         bw = kWaves[b];
       }
     }
   }
   c->timing = timing;
   if (rc != DC_HIP_OK) {
-    c->mark_layers = layers0;
+    c->mark_rows = rows0;
     c->mark_waves = waves0;
     return rc;
   }
-  c->mark_layers = bl;
+  c->mark_rows = bl;
   c->mark_waves = bw;
-  if (best_layers) *best_layers = bl;
+  if (best_rows) *best_rows = bl;
   if (best_waves) *best_waves = bw;
   return DC_HIP_OK;
 }

@@ -5,6 +5,9 @@
 // are concatenated grid-major; offsets live in DcGridDev.
 //
 //   samples   float[nx*ny*nz] per grid, x fastest (caller-owned, DenseSdfGrid layout)
+//   sg        sign planes: 16 bytes per sample pencil (x, y) and block of 64 sample layers,
+//             entry (kb*ny + y)*64*Ws + x = {N, P} (uint64 each), bit t of N (P) set when the
+//             sample (x, y, 64*kb + t) is < 0 (> 0); Ws = ceil(nx/64), Wks = ceil(nz/64)
 //   act       uint64 active-cell plane, one word per 64 cells along x:
 //             word (k*cy + j)*W + i/64, bit i%64; W = ceil(cx/64). Zero padding bits.
 //   wprefix   uint32 per act word: exclusive count of active cells before it (global, so the
@@ -33,17 +36,16 @@
 #pragma once
 #include <stdint.h>
 
-#define DC_MARK_ROWS_MAX 8
-
 struct DcGridDev {
   const float* val;
   int32_t nx, ny, nz;
   int32_t cx, cy, cz;
   int32_t W, Wk;
   int32_t Ws;        // 64-sample words per sample row = ceil(nx/64)
-  int32_t pad_;
+  int32_t Wks;       // 64-layer blocks of sample layers = ceil(nz/64)
   float vx, vy, vz, ox, oy, oz;
   int64_t act_off;   // first act word
+  int64_t sg_off;    // first sign-plane entry
   int64_t ft_off;    // first ft word
   int64_t pen_off;   // first pencil (pcount / pbase index)
   int64_t samp_off;  // first sample in a concatenated gradient buffer (backward)

@@ -5,28 +5,17 @@
 
 #include "dc_internal.h"
 
-// Cell rows per wave of the face transpose (its wave count: 3 * W * ceil(cy / kFtRows) per grid).
-// 4 rows: 36 us at 512^3, against 46 (8 rows: too few waves to hide the loads) and 38-44 (2, 1).
-#ifndef DC_FT_ROWS
-#define DC_FT_ROWS 4
-#endif
-constexpr int kFtRows = DC_FT_ROWS;
-
 struct DcMarkArgs {
   const DcGridDev* grids;
   int n_grids;
   const int64_t* tile_bounds;  // n_grids + 1
   int64_t n_tiles;
-  int layers;           // cell layers per wave tile: 8, 16, 32 or 64
+  int rows;             // sample rows per sign-pass wave tile: 4, 8, 16 or 32
   int waves_per_block;  // 1, 2 or 4
+  uint4* sg;            // sign planes
+  const int64_t* cw_bounds;  // n_grids + 1, cell-pass waves (W * cy per grid)
+  int64_t n_cw;
   uint64_t* act;        // active words in vertex order [k][j][c]
-  uint64_t* act_kcj;    // the same words as the marking pass writes them, [k][c][j]
-  uint64_t* ftx;        // face words over x: 3 planes of plane_words, [k][c][j] each
-  int64_t plane_words;  // act words of the batch
-  const int64_t* tw_bounds;  // n_grids + 1, face transpose waves (3 * W * ceil(cy/kFtRows))
-  int64_t n_tw;
-  const int64_t* at_bounds;  // n_grids + 1, active transpose waves (cz * ceil(cy/64) * ceil(W/8))
-  int64_t n_at;
   uint64_t* ft;
   uint32_t* pcount;
   DcCounters* ctr;
@@ -164,8 +153,9 @@ hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s)
 hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, int n,
                                   const uint32_t* wprefix, const uint32_t* pbase, uint32_t* out,
                                   hipStream_t s);
-int64_t dc_mark_tiles(const DcGridDev& g, int layers);
-// k_mark (active and face words over x, [k][c][j]); then dc_launch_mark_finish: the active plane
+int64_t dc_mark_tiles(const DcGridDev& g, int rows);  // sign-pass wave tiles of a grid
+int64_t dc_cell_waves(const DcGridDev& g);            // cell-pass waves of a grid
+// k_mark, the sign pass (sign planes); then dc_launch_mark_finish, the cell pass: the active plane
 // in vertex order, the face planes over k and the pencil face counts.
 hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s);
 hipError_t dc_launch_mark_finish(const DcMarkArgs& a, hipStream_t s);

@@ -1,10 +1,11 @@
 // dc_kernels.hip — CDNA4 (gfx950) kernels of the dense dual-contouring path.
 //
 // Pipeline per batch of grids (a single extraction is a batch of one):
-//   k_mark      every sample read once (bit-sliced: lane = sample row, 64-bit masks over x):
-//               active-cell plane and the per-axis face words over x.
-//   k_ft_transpose  face words over x -> face planes over k (64x64 bit transposes across a
-//               wave) and the pencil face counts.
+//   k_mark      sign pass: every sample read once; per sample pencil and block of 64 layers,
+//               the masks of its negative and positive samples (bits over k).
+//   k_cells     cell pass: from the sign masks of each cell's four corner pencils, the face
+//               planes over k, the active-cell plane (one 64x64 bit transpose across a wave per
+//               word) and the pencil face counts.
 //   scans       per-tile reduce + one-block tile scan + apply, over
 //               the active plane (k_scan_act: per-word vertex-id prefix, active-cell list in
 //               vertex order), the pencil face counts in the CPU backend's face order
@@ -43,50 +44,44 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// Lane L of four registers := four wave-uniform values (v_writelane_b32), the other lanes
-// unchanged. The values are usually compare results just written by the vector ALU; the leading
-// s_nop gives the wait states a lane write needs after such an SGPR write (the compiler cannot
-// see into the asm to insert them).
-template <int L>
-__device__ __forceinline__ void write_lane4(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
-                                            uint32_t sa, uint32_t sb, uint32_t sc, uint32_t sd) {
-  asm("s_nop 3\n\t"
-      "v_writelane_b32 %0, %4, %8\n\t"
-      "v_writelane_b32 %1, %5, %8\n\t"
-      "v_writelane_b32 %2, %6, %8\n\t"
-      "v_writelane_b32 %3, %7, %8"
-      : "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-      : "s"(sa), "s"(sb), "s"(sc), "s"(sd), "i"(L));
-}
-
-// A read of memory that no kernel writes, at a wave-uniform address: through the constant
-// address space the compiler can issue it as a scalar load (no vector memory instruction).
-__device__ __forceinline__ float load_uniform(gfloat p, int64_t idx) {
-  return ((const __attribute__((address_space(4))) float*)p)[idx];
+// n := 2n + (v < 0), p := 2p + (v > 0): a compare into VCC and an add with VCC as carry-in per
+// mask (the compiler builds the same from a select, a shift and an or, and holds batches of
+// compare results in SGPR pairs).
+__device__ __forceinline__ void shift_in_sign(uint32_t& n, uint32_t& p, float v) {
+  asm("v_cmp_gt_f32 vcc, 0, %2\n\t"
+      "v_addc_co_u32 %0, vcc, %0, %0, vcc\n\t"
+      "v_cmp_lt_f32 vcc, 0, %2\n\t"
+      "v_addc_co_u32 %1, vcc, %1, %1, vcc"
+      : "+v"(n), "+v"(p)
+      : "v"(v)
+      : "vcc");
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_mark: bit-sliced. A wave tile is one 64-cell word along x (c) x 56 cell rows (57 sample
-// rows) x KL cell layers. Lane r holds sample row j0 + r of the current layer as 64-bit masks
-// over x (negative, positive samples), built from one ballot per row of the coalesced loads, so
-// the cell and face logic below is a few dozen 64-bit operations per layer for 3584 cells.
-// Cell row j0 + r's neighbour row comes from lane r + 1; the sample at x = 64c + 64 (the last
-// cell's right corner) is one per-lane load. Outputs per cell layer: the active word (lane r)
-// and the three face words over x (ftx), both stored [k][c][j]; k_act_transpose puts the active
-// words in vertex order ([k][j][c]) and k_ft_transpose turns the face words into the face planes
-// over k. Reference semantics: Cell.cpp:66-76 has_vertex (some corner < 0 and some > 0);
+// Marking, in two passes.
+// k_mark (sign pass) reads every sample exactly once. A wave tile is 64 samples along x (lane =
+// x) x R sample rows x one block of 64 sample layers. For each of its R pencils a lane builds
+// the 64-bit masks over the block's layers of the negative (N) and positive (P) samples, one
+// compare and one shift-in per sample and mask, and stores them as one 16-byte sign entry. There
+// is no halo: the neighbours a cell needs (x+1, y+1, z+1) come from other entries, read by
+// k_cells.
+// k_cells (cell pass): lane = cell i of a 64-cell word, one wave per (word, cell row). Per block
+// of 64 cell layers it reads the sign entries of the cell's four corner pencils (and bit 0 of
+// the next block's), forms the three face words over k (already the face planes' layout) and the
+// active word over k, which one 64x64 bit transpose across the wave turns into the active
+// plane's word over x.
+// Reference semantics: Cell.cpp:66-76 has_vertex (some corner < 0 and some > 0);
 // contouring.cpp:534-539 a quad per strict sign change on an interior edge.
 // ---------------------------------------------------------------------------------------------
-// Cell rows per wave tile (57 sample rows in lanes 0..56). Per layer a wave keeps one load per
-// sample row, the x = 64c + 64 column and 4 stores in flight: 61 vector memory operations, within
-// the 63 the wait counter can track (more would force a full drain every layer).
-constexpr int kMarkRows = 56;
+// Sample loads a sign-pass wave keeps in flight: R rows x (kMarkLoads / R) layers.
+constexpr int kMarkLoads = 32;
 
-template <int KL>
+template <int R>
 __global__ void __launch_bounds__(256)
     k_mark(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ tile_bounds,
-           int64_t n_tiles, uint64_t* __restrict__ act_kcj, uint64_t* __restrict__ ftx,
-           int64_t plane_words, DcCounters* __restrict__ ctr) {
+           int64_t n_tiles, uint4* __restrict__ sg, DcCounters* __restrict__ ctr) {
+  constexpr int D = kMarkLoads / R;  // layers in flight
+  static_assert(D >= 1 && 32 % D == 0, "layers in flight must divide 32");
   const int lane = threadIdx.x & (kWave - 1);
   const int wpb = blockDim.x / kWave;
   const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -95,104 +90,74 @@ __global__ void __launch_bounds__(256)
   const int g = find_bucket(tile_bounds, n_grids, tile);
   const DcGridDev G = grids[g];
   int64_t lt = tile - tile_bounds[g];
-  const int nJB = (G.cy + kMarkRows - 1) / kMarkRows;
-  const int c = (int)(lt % G.W);
-  lt /= G.W;
+  const int nJB = (G.ny + R - 1) / R;
+  const int c = (int)(lt % G.Ws);
+  lt /= G.Ws;
   const int jb = (int)(lt % nJB);
   const int kb = (int)(lt / nJB);
-  const int j0 = jb * kMarkRows;
-  const int k0 = kb * KL;
-  const int k1 = min(k0 + KL, G.cz);
-  const int jr = j0 + lane;  // this lane's sample row (and cell row for lanes < kMarkRows)
-  const bool rowc = lane < kMarkRows && jr < G.cy;
-  const bool rowj1 = rowc && jr >= 1;
-  const int nvalid = min(64, G.cx - 64 * c);  // cells of this word
-  const uint64_t xm = nvalid >= 64 ? ~0ull : ((1ull << nvalid) - 1ull);
-  const uint64_t xm1 = c == 0 ? (xm & ~1ull) : xm;  // cells with i >= 1
-
-  // Samples of layers k0..k1 through a bounded buffer resource: a uniform per-row byte offset
-  // (no address arithmetic per lane), and reads past the tile's layers return 0. Samples past
-  // nx or rows past ny read neighbouring data or 0; only masked bits depend on them.
-  const int64_t lbytes = (int64_t)G.nx * G.ny * 4;
-  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(G.val + (int64_t)k0 * G.nx * G.ny), (short)0,
-      (int)(uint32_t)((int64_t)(k1 - k0 + 1) * lbytes), 0x00020000);
-  const int vx = (64 * c + lane) * 4;                           // row loads: lane = x
-  const int ve = (jr * G.nx + 64 * c + 64) * 4;                 // x = 64c + 64: lane = row
-  auto ld = [&](int voff, int soff) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0));
+  const int j0 = jb * R;
+  const int nl = min(64, G.nz - 64 * kb);  // sample layers of this block
+  const int64_t lfloats = (int64_t)G.nx * G.ny;
+  const uint32_t lbytes = (uint32_t)(lfloats * 4);  // nx * ny < 2^30 - 64 (checked on the host)
+  const float* base = G.val + (int64_t)64 * kb * lfloats;
+  // Per-lane byte offsets of the tile's rows within a layer. Rows past ny read the last row (not
+  // stored); samples past nx read the next row (bits of pencils no cell uses) or, past the
+  // layer's end, 0 from the bounded buffer resource.
+  uint32_t voff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    voff[r] = ((uint32_t)min(j0 + r, G.ny - 1) * (uint32_t)G.nx + (uint32_t)(64 * c + lane)) * 4u;
+  // one buffer resource per layer; layers past the block (t >= nl) have no records: their loads
+  // touch no memory and read 0
+  auto layer = [&](int t) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(base + (int64_t)t * lfloats), (short)0,
+                                             t < nl ? (int)lbytes : 0, 0x00020000);
   };
+  float buf[D][R];
+  auto load_layer = [&](int t, float (&v)[R]) {
+    const __amdgpu_buffer_rsrc_t rs = layer(t);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      v[r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff[r], 0, 0));
+  };
+  // masks of layers 0..31 (lo) and 32..63 (hi), each shifted in from the bottom: the first
+  // layer of a half ends in bit 31 and a bit reverse puts layer t in bit t
+  uint32_t nlo[R], nhi[R], plo[R], phi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) nlo[r] = nhi[r] = plo[r] = phi[r] = 0u;
   float badacc = 0.f;  // becomes NaN once a NaN or Inf sample has been read
-  // Masks of one sample layer for sample row j0 + lane over x: negative (N) and positive (P)
-  // samples from v. All rows of layer sl_next are first loaded into w in one burst, so the whole
-  // layer is in flight while v is taken apart. v and w swap roles every layer (no copies).
-  auto masks = [&](const float (&v)[kMarkRows + 1], float (&w)[kMarkRows + 1], float ev,
-                   float& ew, int sl_next, uint64_t& N, uint64_t& P) {
-    const int lsoff = (int)((int64_t)(sl_next - k0) * lbytes);
-    static_for<kMarkRows + 1>([&](auto rc) {
-      constexpr int r = decltype(rc)::value;
-      w[r] = ld(vx, lsoff + (j0 + r) * G.nx * 4);
-    });
-    ew = ld(ve, lsoff);
-    uint32_t nlo = 0, nhi = 0, plo = 0, phi = 0;
-    static_for<kMarkRows + 1>([&](auto rc) {
-      constexpr int r = decltype(rc)::value;
-      const uint64_t mn = __ballot(v[r] < 0.f);
-      const uint64_t mp = __ballot(v[r] > 0.f);
+  auto consume = [&](const float (&v)[R], uint32_t (&nm)[R], uint32_t (&pm)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
       badacc = __builtin_fmaf(v[r], 0.f, badacc);
-      write_lane4<r>(nlo, nhi, plo, phi, (uint32_t)mn, (uint32_t)(mn >> 32), (uint32_t)mp,
-                     (uint32_t)(mp >> 32));
+      shift_in_sign(nm[r], pm[r], v[r]);
+    }
+  };
+  static_for<D>([&](auto dc) { load_layer(decltype(dc)::value, buf[decltype(dc)::value]); });
+  // layers 0..31, loading D layers ahead (at most layer 31 + D < 64)
+  for (int t0 = 0; t0 < 32; t0 += D) {
+    static_for<D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      consume(buf[d], nlo, plo);
+      load_layer(t0 + d + D, buf[d]);
     });
-    N = ((uint64_t)nhi << 32) | nlo;
-    P = ((uint64_t)phi << 32) | plo;
-    (void)ev;
-  };
-  // outputs in [k][c][j] order (rows fastest), so each store is contiguous across the lanes
-  uint64_t* ftx_x = ftx + G.act_off;
-  uint64_t* ftx_y = ftx + plane_words + G.act_off;
-  uint64_t* ftx_z = ftx + 2 * plane_words + G.act_off;
-  uint64_t* actg = act_kcj + G.act_off;
-
-  uint64_t Np = 0, Pp = 0, CAp = 0, CBp = 0;
-  // one sample layer from (v, ev): its masks, the face words of cell layer sl (x and y edges lie
-  // in sample layer sl) and the active word and z faces of cell layer sl - 1
-  auto step = [&](int sl, const float (&v)[kMarkRows + 1], float ev, float (&w)[kMarkRows + 1],
-                  float& ew) {
-    uint64_t N, P;
-    masks(v, w, ev, ew, min(sl + 1, k1), N, P);  // the last layer reloads itself (unused)
-    const uint64_t N1 = (N >> 1) | ((uint64_t)(ev < 0.f) << 63);  // sample x+1
-    const uint64_t P1 = (P >> 1) | ((uint64_t)(ev > 0.f) << 63);
-    const uint64_t A = N | N1, B = P | P1;                         // x-edge has a - / + end
-    const uint64_t Nn = __shfl_down(N, 1), Pn = __shfl_down(P, 1);  // sample row j + 1
-    const uint64_t CA = A | __shfl_down(A, 1), CB = B | __shfl_down(B, 1);
-    if (sl < k1 && rowc) {
-      const int k = sl;
-      const int64_t wd = ((int64_t)k * G.W + c) * G.cy + jr;
-      ftx_x[wd] = (rowj1 && k >= 1) ? (((N & P1) | (P & N1)) & xm) : 0ull;
-      ftx_y[wd] = (k >= 1) ? (((N & Pn) | (P & Nn)) & xm1) : 0ull;
-    }
-    if (sl > k0 && rowc) {
-      const int k = sl - 1;
-      const int64_t wd = ((int64_t)k * G.W + c) * G.cy + jr;
-      actg[wd] = (CAp | CA) & (CBp | CB) & xm;
-      ftx_z[wd] = rowj1 ? (((Np & P) | (Pp & N)) & xm1) : 0ull;
-    }
-    Np = N;
-    Pp = P;
-    CAp = CA;
-    CBp = CB;
-  };
-  float va[kMarkRows + 1], vb[kMarkRows + 1], ea, eb;
-  static_for<kMarkRows + 1>([&](auto rc) {
-    constexpr int r = decltype(rc)::value;
-    va[r] = ld(vx, (j0 + r) * G.nx * 4);
-  });
-  ea = ld(ve, 0);
-  for (int sl = k0; sl <= k1; sl += 2) {
-    step(sl, va, ea, vb, eb);
-    if (sl + 1 > k1) break;
-    step(sl + 1, vb, eb, va, ea);
   }
+  // layers 32..63: the last D are consumed without loads
+  for (int t0 = 32; t0 < 64 - D; t0 += D) {
+    static_for<D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      consume(buf[d], nhi, phi);
+      load_layer(t0 + d + D, buf[d]);
+    });
+  }
+  static_for<D>([&](auto dc) { consume(buf[decltype(dc)::value], nhi, phi); });
+  const int64_t nxp = 64 * (int64_t)G.Ws;
+  uint4* out = sg + G.sg_off + ((int64_t)kb * G.ny + j0) * nxp + 64 * c + lane;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (j0 + r < G.ny)
+      out[r * nxp] = make_uint4(__builtin_bitreverse32(nlo[r]), __builtin_bitreverse32(nhi[r]),
+                                __builtin_bitreverse32(plo[r]), __builtin_bitreverse32(phi[r]));
   if (__ballot(badacc != badacc) && lane == 0) atomicOr(&ctr->nonfinite, 1u);
 }
 
@@ -212,92 +177,91 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x) {
   return x;
 }
 
-// Face words over x ([a][k][c][j]) -> face planes over k (word (a, k/64, j, i), bit k%64) and
-// the pencil face counts in face order (a, i, j). One wave per (a, c, kFtRows rows): per 64
-// layers, lane = layer loads the rows' words (contiguous) and one wave transpose per row gives
-// lane = cell i.
-
+// Cell pass: sign entries -> face planes over k (word (a, k/64, j, i), bit k%64), the active
+// plane in vertex order (word (k, j, c), bit i%64) and the pencil face counts in face order
+// (a, i, j). One wave per (word c, cell row j), lane = cell i = 64c + lane, looping over the
+// blocks of 64 cell layers; the next block's entries are loaded while the current one is used.
 __global__ void __launch_bounds__(256)
-    k_ft_transpose(const DcGridDev* __restrict__ grids, int n_grids,
-                   const int64_t* __restrict__ tw_bounds, int64_t n_tw,
-                   const uint64_t* __restrict__ ftx, int64_t plane_words,
-                   uint64_t* __restrict__ ft, uint32_t* __restrict__ pcount) {
+    k_cells(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ cw_bounds,
+            int64_t n_cw, const uint4* __restrict__ sg, uint64_t* __restrict__ act,
+            uint64_t* __restrict__ ft, uint32_t* __restrict__ pcount) {
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) +
                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (t >= n_tw) return;  // uniform per wave
-  const int g = find_bucket(tw_bounds, n_grids, t);
+  if (t >= n_cw) return;  // uniform per wave
+  const int g = find_bucket(cw_bounds, n_grids, t);
   const DcGridDev G = grids[g];
-  const int ng = (G.cy + kFtRows - 1) / kFtRows;
-  int64_t lt = t - tw_bounds[g];
-  const int jg = (int)(lt % ng);
-  lt /= ng;
+  const int64_t lt = t - cw_bounds[g];
   const int c = (int)(lt % G.W);
-  const int a = (int)(lt / G.W);
-  const int j0 = jg * kFtRows;
+  const int j = (int)(lt / G.W);
   const int i = 64 * c + lane;
-  const uint64_t* src = ftx + a * plane_words + G.act_off;
-  uint32_t count[kFtRows];
-#pragma unroll
-  for (int r = 0; r < kFtRows; ++r) count[r] = 0;
+  const uint64_t iv = i < G.cx ? ~0ull : 0ull;          // cell exists
+  const uint64_t iv1 = (i >= 1 && i < G.cx) ? ~0ull : 0ull;
+  const uint64_t jv1 = j >= 1 ? ~0ull : 0ull;
+  const int nxp = 64 * G.Ws;
+  const int64_t bstride = (int64_t)G.ny * nxp;  // entries per block of 64 sample layers
+  const uint4* s0 = sg + G.sg_off + (int64_t)j * nxp + i;
+  const int di = min(i + 1, nxp - 1) - i;  // pencil i+1 (0 past the row's entries: unused lanes)
+  // the cell's corner pencils (j, i), (j, i+1), (j+1, i), (j+1, i+1) in one block
+  auto load4 = [&](int kb, uint4 (&q)[4]) {
+    const uint4* p = s0 + kb * bstride;
+    q[0] = p[0];
+    q[1] = p[di];
+    q[2] = p[nxp];
+    q[3] = p[nxp + di];
+  };
+  auto lo64 = [](const uint4& v) { return ((uint64_t)v.y << 32) | v.x; };  // N
+  auto hi64 = [](const uint4& v) { return ((uint64_t)v.w << 32) | v.z; };  // P
+  const int64_t astride = (int64_t)G.Wk * G.cy * G.cx;  // face words per axis
+  uint64_t* ftg = ft + G.ft_off;
+  uint32_t cnt[3] = {0u, 0u, 0u};
+  uint4 cur[4], nxt[4];
+  load4(0, cur);
   for (int kw = 0; kw < G.Wk; ++kw) {
-    const int k = min(64 * kw + lane, G.cz - 1);
-    const uint64_t* p = src + ((int64_t)k * G.W + c) * G.cy + j0;
-    uint64_t wx[kFtRows];
+    if (kw + 1 < G.Wks) {
+      load4(kw + 1, nxt);
+    } else {  // no sample layer 64 (kw + 1): only cell layers past cz would read it
 #pragma unroll
-    for (int r = 0; r < kFtRows; ++r) wx[r] = p[min(r, G.cy - 1 - j0)];
-#pragma unroll
-    for (int r = 0; r < kFtRows; ++r) {
-      const uint64_t wk = transpose64(64 * kw + lane < G.cz ? wx[r] : 0ull);
-      const int j = j0 + r;
-      if (j < G.cy && i < G.cx) {
-        ft[G.ft_off + (((int64_t)a * G.Wk + kw) * G.cy + j) * G.cx + i] = wk;
-        count[r] += (uint32_t)__popcll(wk);
-      }
+      for (int q = 0; q < 4; ++q) nxt[q] = make_uint4(0u, 0u, 0u, 0u);
     }
+    const int kv = min(64, G.cz - 64 * kw);  // cell layers of this block
+    const uint64_t km = kv >= 64 ? ~0ull : ((1ull << kv) - 1ull);
+    const uint64_t km1 = kw == 0 ? (km & ~1ull) : km;  // layers k >= 1
+    const uint64_t N = lo64(cur[0]), P = hi64(cur[0]);
+    const uint64_t Nx = lo64(cur[1]), Px = hi64(cur[1]);
+    const uint64_t Ny = lo64(cur[2]), Py = hi64(cur[2]);
+    // sample layer k + 1 at (j, i): the block shifted down, the next block's bit 0 on top
+    const uint64_t Nz = (N >> 1) | ((uint64_t)(nxt[0].x & 1u) << 63);
+    const uint64_t Pz = (P >> 1) | ((uint64_t)(nxt[0].z & 1u) << 63);
+    const uint64_t fx = ((N & Px) | (P & Nx)) & km1 & jv1 & iv;   // edge (i..i+1, j, k)
+    const uint64_t fy = ((N & Py) | (P & Ny)) & km1 & iv1;        // edge (i, j..j+1, k)
+    const uint64_t fz = ((N & Pz) | (P & Nz)) & km & jv1 & iv1;   // edge (i, j, k..k+1)
+    // some corner < 0 and some > 0 over the 4 pencils and the layers k, k + 1
+    const uint64_t A = N | Nx | Ny | lo64(cur[3]);
+    const uint64_t B = P | Px | Py | hi64(cur[3]);
+    const uint32_t An = nxt[0].x | nxt[1].x | nxt[2].x | nxt[3].x;
+    const uint32_t Bn = nxt[0].z | nxt[1].z | nxt[2].z | nxt[3].z;
+    const uint64_t Ak = A | (A >> 1) | ((uint64_t)(An & 1u) << 63);
+    const uint64_t Bk = B | (B >> 1) | ((uint64_t)(Bn & 1u) << 63);
+    const uint64_t av = Ak & Bk & km & iv;
+    const int64_t fw = ((int64_t)kw * G.cy + j) * G.cx + i;
+    if (iv) {
+      ftg[fw] = fx;
+      ftg[astride + fw] = fy;
+      ftg[2 * astride + fw] = fz;
+    }
+    cnt[0] += (uint32_t)__popcll(fx);
+    cnt[1] += (uint32_t)__popcll(fy);
+    cnt[2] += (uint32_t)__popcll(fz);
+    const uint64_t aw = transpose64(av);  // lane = cell layer 64 kw + lane, bits over i
+    const int k = 64 * kw + lane;
+    if (k < G.cz) act[G.act_off + ((int64_t)k * G.cy + j) * G.W + c] = aw;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
   }
+  if (iv) {
 #pragma unroll
-  for (int r = 0; r < kFtRows; ++r) {
-    const int j = j0 + r;
-    if (j < G.cy && i < G.cx) pcount[G.pen_off + ((int64_t)a * G.cx + i) * G.cy + j] = count[r];
-  }
-}
-
-// Active words [k][c][j] -> vertex order [k][j][c]. One wave per (k, 64 rows, 8 words), staged
-// through LDS so both the loads (lane = row) and the stores (8 consecutive words per row) are
-// contiguous.
-__global__ void __launch_bounds__(256)
-    k_act_transpose(const DcGridDev* __restrict__ grids, int n_grids,
-                    const int64_t* __restrict__ at_bounds, int64_t n_at,
-                    const uint64_t* __restrict__ act_kcj, uint64_t* __restrict__ act) {
-  __shared__ uint64_t s_t[4][64][9];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(wv);
-  if (t >= n_at) return;  // uniform per wave
-  const int g = find_bucket(at_bounds, n_grids, t);
-  const DcGridDev G = grids[g];
-  const int nj = (G.cy + 63) / 64, nc = (G.W + 7) / 8;
-  int64_t lt = t - at_bounds[g];
-  const int cb = (int)(lt % nc);
-  lt /= nc;
-  const int jb = (int)(lt % nj);
-  const int k = (int)(lt / nj);
-  const int j0 = 64 * jb, c0 = 8 * cb;
-  const int nrows = min(64, G.cy - j0), nw = min(8, G.W - c0);
-  const uint64_t* src = act_kcj + G.act_off + (int64_t)k * G.W * G.cy;
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-    if (q < nw && lane < nrows) s_t[wv][lane][q] = src[(int64_t)(c0 + q) * G.cy + j0 + lane];
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  uint64_t* dst = act + G.act_off + ((int64_t)k * G.cy + j0) * G.W + c0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int e = 64 * q + lane;  // (row e / 8, word e % 8)
-    const int r = e >> 3, w = e & 7;
-    if (r < nrows && w < nw) dst[(int64_t)r * G.W + w] = s_t[wv][r][w];
+    for (int a = 0; a < 3; ++a) pcount[G.pen_off + ((int64_t)a * G.cx + i) * G.cy + j] = cnt[a];
   }
 }
 
@@ -616,11 +580,11 @@ hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, in
   return hipGetLastError();
 }
 
-int64_t dc_mark_tiles(const DcGridDev& g, int layers) {
-  const int64_t nJB = (g.cy + kMarkRows - 1) / kMarkRows;
-  const int64_t nKB = (g.cz + layers - 1) / layers;
-  return (int64_t)g.W * nJB * nKB;
+int64_t dc_mark_tiles(const DcGridDev& g, int rows) {
+  return (int64_t)g.Ws * ((g.ny + rows - 1) / rows) * g.Wks;
 }
+
+int64_t dc_cell_waves(const DcGridDev& g) { return (int64_t)g.W * g.cy; }
 
 hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
   const int wpb = a.waves_per_block;
@@ -628,14 +592,14 @@ hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
   const int64_t blocks = (a.n_tiles + wpb - 1) / wpb;
   if (blocks <= 0) return hipSuccess;
   const dim3 grid((unsigned)blocks), block((unsigned)(64 * wpb));
-#define DC_MARK(KL)                                                                                \
-  hipLaunchKernelGGL(k_mark<KL>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds, a.n_tiles, \
-                     a.act_kcj, a.ftx, a.plane_words, a.ctr)
-  switch (a.layers) {
+#define DC_MARK(R)                                                                                \
+  hipLaunchKernelGGL(k_mark<R>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds, a.n_tiles,  \
+                     a.sg, a.ctr)
+  switch (a.rows) {
+    case 4: DC_MARK(4); break;
     case 8: DC_MARK(8); break;
     case 16: DC_MARK(16); break;
     case 32: DC_MARK(32); break;
-    case 64: DC_MARK(64); break;
     default: return hipErrorInvalidValue;
   }
 #undef DC_MARK
@@ -643,13 +607,9 @@ hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
 }
 
 hipError_t dc_launch_mark_finish(const DcMarkArgs& a, hipStream_t s) {
-  if (a.n_at > 0)
-    hipLaunchKernelGGL(k_act_transpose, dim3((unsigned)((a.n_at + 3) / 4)), dim3(256), 0, s,
-                       a.grids, a.n_grids, a.at_bounds, a.n_at, a.act_kcj, a.act);
-  if (a.n_tw > 0)
-    hipLaunchKernelGGL(k_ft_transpose, dim3((unsigned)((a.n_tw + 3) / 4)), dim3(256), 0, s,
-                       a.grids, a.n_grids, a.tw_bounds, a.n_tw, a.ftx, a.plane_words, a.ft,
-                       a.pcount);
+  if (a.n_cw > 0)
+    hipLaunchKernelGGL(k_cells, dim3((unsigned)((a.n_cw + 3) / 4)), dim3(256), 0, s, a.grids,
+                       a.n_grids, a.cw_bounds, a.n_cw, a.sg, a.act, a.ft, a.pcount);
   return hipGetLastError();
 }
 

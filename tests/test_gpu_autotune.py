@@ -33,4 +33,4 @@ def test_autotune_keeps_results_and_picks_fastest(tmp_path):
     assert st.marking_ms == 0  # timing stayed off
     # the disk cache gives the same choice without a sweep
     best2, table2 = autotune_mark(be, dev, cache_path=str(tmp_path / "tune.json"), use_cache=True)
-    assert table2 == [] and (best2["layers"], best2["waves"]) == (best["layers"], best["waves"])
+    assert table2 == [] and (best2["rows"], best2["waves"]) == (best["rows"], best["waves"])
