@@ -105,7 +105,7 @@ def main():
     if args.tune:
         tile, table = be.autotune(gdev, repeats=3, use_cache=False)
     else:
-        tile, table = {"rows": 8, "waves": 4}, []  # the library's defaults
+        tile, table = {"rows": 16, "waves": 4}, []  # the library's defaults
     if args.calibrate:
         # a kernel of known traffic: its FETCH_SIZE / WRITE_SIZE scale the marking counters
         dst = torch.empty_like(phi)

@@ -55,7 +55,7 @@ struct dc_hip_context {
   hipStream_t own = nullptr;     // captures sequences meant for the null stream
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_rows = 8;   // default sign-pass tile: 8 sample rows x 4 waves per block
+  int mark_rows = 16;  // default sign-pass tile: 16 sample rows x 4 waves per block (fastest at 512^3)
   int mark_waves = 4;
   int gather_rows = 1;
   int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
