@@ -6,8 +6,10 @@ FP64 mode through the C ABI (dc_hip_extract_device). The mesh stays in device me
 ends with the pipeline's one host synchronization (counts readback).
 
 Prints one JSON line. `value` is ms per extraction (lower is better). `roofline` is the marking
-kernel k_mark (the pass that reads every sample): algorithmic bytes 4*N_s + N_c/8 per launch over
-its hipEvent-measured duration; `traffic` is its measured HBM bytes per launch from the committed
+sign pass k_mark (the pass that reads every sample): algorithmic bytes per launch = every sample
+read once (4*N_s) plus its sign masks written (16 bytes per sample pencil and block of 64 layers),
+over its hipEvent-measured duration; `marking_stage` prices the whole marking stage (sign and
+cell passes) at 4*N_s + N_c/8; `traffic` is its measured HBM bytes per launch from the committed
 rocprofv3 counter summary (profiles/k_mark_csg8_512_traffic.json). `cpu_baseline` is the oracle
 (the C++ restatement of the reference CPU backend) on the same grid, single thread, on this host.
 The mark tile shape is the library default; --tune sweeps the shapes on this grid first.
@@ -174,9 +176,13 @@ def main():
 
     Ns = n ** 3
     Nc = (n - 1) ** 3
-    mark_bytes = 4 * Ns + Nc / 8
+    # sign pass: samples in, one 16-byte sign entry per pencil (x padded to 64) per 64 layers out
+    mark_bytes = 4 * Ns + 16 * (64 * -(-n // 64)) * n * -(-n // 64)
     mark_s = passes["mark_kernel_ms"] * 1e-3
     achieved = mark_bytes / mark_s / 1e9 if mark_s > 0 else None
+    stage_bytes = 4 * Ns + Nc / 8
+    stage_s = passes["marking_ms"] * 1e-3
+    stage_gbs = stage_bytes / stage_s / 1e9 if stage_s > 0 else None
     traffic = parse_traffic(n)
 
     cpu = None
@@ -222,6 +228,9 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "kernel": "k_mark",
                      "algorithmic_bytes": mark_bytes},
+        "marking_stage": {"kernels": ["k_mark", "k_cells"], "ms": passes["marking_ms"],
+                          "algorithmic_bytes": stage_bytes, "achieved_gbs": stage_gbs,
+                          "frac": (stage_gbs / HBM_PEAK_GBS) if stage_gbs else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -248,7 +257,8 @@ def _device_grid(g):
 
 
 def _fwd_bwd_ms(n, steps, warmup, options=None):
-    """Plain forward ms and FP32+fallback forward+adjoint ms on the csg8 grid of size n."""
+    """Plain forward ms and FP32+fallback forward+adjoint ms on the csg8 grid of size n: medians
+    over interleaved rounds of 5 timed steps each (at least 10 rounds)."""
     import torch
     from contouring import HipDualContouringBackend, _lib, synth
     g = synth.csg8(n, seed=1234)
@@ -269,7 +279,6 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
     def fwd(prec):
         be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st)))
 
-    plain = _timed(lambda: fwd(_lib.DC_HIP_PLAIN_FP64), steps, max(warmup, 3))
     fwd(_lib.DC_HIP_FP32_FALLBACK)
     nv = int(st.vertex_count)
     gen = torch.Generator(device="cuda:0")
@@ -283,8 +292,14 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
         be._check(lib.dc_hip_backward(be._ctx, ctypes.c_void_p(gv.data_ptr()),
                                       ctypes.c_void_p(gphi.data_ptr()), None))
 
-    both = _timed(fb, steps, max(warmup, 3))
-    return plain, both, nv
+    # interleaved rounds, median per configuration: host and clock drift hit both alike
+    plain_r, both_r = [], []
+    for _ in range(max(steps, 10)):
+        plain_r.append(_timed(lambda: fwd(_lib.DC_HIP_PLAIN_FP64), 5, 2))
+        both_r.append(_timed(fb, 5, 2))
+    plain_r.sort()
+    both_r.sort()
+    return plain_r[len(plain_r) // 2], both_r[len(both_r) // 2], nv
 
 
 def bench_c5(args):

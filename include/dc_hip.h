@@ -83,9 +83,9 @@ void dc_hip_destroy(dc_hip_context* ctx);
 /* Runs all later work on `stream` (a hipStream_t). NULL, the default, is the null stream: each
  * call runs there and returns when its work is complete. */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
-/* Options: "mark_rows" (4, 8, 16, 32: sample rows per wave tile of the marking sign pass),
- * "mark_waves" (1, 2, 4: waves per marking block), "gather_rows" (1, 2, 4, 8: sample rows per wave of the
- * adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
+/* Options: "mark_rows" (4, 8, 16, 32, default 16: sample rows per wave tile of the marking sign
+ * pass), "mark_waves" (1, 2, 4, default 4: waves per marking block), "gather_rows" (1, 2, 4, 8:
+ * sample rows per wave of the adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
  * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
  * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
  * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve),
