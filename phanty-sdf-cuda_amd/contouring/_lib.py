@@ -68,6 +68,14 @@ def load_hip():
     path = lib_path()
     if not os.path.exists(path):
         raise RuntimeError(f"HIP extension not built: {path} is missing (run __graft_entry__.build())")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and loads it by path. Once
+    # it is loaded, the library's dependency on that soname resolves to it; loaded first, the
+    # library would bring in the system runtime beside it, and whichever of the two initialized
+    # the device second would find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     sig = {

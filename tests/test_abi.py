@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -98,3 +99,19 @@ def test_ctypes_mirrors_match_the_header(struct, mirror, tmp_path):
 def test_autotune_rejects_bad_arguments():
     lib = _lib.load_hip()
     assert lib.dc_hip_autotune(None, None, 0, 0, 1, None, None, None) == _lib.DC_HIP_ERR_INVALID_ARG
+
+
+def test_one_hip_runtime_per_process():
+    """Loaded from Python before torch has touched the device, the library shares torch's HIP
+    runtime: a second runtime beside it would leave whichever initialized the GPU second with no
+    device."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from contouring import _lib\n"
+            "_lib.load_hip()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n"
+            % os.path.join(ROOT, "phanty-sdf-cuda_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=300, check=True).stdout
+    assert out.strip().splitlines()[-1] == "1"
