@@ -720,6 +720,15 @@ __device__ __forceinline__ void add_cells(float* __restrict__ buf, uint2 d, floa
   }
 }
 
+// Gradient stores are non-temporal (streamed to HBM): the dense gradient is never read back by
+// the library, and cached, its dirty lines were written back during the next extraction's
+// sign pass (at 512^3: sign pass 118 -> 98 us after an adjoint, the gather 155 -> 150 us).
+__device__ __forceinline__ void gstore(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void gstore(float4* p, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __builtin_nontemporal_store(f4{v.x, v.y, v.z, v.w}, reinterpret_cast<f4*>(p));
+}
+
 template <int R>
 __global__ void __launch_bounds__(256)
     k_bwd_gather(const DcGridDev* __restrict__ grids, int n_grids,
@@ -790,14 +799,14 @@ __global__ void __launch_bounds__(256)
         float4* o4 = reinterpret_cast<float4*>(out);
         const float4* b4 = reinterpret_cast<const float4*>(buf);
         if (summed)
-          for (int x = lane; x < (nsamp >> 2); x += 64) o4[x] = b4[x];
+          for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, b4[x]);
         else
-          for (int x = lane; x < (nsamp >> 2); x += 64) o4[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+          for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, make_float4(0.f, 0.f, 0.f, 0.f));
       } else {
         if (summed)
-          for (int x = lane; x < nsamp; x += 64) out[x] = buf[x];
+          for (int x = lane; x < nsamp; x += 64) gstore(out + x, buf[x]);
         else
-          for (int x = lane; x < nsamp; x += 64) out[x] = 0.f;
+          for (int x = lane; x < nsamp; x += 64) gstore(out + x, 0.f);
       }
       if (summed) carry[r] = buf[64 * nstep];
     };
