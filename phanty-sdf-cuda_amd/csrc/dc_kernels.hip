@@ -186,7 +186,8 @@ __global__ void __launch_bounds__(256)
             int64_t n_cw, const uint4* __restrict__ sg, uint64_t* __restrict__ act,
             uint64_t* __restrict__ ft, uint32_t* __restrict__ pcount) {
   const int lane = threadIdx.x & 63;
-  const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+  // consecutive waves (rows j, j + 1 of one word, whose sign entries overlap) share an L2
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (t >= n_cw) return;  // uniform per wave
   const int g = find_bucket(cw_bounds, n_grids, t);
