@@ -46,7 +46,7 @@ def parse_traffic(n):
 
 def run_steps(be, lib, cg, prec, steps):
     import ctypes
-    from contouring import _lib
+    from contouring_hip import _lib
     st = _lib.DcHipStats()
     acc = {"marking_ms": 0.0, "mark_kernel_ms": 0.0, "compaction_ms": 0.0,
            "face_prefix_sum_ms": 0.0, "normals_ms": 0.0, "qef_ms": 0.0, "face_fill_ms": 0.0}
@@ -85,7 +85,7 @@ def main():
 
     import torch
 
-    from contouring import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
+    from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
 
     if args.config != "c2":
         {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
@@ -250,7 +250,7 @@ def _timed(fn, steps, warmup):
 
 def _device_grid(g):
     import torch
-    from contouring import DenseSdfGridDevice
+    from contouring_hip import DenseSdfGridDevice
     phi = torch.from_numpy(g.values).to("cuda:0")
     return phi, DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz,
                                    d_values=phi)
@@ -260,7 +260,7 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
     """Plain forward ms and FP32+fallback forward+adjoint ms on the csg8 grid of size n: medians
     over interleaved rounds of 5 timed steps each (at least 10 rounds)."""
     import torch
-    from contouring import HipDualContouringBackend, _lib, synth
+    from contouring_hip import HipDualContouringBackend, _lib, synth
     g = synth.csg8(n, seed=1234)
     phi, _ = _device_grid(g)
     be = HipDualContouringBackend("plain-fp64", device=0)
@@ -320,7 +320,7 @@ def bench_c1(args):
     with the oracle: quads and FP32 vertices exact, FP64 vertices within 1e-6 h."""
     import numpy as np
     import torch
-    from contouring import HipDualContouringBackend, _lib, synth
+    from contouring_hip import HipDualContouringBackend, _lib, synth
     import oracle
     rows = {}
     for g in (synth.sphere_unit(64), synth.cli_generate("sphere", 64)):
@@ -360,7 +360,7 @@ def bench_c3(args):
     """1024^3 narrow band (64-sphere smooth-min, clamped to +-4h): plain vs FP32+fallback time and
     the FP32 vertex error against the plain FP64 vertices."""
     import torch
-    from contouring import HipDualContouringBackend, _lib, synth
+    from contouring_hip import HipDualContouringBackend, _lib, synth
     n = 1024
     t = time.time()
     g = synth.blend64(n)
@@ -403,7 +403,7 @@ def bench_c4(args):
     """256 uneven CSG grids (32^3..256^3): one extract_batch launch sequence vs 256 sequential
     extract_device calls on the same device-resident grids; per-grid spot parity."""
     import torch
-    from contouring import HipDualContouringBackend, _lib, synth
+    from contouring_hip import HipDualContouringBackend, _lib, synth
     t = time.time()
     grids = synth.batch_grids(256)
     log(f"[c4] generated 256 grids ({sum(g.values.size for g in grids) / 1e9:.2f} G samples) in "

@@ -1,21 +1,34 @@
 // dc_hip_backend.h — C++ backend class over the C ABI (dc_hip.h); takes the place of
 // CudaDualContouringBackend (reference include/dc_cuda.h:31-35) behind IDualContouringBackend.
+//
+// The grid, mesh and stats types come from <dc_backend.h>: inside the reference tree that is the
+// reference's include/dc_backend.h (put this directory after the reference's include directory);
+// this repository's own builds add include/standalone, which restates it. DenseSdfGridDevice is
+// the reference's (include/dc_cuda.h:24-29) whenever the build enables the CUDA backends
+// (DC_ENABLE_CUDA, CMakeLists.txt:52-72), and a field-for-field equal struct otherwise.
+// This header defines nothing the reference's dc_core also defines (DenseSdfGrid::sample lives in
+// the reference's src/cpp/dc_backend.cpp:8-49, or in this repository's dc_standalone.cpp).
 #pragma once
 
 #include <cstdint>
 #include <string>
 #include <vector>
 
-#include "dc_backend.h"
+#include <dc_backend.h>
+
 #include "dc_hip.h"
 
+#ifdef DC_ENABLE_CUDA
+#include <dc_cuda.h>
+#else
 // include/dc_cuda.h:24-29: a grid whose values already live in device memory (caller-owned).
 struct DenseSdfGridDevice {
-  int nx = 0, ny = 0, nz = 0;
-  float vx = 0.f, vy = 0.f, vz = 0.f;
-  float ox = 0.f, oy = 0.f, oz = 0.f;
-  const float* d_values = nullptr;
+  int nx, ny, nz;
+  float vx, vy, vz;
+  float ox, oy, oz;
+  const float* d_values;
 };
+#endif
 
 class HipDualContouringBackend : public IDualContouringBackend {
  public:

@@ -1,0 +1,63 @@
+// dc_hip_pybind.h — pybind11 registration of HipDualContouringBackend for the reference's Python
+// module (_contouring_cpp_module, src/cpp/bindings.cpp:73). Inside PYBIND11_MODULE, after the
+// reference has registered DenseSdfGrid, DualContouringMesh and DualContouringStats
+// (bindings.cpp:126-185), one call adds the class:
+//
+//   #include "dc_hip_pybind.h"
+//   ...
+//   bind_hip_backend(m);
+//
+// The class mirrors the reference's CudaDualContouringBackend binding (bindings.cpp:216-228):
+// extract(grid) and extract_device(grid) return (mesh, stats). DenseSdfGridDevice is registered
+// here only when the build has no CUDA backends (with DC_ENABLE_CUDA the reference registers it,
+// bindings.cpp:195-207); this registration adds d_values as an integer device address, so a
+// torch tensor's data_ptr() can be passed.
+#pragma once
+
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <string>
+
+#include "dc_hip_backend.h"
+
+inline void bind_hip_backend(pybind11::module_& m) {
+  namespace py = pybind11;
+#ifndef DC_ENABLE_CUDA
+  py::class_<DenseSdfGridDevice>(m, "DenseSdfGridDevice")
+      .def(py::init([]() { return DenseSdfGridDevice{0, 0, 0, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                                     nullptr}; }))
+      .def_readwrite("nx", &DenseSdfGridDevice::nx)
+      .def_readwrite("ny", &DenseSdfGridDevice::ny)
+      .def_readwrite("nz", &DenseSdfGridDevice::nz)
+      .def_readwrite("vx", &DenseSdfGridDevice::vx)
+      .def_readwrite("vy", &DenseSdfGridDevice::vy)
+      .def_readwrite("vz", &DenseSdfGridDevice::vz)
+      .def_readwrite("ox", &DenseSdfGridDevice::ox)
+      .def_readwrite("oy", &DenseSdfGridDevice::oy)
+      .def_readwrite("oz", &DenseSdfGridDevice::oz)
+      .def_property(
+          "d_values",
+          [](const DenseSdfGridDevice& g) { return (uintptr_t)g.d_values; },
+          [](DenseSdfGridDevice& g, uintptr_t p) { g.d_values = (const float*)p; });
+#endif
+  py::class_<HipDualContouringBackend>(m, "HipDualContouringBackend")
+      .def(py::init<int, int>(), py::arg("device") = 0, py::arg("precision") = 0)
+      .def("extract",
+           [](HipDualContouringBackend& self, const DenseSdfGrid& grid) {
+             DualContouringStats stats;
+             DualContouringMesh mesh = self.extract(grid, stats);
+             return py::make_tuple(mesh, stats);
+           })
+      .def("extract_device",
+           [](HipDualContouringBackend& self, const DenseSdfGridDevice& grid) {
+             DualContouringStats stats;
+             DualContouringMesh mesh = self.extract_device(grid, stats);
+             return py::make_tuple(mesh, stats);
+           })
+      .def("last_vertices_f64", &HipDualContouringBackend::last_vertices_f64)
+      .def("last_triangles", &HipDualContouringBackend::last_triangles)
+      .def("set_option", &HipDualContouringBackend::set_option)
+      .def_property_readonly("precision", &HipDualContouringBackend::precision);
+}

@@ -1,7 +1,11 @@
-// dc_backend.h — grid, mesh, stats and backend interface of the dense dual-contouring path,
-// restated API-compatibly from the reference (include/dc_backend.h:1-89) so that code written
-// against the reference compiles against this backend. Only the types on the dense path are
-// declared; the CPU backend and mesh post-processing stay in the reference.
+// dc_backend.h (standalone) — grid, mesh, stats and backend interface of the dense
+// dual-contouring path, restated API-compatibly from the reference (include/dc_backend.h:1-89).
+//
+// Only this repository's own builds (its dc_cli, the Python-free C++ users) put this directory on
+// the include path. A build inside the reference tree includes the reference's own
+// include/dc_backend.h instead: dc_hip_backend.h names <dc_backend.h> and the include path picks
+// one, so the two never meet in one translation unit. Only the types on the dense path are
+// declared; the CPU backend stays in the reference.
 #pragma once
 
 #include <memory>

@@ -14,6 +14,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "build")
 INCLUDE = os.path.join(ROOT, "include")
+# the restated reference types (dc_backend.h) for this repository's own C++ builds; a build inside
+# the reference tree uses the reference's include/dc_backend.h instead
+STANDALONE = os.path.join(INCLUDE, "standalone")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -47,8 +50,10 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib, *objs], verbose)
     _run(["g++", "-std=c++17", "-O3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
           "-o", os.path.join(BUILD, "libdc_synth.so"), os.path.join(CSRC, "synth.cpp")], verbose)
-    cli_srcs = [os.path.join(CSRC, f) for f in ("dc_cli.cpp", "dc_hip_backend.cpp", "synth.cpp")]
-    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-I", INCLUDE, "-I", CSRC,
+    cli_srcs = [os.path.join(CSRC, f)
+                for f in ("dc_cli.cpp", "dc_hip_backend.cpp", "dc_standalone.cpp", "synth.cpp")]
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-I", STANDALONE,
+          "-I", INCLUDE, "-I", CSRC,
           "-o", os.path.join(BUILD, "dc_cli"), *cli_srcs, "-L", BUILD, "-ldc_hip",
           "-Wl,-rpath,$ORIGIN"], verbose)
     return lib

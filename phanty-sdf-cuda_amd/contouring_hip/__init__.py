@@ -1,7 +1,10 @@
 """MI355X dual contouring of dense SDF grids — drop-in for the reference's dense GPU backend.
 
 Mirrors the reference Python module (src/python/contouring, src/cpp/bindings.cpp): grid, mesh
-and stats classes plus a backend class whose extract(grid) returns (mesh, stats).
+and stats classes plus a backend class whose extract(grid) returns (mesh, stats). It is a ctypes
+package over libdc_hip.so with its own name, so it installs beside the reference's `contouring`
+package; inside the reference's own module the same class is bound with pybind11 by
+include/dc_hip_pybind.h.
 """
 from .backend import (  # noqa: F401
     DenseSdfGrid,

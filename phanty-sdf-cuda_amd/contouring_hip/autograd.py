@@ -39,7 +39,10 @@ class DualContouringFunction(torch.autograd.Function):
         if be.generation != ctx.generation:
             # another extraction used the context since: rebuild this forward's topology
             be.extract_tensor(phi.detach(), ctx.spacing, ctx.origin)
-        gphi = torch.empty_like(phi)
+        # The forward extracted phi.contiguous() and the adjoint writes dL/dphi in that layout
+        # (x fastest): allocate it contiguous whatever phi's strides are (autograd accepts any).
+        gphi = torch.empty(phi.shape, dtype=torch.float32, device=phi.device,
+                           memory_format=torch.contiguous_format)
         be.backward(gV.contiguous(), gphi)
         return gphi, None, None, None
 

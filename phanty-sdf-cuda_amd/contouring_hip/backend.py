@@ -97,6 +97,23 @@ def _ptr(x):
     return x.data_ptr()
 
 
+def _check_tensor(t, name, numel, device, dtype="float32"):
+    """Raises ValueError unless t is a contiguous CUDA tensor of `dtype` with `numel` elements on
+    device index `device`: its data pointer is handed to kernels that index it by those sizes."""
+    import torch
+    want = getattr(torch, dtype)
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor (or an int device pointer)")
+    if t.dtype != want:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_cuda or (t.device.index or 0) != device:
+        raise ValueError(f"{name} must live on cuda:{device}, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() != numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+
+
 class HipDualContouringBackend:
     """MI355X dense dual contouring (replaces CudaDualContouringBackend, dc_cuda.h:31-35).
 
@@ -116,6 +133,7 @@ class HipDualContouringBackend:
             raise _lib.DcHipError(rc, self._lib.dc_hip_status_string(rc).decode())
         self._ctx = ctx
         self.generation = 0
+        self._samples = 0  # samples of the last forward (single grid, or the whole batch)
         if mark_rows is not None:
             self.set_option("mark_rows", mark_rows)
         if mark_waves is not None:
@@ -138,7 +156,7 @@ class HipDualContouringBackend:
 
     def autotune(self, grids, **kw):
         """Time every mark tile shape on `grids` (device grids) and keep the fastest; see
-        contouring.autotune.autotune_mark."""
+        contouring_hip.autotune.autotune_mark."""
         from .autotune import autotune_mark
         return autotune_mark(self, grids, **kw)
 
@@ -184,6 +202,7 @@ class HipDualContouringBackend:
         self._check(self._lib.dc_hip_extract(self._ctx, ctypes.byref(cg),
                                              PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
+        self._samples = vals.size
         t0 = time.perf_counter()
         if with_f64 is None:
             with_f64 = self.precision == "plain-fp64"
@@ -194,8 +213,13 @@ class HipDualContouringBackend:
         return mesh, stats
 
     def extract_device(self, grid, with_f64=None):
-        """Device grid -> host mesh (dc_cuda.cu:651-847). grid: DenseSdfGridDevice."""
+        """Device grid -> host mesh (dc_cuda.cu:651-847). grid: DenseSdfGridDevice whose
+        d_values is a float32 CUDA tensor (checked: dtype, device, contiguity, nx*ny*nz
+        elements) or an int device pointer (the caller vouches for nx*ny*nz floats)."""
         import time
+        n = int(grid.nx) * int(grid.ny) * int(grid.nz)
+        if not isinstance(grid.d_values, int):
+            _check_tensor(grid.d_values, "d_values", n, self.device)
         st = _lib.DcHipStats()
         cg = self._cgrid(grid, _ptr(grid.d_values))
         if not isinstance(grid.d_values, int):
@@ -203,6 +227,7 @@ class HipDualContouringBackend:
         self._check(self._lib.dc_hip_extract_device(self._ctx, ctypes.byref(cg),
                                                     PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
+        self._samples = n
         t0 = time.perf_counter()
         if with_f64 is None:
             with_f64 = self.precision == "plain-fp64"
@@ -220,6 +245,8 @@ class HipDualContouringBackend:
         import torch
         if phi.dtype != torch.float32 or not phi.is_cuda or phi.dim() != 3:
             raise ValueError("phi must be a float32 CUDA tensor of shape (nz, ny, nx)")
+        if (phi.device.index or 0) != self.device:
+            raise ValueError(f"phi must live on cuda:{self.device}, got {phi.device}")
         phi = phi.contiguous()
         nz, ny, nx = phi.shape
         g = DenseSdfGridDevice(nx, ny, nz, *spacing, *origin, d_values=phi)
@@ -229,6 +256,7 @@ class HipDualContouringBackend:
         self._check(self._lib.dc_hip_extract_device(self._ctx, ctypes.byref(cg),
                                                     PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
+        self._samples = phi.numel()
         nv, nf = self.counts()
         V = torch.empty((nv, 3), dtype=torch.float32, device=phi.device)
         F = torch.empty((nf, 4), dtype=torch.int32, device=phi.device)
@@ -241,9 +269,19 @@ class HipDualContouringBackend:
         return out, DualContouringStats.from_c(st)
 
     def extract_batch(self, grids, with_f64=False):
-        """List of DenseSdfGridDevice (tensor-backed) -> list of (V, F) CUDA tensor pairs."""
+        """List of DenseSdfGridDevice (tensor-backed) -> list of (V, F) CUDA tensor pairs.
+        Every tensor is checked like extract_device's; a gradient for backward() after a batch
+        is the concatenation of all grids' samples, grid-major."""
         import torch
         n = len(grids)
+        if n == 0:
+            raise ValueError("empty batch")
+        total = 0
+        for q, g in enumerate(grids):
+            ng = int(g.nx) * int(g.ny) * int(g.nz)
+            if not isinstance(g.d_values, int):
+                _check_tensor(g.d_values, f"grids[{q}].d_values", ng, self.device)
+            total += ng
         arr = (_lib.DcHipGrid * n)(*[self._cgrid(g, _ptr(g.d_values)) for g in grids])
         voff = np.zeros(n + 1, np.int64)
         foff = np.zeros(n + 1, np.int64)
@@ -253,6 +291,7 @@ class HipDualContouringBackend:
             self._ctx, arr, n, PRECISIONS[self.precision], voff.ctypes.data_as(ctypes.c_void_p),
             foff.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
         self.generation += 1
+        self._samples = total
         nv, nf = self.counts()
         dev = torch.device("cuda", self.device)
         V = torch.empty((nv, 3), dtype=torch.float32, device=dev)
@@ -282,11 +321,18 @@ class HipDualContouringBackend:
         return T
 
     def backward(self, grad_vertices, grad_phi):
-        """dL/dphi (written into grad_phi, float32 CUDA) for dL/dV of the last forward."""
+        """dL/dphi (written into grad_phi: a contiguous float32 CUDA tensor with one element per
+        sample of the last forward, all grids' samples concatenated after a batch) for dL/dV
+        (3 per vertex of the last forward)."""
         import torch
+        nv, _ = self.counts()
+        if not isinstance(grad_vertices, torch.Tensor):
+            raise ValueError("grad_vertices must be a torch tensor")
+        gv = grad_vertices.contiguous().to(torch.float32)
+        _check_tensor(gv, "grad_vertices", 3 * nv, self.device)
+        _check_tensor(grad_phi, "grad_phi", self._samples, self.device)
         self._use_torch_stream()
         st = _lib.DcHipStats()
-        gv = grad_vertices.contiguous().to(torch.float32)
         self._check(self._lib.dc_hip_backward(self._ctx, ctypes.c_void_p(gv.data_ptr()),
                                               ctypes.c_void_p(grad_phi.data_ptr()),
                                               ctypes.byref(st)))

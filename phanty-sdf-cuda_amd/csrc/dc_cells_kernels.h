@@ -22,6 +22,7 @@
 
 #include "dc_cell.h"
 #include "dc_common.h"
+#include "dc_decode.h"
 #include "dc_internal.h"
 #include "dc_kernels.h"
 
@@ -30,16 +31,10 @@ namespace {
 using namespace dccell;
 using dccommon::find_bucket;
 using dccommon::wave_scan_incl;
-
-__device__ __forceinline__ void decode_cell(const DcGridDev& G, uint64_t entry, int& i, int& j,
-                                            int& k) {
-  const int64_t lw = (int64_t)(entry >> 6) - G.act_off;
-  const int64_t row = lw / G.W;
-  const int c = (int)(lw - row * G.W);
-  i = 64 * c + (int)(entry & 63);
-  j = (int)(row % G.cy);
-  k = (int)(row / G.cy);
-}
+using dcdecode::decode_cell;
+using dcdecode::decode_face;
+using dcdecode::kErrCellRange;
+using dcdecode::kErrFaceRange;
 
 // ---------------------------------------------------------------------------------------------
 // Face enumeration
@@ -68,27 +63,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__device__ __forceinline__ void decode_face(const DcGridDev& G, uint64_t entry, int& a, int& i,
-                                            int& j, int& k) {
-  const int64_t plane = (int64_t)G.cy * G.cx;
-  const int64_t local = (int64_t)(entry >> 6) - G.ft_off;
-  const int64_t aw = local / plane;  // a * Wk + kw
-  const int64_t rem = local - aw * plane;
-  a = (int)(aw / G.Wk);
-  const int kw = (int)(aw - (int64_t)a * G.Wk);
-  j = (int)(rem / G.cx);
-  i = (int)(rem - (int64_t)j * G.cx);
-  k = 64 * kw + (int)(entry & 63);
-}
-
 // fn(G, a, i, j, k, face id) for every face, one thread per face-list entry (grid-stride over
-// the count).
+// the count). An entry that names no quad edge of its grid is skipped and flagged in *err.
 template <typename Fn>
 __device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, int n_grids,
                                           const int64_t* __restrict__ ft_bounds,
                                           const uint64_t* __restrict__ flist,
                                           const uint64_t* __restrict__ ft,
-                                          const uint32_t* __restrict__ pw, uint64_t n, Fn&& fn) {
+                                          const uint32_t* __restrict__ pw, uint64_t n,
+                                          uint32_t* __restrict__ err, Fn&& fn) {
   for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n;
        f += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t entry = flist[f];
@@ -98,7 +81,10 @@ __device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, i
     const int g = find_bucket(ft_bounds, n_grids, word);
     const DcGridDev G = grids[g];
     int a, i, j, k;
-    decode_face(G, entry, a, i, j, k);
+    if (!decode_face(G, entry, a, i, j, k)) {
+      atomicOr(err, kErrFaceRange);
+      continue;
+    }
     fn(G, a, i, j, k, face);
   }
 }
@@ -110,8 +96,8 @@ __global__ void __launch_bounds__(256)
     k_face_normals(const DcGridDev* __restrict__ grids, int n_grids,
                    const int64_t* __restrict__ ft_bounds, const uint64_t* __restrict__ flist,
                    const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-                   const DcCounters* __restrict__ ctr, uint64_t capF, T* __restrict__ nrm) {
-  each_face(grids, n_grids, ft_bounds, flist, ft, pw, min(ctr->n_faces, capF),
+                   DcCounters* __restrict__ ctr, uint64_t capF, T* __restrict__ nrm) {
+  each_face(grids, n_grids, ft_bounds, flist, ft, pw, min(ctr->n_faces, capF), &ctr->error,
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
               if (face >= capF) return;
               T n[3];
@@ -155,7 +141,10 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 3)
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
-    decode_cell(G, entry, i, j, k);
+    if (!decode_cell(G, entry, i, j, k)) {
+      atomicOr(&ctr->error, kErrCellRange);
+      continue;
+    }
     if (PREC == DC_PREC_FP64) {
       double x[3];
       solve_cell_fp64(G, act, fn64, i, j, k, x);
@@ -220,7 +209,7 @@ __global__ void __launch_bounds__(256)
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
-    decode_cell(G, entry, i, j, k);
+    if (!decode_cell(G, entry, i, j, k)) continue;  // k_qef flagged it (the group skips together)
     Cell<double> own;
     load_cell(G, i, j, k, own);
     const uint32_t cross = crossing_mask(own);
@@ -300,11 +289,11 @@ __global__ void __launch_bounds__(256)
            const uint64_t* __restrict__ flist, const uint64_t* __restrict__ ft,
            const uint32_t* __restrict__ pw, const uint64_t* __restrict__ act,
            const uint32_t* __restrict__ wprefix, const double* __restrict__ v64,
-           const float* __restrict__ v32, int precision, const DcCounters* __restrict__ ctr,
+           const float* __restrict__ v32, int precision, DcCounters* __restrict__ ctr,
            uint64_t capA, uint64_t capF, int32_t* __restrict__ faces) {
   // an overflowed run is repeated by the host; never index past the buffers
   if (ctr->n_active > capA || ctr->n_faces > capF) return;
-  each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces,
+  each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces, &ctr->error,
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
     // the edge's far endpoint; the quad's cells: c0 = (u-1, v-1), c1 = (u, v-1), c2 = (u, v),
     // c3 = (u-1, v) over the two axes (u, v) across the edge (x: (y, z); y: (x, z); z: (x, y))
@@ -488,7 +477,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
-    decode_cell(G, entry, i, j, k);
+    if (!decode_cell(G, entry, i, j, k)) continue;  // the forward flagged it and failed
     T Sb[8];
     float4* out = reinterpret_cast<float4*>(contrib + 8 * v);
     if (PHASE == 1) {
@@ -552,7 +541,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];
     int i, j, k;
-    decode_cell(G, entry, i, j, k);
+    if (!decode_cell(G, entry, i, j, k)) continue;  // the forward flagged it (the group skips)
     // L = 16: the lane's one row, its kind loaded with the cell
     const uint32_t kind16 = L == 16 ? rows[12 * v + (uint64_t)min(sub, 11)] : 0u;
     Cell<T> own;

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "dc_decode.h"
 #include "dc_hip.h"
 #include "dc_internal.h"
 #include "dc_kernels.h"
@@ -546,7 +547,11 @@ int run_pipeline(dc_hip_context* c, const dc_hip_grid* grids, int n, int precisi
     rc = run_once(c, precision);
     if (rc) return rc;
     const DcCounters& h = *c->h_counters;
-    if (h.error) return fail(c, DC_HIP_ERR_INTERNAL, "device error flag set");
+    if (h.error)
+      return fail(c, DC_HIP_ERR_INTERNAL,
+                  std::string("device consistency check failed:") +
+                      ((h.error & dcdecode::kErrCellRange) ? " an active-cell entry outside its grid" : "") +
+                      ((h.error & dcdecode::kErrFaceRange) ? " a face entry outside its grid" : ""));
     if (h.n_list != h.n_faces) return fail(c, DC_HIP_ERR_INTERNAL, "face list count mismatch");
     if (h.nonfinite) return fail(c, DC_HIP_ERR_NONFINITE, "grid contains NaN or Inf samples");
     if (h.n_active >= (1ull << 31) || h.n_faces >= (1ull << 31))

@@ -59,7 +59,7 @@ struct DcCounters {
   uint64_t fallback;      // FP32 mode: cells solved again in FP64
   uint64_t n_list;        // face list entries (equals n_faces)
   uint32_t nonfinite;     // a NaN or Inf sample was seen
-  uint32_t error;         // internal failure bits (none defined at present)
+  uint32_t error;         // internal failure bits: a list entry outside its grid (dc_decode.h)
 };
 
 enum DcPrecision { DC_PREC_FP64 = 0, DC_PREC_FP32 = 1 };

@@ -102,7 +102,7 @@ struct DcEmitArgs {
   const double* v64;
   const float* v32;
   int precision;
-  const DcCounters* ctr;
+  DcCounters* ctr;  // read: the counts; written: error bits of out-of-grid face entries
   uint64_t capA;
   uint64_t capF;
   int32_t* faces;

@@ -25,5 +25,5 @@ def gpu_available():
 
 @pytest.fixture(scope="session")
 def hip_backend():
-    from contouring import HipDualContouringBackend
+    from contouring_hip import HipDualContouringBackend
     return HipDualContouringBackend("plain-fp64")

@@ -16,7 +16,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "phanty-sdf-cuda_amd"), os.path.dirname
 
 import oracle  # noqa: E402
 from grids import ragged_sphere, ref_sphere, ref_thin_gap, zeros_grid  # noqa: E402
-from contouring import synth  # noqa: E402
+from contouring_hip import synth  # noqa: E402
 
 
 def main():

@@ -3,8 +3,8 @@ configurations at small sizes, and edge cases (exact zeros, ragged dimensions, 6
 boundaries). FP32 arithmetic follows the reference's host loops op by op."""
 import numpy as np
 
-from contouring import synth
-from contouring.synth import Grid
+from contouring_hip import synth
+from contouring_hip.synth import Grid
 
 
 def _axis(res, o=-1.0):

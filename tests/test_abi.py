@@ -9,7 +9,7 @@ import sys
 import pytest
 
 from conftest import ROOT, gpu_available
-from contouring import _lib
+from contouring_hip import _lib
 
 HEADER = os.path.join(ROOT, "include", "dc_hip.h")
 
@@ -66,7 +66,7 @@ def test_null_arguments_are_rejected():
 def test_backend_raises_without_device():
     if gpu_available():
         pytest.skip("checks the no-device path")
-    from contouring import HipDualContouringBackend
+    from contouring_hip import HipDualContouringBackend
     with pytest.raises(_lib.DcHipError):
         HipDualContouringBackend()
 
@@ -106,7 +106,7 @@ def test_one_hip_runtime_per_process():
     runtime: a second runtime beside it would leave whichever initialized the GPU second with no
     device."""
     code = ("import sys; sys.path.insert(0, %r)\n"
-            "from contouring import _lib\n"
+            "from contouring_hip import _lib\n"
             "_lib.load_hip()\n"
             "import torch\n"
             "maps = open('/proc/self/maps').read().splitlines()\n"

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from contouring import synth
+from contouring_hip import synth
 
 
 def _loss_grad_fd(g, w, idx, step=1e-6):

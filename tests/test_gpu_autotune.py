@@ -1,4 +1,4 @@
-"""The marking pass's tile-shape sweep (dc_hip_autotune through contouring.autotune_mark).
+"""The marking pass's tile-shape sweep (dc_hip_autotune through contouring_hip.autotune_mark).
 
 The tile shape changes only the launch geometry: the tuned context must give the same mesh as the
 default one, bit for bit. The sweep times all 12 candidates, keeps the fastest on the context and
@@ -7,7 +7,7 @@ restores the caller's timing option.
 import numpy as np
 import pytest
 
-from contouring import DenseSdfGridDevice, HipDualContouringBackend, autotune_mark, synth
+from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, autotune_mark, synth
 
 pytestmark = pytest.mark.gpu
 

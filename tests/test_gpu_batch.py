@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from grids import small_suite
-from contouring import DenseSdfGridDevice, HipDualContouringBackend, synth
+from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, synth
 
 pytestmark = pytest.mark.gpu
 

@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from conftest import PKG
-from contouring import synth
+from contouring_hip import synth
 
 pytestmark = pytest.mark.gpu
 

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
-from contouring import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
-from contouring.synth import Grid
+from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
+from contouring_hip.synth import Grid
 
 pytestmark = pytest.mark.gpu
 

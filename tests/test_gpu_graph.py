@@ -8,7 +8,7 @@ adjoint after a replayed forward must equal the adjoint after an eager forward.
 import numpy as np
 import pytest
 
-from contouring import DenseSdfGridDevice, HipDualContouringBackend, synth
+from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, synth
 
 pytestmark = pytest.mark.gpu
 

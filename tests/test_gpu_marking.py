@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 from grids import ragged_sphere
-from contouring import HipDualContouringBackend
+from contouring_hip import HipDualContouringBackend
 
 pytestmark = pytest.mark.gpu
 

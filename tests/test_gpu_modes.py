@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from contouring import DenseSdfGrid, HipDualContouringBackend, synth
+from contouring_hip import DenseSdfGrid, HipDualContouringBackend, synth
 from grids import ragged_sphere
 
 pytestmark = pytest.mark.gpu
@@ -91,7 +91,7 @@ def test_adjoint_matches_oracle_gradient(precision, tol):
 
 def test_adjoint_deterministic_and_autograd():
     import torch
-    from contouring.autograd import dual_contour
+    from contouring_hip.autograd import dual_contour
     g = synth.csg8(48, seed=3)
     phi = torch.from_numpy(np.asarray(g.values, np.float32)).reshape(g.nz, g.ny, g.nx).cuda()
     phi.requires_grad_(True)

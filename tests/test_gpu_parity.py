@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from grids import small_suite
-from contouring import synth
+from contouring_hip import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -16,7 +16,7 @@ def _h(g):
 
 
 def _compare(hip_backend, g, label):
-    from contouring import DenseSdfGrid
+    from contouring_hip import DenseSdfGrid
     print(f"[parity] {label} dims={g.nx}x{g.ny}x{g.nz}", flush=True)
     ref = oracle.run_grid(g)
     grid = DenseSdfGrid(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, g.values)
@@ -52,7 +52,7 @@ def test_parity_csg_medium(hip_backend, n, seed):
 
 
 def test_repeat_is_deterministic(hip_backend):
-    from contouring import DenseSdfGrid
+    from contouring_hip import DenseSdfGrid
     g = synth.csg8(80, seed=5)
     grid = DenseSdfGrid(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, g.values)
     m1, _ = hip_backend.extract(grid)
@@ -64,7 +64,7 @@ def test_repeat_is_deterministic(hip_backend):
 @pytest.mark.parametrize("idx", [0, 1, 3, 6, 10, 13])
 def test_triangles_match_oracle(hip_backend, idx):
     """dc_hip_triangulate vs the oracle's triangulate_v1 on the same quads and FP64 vertices."""
-    from contouring import DenseSdfGrid
+    from contouring_hip import DenseSdfGrid
     g = small_suite()[idx]
     print(f"[triangles] {g.name}", flush=True)
     ref = oracle.run_grid(g)

@@ -16,7 +16,7 @@ import oracle
 from oracle import twin
 from grids import ref_box, ref_const, ref_plane, ref_sphere, ref_thin_gap, ragged_sphere, \
     zeros_grid
-from contouring import synth
+from contouring_hip import synth
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
