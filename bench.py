@@ -7,8 +7,8 @@ ends with the pipeline's one host synchronization (counts readback).
 
 Prints one JSON line. `value` is ms per extraction (lower is better). `roofline` is the marking
 sign pass k_mark (the pass that reads every sample): algorithmic bytes per launch = every sample
-read once (4*N_s) plus its sign masks written (16 bytes per sample pencil and block of 64 layers),
-over its hipEvent-measured duration; `marking_stage` prices the whole marking stage (sign and
+read once (4*N_s; its sign-mask output is an intermediate of this design and not counted), over
+its hipEvent-measured duration; `marking_stage` prices the whole marking stage (sign and
 cell passes) at 4*N_s + N_c/8; `traffic` is its measured HBM bytes per launch from the committed
 rocprofv3 counter summary (profiles/k_mark_csg8_512_traffic.json). `cpu_baseline` is the oracle
 (the C++ restatement of the reference CPU backend) on the same grid, single thread, on this host.
@@ -20,6 +20,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "phanty-sdf-cuda_amd")]
@@ -66,6 +68,11 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed single-thread oracle runs (after one warm-up run)")
+    ap.add_argument("--no-host-input", action="store_true",
+                    help="skip the host-input (upload + download) timing")
+    ap.add_argument("--host-steps", type=int, default=5)
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
     ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"),
@@ -136,8 +143,11 @@ def main():
 
     # plain-mode FP64 vertices (bit-identical to the CPU backend's) for the FP32 error below
     nv = int(st.vertex_count)
+    nf = int(st.face_count)
     v64 = torch.empty((nv, 3), dtype=torch.float64, device="cuda:0")
-    be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(v64.data_ptr()), None))
+    f_plain = torch.empty((nf, 4), dtype=torch.int32, device="cuda:0")
+    be._check(lib.dc_hip_copy_result(be._ctx, None, ctypes.c_void_p(v64.data_ptr()),
+                                     ctypes.c_void_p(f_plain.data_ptr())))
 
     # FP32 + FP64-fallback forward followed by the adjoint (dL/dphi for a fixed dL/dV)
     prec32 = _lib.DC_HIP_FP32_FALLBACK
@@ -167,17 +177,22 @@ def main():
     fwd_bwd(1)
     fallback = int(st.qef_fallback_count)
     v32 = torch.empty((nv, 3), dtype=torch.float32, device="cuda:0")
-    be._check(lib.dc_hip_copy_result(be._ctx, ctypes.c_void_p(v32.data_ptr()), None, None))
+    f32 = torch.empty((int(st.face_count), 4), dtype=torch.int32, device="cuda:0")
+    be._check(lib.dc_hip_copy_result(be._ctx, ctypes.c_void_p(v32.data_ptr()), None,
+                                     ctypes.c_void_p(f32.data_ptr())))
     err = (v32.double() - v64).abs() / g.h
     err_max, err_rms = float(err.max()), float(err.pow(2).mean().sqrt())
+    # quads whose orientation differs from the plain (CPU-exact) mesh
+    flips = int((f32 != f_plain).any(dim=1).sum()) if f32.shape == f_plain.shape else -1
     log(f"[bench] fp32+fallback fwd+bwd: {fb_ms:.3f} ms ({fb_ms / ms:.2f}x plain), fallback "
         f"{fallback}/{nv}, vertex err max {err_max:.2e} h rms {err_rms:.2e} h, "
-        f"bwd {bst.backward_ms:.3f} ms")
+        f"bwd {bst.backward_ms:.3f} ms, quads oriented unlike plain: {flips}")
 
     Ns = n ** 3
     Nc = (n - 1) ** 3
-    # sign pass: samples in, one 16-byte sign entry per pencil (x padded to 64) per 64 layers out
-    mark_bytes = 4 * Ns + 16 * (64 * -(-n // 64)) * n * -(-n // 64)
+    # sign pass: the algorithmic bytes are the samples it reads (4 N_s); its 16-byte sign entries
+    # per pencil and 64 layers are this design's intermediate, not counted
+    mark_bytes = 4 * Ns
     mark_s = passes["mark_kernel_ms"] * 1e-3
     achieved = mark_bytes / mark_s / 1e9 if mark_s > 0 else None
     stage_bytes = 4 * Ns + Nc / 8
@@ -188,15 +203,38 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        threads = 1
+        host = host_cpu()
+        # the reference CPU backend is single-threaded: one thread, median of cpu_runs after a
+        # warm-up run
+        oracle.set_threads(1)
+        times = []
+        for q in range(args.cpu_runs + 1):
+            t = time.perf_counter()
+            r = oracle.run_grid(g)
+            if q > 0:
+                times.append((time.perf_counter() - t) * 1e3)
+            del r
+            log(f"[bench] oracle run {q} (1 thread){' warm-up' if q == 0 else ''}")
+        cpu_ms = float(np.median(times))
+        # the same restatement over all host cores (its loops are OpenMP), reported beside it
+        oracle.set_threads(host["nproc"])
         t = time.perf_counter()
         r = oracle.run_grid(g)
-        cpu_ms = (time.perf_counter() - t) * 1e3
+        all_ms = (time.perf_counter() - t) * 1e3
         ok = (r.n_vertices == st.vertex_count and r.n_faces == st.face_count)
-        log(f"[bench] oracle (1 thread): {cpu_ms:.0f} ms, counts match GPU: {ok}")
-        cpu = {"value": cpu_ms, "unit": "ms per extraction", "cores": threads, "kind": "port",
-               "sample": f"one full csg8 {n}^3 extraction (seed {args.seed}) by the C++ "
-                         "restatement of CpuDualContouringBackend, single thread"}
+        del r
+        log(f"[bench] oracle: {cpu_ms:.0f} ms (1 thread, median of {len(times)}), "
+            f"{all_ms:.0f} ms ({host['nproc']} threads), counts match GPU: {ok}")
+        cpu = {"value": cpu_ms, "unit": "ms per extraction", "cores": 1, "kind": "port",
+               "runs_ms": times, "host_nproc": host["nproc"], "host_cpu": host["model"],
+               "all_cores_ms": all_ms,
+               "sample": f"full csg8 {n}^3 extractions (seed {args.seed}) by the C++ "
+                         f"restatement of CpuDualContouringBackend: median of {len(times)} "
+                         "single-thread runs after one warm-up run"}
+
+    host_input = None
+    if not args.no_host_input:
+        host_input = time_host_input(be, g, args)
 
     out = {
         "metric": "512^3 DC ms (plain vs FP32+bwd, <=1e-4*h vertex err); %HBM peak; CPU ms",
@@ -224,6 +262,7 @@ def main():
         "fp32_bwd_over_plain": fb_ms / ms,
         "fp32_vertex_err_h": {"max": err_max, "rms": err_rms},
         "fp32_fallback_cells": fallback,
+        "fp32_orientation_flips": flips,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "kernel": "k_mark",
@@ -232,8 +271,60 @@ def main():
                           "algorithmic_bytes": stage_bytes, "achieved_gbs": stage_gbs,
                           "frac": (stage_gbs / HBM_PEAK_GBS) if stage_gbs else None},
         "cpu_baseline": cpu,
+        "host_input": host_input,
     }
     print(json.dumps(out), flush=True)
+
+
+def host_cpu():
+    """Host core count and CPU model (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"nproc": nproc, "model": model}
+
+
+def time_host_input(be, g, args):
+    """The reference's extract path: host grid in (upload), host mesh out (download), as
+    dc_hip_extract + dc_hip_copy_result into host arrays (CudaDualContouringBackend::extract,
+    dc_cuda.cu:614-649, 797-803). Pageable numpy input, as the reference's std::vector."""
+    import ctypes
+
+    from contouring_hip import _lib
+    lib = be._lib
+    vals = np.ascontiguousarray(g.values, np.float32)
+    cg = be._cgrid(g, vals.ctypes.data)
+    be.set_stream(None)
+    be.set_option("timing", 1)
+    st = _lib.DcHipStats()
+    rows = []
+    for q in range(max(2, args.host_steps) + 1):
+        t0 = time.perf_counter()
+        be._check(lib.dc_hip_extract(be._ctx, ctypes.byref(cg), _lib.DC_HIP_PLAIN_FP64,
+                                     ctypes.byref(st)))
+        t1 = time.perf_counter()
+        V = np.empty(3 * st.vertex_count, np.float32)
+        F = np.empty(4 * st.face_count, np.int32)
+        be._check(lib.dc_hip_copy_result(be._ctx, V.ctypes.data_as(ctypes.c_void_p), None,
+                                         F.ctypes.data_as(ctypes.c_void_p)))
+        t2 = time.perf_counter()
+        if q > 0:
+            rows.append(((t2 - t0) * 1e3, st.upload_ms, (t2 - t1) * 1e3))
+    tot, up, down = (float(np.median([r[i] for r in rows])) for i in range(3))
+    log(f"[bench] host input: {tot:.2f} ms per extraction (upload {up:.2f} ms, mesh download "
+        f"{down:.2f} ms)")
+    return {"ms": tot, "upload_ms": up, "download_ms": down, "steps": len(rows),
+            "note": "host grid in (pageable), host mesh out; never the headline value"}
 
 
 def _timed(fn, steps, warmup):

@@ -38,8 +38,14 @@ def load():
     lib.oracle_sym3_eig.argtypes = [P, P, P]
     lib.oracle_qef_solve.argtypes = [I, P, P, P]
     lib.oracle_triangulate.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64, P]
+    lib.oracle_set_threads.argtypes = [I]
     _lib = lib
     return lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's forward loops (1 = the reference's single-threaded run)."""
+    load().oracle_set_threads(int(n))
 
 
 def _p(a):

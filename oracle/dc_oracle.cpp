@@ -47,6 +47,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // Cell corner order (contouring.cpp:232-241) and edge pairs (include/Cell.h:26-30).
@@ -363,18 +365,19 @@ int edge_idx_in_cell(int ci, int cj, int ck, int i0, int j0, int k0, int i1, int
 
 // hermite_update.cpp:171-332 process_edge + the averaging loop at :406-437, applied at once
 // (each (cell, local edge) slot belongs to exactly one grid edge, so order does not matter).
-void process_edge(OracleResult& R, int i0, int j0, int k0, int i1, int j1, int k1) {
+// Returns 1 when the edge's normals were averaged (for the count), else 0.
+int process_edge(OracleResult& R, int i0, int j0, int k0, int i1, int j1, int k1) {
   const Grid& g = R.g;
   auto inside = [&](int a, int b, int c) {
     return a > 0 && a < g.nx - 1 && b > 0 && b < g.ny - 1 && c > 0 && c < g.nz - 1;
   };
-  if (!inside(i0, j0, k0) && !inside(i1, j1, k1)) return;
+  if (!inside(i0, j0, k0) && !inside(i1, j1, k1)) return 0;
   // With one endpoint strictly interior, the cell whose min corner is (i0,j0,k0) exists.
   const int64_t cell = g.cidx(i0, j0, k0);
   const int v0 = R.vid[cell];
-  if (v0 < 0) return;  // inactive cell: no hermite data
+  if (v0 < 0) return 0;  // inactive cell: no hermite data
   const int e0 = edge_idx_in_cell(i0, j0, k0, i0, j0, k0, i1, j1, k1);
-  if (!(R.herm[v0].mask & (1u << e0))) return;
+  if (!(R.herm[v0].mask & (1u << e0))) return 0;
   const int* d1 = kDirs[kEdgeFaces[e0][0]];
   const int* d2 = kDirs[kEdgeFaces[e0][1]];
   const int cc[4][3] = {{i0, j0, k0},
@@ -384,7 +387,7 @@ void process_edge(OracleResult& R, int i0, int j0, int k0, int i1, int j1, int k
   int vids[4], eids[4];
   for (int q = 0; q < 4; ++q) {
     vids[q] = R.vid[g.cidx(cc[q][0], cc[q][1], cc[q][2])];
-    if (vids[q] < 0) return;
+    if (vids[q] < 0) return 0;
   }
   for (int q = 0; q < 4; ++q)
     eids[q] = edge_idx_in_cell(cc[q][0], cc[q][1], cc[q][2], i0, j0, k0, i1, j1, k1);
@@ -404,17 +407,25 @@ void process_edge(OracleResult& R, int i0, int j0, int k0, int i1, int j1, int k
     n[1] = acc[1];
     n[2] = acc[2];
   }
-  R.averaged_edges++;
+  return 1;
 }
 
+// The loops below run over OpenMP threads (oracle_set_threads; one thread reproduces the
+// reference's single-threaded run). Nothing they compute depends on the thread count: vertex ids
+// come from per-layer counts and a prefix over the layers, every (cell, local edge) normal slot is
+// written by the one grid edge it belongs to, and per-cell work is independent.
 void run_forward(OracleResult& R) {
   const Grid& g = R.g;
   const int cx = g.cx(), cy = g.cy(), cz = g.cz();
   const int64_t ncell = (int64_t)cx * cy * cz;
+  const int64_t layer = (int64_t)cx * cy;
   R.vid.assign(ncell, -1);
   // generate_cells: has_vertex per cell (Cell.cpp:66-76), vertex ids in (k,j,i) order
-  // (contouring.cpp:509-524).
-  for (int k = 0; k < cz; ++k)
+  // (contouring.cpp:509-524): flags and per-layer counts, then ids from the layers' prefix.
+  std::vector<int64_t> layer_base(cz + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < cz; ++k) {
+    int64_t n = 0;
     for (int j = 0; j < cy; ++j)
       for (int i = 0; i < cx; ++i) {
         bool neg = false, pos = false;
@@ -424,13 +435,26 @@ void run_forward(OracleResult& R) {
           if (s > 0) pos = true;
         }
         if (neg && pos) {
-          const int64_t ci = g.cidx(i, j, k);
-          R.vid[ci] = (int32_t)R.active.size();
-          R.active.push_back(ci);
+          R.vid[g.cidx(i, j, k)] = 0;
+          ++n;
         }
       }
-  const int64_t A = (int64_t)R.active.size();
+    layer_base[k + 1] = n;
+  }
+  for (int k = 0; k < cz; ++k) layer_base[k + 1] += layer_base[k];
+  const int64_t A = layer_base[cz];
+  R.active.assign(A, 0);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < cz; ++k) {
+    int64_t v = layer_base[k];
+    for (int64_t ci = (int64_t)k * layer; ci < (int64_t)(k + 1) * layer; ++ci)
+      if (R.vid[ci] == 0) {
+        R.vid[ci] = (int32_t)v;
+        R.active[v++] = ci;
+      }
+  }
   R.herm.resize(A);
+#pragma omp parallel for schedule(static)
   for (int64_t v = 0; v < A; ++v) {
     const int64_t ci = R.active[v];
     const int i = (int)(ci % cx), j = (int)((ci / cx) % cy), k = (int)(ci / ((int64_t)cx * cy));
@@ -449,15 +473,19 @@ void run_forward(OracleResult& R) {
   R.herm_raw = R.herm;
   // average_hermite_normals (hermite_update.cpp:334-438): grid points in (k,j,i) order,
   // X, Y, Z edge per point.
+  int64_t averaged = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : averaged)
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
-        if (i < g.nx - 1) process_edge(R, i, j, k, i + 1, j, k);
-        if (j < g.ny - 1) process_edge(R, i, j, k, i, j + 1, k);
-        if (k < g.nz - 1) process_edge(R, i, j, k, i, j, k + 1);
+        if (i < g.nx - 1) averaged += process_edge(R, i, j, k, i + 1, j, k);
+        if (j < g.ny - 1) averaged += process_edge(R, i, j, k, i, j + 1, k);
+        if (k < g.nz - 1) averaged += process_edge(R, i, j, k, i, j, k + 1);
       }
+  R.averaged_edges = averaged;
   // Cell::update for every active cell.
   R.V.resize(3 * A);
+#pragma omp parallel for schedule(static)
   for (int64_t v = 0; v < A; ++v) solve_cell(R.herm[v], &R.V[3 * v], nullptr);
   // extract_mesh_from_cells (contouring.cpp:607-729).
   auto cellv = [&](int i, int j, int k) -> int {
@@ -496,33 +524,54 @@ void run_forward(OracleResult& R) {
   auto sign_change = [&](int a0, int b0, int c0, int a1, int b1, int c1) {
     return g.S(a0, b0, c0) * g.S(a1, b1, c1) < 0.0;
   };
-  for (int i = 0; i < cx; ++i)
-    for (int j = 1; j < cy; ++j)
-      for (int k = 1; k < cz; ++k) {
-        if (!sign_change(i, j, k, i + 1, j, k)) continue;
-        const int c0 = cellv(i, j - 1, k - 1), c1 = cellv(i, j, k - 1), c2 = cellv(i, j, k),
-                  c3 = cellv(i, j - 1, k);
-        if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
-        emit(c0, c1, c2, c3, i, j, k, i + 1, j, k);
-      }
-  for (int i = 1; i < cx; ++i)
-    for (int j = 0; j < cy; ++j)
-      for (int k = 1; k < cz; ++k) {
-        if (!sign_change(i, j, k, i, j + 1, k)) continue;
-        const int c0 = cellv(i - 1, j, k - 1), c1 = cellv(i, j, k - 1), c2 = cellv(i, j, k),
-                  c3 = cellv(i - 1, j, k);
-        if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
-        emit(c0, c1, c2, c3, i, j, k, i, j + 1, k);
-      }
-  for (int i = 1; i < cx; ++i)
-    for (int j = 1; j < cy; ++j)
-      for (int k = 0; k < cz; ++k) {
-        if (!sign_change(i, j, k, i, j, k + 1)) continue;
-        const int c0 = cellv(i - 1, j - 1, k), c1 = cellv(i, j - 1, k), c2 = cellv(i, j, k),
-                  c3 = cellv(i - 1, j, k);
-        if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
-        emit(c0, c1, c2, c3, i, j, k, i, j, k + 1);
-      }
+  // The reference visits each axis's edges in (i, j, k) order, k innermost (contouring.cpp:
+  // 607-729), a stride of nx*ny samples per step. The sign-changing edges are found here in
+  // memory order (k, j, i) and sorted into the reference's (i, j, k) order before emission, which
+  // is the same sequence of quads at a fraction of the cache misses on large grids.
+  auto edges_in_order = [&](int axis, int ilo, int jlo, int klo) {
+    const int di = axis == 0, dj = axis == 1, dk = axis == 2;
+    std::vector<std::vector<uint64_t>> per_k(cz);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = klo; k < cz; ++k)
+      for (int j = jlo; j < cy; ++j)
+        for (int i = ilo; i < cx; ++i)
+          if (sign_change(i, j, k, i + di, j + dj, k + dk))
+            per_k[k].push_back(((uint64_t)i * (uint64_t)cy + (uint64_t)j) * (uint64_t)cz +
+                               (uint64_t)k);
+    std::vector<uint64_t> keys;
+    for (auto& v : per_k) keys.insert(keys.end(), v.begin(), v.end());
+    std::sort(keys.begin(), keys.end());
+    return keys;
+  };
+  auto unkey = [&](uint64_t key, int& i, int& j, int& k) {
+    k = (int)(key % (uint64_t)cz);
+    j = (int)((key / (uint64_t)cz) % (uint64_t)cy);
+    i = (int)(key / ((uint64_t)cz * (uint64_t)cy));
+  };
+  for (uint64_t key : edges_in_order(0, 0, 1, 1)) {  // x: i in [0,cx), j in [1,cy), k in [1,cz)
+    int i, j, k;
+    unkey(key, i, j, k);
+    const int c0 = cellv(i, j - 1, k - 1), c1 = cellv(i, j, k - 1), c2 = cellv(i, j, k),
+              c3 = cellv(i, j - 1, k);
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
+    emit(c0, c1, c2, c3, i, j, k, i + 1, j, k);
+  }
+  for (uint64_t key : edges_in_order(1, 1, 0, 1)) {  // y: i in [1,cx), j in [0,cy), k in [1,cz)
+    int i, j, k;
+    unkey(key, i, j, k);
+    const int c0 = cellv(i - 1, j, k - 1), c1 = cellv(i, j, k - 1), c2 = cellv(i, j, k),
+              c3 = cellv(i - 1, j, k);
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
+    emit(c0, c1, c2, c3, i, j, k, i, j + 1, k);
+  }
+  for (uint64_t key : edges_in_order(2, 1, 1, 0)) {  // z: i in [1,cx), j in [1,cy), k in [0,cz)
+    int i, j, k;
+    unkey(key, i, j, k);
+    const int c0 = cellv(i - 1, j - 1, k), c1 = cellv(i, j - 1, k), c2 = cellv(i, j, k),
+              c3 = cellv(i - 1, j, k);
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0) continue;
+    emit(c0, c1, c2, c3, i, j, k, i, j, k + 1);
+  }
 }
 
 }  // namespace
@@ -822,6 +871,9 @@ void oracle_dc_backward(const OracleResult* R, const double* gV, double* gphi) {
 }
 
 void oracle_dc_free(OracleResult* R) { delete R; }
+
+// Threads of the forward's loops (1: the reference's single-threaded run).
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 // triangulate_v1 (contouring.cpp:88-113) with orient_triangles_for_quad (contouring.cpp:40-86):
 // quad f = (a, b, c, d) -> triangles 2f = (a, b, c) and 2f+1 = (a, c, d). The quad normal is
