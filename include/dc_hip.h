@@ -106,6 +106,23 @@ int dc_hip_extract(dc_hip_context* ctx, const dc_hip_grid* grid, int precision,
  * device pointer, read on the context's stream; the call returns when the mesh is complete. */
 int dc_hip_extract_device(dc_hip_context* ctx, const dc_hip_grid* grid, int precision,
                           dc_hip_stats* stats);
+/* No counterpart in the reference: an extract_device that allocates nothing, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). values is a device pointer; the mesh goes straight into the caller's device
+ * buffers: vertices (3 * vertex_capacity floats), vertices_f64 (optional, plain FP64 mode only,
+ * 3 * vertex_capacity doubles) and faces (4 * face_capacity int32). Every intermediate buffer
+ * lives in the library's static device scratch (64 MiB per device, held until the process
+ * ends): a grid whose marking planes plus capacity-sized lists exceed it (about 256^3 with
+ * meshes of a few hundred thousand quads fit) is rejected with DC_HIP_ERR_INVALID_GRID before
+ * anything runs. *n_vertices and *n_faces receive the mesh's counts; when they exceed the
+ * capacities the call returns DC_HIP_ERR_INVALID_ARG with the counts set (nothing past the
+ * capacities is written). Runs on the context's stream without per-pass timing and returns when
+ * the mesh is complete. Calls on one device share the scratch: calls that overlap on one device
+ * race. The context's last result (dc_hip_result_*, dc_hip_backward) is cleared. */
+int dc_hip_extract_device_noalloc(dc_hip_context* ctx, const dc_hip_grid* grid, int precision,
+                                  float* vertices, double* vertices_f64, int64_t vertex_capacity,
+                                  int32_t* faces, int64_t face_capacity, int64_t* n_vertices,
+                                  int64_t* n_faces);
 /* Counts of the last successful extraction (single grid or whole batch). */
 int dc_hip_result_counts(dc_hip_context* ctx, int64_t* n_vertices, int64_t* n_faces);
 /* Copies the last result (any pointer may be NULL; host or device memory). vertices:

@@ -24,7 +24,8 @@ DC_HIP_FP32_FALLBACK = 1
 # Every symbol include/dc_hip.h declares; the CPU tests check the library exports all of them.
 EXPORTS = (
     "dc_hip_create", "dc_hip_destroy", "dc_hip_set_stream", "dc_hip_set_option",
-    "dc_hip_extract", "dc_hip_extract_device", "dc_hip_result_counts", "dc_hip_copy_result",
+    "dc_hip_extract", "dc_hip_extract_device", "dc_hip_extract_device_noalloc",
+    "dc_hip_result_counts", "dc_hip_copy_result",
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
     "dc_hip_triangulate",
     "dc_hip_last_error", "dc_hip_status_string",
@@ -86,6 +87,8 @@ def load_hip():
         "dc_hip_extract": (I, [P, ctypes.POINTER(DcHipGrid), I, ctypes.POINTER(DcHipStats)]),
         "dc_hip_extract_device": (I, [P, ctypes.POINTER(DcHipGrid), I,
                                       ctypes.POINTER(DcHipStats)]),
+        "dc_hip_extract_device_noalloc": (I, [P, ctypes.POINTER(DcHipGrid), I, P, P, I64, P, I64,
+                                              ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "dc_hip_result_counts": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "dc_hip_copy_result": (I, [P, P, P, P]),
         "dc_hip_result_device": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]),

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,7 +28,9 @@ struct DevBuf {
   size_t cap = 0;
 };
 
-constexpr int kNumEvents = 10;  // 0..7 forward passes, 8..9 upload / backward
+constexpr int kNumEvents = 10;
+// Pinned staging of one grid's descriptor and bounds (dc_hip_extract_device_noalloc).
+constexpr size_t kStageBytes = 1024;  // 0..7 forward passes, 8..9 upload / backward
 // Adjoint: active cells per CU up to which the second per-cell kernel runs 4 lanes per cell
 // (its crossing edges split over the lanes) rather than one thread per cell.
 constexpr int64_t kWideCellsPerCu = 1024;
@@ -46,6 +49,87 @@ struct Captured {
     exec = nullptr;
     graph = nullptr;
     key.clear();
+  }
+};
+
+// The device buffers of one forward launch sequence: the context's grow-only buffers, or
+// carvings of the static scratch array (dc_hip_extract_device_noalloc).
+struct Bufs {
+  DcGridDev* grids;
+  int64_t* bounds;      // 7 * (n + 1): tiles, act words, pencils, samples, gather rows, face
+                        // words, cell-pass waves
+  uint64_t* act;
+  uint4* sg;
+  uint32_t* wprefix;
+  uint64_t* ft;
+  uint32_t* pcount;
+  uint32_t* pbase;
+  uint32_t* scan_sum;
+  uint64_t* scan_base;
+  DcCounters* ctr;
+  uint64_t* alist;
+  double* v64;
+  float* v32;
+  uint32_t* flags;
+  uint32_t* fblist;
+  int32_t* faces;
+  uint32_t* pw;
+  void* fnrm;
+  uint64_t* flist;
+  uint32_t* goff;
+  float* qstate;      // FP32 mode's adjoint inputs, or null
+  uint32_t* rows;
+  double* qstate64;
+  uint64_t capA, capF, cap64;
+  bool timing;        // per-pass events
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Static scratch of dc_hip_extract_device_noalloc: every intermediate buffer of one single-grid
+// forward (sign and face planes, the active plane, scan tiles, lists, face normals, counters and
+// descriptors) is carved from this one file-scope device array, so that entry point calls no
+// allocator. Each device holds its own copy of kScratchBytes for the life of the process; host
+// code never uses the name `scratch` as an address (on the host it is a stand-in): the device
+// address comes from hipGetSymbolAddress, once per device, at dc_hip_create with that device
+// current. Calls on one device share the array (and keep the last call's contents): calls that
+// overlap on one device race.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current), looked up once.
+void init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) (void)hipGetLastError();  // leave no sticky error for later launches
+  g_scratch_rc[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? static_cast<unsigned char*>(p) : nullptr;
+  g_scratch_done[device] = true;
+}
+
+// Consecutive 256-byte aligned pieces of a base address (null base: sizes only).
+struct Carver {
+  unsigned char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += std::max<size_t>(bytes, 256);
+    return p;
   }
 };
 
@@ -72,6 +156,7 @@ struct dc_hip_context {
       counters, upload, flags, bwd_contrib, pw, fnrm, ubar, ubar_e, bwd_rows, fblist, flist, goff,
       qstate, qstate64, tris;
   DcCounters* h_counters = nullptr;
+  unsigned char* h_stage = nullptr;  // pinned: descriptors of the allocation-free entry point
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
   int h_goff_cap = 0;
   hipEvent_t ev[kNumEvents] = {};
@@ -96,6 +181,11 @@ namespace {
 
 int fail(dc_hip_context* c, int code, const std::string& msg) {
   if (c) c->err = msg;
+  return code;
+}
+// A literal message: assigned within the error string's reserved capacity (no allocation).
+int fail(dc_hip_context* c, int code, const char* msg) {
+  if (c) c->err.assign(msg);
   return code;
 }
 
@@ -235,7 +325,10 @@ float elapsed(dc_hip_context* c, int a, int b) {
 // The launch sequence after the buffers and descriptors are in place: kernels from the counter
 // clear to the counters' (and batch offsets') read-back into pinned memory. No host
 // synchronisation and no memset nodes, so it can be captured in a graph and replayed.
-int enqueue(dc_hip_context* c, int precision, hipStream_t s);
+int enqueue(dc_hip_context* c, int precision, hipStream_t s, const Bufs& b);
+
+// The context's own buffers as a launch sequence's Bufs.
+Bufs context_bufs(dc_hip_context* c, int precision);
 
 void drop_graph(dc_hip_context* c) {
   c->fwd.drop();
@@ -370,33 +463,68 @@ int run_once(dc_hip_context* c, int precision) {
                              hipMemcpyHostToDevice, s), "upload bounds");
     c->dev_key = key;
   }
+  const Bufs b = context_bufs(c, precision);
   rc = run_captured(c, c->fwd, std::move(key), c->use_graph && !c->timing, s,
-                    [&](hipStream_t st) { return enqueue(c, precision, st); });
+                    [&](hipStream_t st) { return enqueue(c, precision, st, b); });
   if (rc) return rc;
   DC_TRY(c, hipStreamSynchronize(s), "pipeline");
   return DC_HIP_OK;
 }
 
-int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
+Bufs context_bufs(dc_hip_context* c, int precision) {
+  Bufs b;
+  b.grids = as<DcGridDev>(c->grids);
+  b.bounds = as<int64_t>(c->bounds);
+  b.act = as<uint64_t>(c->act);
+  b.sg = as<uint4>(c->sg);
+  b.wprefix = as<uint32_t>(c->wprefix);
+  b.ft = as<uint64_t>(c->ft);
+  b.pcount = as<uint32_t>(c->pcount);
+  b.pbase = as<uint32_t>(c->pbase);
+  b.scan_sum = as<uint32_t>(c->scan_sum);
+  b.scan_base = as<uint64_t>(c->scan_base);
+  b.ctr = as<DcCounters>(c->counters);
+  b.alist = as<uint64_t>(c->alist);
+  b.v64 = as<double>(c->v64);
+  b.v32 = as<float>(c->v32);
+  b.flags = as<uint32_t>(c->flags);
+  b.fblist = as<uint32_t>(c->fblist);
+  b.faces = as<int32_t>(c->faces);
+  b.pw = as<uint32_t>(c->pw);
+  b.fnrm = c->fnrm.p;
+  b.flist = as<uint64_t>(c->flist);
+  b.goff = as<uint32_t>(c->goff);
+  const bool fp32 = precision == DC_HIP_FP32_FALLBACK;
+  b.qstate = fp32 ? as<float>(c->qstate) : nullptr;
+  b.rows = fp32 ? as<uint32_t>(c->bwd_rows) : nullptr;
+  b.qstate64 = fp32 ? as<double>(c->qstate64) : nullptr;
+  b.capA = c->capA;
+  b.capF = c->capF;
+  b.cap64 = c->cap64;
+  b.timing = c->timing;
+  return b;
+}
+
+int enqueue(dc_hip_context* c, int precision, hipStream_t s, const Bufs& b) {
   const int n = (int)c->hgrids.size();
   const int64_t t_a = dc_scan_tiles(c->n_words), t_p = dc_scan_tiles(c->n_pen);
   const int64_t face_blocks =
-      std::max<int64_t>(1, std::min<int64_t>(((int64_t)c->capF + 255) / 256,
+      std::max<int64_t>(1, std::min<int64_t>(((int64_t)b.capF + 255) / 256,
                                              (int64_t)c->cu_count * 16));
-  const int64_t* d_tile_b = as<int64_t>(c->bounds);
+  const int64_t* d_tile_b = b.bounds;
   const int64_t* d_act_b = d_tile_b + (n + 1);
   const int64_t* d_pen_b = d_act_b + (n + 1);
   const int64_t* d_ft_b = d_tile_b + 5 * (n + 1);
-  const DcGridDev* d_grids = as<DcGridDev>(c->grids);
-  DcCounters* d_ctr = as<DcCounters>(c->counters);
-  uint32_t* tsum = as<uint32_t>(c->scan_sum);
-  uint64_t* tbase = as<uint64_t>(c->scan_base);
+  const DcGridDev* d_grids = b.grids;
+  DcCounters* d_ctr = b.ctr;
+  uint32_t* tsum = b.scan_sum;
+  uint64_t* tbase = b.scan_base;
 
   // clears: kernels rather than memset nodes, so the sequence can be captured in a graph
   DC_TRY(c, dc_launch_fill_u32(reinterpret_cast<uint32_t*>(d_ctr),
                                (int64_t)(sizeof(DcCounters) / 4), 0u, s), "clear counters");
 
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[0], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[0], s), "event");
   DcMarkArgs ma;
   ma.grids = d_grids;
   ma.n_grids = n;
@@ -404,35 +532,35 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   ma.n_tiles = c->n_tiles;
   ma.rows = c->mark_rows;
   ma.waves_per_block = c->mark_waves;
-  ma.sg = as<uint4>(c->sg);
+  ma.sg = b.sg;
   ma.cw_bounds = d_tile_b + 6 * (n + 1);
   ma.n_cw = c->n_cw;
-  ma.act = as<uint64_t>(c->act);
-  ma.ft = as<uint64_t>(c->ft);
-  ma.pcount = as<uint32_t>(c->pcount);
+  ma.act = b.act;
+  ma.ft = b.ft;
+  ma.pcount = b.pcount;
   ma.ctr = d_ctr;
   DC_TRY(c, dc_launch_mark(ma, s), "mark");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[7], s), "event");
   DC_TRY(c, dc_launch_mark_finish(ma, s), "cell pass");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[1], s), "event");
   // With per-pass timing off the three scans (active words, pencil counts, face words) share
   // their launches (reduce, tile scan, apply: 3 launches instead of 9); with it on each runs by
   // itself between its pass's events.
-  const bool shared_scans = !c->timing;
+  const bool shared_scans = !b.timing;
   if (shared_scans) {
     DcScan3Args sa;
-    sa.act = as<uint64_t>(c->act);
+    sa.act = b.act;
     sa.n_words = c->n_words;
-    sa.wprefix = as<uint32_t>(c->wprefix);
-    sa.alist = as<uint64_t>(c->alist);
-    sa.capA = c->capA;
-    sa.pcount = as<uint32_t>(c->pcount);
+    sa.wprefix = b.wprefix;
+    sa.alist = b.alist;
+    sa.capA = b.capA;
+    sa.pcount = b.pcount;
     sa.n_pen = c->n_pen;
-    sa.pbase = as<uint32_t>(c->pbase);
-    sa.ft = as<uint64_t>(c->ft);
+    sa.pbase = b.pbase;
+    sa.ft = b.ft;
     sa.n_ft = c->n_ft;
-    sa.flist = as<uint64_t>(c->flist);
-    sa.capF = c->capF;
+    sa.flist = b.flist;
+    sa.capF = b.capF;
     sa.tiles_a = t_a;
     sa.tiles_p = t_p;
     sa.tiles_f = dc_scan_tiles(c->n_ft);
@@ -441,16 +569,15 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
     sa.ctr = d_ctr;
     DC_TRY(c, dc_launch_scan3(sa, s), "scans");
   } else {
-    DC_TRY(c, dc_launch_scan_act(as<uint64_t>(c->act), c->n_words, tsum, tbase,
-                                 as<uint32_t>(c->wprefix), as<uint64_t>(c->alist), c->capA,
+    DC_TRY(c, dc_launch_scan_act(b.act, c->n_words, tsum, tbase, b.wprefix, b.alist, b.capA,
                                  &d_ctr->n_active, s),
            "active scan");
-    if (c->timing) DC_TRY(c, hipEventRecord(c->ev[2], s), "event");
-    DC_TRY(c, dc_launch_scan_pen(as<uint32_t>(c->pcount), c->n_pen, tsum + t_a, tbase + t_a,
-                                 as<uint32_t>(c->pbase), &d_ctr->n_faces, s),
+    if (b.timing) DC_TRY(c, hipEventRecord(c->ev[2], s), "event");
+    DC_TRY(c, dc_launch_scan_pen(b.pcount, c->n_pen, tsum + t_a, tbase + t_a, b.pbase,
+                                 &d_ctr->n_faces, s),
            "pencil scan");
   }
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[3], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[3], s), "event");
   DcFaceNormalArgs fa;
   fa.grids = d_grids;
   fa.n_grids = n;
@@ -458,70 +585,69 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s) {
   fa.n_pen = c->n_pen;
   fa.ft_bounds = d_ft_b;
   fa.n_ft = c->n_ft;
-  fa.ft = as<uint64_t>(c->ft);
-  fa.pbase = as<uint32_t>(c->pbase);
-  fa.pw = as<uint32_t>(c->pw);
-  fa.flist = as<uint64_t>(c->flist);
+  fa.ft = b.ft;
+  fa.pbase = b.pbase;
+  fa.pw = b.pw;
+  fa.flist = b.flist;
   fa.tile_sum = tsum + t_a + t_p;
   fa.tile_base = tbase + t_a + t_p;
   fa.ctr = d_ctr;
-  fa.capF = c->capF;
+  fa.capF = b.capF;
   fa.face_blocks = face_blocks;
   fa.precision = precision;
-  fa.nrm = c->fnrm.p;
+  fa.nrm = b.fnrm;
   fa.list_done = shared_scans ? 1 : 0;
   DC_TRY(c, dc_launch_face_normals(fa, s), "face normals");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[4], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[4], s), "event");
   DcQefArgs qa;
   qa.grids = d_grids;
   qa.n_grids = n;
   qa.act_bounds = d_act_b;
-  qa.act = as<uint64_t>(c->act);
-  qa.wprefix = as<uint32_t>(c->wprefix);
-  qa.alist = as<uint64_t>(c->alist);
+  qa.act = b.act;
+  qa.wprefix = b.wprefix;
+  qa.alist = b.alist;
   qa.ctr = d_ctr;
-  qa.capA = c->capA;
+  qa.capA = b.capA;
   qa.blocks = c->qef_blocks > 0 ? c->qef_blocks : (int64_t)c->cu_count * 8;
   qa.precision = precision;
-  qa.v64 = as<double>(c->v64);
-  qa.v32 = as<float>(c->v32);
-  qa.flags = as<uint32_t>(c->flags);
-  qa.fblist = as<uint32_t>(c->fblist);
+  qa.v64 = b.v64;
+  qa.v32 = b.v32;
+  qa.flags = b.flags;
+  qa.fblist = b.fblist;
   qa.ctr_rw = d_ctr;
-  qa.ft = as<uint64_t>(c->ft);
-  qa.pw = as<uint32_t>(c->pw);
-  qa.nrm = c->fnrm.p;
-  qa.capF = c->capF;
-  qa.qstate = precision == DC_HIP_FP32_FALLBACK ? as<float>(c->qstate) : nullptr;
-  qa.rows = precision == DC_HIP_FP32_FALLBACK ? as<uint32_t>(c->bwd_rows) : nullptr;
-  qa.qstate64 = precision == DC_HIP_FP32_FALLBACK ? as<double>(c->qstate64) : nullptr;
-  qa.cap64 = c->cap64;
+  qa.ft = b.ft;
+  qa.pw = b.pw;
+  qa.nrm = b.fnrm;
+  qa.capF = b.capF;
+  qa.qstate = b.qstate;
+  qa.rows = b.rows;
+  qa.qstate64 = b.qstate64;
+  qa.cap64 = b.qstate64 ? b.cap64 : 0;
   DC_TRY(c, dc_launch_qef(qa, s), "qef");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[5], s), "event");
   DcEmitArgs ea;
   ea.grids = d_grids;
   ea.n_grids = n;
   ea.ft_bounds = d_ft_b;
-  ea.flist = as<uint64_t>(c->flist);
+  ea.flist = b.flist;
   ea.face_blocks = face_blocks;
-  ea.ft = as<uint64_t>(c->ft);
-  ea.pw = as<uint32_t>(c->pw);
-  ea.act = as<uint64_t>(c->act);
-  ea.wprefix = as<uint32_t>(c->wprefix);
-  ea.v64 = as<double>(c->v64);
-  ea.v32 = as<float>(c->v32);
+  ea.ft = b.ft;
+  ea.pw = b.pw;
+  ea.act = b.act;
+  ea.wprefix = b.wprefix;
+  ea.v64 = b.v64;
+  ea.v32 = b.v32;
   ea.precision = precision;
   ea.ctr = d_ctr;
-  ea.capF = c->capF;
-  ea.faces = as<int32_t>(c->faces);
-  ea.capA = c->capA;
+  ea.capF = b.capF;
+  ea.faces = b.faces;
+  ea.capA = b.capA;
   DC_TRY(c, dc_launch_emit(ea, s), "emit");
-  if (c->timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
+  if (b.timing) DC_TRY(c, hipEventRecord(c->ev[6], s), "event");
   if (n > 1) {
-    DC_TRY(c, dc_launch_grid_offsets(d_act_b, d_pen_b, n, as<uint32_t>(c->wprefix),
-                                     as<uint32_t>(c->pbase), as<uint32_t>(c->goff), s),
+    DC_TRY(c, dc_launch_grid_offsets(d_act_b, d_pen_b, n, b.wprefix, b.pbase, b.goff, s),
            "grid offsets");
-    DC_TRY(c, hipMemcpyAsync(c->h_goff, c->goff.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s),
+    DC_TRY(c, hipMemcpyAsync(c->h_goff, b.goff, 8 * (size_t)n, hipMemcpyDeviceToHost, s),
            "read grid offsets");
   }
   DC_TRY(c, hipMemcpyAsync(c->h_counters, d_ctr, sizeof(DcCounters), hipMemcpyDeviceToHost, s),
@@ -652,6 +778,17 @@ int dc_hip_create(int device, dc_hip_context** out) {
     return DC_HIP_ERR_OUT_OF_MEMORY;
   }
   std::memset(c->h_counters, 0, sizeof(DcCounters));
+  if (hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+    dc_hip_destroy(c);
+    return DC_HIP_ERR_OUT_OF_MEMORY;
+  }
+  // the allocation-free entry point's set-up: the device address of the static scratch, and
+  // host-side capacity for a single-grid plan and error messages, so that it allocates nothing
+  init_scratch(device);
+  c->hgrids.reserve(1);
+  for (auto* v : {&c->tile_b, &c->act_b, &c->pen_b, &c->samp_b, &c->sr_b, &c->ft_b, &c->cw_b})
+    v->reserve(2);
+  c->err.reserve(512);
   *out = c;
   return DC_HIP_OK;
 }
@@ -674,6 +811,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   if (c->ev_side) hipEventDestroy(c->ev_side);
   drop_graph(c);
   if (c->h_counters) hipHostFree(c->h_counters);
+  if (c->h_stage) hipHostFree(c->h_stage);
   if (c->h_goff) hipHostFree(c->h_goff);
   if (c->side) {
     hipStreamSynchronize(c->side);
@@ -779,6 +917,108 @@ int dc_hip_extract_batch(dc_hip_context* c, const dc_hip_grid* grids, int32_t n_
     if (vertex_offsets) vertex_offsets[n_grids] = c->nv;
     if (face_offsets) face_offsets[n_grids] = c->nf;
   }
+  return DC_HIP_OK;
+}
+
+int dc_hip_extract_device_noalloc(dc_hip_context* c, const dc_hip_grid* grid, int precision,
+                                  float* vertices, double* vertices_f64, int64_t vertex_capacity,
+                                  int32_t* faces, int64_t face_capacity, int64_t* n_vertices,
+                                  int64_t* n_faces) {
+  if (!c || !grid || !n_vertices || !n_faces)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  *n_vertices = 0;
+  *n_faces = 0;
+  c->have = false;  // this call's result lives in the caller's buffers, not in the context
+  if (precision != DC_HIP_PLAIN_FP64 && precision != DC_HIP_FP32_FALLBACK)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "unknown precision");
+  if (vertex_capacity < 0 || face_capacity < 0)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "negative output capacity");
+  if ((vertex_capacity > 0 && !vertices) || (face_capacity > 0 && !faces))
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL output buffer with a nonzero capacity");
+  if (vertices_f64 && precision != DC_HIP_PLAIN_FP64)
+    return fail(c, DC_HIP_ERR_INVALID_ARG, "FP64 vertices exist only in plain FP64 mode");
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+    return fail(c, DC_HIP_ERR_HIP, "static scratch unavailable on this device");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  int rc = plan(c, grid, 1);
+  if (rc) return rc;
+  const DcGridDev& g = c->hgrids[0];
+  const int64_t cells = (int64_t)g.cx * g.cy * g.cz;
+  const uint64_t capA = (uint64_t)std::min<int64_t>(vertex_capacity, cells);
+  const uint64_t capF = (uint64_t)std::min<int64_t>(face_capacity, 3 * cells);
+  const int64_t t_a = dc_scan_tiles(c->n_words), t_p = dc_scan_tiles(c->n_pen),
+                t_f = dc_scan_tiles(c->n_ft);
+  const bool plain = precision == DC_HIP_PLAIN_FP64;
+  // the layout: sized with a null base first, then carved from this device's copy
+  auto layout = [&](unsigned char* base, Bufs& b) {
+    Carver cv{base};
+    b.grids = cv.take<DcGridDev>(sizeof(DcGridDev));
+    b.bounds = cv.take<int64_t>(7 * 2 * sizeof(int64_t));
+    b.ctr = cv.take<DcCounters>(sizeof(DcCounters));
+    b.act = cv.take<uint64_t>(c->n_words * 8);
+    b.sg = cv.take<uint4>(c->n_sg * 16);
+    b.wprefix = cv.take<uint32_t>(c->n_words * 4);
+    b.ft = cv.take<uint64_t>(c->n_ft * 8);
+    b.pcount = cv.take<uint32_t>(c->n_pen * 4);
+    b.pbase = cv.take<uint32_t>(c->n_pen * 4);
+    b.scan_sum = cv.take<uint32_t>((t_a + t_p + t_f) * 4);
+    b.scan_base = cv.take<uint64_t>((t_a + t_p + t_f) * 8);
+    b.alist = cv.take<uint64_t>(capA * 8);
+    b.v64 = (plain && !vertices_f64) ? cv.take<double>(capA * 24) : vertices_f64;
+    b.flags = cv.take<uint32_t>(plain ? 0 : capA * 4);
+    b.fblist = cv.take<uint32_t>(plain ? 0 : capA * 4);
+    b.pw = cv.take<uint32_t>(c->n_ft * 4);
+    b.fnrm = cv.take<unsigned char>(capF * 32);
+    b.flist = cv.take<uint64_t>(capF * 8);
+    return cv.off;
+  };
+  Bufs b{};
+  const size_t need = layout(nullptr, b);
+  if (need > kScratchBytes) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg),
+                  "grid and output capacities need %zu bytes of scratch; the static scratch "
+                  "holds %zu",
+                  need, kScratchBytes);
+    return fail(c, DC_HIP_ERR_INVALID_GRID, (const char*)msg);
+  }
+  layout(g_scratch_ptr[c->device], b);
+  b.v32 = vertices;
+  b.faces = faces;
+  b.goff = nullptr;  // one grid: no batch offsets
+  b.qstate = nullptr;  // no adjoint state: the adjoint needs a regular forward
+  b.rows = nullptr;
+  b.qstate64 = nullptr;
+  b.capA = capA;
+  b.capF = capF;
+  b.cap64 = 0;
+  b.timing = false;
+  // descriptor and bounds through the pinned staging (no host allocation, no pageable copy)
+  const hipStream_t s = c->stream;
+  std::memcpy(c->h_stage, &g, sizeof(DcGridDev));
+  int64_t* hb = reinterpret_cast<int64_t*>(c->h_stage + 256);
+  const std::vector<int64_t>* parts[7] = {&c->tile_b, &c->act_b, &c->pen_b, &c->samp_b,
+                                          &c->sr_b, &c->ft_b, &c->cw_b};
+  for (int q = 0; q < 7; ++q) {
+    hb[2 * q] = (*parts[q])[0];
+    hb[2 * q + 1] = (*parts[q])[1];
+  }
+  DC_TRY(c, hipMemcpyAsync(b.grids, c->h_stage, sizeof(DcGridDev), hipMemcpyHostToDevice, s),
+         "upload descriptor");
+  DC_TRY(c, hipMemcpyAsync(b.bounds, hb, 14 * sizeof(int64_t), hipMemcpyHostToDevice, s),
+         "upload bounds");
+  rc = enqueue(c, precision, s, b);
+  if (rc) return rc;
+  DC_TRY(c, hipStreamSynchronize(s), "pipeline");
+  const DcCounters& h = *c->h_counters;
+  if (h.error) return fail(c, DC_HIP_ERR_INTERNAL, "device consistency check failed");
+  if (h.n_list != h.n_faces) return fail(c, DC_HIP_ERR_INTERNAL, "face list count mismatch");
+  if (h.nonfinite) return fail(c, DC_HIP_ERR_NONFINITE, "grid contains NaN or Inf samples");
+  *n_vertices = (int64_t)h.n_active;
+  *n_faces = (int64_t)h.n_faces;
+  if (h.n_active > (uint64_t)vertex_capacity || h.n_faces > (uint64_t)face_capacity)
+    return fail(c, DC_HIP_ERR_INVALID_ARG,
+                "output capacity too small for the mesh (required counts returned)");
   return DC_HIP_OK;
 }
 
