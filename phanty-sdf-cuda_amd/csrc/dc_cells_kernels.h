@@ -63,8 +63,9 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// fn(G, a, i, j, k, face id) for every face, one thread per face-list entry (grid-stride over
-// the count). An entry that names no quad edge of its grid is skipped and flagged in *err.
+// fn(G, a, i, j, k, face id) for every face, one thread per face-list entry; each block takes a
+// contiguous XCD-aware share of the list (at 512^3 this halves the kernels' HBM fetch: list
+// neighbours share stencil rows, and adjacent shares run under one L2). An entry that names no quad edge of its grid is skipped and flagged in *err.
 template <typename Fn>
 __device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, int n_grids,
                                           const int64_t* __restrict__ ft_bounds,
@@ -72,8 +73,9 @@ __device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, i
                                           const uint64_t* __restrict__ ft,
                                           const uint32_t* __restrict__ pw, uint64_t n,
                                           uint32_t* __restrict__ err, Fn&& fn) {
-  for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n;
-       f += (uint64_t)gridDim.x * blockDim.x) {
+  uint64_t lo, hi;
+  dccommon::block_share(n, lo, hi);
+  for (uint64_t f = lo + threadIdx.x; f < hi; f += blockDim.x) {
     const uint64_t entry = flist[f];
     const int64_t word = (int64_t)(entry >> 6);
     const uint64_t below = (1ull << (entry & 63)) - 1ull;
@@ -135,8 +137,19 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 3)
   // FP32 mode: each thread's row kinds gathered in its LDS column (e * 256 + thread), then written
   // as three 16-byte stores, so a wave writes whole lines
   __shared__ uint32_t s_kinds[PREC == DC_PREC_FP32 ? 12 * kKindStride : 1];
-  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (uint64_t)gridDim.x * blockDim.x) {
+  // FP32 solve: contiguous XCD-aware shares (106 against 116 us at 512^3, half the HBM fetch);
+  // the FP64 solve is faster on the plain grid stride (110 against 120 us)
+  uint64_t lo, hi, step;
+  if (PREC == DC_PREC_FP32) {
+    dccommon::block_share(n, lo, hi);
+    lo += threadIdx.x;
+    step = blockDim.x;
+  } else {
+    lo = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    hi = n;
+    step = (uint64_t)gridDim.x * blockDim.x;
+  }
+  for (uint64_t v = lo; v < hi; v += step) {
     const uint64_t entry = alist[v];
     const int g = find_bucket(act_bounds, n_grids, (int64_t)(entry >> 6));
     const DcGridDev G = grids[g];

@@ -35,6 +35,17 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + s;
 }
 
+// This block's share [lo, hi) of n items split into gridDim.x contiguous shares, the shares handed
+// out XCD-aware (the blocks of one XCD take consecutive shares): list neighbours, which touch the
+// same samples, are then processed under one L2. Every block gets a share however small n is.
+__device__ __forceinline__ void block_share(uint64_t n, uint64_t& lo, uint64_t& hi) {
+  const uint64_t nb = gridDim.x;
+  const uint64_t b = (uint64_t)xcd_remap(blockIdx.x, (int64_t)nb);
+  const uint64_t per = (n + nb - 1) / nb;
+  lo = b * per < n ? b * per : n;
+  hi = lo + per < n ? lo + per : n;
+}
+
 // Inclusive prefix sum over the 64 lanes: DPP row shifts within each row of 16 lanes, then the
 // totals of the lower rows.
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v, int lane) {

@@ -36,12 +36,20 @@ DC_HD inline bool decode_cell(const DcGridDev& G, uint64_t entry, int& i, int& j
   return lw >= 0 && row < (int64_t)G.cy * G.cz && i < G.cx;
 }
 
-// Face-list entry (global face word * 64 + bit) -> the quad's grid edge: axis a and lower endpoint
+// Does the axis-a grid edge with lower endpoint (i, j, k) emit a quad when its sign changes
+// (contouring.cpp:607-729)? x edges i < cx, 1 <= j < cy, 1 <= k < cz; y edges 1 <= i < cx,
+// j < cy, 1 <= k < cz; z edges 1 <= i < cx, 1 <= j < cy, k < cz. Their four cells (one step back
+// across the edge on each of the other two axes) all lie in the grid.
+DC_HD inline bool face_edge_ok(const DcGridDev& G, int a, int i, int j, int k) {
+  if (a < 0 || a > 2 || i >= G.cx || j >= G.cy || k >= G.cz) return false;
+  const int lo_i = a == 0 ? 0 : 1, lo_j = a == 1 ? 0 : 1, lo_k = a == 2 ? 0 : 1;
+  return i >= lo_i && j >= lo_j && k >= lo_k;
+}
+
+// Face-plane bit (global face word * 64 + bit) -> the quad's grid edge: axis a and lower endpoint
 // (i, j, k). The face word of axis a, k-block kw and pencil (i, j) is
-// ft_off + ((a*Wk + kw)*cy + j)*cx + i, bit k % 64. True only for the edges that can emit a quad
-// (contouring.cpp:607-729): x edges i < cx, 1 <= j < cy, 1 <= k < cz; y edges 1 <= i < cx,
-// j < cy, 1 <= k < cz; z edges 1 <= i < cx, 1 <= j < cy, k < cz. Their four cells
-// (one step back across the edge on each of the other two axes) all lie in the grid.
+// ft_off + ((a*Wk + kw)*cy + j)*cx + i, bit k % 64. True only for an edge in G's planes that can
+// emit a quad (face_edge_ok).
 DC_HD inline bool decode_face(const DcGridDev& G, uint64_t entry, int& a, int& i, int& j, int& k) {
   const int64_t plane = (int64_t)G.cy * G.cx;
   const int64_t local = (int64_t)(entry >> 6) - G.ft_off;
@@ -52,9 +60,7 @@ DC_HD inline bool decode_face(const DcGridDev& G, uint64_t entry, int& a, int& i
   j = (int)(rem / G.cx);
   i = (int)(rem - (int64_t)j * G.cx);
   k = 64 * kw + (int)(entry & 63);
-  if (local < 0 || a > 2 || k >= G.cz) return false;
-  const int lo_i = a == 0 ? 0 : 1, lo_j = a == 1 ? 0 : 1, lo_k = a == 2 ? 0 : 1;
-  return i >= lo_i && j >= lo_j && k >= lo_k;
+  return local >= 0 && face_edge_ok(G, a, i, j, k);
 }
 
 }  // namespace dcdecode
