@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "build", "liboracle_dc.so")
+# ORACLE_LIB: a library built from a modified copy of dc_oracle.cpp (the mutation tests)
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle_dc.so")
 _lib = None
 
 
