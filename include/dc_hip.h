@@ -151,7 +151,8 @@ int dc_hip_extract_batch(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t 
 
 /* Adjoint of the last extraction at fixed topology (no counterpart in the reference):
  * grad_values[N_s] (device, overwritten) = dL/dvalues for grad_vertices[3*n_vertices]
- * (device) = dL/dvertices. Deterministic: per-cell stencil contributions are gathered in a
+ * (device) = dL/dvertices. The adjoint reads the last forward's samples again: they must be alive
+ * and unchanged, and grad_values must not overlap them (DC_HIP_ERR_INVALID_ARG otherwise). Deterministic: per-cell stencil contributions are gathered in a
  * fixed order, no floating-point atomics. Returns when the gradient is complete, except with
  * the "timing" option off on a stream set by dc_hip_set_stream: then it returns once the work is
  * enqueued on that stream (later work on the stream sees the gradient). */

@@ -23,7 +23,12 @@ def default_backend(precision="fp32-fallback", device=0):
 class DualContouringFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, phi, backend, spacing, origin):
-        (V, F), _ = backend.extract_tensor(phi.detach(), spacing, origin)
+        # the adjoint reads the forward's samples again: keep the contiguous copy the context
+        # points at alive until backward (a temporary would be freed, and torch's allocator could
+        # hand its memory to the gradient)
+        phic = phi.detach().contiguous()
+        (V, F), _ = backend.extract_tensor(phic, spacing, origin)
+        ctx.phic = phic
         ctx.backend = backend
         ctx.generation = backend.generation
         ctx.spacing = spacing
@@ -38,12 +43,13 @@ class DualContouringFunction(torch.autograd.Function):
         be = ctx.backend
         if be.generation != ctx.generation:
             # another extraction used the context since: rebuild this forward's topology
-            be.extract_tensor(phi.detach(), ctx.spacing, ctx.origin)
+            be.extract_tensor(ctx.phic, ctx.spacing, ctx.origin)
         # The forward extracted phi.contiguous() and the adjoint writes dL/dphi in that layout
         # (x fastest): allocate it contiguous whatever phi's strides are (autograd accepts any).
         gphi = torch.empty(phi.shape, dtype=torch.float32, device=phi.device,
                            memory_format=torch.contiguous_format)
         be.backward(gV.contiguous(), gphi)
+        ctx.phic = None
         return gphi, None, None, None
 
 

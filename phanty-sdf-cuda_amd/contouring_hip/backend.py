@@ -134,6 +134,8 @@ class HipDualContouringBackend:
         self._ctx = ctx
         self.generation = 0
         self._samples = 0  # samples of the last forward (single grid, or the whole batch)
+        # the device tensors the context's last forward points at: the adjoint reads them again
+        self._inputs = ()
         if mark_rows is not None:
             self.set_option("mark_rows", mark_rows)
         if mark_waves is not None:
@@ -203,6 +205,7 @@ class HipDualContouringBackend:
                                              PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
         self._samples = vals.size
+        self._inputs = ()  # the samples were uploaded into the context's own buffer
         t0 = time.perf_counter()
         if with_f64 is None:
             with_f64 = self.precision == "plain-fp64"
@@ -228,6 +231,7 @@ class HipDualContouringBackend:
                                                     PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
         self._samples = n
+        self._inputs = () if isinstance(grid.d_values, int) else (grid.d_values,)
         t0 = time.perf_counter()
         if with_f64 is None:
             with_f64 = self.precision == "plain-fp64"
@@ -257,6 +261,7 @@ class HipDualContouringBackend:
                                                     PRECISIONS[self.precision], ctypes.byref(st)))
         self.generation += 1
         self._samples = phi.numel()
+        self._inputs = (phi,)
         nv, nf = self.counts()
         V = torch.empty((nv, 3), dtype=torch.float32, device=phi.device)
         F = torch.empty((nf, 4), dtype=torch.int32, device=phi.device)
@@ -292,6 +297,7 @@ class HipDualContouringBackend:
             foff.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
         self.generation += 1
         self._samples = total
+        self._inputs = tuple(g.d_values for g in grids if not isinstance(g.d_values, int))
         nv, nf = self.counts()
         dev = torch.device("cuda", self.device)
         V = torch.empty((nv, 3), dtype=torch.float32, device=dev)
@@ -323,7 +329,9 @@ class HipDualContouringBackend:
     def backward(self, grad_vertices, grad_phi):
         """dL/dphi (written into grad_phi: a contiguous float32 CUDA tensor with one element per
         sample of the last forward, all grids' samples concatenated after a batch) for dL/dV
-        (3 per vertex of the last forward)."""
+        (3 per vertex of the last forward). The adjoint reads the last forward's samples again:
+        the backend keeps the device tensors it was given alive (a contiguous copy, for
+        extract_tensor), and they must be unchanged; grad_phi must be a different tensor."""
         import torch
         nv, _ = self.counts()
         if not isinstance(grad_vertices, torch.Tensor):

@@ -973,13 +973,16 @@ __device__ __forceinline__ uint32_t row_kind(const uint4 r[3], int e) {
 }
 
 // Crossing rows with a face (fmask) and with an edge slot (smask), from the 12 kinds.
-__device__ __forceinline__ void kind_masks(const uint4 kinds[3], uint32_t cross, uint32_t& fmask,
-                                           uint32_t& smask) {
+// Rows of the crossing mask with a face (fmask) and with an edge slot (smask). A kind is a face id
+// only below nf (the face count): a kind outside that range (never written by a consistent
+// forward) names no slot and the row is taken as the cell's own normal.
+__device__ __forceinline__ void kind_masks(const uint4 kinds[3], uint32_t cross, uint64_t nf,
+                                           uint32_t& fmask, uint32_t& smask) {
   fmask = smask = 0;
 #pragma unroll
   for (int e = 0; e < 12; ++e) {
     const uint32_t kind = row_kind(kinds, e);
-    fmask |= (uint32_t)(kind < kRowEdgeSlot) << e;
+    fmask |= (uint32_t)((uint64_t)kind < nf) << e;
     smask |= (uint32_t)(kind == kRowEdgeSlot) << e;
   }
   fmask &= cross;
@@ -1146,7 +1149,7 @@ __device__ __forceinline__ void backward_rows(const DcGridDev& G, const uint64_t
   if constexpr (kGiven) {
     constexpr int kPre = 3;
     uint32_t fmask, smask;
-    kind_masks(kinds, cross, fmask, smask);
+    kind_masks(kinds, cross, fn.cap, fmask, smask);
     T pre[kPre][3];
     uint32_t pf[kPre];
     {
@@ -1221,13 +1224,14 @@ __device__ __forceinline__ void backward_row_given(const DcGridDev& G,
                                                    float* __restrict__ ubar_e, T Sb[8]) {
   T nf[3];
   float* slot;
-  if (kind < kRowEdgeSlot) {
+  if ((uint64_t)kind < fn.cap) {
     nf[0] = fn.at(4 * (uint64_t)kind + 0);
     nf[1] = fn.at(4 * (uint64_t)kind + 1);
     nf[2] = fn.at(4 * (uint64_t)kind + 2);
     slot = face_slot(G, act, i, j, k, e, kind, ubar);
-  } else {
-    slot = row_nonface(G, act, i, j, k, own, e, kind, v, ubar_e, nf);
+  } else {  // an edge slot, or the own normal (also for a kind past the faces)
+    slot = row_nonface(G, act, i, j, k, own, e, kind == kRowEdgeSlot ? kRowEdgeSlot : kRowOwn, v,
+                       ubar_e, nf);
   }
   row_adjoint(own, R, e, nf, slot, Sb);
 }
@@ -1290,14 +1294,14 @@ template <typename T, typename S>
 __device__ __forceinline__ void backward_faces(const DcGridDev& G, const uint64_t* __restrict__ act,
                                const uint32_t* __restrict__ wprefix, int i, int j, int k,
                                const uint32_t* __restrict__ rows, const float* __restrict__ ubar,
-                               const float* __restrict__ ubar_e, T Sb[8]) {
+                               const float* __restrict__ ubar_e, uint64_t nf, T Sb[8]) {
   constexpr int kPre = 3;
   const uint4* r4 = reinterpret_cast<const uint4*>(rows);
   const uint4 kinds[3] = {r4[0], r4[1], r4[2]};
   Cell<T> own;
   load_cell(G, i, j, k, own);
   uint32_t fmask, smask;  // crossing rows with a face, with an edge slot
-  kind_masks(kinds, crossing_mask(own), fmask, smask);
+  kind_masks(kinds, crossing_mask(own), nf, fmask, smask);
   T pre[kPre][3];
   {
     uint32_t pm = fmask;
@@ -1349,7 +1353,9 @@ __device__ __forceinline__ void backward_face_row(const DcGridDev& G,
                                                   const uint32_t* __restrict__ wprefix, int i,
                                                   int j, int k, const Cell<T>& own, int e,
                                                   uint32_t kind, const float* __restrict__ ubar,
-                                                  const float* __restrict__ ubar_e, T Sb[8]) {
+                                                  const float* __restrict__ ubar_e, uint64_t nf,
+                                                  T Sb[8]) {
+  if ((uint64_t)kind >= nf && kind != kRowEdgeSlot) return;  // the own normal: no face term
   T accb[3];
   if (kind < kRowEdgeSlot) {
     const S* a = reinterpret_cast<const S*>(ubar + 12 * (uint64_t)kind);

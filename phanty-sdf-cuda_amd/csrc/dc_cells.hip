@@ -1,6 +1,7 @@
 // dc_cells.hip — host launchers of the per-face and per-active-cell kernels, exact-arithmetic
 // half (built with -ffp-contract=off): pencil words, face normals and the QEF solve of the plain
-// FP64 mode (vertices bit-identical to the CPU backend's), quad emission (orientation tests) and
+// FP64 mode (vertices bit-identical to the CPU backend's), quad emission (orientation tests, and
+// FP32 mode's near-tie quads oriented again from FP64 vertices solved here) and
 // the adjoint's gather. The FP32 mode and the adjoint cell kernels are launched from
 // dc_cells_fast.hip.
 
@@ -20,10 +21,12 @@ hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s) {
                             &a.ctr->n_list, s);
     if (e != hipSuccess) return e;
   }
-  if (a.precision != DC_PREC_FP64) return dc_launch_face_normals_fp32(a, s);
+  // the table is FP64 in both modes (the FP32 mode's FP64 cells and near-tie quads read it like
+  // the plain solve); FP32 mode also writes a rounded copy for its FP32 solve and adjoint, whose
+  // gathers run faster on half the bytes
   hipLaunchKernelGGL(k_face_normals<double>, dim3((unsigned)a.face_blocks), dim3(256), 0, s,
                      a.grids, a.n_grids, a.ft_bounds, a.flist, a.ft, a.pw, a.ctr, a.capF,
-                     reinterpret_cast<double*>(a.nrm));
+                     reinterpret_cast<double*>(a.nrm), a.nrm32);
   return hipGetLastError();
 }
 
@@ -39,9 +42,18 @@ hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s) {
 }
 
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s) {
+  uint64_t* reo = a.precision != DC_PREC_FP64 ? a.reorient : nullptr;
   hipLaunchKernelGGL(k_emit, dim3((unsigned)a.face_blocks), dim3(256), 0, s, a.grids, a.n_grids,
                      a.ft_bounds, a.flist, a.ft, a.pw, a.act, a.wprefix, a.v64, a.v32,
-                     a.precision, a.ctr, a.capA, a.capF, a.faces);
+                     a.precision, a.ctr, a.capA, a.capF, a.faces, reo);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !reo) return e;
+  // one wave per listed quad (each is four FP64 solves in a row of dependent loads): a quarter of
+  // the face blocks, i.e. one wave per 256 quads of capacity (4096 waves at 512^3); waves past the
+  // count exit after reading it
+  hipLaunchKernelGGL(k_reorient, dim3((unsigned)std::max<int64_t>(1, a.face_blocks / 4)),
+                     dim3(256), 0, s, a.grids, a.n_grids, a.ft_bounds, a.ft, a.pw, a.act,
+                     a.wprefix, a.ctr, a.capF, reo, a.nrm64, a.faces);
   return hipGetLastError();
 }
 

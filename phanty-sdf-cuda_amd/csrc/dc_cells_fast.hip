@@ -1,15 +1,8 @@
 // dc_cells_fast.hip — host launchers of the kernels whose results are held to tolerances, built
-// with floating-point contraction (FMA) allowed: the FP32 mode's face normals and solve, its FP64
-// fallback cells, and the adjoint's per-cell kernels (both precisions).
+// with floating-point contraction (FMA) allowed: the FP32 mode's solve, its FP64 fallback cells,
+// and the adjoint's per-cell kernels (both precisions).
 
 #include "dc_cells_kernels.h"
-
-hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_face_normals<float>, dim3((unsigned)a.face_blocks), dim3(256), 0, s,
-                     a.grids, a.n_grids, a.ft_bounds, a.flist, a.ft, a.pw, a.ctr, a.capF,
-                     reinterpret_cast<float*>(a.nrm));
-  return hipGetLastError();
-}
 
 hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.blocks), block(256);
@@ -20,7 +13,7 @@ hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_qef_fallback, dim3((unsigned)std::max<int64_t>(1, a.blocks * 2)), block,
                      0, s, a.grids, a.n_grids, a.act_bounds, a.act, a.alist, a.ctr_rw, a.capA,
-                     a.fblist, a.v32, a.qstate64, a.cap64);
+                     a.fblist, a.v32, a.qstate64, a.cap64, a.ft, a.pw, a.nrm64, a.capF);
   return hipGetLastError();
 }
 
@@ -31,8 +24,9 @@ dim3 lane_grid(int64_t cells, int lanes) {
 }  // namespace
 
 hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hipStream_t s64) {
-  const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
-  const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
+  // the FP64 face normal table; FP32 mode's FP32 cells read its FP32 copy instead
+  const double* n64 = reinterpret_cast<const double*>(a.nrm);
+  const float* n32 = nullptr;
   const dim3 block(256);
   const int lanes = phase == 1 ? a.lanes1 : a.lanes2;  // for the active cells' kernel
   const dim3 grid = lane_grid(a.n_cells, lanes);
@@ -83,6 +77,8 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  n64 = nullptr;
+  n32 = a.nrm32;
   if (phase == 1) {
     DC_BWD_CELLS(1, float, float, true, 1, a.qstate)
   } else {
@@ -96,9 +92,9 @@ hipError_t dc_launch_bwd_phase(const DcBwdArgs& a, int phase, hipStream_t s, hip
 }
 
 hipError_t dc_launch_bwd_face_adjoint(const DcBwdArgs& a, hipStream_t s) {
-  const double* n64 = a.precision == DC_PREC_FP64 ? reinterpret_cast<const double*>(a.nrm) : nullptr;
-  const float* n32 = a.precision == DC_PREC_FP64 ? nullptr : reinterpret_cast<const float*>(a.nrm);
-  if (n64)
+  const double* n64 = reinterpret_cast<const double*>(a.nrm);  // FP64 table in both modes
+  const float* n32 = nullptr;
+  if (a.precision == DC_PREC_FP64)
     hipLaunchKernelGGL(k_bwd_face_adjoint<double>, dim3((unsigned)a.face_blocks), dim3(256), 0, s,
                        a.ctr, a.capF, n64, n32, a.ubar);
   else

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// fn(G, a, i, j, k, face id) for every face, one thread per face-list entry; each block takes a
+// fn(G, a, i, j, k, face id, list entry) for every face, one thread per face-list entry; each block takes a
 // contiguous XCD-aware share of the list (at 512^3 this halves the kernels' HBM fetch: list
 // neighbours share stencil rows, and adjacent shares run under one L2). An entry that names no quad edge of its grid is skipped and flagged in *err.
 template <typename Fn>
@@ -87,7 +87,7 @@ __device__ __forceinline__ void each_face(const DcGridDev* __restrict__ grids, i
       atomicOr(err, kErrFaceRange);
       continue;
     }
-    fn(G, a, i, j, k, face);
+    fn(G, a, i, j, k, face, entry);
   }
 }
 
@@ -98,9 +98,10 @@ __global__ void __launch_bounds__(256)
     k_face_normals(const DcGridDev* __restrict__ grids, int n_grids,
                    const int64_t* __restrict__ ft_bounds, const uint64_t* __restrict__ flist,
                    const uint64_t* __restrict__ ft, const uint32_t* __restrict__ pw,
-                   DcCounters* __restrict__ ctr, uint64_t capF, T* __restrict__ nrm) {
+                   DcCounters* __restrict__ ctr, uint64_t capF, T* __restrict__ nrm,
+                   float* __restrict__ nrm32) {
   each_face(grids, n_grids, ft_bounds, flist, ft, pw, min(ctr->n_faces, capF), &ctr->error,
-            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face) {
+            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face, uint64_t) {
               if (face >= capF) return;
               T n[3];
               group_sum<T, true>(G, a, i, j, k, n);
@@ -110,6 +111,9 @@ __global__ void __launch_bounds__(256)
               nrm[4 * face + 1] = n[1];
               nrm[4 * face + 2] = n[2];
               nrm[4 * face + 3] = len;
+              if (nrm32)  // FP32 mode: the same normals rounded, for the FP32 solve and adjoint
+                reinterpret_cast<float4*>(nrm32)[face] =
+                    make_float4((float)n[0], (float)n[1], (float)n[2], (float)len);
             });
 }
 
@@ -132,6 +136,7 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 3)
           uint32_t* __restrict__ rows) {
   const uint64_t n = min(ctr->n_active, cap);
   const uint64_t nf = min(ctr->n_faces, capF);
+  // nrm: the FP64 table (FP64 instance) or its FP32 copy (FP32 instance)
   const FaceNormals<double> fn64{ft, pw, reinterpret_cast<const double*>(nrm), nullptr, nf};
   const FaceNormals<float> fn32{ft, pw, nullptr, reinterpret_cast<const float*>(nrm), nf};
   // FP32 mode: each thread's row kinds gathered in its LDS column (e * 256 + thread), then written
@@ -194,7 +199,7 @@ __global__ void __launch_bounds__(256, PREC == DC_PREC_FP32 ? 4 : 3)
   }
 }
 
-// FP32 mode: the listed cells solved in FP64 (inline averaged normals, the plain arithmetic). The
+// FP32 mode: the listed cells solved in FP64 (the plain arithmetic, with the FP64 face table). The
 // solve's state of list entry t < cap_state goes to state + kQefState * t for the adjoint.
 // 16 lanes per cell: lane e computes the row of edge e (an averaged normal takes up to 4 raw
 // normals), then every lane of the group sums the rows in ascending edge order through shuffles
@@ -208,9 +213,11 @@ __global__ void __launch_bounds__(256)
                    const int64_t* __restrict__ act_bounds, const uint64_t* __restrict__ act,
                    const uint64_t* __restrict__ alist, const DcCounters* __restrict__ ctr,
                    uint64_t cap, const uint32_t* __restrict__ fblist, float* __restrict__ v32,
-                   double* __restrict__ state, uint64_t cap_state) {
+                   double* __restrict__ state, uint64_t cap_state, const uint64_t* __restrict__ ft,
+                   const uint32_t* __restrict__ pw, const double* __restrict__ nrm64,
+                   uint64_t capF) {
   const uint64_t n = min(ctr->fallback, cap);
-  const FaceNormals<double> inline64{nullptr, nullptr, nullptr, nullptr, 0ull};
+  const FaceNormals<double> fn64{ft, pw, nrm64, nullptr, min(ctr->n_faces, capF)};
   const int lane = threadIdx.x & 63;
   const int sub = lane & (kFbLanes - 1);
   const int base = lane - sub;
@@ -227,7 +234,7 @@ __global__ void __launch_bounds__(256)
     load_cell(G, i, j, k, own);
     const uint32_t cross = crossing_mask(own);
     double p[3] = {0.0, 0.0, 0.0}, nf[3] = {0.0, 0.0, 0.0};
-    if (sub < 12 && ((cross >> sub) & 1u)) fp64_row(G, act, inline64, i, j, k, own, sub, p, nf);
+    if (sub < 12 && ((cross >> sub) & 1u)) fp64_row(G, act, fn64, i, j, k, own, sub, p, nf);
     double x[3];
     solve_rows_fp64(
         cross,
@@ -297,22 +304,74 @@ __device__ __forceinline__ int32_t vertex_id(const DcGridDev& G, const uint64_t*
   return (int32_t)(wprefix[w] - vbase + (uint32_t)__popcll(act[w] & below));
 }
 
+// contouring.cpp:607-729: the quad (c0, c1, c2, c3) turned to (c0, c3, c2, c1) when its normal
+// (pb - pa) x (pd - pa), or (pc - pa) x (pd - pa) when that has squared length below 1e-24, points
+// against gn, the sum of the edge endpoints' normalized gradients. Returns q . gn (0 when either
+// vector is zero and the quad is left as it is); *q1sq receives the squared length of the first
+// normal (the one the 1e-24 test is made on).
+__device__ __forceinline__ double orient_quad(const double pa[3], const double pb[3],
+                                              const double pc[3], const double pd[3], double gn0,
+                                              double gn1, double gn2, int& c1, int& c3,
+                                              double* q1sq = nullptr) {
+  const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
+  const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
+  double q0 = ba1 * da2 - ba2 * da1;
+  double q1 = ba2 * da0 - ba0 * da2;
+  double q2 = ba0 * da1 - ba1 * da0;
+  const double first = sq3(q0, q1, q2);
+  if (q1sq) *q1sq = first;
+  if (first < 1e-24) {
+    const double ca0 = pc[0] - pa[0], ca1 = pc[1] - pa[1], ca2 = pc[2] - pa[2];
+    q0 = ca1 * da2 - ca2 * da1;
+    q1 = ca2 * da0 - ca0 * da2;
+    q2 = ca0 * da1 - ca1 * da0;
+  }
+  double dot = 0.0;
+  if (sq3(gn0, gn1, gn2) > 0 && sq3(q0, q1, q2) > 0) {
+    dot = (q0 * gn0 + q1 * gn1) + q2 * gn2;
+    if (dot < 0.0) {
+      const int t = c1;
+      c1 = c3;
+      c3 = t;
+    }
+  }
+  return dot;
+}
+
+// A quad's four cells (c0..c3 in orient_quad's order) around the axis-a edge at (i, j, k):
+// c0 = (u-1, v-1), c1 = (u, v-1), c2 = (u, v), c3 = (u-1, v) over the two axes (u, v) across the
+// edge (x: (y, z); y: (x, z); z: (x, y)).
+__device__ __forceinline__ void quad_cell(int a, int i, int j, int k, int q, int& x, int& y,
+                                          int& z) {
+  const int ux = (a != 0), uy = (a == 0);  // unit step of u
+  const int vy = (a == 2), vz = (a != 2);  // unit step of v
+  const int du = (q == 0 || q == 3), dv = (q == 0 || q == 1);
+  x = i - du * ux;
+  y = j - du * uy - dv * vy;
+  z = k - dv * vz;
+}
+
+// FP32 mode's near-tie bound: an FP32 vertex is within kVertexTol * h of the FP64 one (the mode's
+// stated tolerance), so a quad normal from FP32 vertices is within about
+// 4 kVertexTol h (|b - a| + |c - a| + |d - a|) of the FP64 one. A decision is made again from FP64
+// vertices when, with kTieMargin times that error, its dot product could change sign, or the first
+// normal's length could fall on the other side of the degenerate-quad test (1e-12).
+constexpr double kVertexTol = 1e-4;
+constexpr double kTieMargin = 8.0;
+
 __global__ void __launch_bounds__(256)
     k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ ft_bounds,
            const uint64_t* __restrict__ flist, const uint64_t* __restrict__ ft,
            const uint32_t* __restrict__ pw, const uint64_t* __restrict__ act,
            const uint32_t* __restrict__ wprefix, const double* __restrict__ v64,
            const float* __restrict__ v32, int precision, DcCounters* __restrict__ ctr,
-           uint64_t capA, uint64_t capF, int32_t* __restrict__ faces) {
+           uint64_t capA, uint64_t capF, int32_t* __restrict__ faces,
+           uint64_t* __restrict__ reorient) {
   // an overflowed run is repeated by the host; never index past the buffers
   if (ctr->n_active > capA || ctr->n_faces > capF) return;
   each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces, &ctr->error,
-            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos) {
-    // the edge's far endpoint; the quad's cells: c0 = (u-1, v-1), c1 = (u, v-1), c2 = (u, v),
-    // c3 = (u-1, v) over the two axes (u, v) across the edge (x: (y, z); y: (x, z); z: (x, y))
-    const int e1x = i + (a == 0), e1y = j + (a == 1), e1z = k + (a == 2);
-    const int ux = (a != 0), uy = (a == 0);  // unit step of u
-    const int vy = (a == 2), vz = (a != 2);  // unit step of v
+            [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos, uint64_t entry) {
+    const int e1x = i + (a == 0), e1y = j + (a == 1), e1z = k + (a == 2);  // far endpoint
     // all loads of the quad are issued before its arithmetic: the stencil samples of both
     // endpoints, then the four cells' act words and prefixes
     float f0[6], f1[6];
@@ -320,42 +379,115 @@ __global__ void __launch_bounds__(256)
     gradient_samples(G, e1x, e1y, e1z, f1);
     const uint32_t vbase = wprefix[G.act_off];
     const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
-    int c0 = vertex_id(G, act, wprefix, vbase, i - ux, j - uy - vy, k - vz);
-    int c1 = vertex_id(G, act, wprefix, vbase, i, j - vy, k - vz);
-    const int c2 = vertex_id(G, act, wprefix, vbase, i, j, k);
-    int c3 = vertex_id(G, act, wprefix, vbase, i - ux, j - uy, k);
+    int c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int x, y, z;
+      quad_cell(a, i, j, k, q, x, y, z);
+      c[q] = vertex_id(G, act, wprefix, vbase, x, y, z);
+    }
     double g0[3], g1[3];
     grid_gradient_from(G, i, j, k, f0, g0);
     grid_gradient_from(G, e1x, e1y, e1z, f1, g1);
     const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
     double pa[3], pb[3], pc[3], pd[3];
-    const int64_t vv[4] = {vglob + c0, vglob + c1, vglob + c2, vglob + c3};
     double* dst[4] = {pa, pb, pc, pd};
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * vv[q] + d] : (double)v32[3 * vv[q] + d];
-    const double ba0 = pb[0] - pa[0], ba1 = pb[1] - pa[1], ba2 = pb[2] - pa[2];
-    const double da0 = pd[0] - pa[0], da1 = pd[1] - pa[1], da2 = pd[2] - pa[2];
-    double q0 = ba1 * da2 - ba2 * da1;
-    double q1 = ba2 * da0 - ba0 * da2;
-    double q2 = ba0 * da1 - ba1 * da0;
-    if (sq3(q0, q1, q2) < 1e-24) {
-      const double ca0 = pc[0] - pa[0], ca1 = pc[1] - pa[1], ca2 = pc[2] - pa[2];
-      q0 = ca1 * da2 - ca2 * da1;
-      q1 = ca2 * da0 - ca0 * da2;
-      q2 = ca0 * da1 - ca1 * da0;
-    }
-    if (sq3(gn0, gn1, gn2) > 0 && sq3(q0, q1, q2) > 0) {
-      if ((q0 * gn0 + q1 * gn1) + q2 * gn2 < 0.0) {
-        const int t = c1;
-        c1 = c3;
-        c3 = t;
+        dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * (vglob + c[q]) + d]
+                                                : (double)v32[3 * (vglob + c[q]) + d];
+    int c1 = c[1], c3 = c[3];
+    double q1sq = 0.0;
+    const double dot = orient_quad(pa, pb, pc, pd, gn0, gn1, gn2, c1, c3, &q1sq);
+    if (precision != DC_PREC_FP64 && reorient) {
+      const double ba = tsqrt(sq3(pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]));
+      const double ca = tsqrt(sq3(pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]));
+      const double da = tsqrt(sq3(pd[0] - pa[0], pd[1] - pa[1], pd[2] - pa[2]));
+      const double gl = tsqrt(sq3(gn0, gn1, gn2));
+      const double err =
+          kTieMargin * 4.0 * kVertexTol * fmin(fmin(G.vx, G.vy), G.vz) * (ba + ca + da) + 2e-12;
+      if (gl > 0.0 && (tsqrt(q1sq) <= 1e-12 + err || fabs(dot) <= err * gl)) {
+        const uint64_t slot = atomicAdd((unsigned long long*)&ctr->n_reorient, 1ull);
+        if (slot < capF) reorient[slot] = entry;
       }
     }
-    reinterpret_cast<int4*>(faces)[pos] = make_int4(c0, c1, c2, c3);
+    reinterpret_cast<int4*>(faces)[pos] = make_int4(c[0], c1, c[2], c3);
   });
+}
+
+// FP32 mode: the quads k_emit listed as near ties, oriented again from FP64 vertices of their four
+// cells, solved here with the plain mode's arithmetic (this unit has no FMA contraction; averaged
+// normals from the FP64 face table, as in the plain solve). One wave per quad:
+// lanes 16q..16q+15 solve cell c_q, one crossing row per lane summed in ascending edge order (the
+// serial solve's operation order); lane 0 then orients the quad like the plain mode's k_emit. The
+// result equals the plain mode's quad.
+__global__ void __launch_bounds__(256)
+    k_reorient(const DcGridDev* __restrict__ grids, int n_grids,
+               const int64_t* __restrict__ ft_bounds, const uint64_t* __restrict__ ft,
+               const uint32_t* __restrict__ pw, const uint64_t* __restrict__ act,
+               const uint32_t* __restrict__ wprefix, const DcCounters* __restrict__ ctr,
+               uint64_t capF, const uint64_t* __restrict__ reorient,
+               const double* __restrict__ nrm64, int32_t* __restrict__ faces) {
+  const uint64_t n = min(ctr->n_reorient, capF);
+  const FaceNormals<double> fn64{ft, pw, nrm64, nullptr, min(ctr->n_faces, capF)};
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, sub = lane & 15;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < n;
+       t += waves) {
+    const uint64_t entry = reorient[t];
+    const int64_t word = (int64_t)(entry >> 6);
+    const uint64_t pos = (uint64_t)pw[word] + (uint64_t)__popcll(ft[word] & ((1ull << (entry & 63)) - 1ull));
+    const DcGridDev G = grids[find_bucket(ft_bounds, n_grids, word)];
+    int a, i, j, k;
+    if (!decode_face(G, entry, a, i, j, k) || pos >= capF) continue;  // uniform per wave
+    int x, y, z;
+    quad_cell(a, i, j, k, q, x, y, z);
+    Cell<double> own;
+    load_cell(G, x, y, z, own);
+    const uint32_t cross = crossing_mask(own);
+    double p[3] = {0.0, 0.0, 0.0}, nf[3] = {0.0, 0.0, 0.0};
+    if (sub < 12 && ((cross >> sub) & 1u)) fp64_row(G, act, fn64, x, y, z, own, sub, p, nf);
+    double v[3];
+    solve_rows_fp64(
+        cross,
+        [&](int e, double pe[3], double ne[3]) {
+          const int src = 16 * q + e;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            pe[d] = __shfl(p[d], src);
+            ne[d] = __shfl(nf[d], src);
+          }
+        },
+        v);
+    double P[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) P[m][d] = __shfl(v[d], 16 * m);
+    if (lane == 0) {
+      const int e1x = i + (a == 0), e1y = j + (a == 1), e1z = k + (a == 2);
+      float f0[6], f1[6];
+      gradient_samples(G, i, j, k, f0);
+      gradient_samples(G, e1x, e1y, e1z, f1);
+      double g0[3], g1[3];
+      grid_gradient_from(G, i, j, k, f0, g0);
+      grid_gradient_from(G, e1x, e1y, e1z, f1, g1);
+      const uint32_t vbase = wprefix[G.act_off];
+      int c[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        int cx, cy, cz;
+        quad_cell(a, i, j, k, m, cx, cy, cz);
+        c[m] = vertex_id(G, act, wprefix, vbase, cx, cy, cz);
+      }
+      int c1 = c[1], c3 = c[3];
+      orient_quad(P[0], P[1], P[2], P[3], g0[0] + g1[0], g0[1] + g1[1], g0[2] + g1[2], c1, c3);
+      reinterpret_cast<int4*>(faces)[pos] = make_int4(c[0], c1, c[2], c3);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -511,7 +643,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q) Sb[q] = T(0);
-      backward_faces<T, S>(G, act, wprefix, i, j, k, rows + 12 * v, ubar, ubar_e, Sb);
+      backward_faces<T, S>(G, act, wprefix, i, j, k, rows + 12 * v, ubar, ubar_e, nf, Sb);
       const float4 o0 = out[0], o1 = out[1];
       out[0] = make_float4(o0.x + (float)Sb[0], o0.y + (float)Sb[1], o0.z + (float)Sb[2],
                            o0.w + (float)Sb[3]);
@@ -583,14 +715,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
     } else {
       if constexpr (L == 16) {
         if (sub < 12 && ((cross >> sub) & 1u) && kind16 != kRowOwn)
-          backward_face_row<T, S>(G, act, wprefix, i, j, k, own, sub, kind16, ubar, ubar_e, Sb);
+          backward_face_row<T, S>(G, act, wprefix, i, j, k, own, sub, kind16, ubar, ubar_e, nf,
+                                  Sb);
       } else {
 #pragma unroll 1
         for (int e = sub; e < 12; e += L) {
           if (!((cross >> e) & 1u)) continue;
           const uint32_t kind = rows[12 * v + e];
           if (kind != kRowOwn)
-            backward_face_row<T, S>(G, act, wprefix, i, j, k, own, e, kind, ubar, ubar_e, Sb);
+            backward_face_row<T, S>(G, act, wprefix, i, j, k, own, e, kind, ubar, ubar_e, nf, Sb);
         }
       }
     }

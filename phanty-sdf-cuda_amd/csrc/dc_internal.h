@@ -58,6 +58,7 @@ struct DcCounters {
   uint64_t n_faces;       // total quads, all grids
   uint64_t fallback;      // FP32 mode: cells solved again in FP64
   uint64_t n_list;        // face list entries (equals n_faces)
+  uint64_t n_reorient;    // FP32 mode: near-tie quads listed for FP64 orientation
   uint32_t nonfinite;     // a NaN or Inf sample was seen
   uint32_t error;         // internal failure bits: a list entry outside its grid (dc_decode.h)
 };

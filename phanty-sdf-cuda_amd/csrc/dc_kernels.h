@@ -39,7 +39,8 @@ struct DcQefArgs {
   DcCounters* ctr_rw;
   const uint64_t* ft;
   const uint32_t* pw;
-  const void* nrm;  // per-face averaged normals (double in FP64 mode, float in FP32 mode)
+  const void* nrm;  // per-face averaged normals for the solve (FP64 table, or FP32 mode's copy)
+  const double* nrm64;  // the FP64 table (FP32 mode's FP64 cells)
   uint64_t capF;
   float* qstate;    // FP32 mode: per-vertex solve state for the adjoint (24 floats), or null
   uint32_t* rows;   // FP32 mode: per-vertex row kinds for the adjoint (12), or null
@@ -85,7 +86,8 @@ struct DcFaceNormalArgs {
   uint64_t capF;
   int64_t face_blocks;  // grid-stride blocks of the per-face kernels
   int precision;
-  void* nrm;
+  void* nrm;     // FP64 table, 4 doubles per face
+  float* nrm32;  // FP32 mode: its FP32 copy, 4 floats per face (null in plain mode)
   int list_done;        // the face list is already built (dc_launch_scan3)
 };
 
@@ -106,6 +108,8 @@ struct DcEmitArgs {
   uint64_t capA;
   uint64_t capF;
   int32_t* faces;
+  uint64_t* reorient;  // FP32 mode: capF entries for the near-tie quads' list entries, or null
+  const double* nrm64;  // the FP64 face normal table (both modes)
 };
 
 // Adjoint gather: cell entries per pass in each wave's LDS list (8 bytes each; a multiple of 64).
@@ -134,7 +138,8 @@ struct DcBwdArgs {
   const float* grad_vertices;
   const uint64_t* ft;
   const uint32_t* pw;
-  const void* nrm;
+  const void* nrm;      // FP64 face normal table
+  const float* nrm32;   // FP32 mode: its FP32 copy (the FP32 cells' kernels read it)
   uint64_t capF;
   float* ubar;    // 4 slots x 3 floats per face (after the face pass: 3 doubles per face)
   float* ubar_e;  // 12 slots x 3 floats per active cell
@@ -175,6 +180,7 @@ hipError_t dc_launch_face_list(const uint64_t* ft, int64_t n_ft, uint32_t* tile_
                                uint64_t* total, hipStream_t s);
 hipError_t dc_launch_face_normals(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef(const DcQefArgs& a, hipStream_t s);
+// Quad emission; in FP32 mode (a.reorient set) followed by the near-tie quads' FP64 orientation.
 hipError_t dc_launch_emit(const DcEmitArgs& a, hipStream_t s);
 // Adjoint: per-cell phase 1 or 2 (FP32 mode: the FP32-solved cells on s, the FP64 cells on s64,
 // which may be another stream: the two sets are disjoint), then the gradient gather.
@@ -187,6 +193,5 @@ hipError_t dc_launch_triangulate(const int32_t* faces, uint64_t nf, int n_grids,
                                  const uint32_t* goff, const double* v64, const float* v32,
                                  int use64, int32_t* tris, int64_t blocks, hipStream_t s);
 // the contraction-enabled launchers (dc_cells_fast.hip)
-hipError_t dc_launch_face_normals_fp32(const DcFaceNormalArgs& a, hipStream_t s);
 hipError_t dc_launch_qef_fp32(const DcQefArgs& a, hipStream_t s);
 

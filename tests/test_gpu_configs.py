@@ -1,10 +1,12 @@
 """The benchmark configurations at their full sizes, on the GPU.
 
 - csg8 (8 rotated boxes minus a sphere) at 256^3 and 512^3: full-array parity with the oracle
-  (vertex count, every quad, FP32 vertices bit-exact, FP64 vertices within 1e-6*h);
+  (vertex count, every quad, FP32 vertices bit-exact, FP64 vertices within 1e-6*h), and the
+  FP32 + FP64-fallback mode's quads equal to the oracle's with vertices within 1e-4*h;
 - blend64 (smooth-min of 64 spheres, clamped to +-4h) at 1024^3: the active-cell and quad counts
   against the NumPy count, full parity with the oracle, and the FP32 + FP64-fallback mode's max
-  and RMS vertex error against the plain FP64 vertices (tolerance 1e-4*h);
+  and RMS vertex error against the plain FP64 vertices (tolerance 1e-4*h), its quads equal to the
+  plain mode's;
 - 256 uneven CSG grids (32^3 ... 256^3) in one batch: every grid's counts against the NumPy
   count and every 16th grid against the oracle, array for array.
 """
@@ -54,6 +56,12 @@ def test_csg8_full_arrays(n):
     _full_parity(f"csg8_{n}", g, ref, mesh.vertices.reshape(-1, 3),
                  mesh.vertices_f64.reshape(-1, 3), mesh.faces.reshape(-1, 4))
     assert (st.vertex_count, st.face_count) == mesh_counts(g.values, g.nx, g.ny, g.nz)
+    # FP32 + FP64-fallback mode: the same quads, vertices within 1e-4 h
+    m32, _ = HipDualContouringBackend("fp32-fallback").extract(_host_grid(g))
+    np.testing.assert_array_equal(m32.faces.reshape(-1, 4), ref.F)
+    err = np.abs(m32.vertices.reshape(-1, 3).astype(np.float64) - ref.V64).max() / g.h
+    _say(f"[configs] csg8 {n}^3: fp32 max err {err:.3e} h")
+    assert err <= 1e-4
 
 
 @pytest.mark.timeout(900)
@@ -76,6 +84,7 @@ def test_blend64_1024_counts_parity_and_fp32_error():
     m32, st32 = fast.extract_device(dg)
     V32f = m32.vertices.reshape(-1, 3).astype(np.float64)
     assert V32f.shape == V64.shape
+    np.testing.assert_array_equal(m32.faces.reshape(-1, 4), F)  # topology exact in FP32 mode
     d = np.abs(V32f - V64).max(axis=1)
     _say(f"[configs] blend64 1024^3: fp32 max err {d.max() / h:.3e} h, rms "
          f"{np.sqrt((d ** 2).mean()) / h:.3e} h, fallback cells {st32.qef_fallback_count}")

@@ -1,7 +1,9 @@
 """FP32+FP64-fallback mode and the adjoint on the GPU, against the FP64 oracle.
 
 Tolerances: FP32 mode vertices within 1e-4*h of the FP64 oracle (the stated target); topology
-(active set, vertex ids, quad sets) exact, quad orientation may differ only at near-ties (counted).
+exact: active set, vertex ids and every quad with its orientation equal to the oracle's (quads
+whose orientation test is within the FP32 vertices' error of a tie are oriented again from FP64
+vertices, k_reorient).
 Adjoint: the plain-FP64 adjoint within 1e-5 relative (max-norm) of the oracle's FP64 analytic
 gradient, the FP32 adjoint within 2e-3; the gradient is checked at fixed topology.
 """
@@ -47,13 +49,12 @@ def test_fp32_fallback_mode_error(make):
     F = mesh.faces.reshape(-1, 4)
     np.testing.assert_array_equal(canon(F), canon(ref.F))
     flips = int(np.sum(np.any(F != ref.F, axis=1)))
+    np.testing.assert_array_equal(F, ref.F)
     err = np.abs(mesh.vertices.reshape(-1, 3).astype(np.float64) - ref.V64) / g.h
     print(f"[fp32] {g.name}: max {err.max():.3e} h, rms {np.sqrt((err ** 2).mean()):.3e} h, "
           f"fallback {st.qef_fallback_count}/{st.vertex_count}, orientation flips {flips}",
           flush=True)
     assert err.max() <= 1e-4
-    # orientation uses the FP32 vertices: only near-tie quads may flip (reported, bounded)
-    assert flips <= max(4, ref.n_faces // 1000)
 
 
 @pytest.mark.parametrize("precision,tol", [("plain-fp64", 1e-5), ("fp32-fallback", 2e-3)])

@@ -108,3 +108,31 @@ def test_bad_tensors_rejected_before_launch():
     want = oracle.run_grid(g).backward(w)
     got = good.cpu().numpy().ravel().astype(np.float64)
     assert np.abs(got - want).max() / np.abs(want).max() <= 1e-5
+
+
+def test_gradient_aliasing_the_samples_is_rejected():
+    """The adjoint reads the forward's samples again while it writes the gradient: a gradient
+    buffer that is the forward's own sample tensor is refused before anything is launched."""
+    from contouring_hip import _lib
+    g = synth.csg8(24, seed=4)
+    print("[boundary] aliasing gradient", flush=True)
+    phi = _phi(g)
+    be = HipDualContouringBackend("plain-fp64")
+    (V, F), _ = be.extract_tensor(phi, (g.vx, g.vy, g.vz), (g.ox, g.oy, g.oz))
+    w, wt = _weights(V.shape[0], seed=6)
+    with pytest.raises(_lib.DcHipError) as ei:
+        be.backward(wt, phi)
+    assert ei.value.status == _lib.DC_HIP_ERR_INVALID_ARG
+    # a separate gradient still works, and the samples are unchanged
+    before = phi.clone()
+    good = torch_empty_like(phi)
+    be.backward(wt, good)
+    assert bool((phi == before).all())
+    want = oracle.run_grid(g).backward(w)
+    got = good.cpu().numpy().ravel().astype(np.float64)
+    assert np.abs(got - want).max() / np.abs(want).max() <= 1e-5
+
+
+def torch_empty_like(t):
+    import torch
+    return torch.empty_like(t)
