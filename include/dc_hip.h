@@ -169,6 +169,44 @@ int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad
 int dc_hip_autotune(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t n_grids, int precision,
                     int32_t repeats, int32_t* best_rows, int32_t* best_waves, float* table_ms);
 
+/* Counts and times of one mesh post-processing call (dc_hip_postprocess). */
+typedef struct dc_hip_post_stats {
+  int64_t welded_vertices;  /* after the first weld */
+  int64_t welded_faces;     /* quads left by the first weld (fewer than 3 distinct vertices: dropped) */
+  int64_t components;       /* edge-connected face components (remove_floaters; else 0) */
+  int64_t kept_faces;       /* faces of the largest component (all welded faces without it) */
+  int64_t boundary_edges;   /* directed boundary edges given to the hole walk (close_holes) */
+  int64_t loops_filled;     /* boundary loops closed */
+  int64_t added_vertices;   /* vertices made by the hole fill */
+  int64_t added_faces;      /* faces made by the hole fill */
+  int64_t vertex_count;     /* result */
+  int64_t face_count;
+  float walk_ms;            /* host time of the boundary-loop walk and fill */
+  float total_ms;           /* host wall time of the call */
+} dc_hip_post_stats;
+
+/* Replaces postprocess_mesh (src/cpp/dc_backend.cpp:217-558, declared include/dc_backend.h:28,
+ * bound at src/cpp/bindings.cpp:145-146) for a quad mesh: vertices welded on a quantised grid of
+ * 1e-5 x the bounding-box diagonal (quads left with fewer than 3 distinct vertices dropped), with
+ * remove_floaters only the largest edge-connected face component kept (ties: the one holding the
+ * lowest face), with close_holes every boundary loop of 3..4096 edges closed by concentric rings
+ * and a centre fan (triangles as quads with a repeated last vertex), vertices renumbered by first
+ * use and, with close_holes, welded again. Output equals the reference's for the same input.
+ * vertices (3 * n_vertices floats) and faces (4 * n_faces int32, indices < n_vertices) may be host
+ * or device memory; both NULL with zero counts: the context's last extraction result (single
+ * grid). With neither flag set, or an empty mesh, the result is the input. The result stays in the
+ * context beside the last extraction (which stays valid for dc_hip_backward); *out_n_vertices and
+ * *out_n_faces (may be NULL) receive its counts; stats may be NULL. Returns when complete. */
+int dc_hip_postprocess(dc_hip_context* ctx, const float* vertices, int64_t n_vertices,
+                       const int32_t* faces, int64_t n_faces, int close_holes,
+                       int remove_floaters, int64_t* out_n_vertices, int64_t* out_n_faces,
+                       dc_hip_post_stats* stats);
+/* Copies the last post-processing result (host or device memory; either pointer may be NULL):
+ * 3 * vertex_count floats, 4 * face_count int32. */
+int dc_hip_postprocess_copy(dc_hip_context* ctx, float* vertices, int32_t* faces);
+/* Device pointers of the last post-processing result, valid until the next dc_hip_postprocess. */
+int dc_hip_postprocess_device(dc_hip_context* ctx, const float** vertices, const int32_t** faces);
+
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);
 

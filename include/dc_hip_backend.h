@@ -47,10 +47,18 @@ class HipDualContouringBackend : public IDualContouringBackend {
   // The last extraction's quads as triangles (3 ints each), split and oriented like
   // triangulate_v1 (src/cpp/contouring.cpp:88-113).
   std::vector<int> last_triangles();
+  // postprocess_mesh (src/cpp/dc_backend.cpp:217-558, declared include/dc_backend.h:28) on the
+  // GPU: weld, largest component (remove_floaters), closed boundary loops (close_holes); the
+  // result equals the reference's.
+  DualContouringMesh postprocess(const DualContouringMesh& mesh, bool close_holes,
+                                 bool remove_floaters);
+  // The same on the last extraction's mesh, without copying it to the host first (single grid).
+  DualContouringMesh postprocess_last(bool close_holes, bool remove_floaters);
   void set_option(const std::string& key, int64_t value);
   int precision() const { return precision_; }
 
  private:
+  DualContouringMesh post_copy(int64_t nv, int64_t nf);
   DualContouringMesh download(const dc_hip_stats& st, DualContouringStats& stats);
   void check(int rc);
   dc_hip_context* ctx_ = nullptr;

@@ -40,6 +40,14 @@ def load():
     lib.oracle_qef_solve.argtypes = [I, P, P, P]
     lib.oracle_triangulate.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64, P]
     lib.oracle_set_threads.argtypes = [I]
+    i64 = ctypes.c_int64
+    lib.oracle_postprocess.restype = P
+    lib.oracle_postprocess.argtypes = [P, i64, P, i64, I, I]
+    lib.oracle_post_counts.argtypes = [P, P, P, P]
+    lib.oracle_post_copy.argtypes = [P, P, P]
+    lib.oracle_post_free.argtypes = [P]
+    lib.oracle_weld.restype = P
+    lib.oracle_weld.argtypes = [P, i64, P, i64]
     _lib = lib
     return lib
 
@@ -135,3 +143,45 @@ def triangulate(F, V):
     T = np.empty((2 * F.shape[0], 3), np.int32)
     load().oracle_triangulate(_p(F), F.shape[0], _p(V), V.shape[0], _p(T))
     return T
+
+
+POST_INFO = ("welded_vertices", "welded_faces", "components", "kept_faces", "boundary_edges",
+             "loops_filled", "added_vertices", "added_faces")
+
+
+def postprocess(V, F, close_holes=True, remove_floaters=True):
+    """The reference's postprocess_mesh (src/cpp/dc_backend.cpp:217-558) restated on the CPU.
+
+    V: (n, 3) float32, F: (m, 4) int32. Returns (V2, F2, info) with info a dict of POST_INFO.
+    """
+    lib = load()
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    F = np.ascontiguousarray(F, np.int32).reshape(-1, 4)
+    h = lib.oracle_postprocess(_p(V), V.shape[0], _p(F), F.shape[0], int(bool(close_holes)),
+                               int(bool(remove_floaters)))
+    if not h:
+        raise RuntimeError("oracle postprocess: a hole fill did not converge")
+    return _post_result(h)
+
+
+def weld(V, F):
+    """The reference's merge_vertices_in_place (src/cpp/dc_backend.cpp:127-215): (V2, F2)."""
+    lib = load()
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    F = np.ascontiguousarray(F, np.int32).reshape(-1, 4)
+    V2, F2, _ = _post_result(lib.oracle_weld(_p(V), V.shape[0], _p(F), F.shape[0]))
+    return V2, F2
+
+
+def _post_result(h):
+    lib = load()
+    try:
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        info = np.zeros(8, np.int64)
+        lib.oracle_post_counts(h, ctypes.byref(nv), ctypes.byref(nf), _p(info))
+        V2 = np.empty((nv.value, 3), np.float32)
+        F2 = np.empty((nf.value, 4), np.int32)
+        lib.oracle_post_copy(h, _p(V2), _p(F2))
+    finally:
+        lib.oracle_post_free(h)
+    return V2, F2, dict(zip(POST_INFO, (int(x) for x in info)))

@@ -39,7 +39,8 @@ def _run(cmd, verbose):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("dc_kernels.hip", "dc_cells.hip", "dc_cells_fast.hip", "dc_hip.cpp"):
+    for src in ("dc_kernels.hip", "dc_cells.hip", "dc_cells_fast.hip", "dc_post.hip",
+                "dc_hip.cpp"):
         obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         flags = list(HIP_FLAGS)
         if src == "dc_cells_fast.hip":  # results held to tolerances: FMA contraction allowed

@@ -13,6 +13,8 @@ from .backend import (  # noqa: F401
     DualContouringStats,
     HipDualContouringBackend,
     PRECISIONS,
+    postprocess_mesh,
 )
 from . import synth  # noqa: F401
+from .io import load_obj, load_sdf, save_obj, save_sdf  # noqa: F401
 from .autotune import autotune_mark  # noqa: F401

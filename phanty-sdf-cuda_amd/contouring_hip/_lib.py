@@ -27,7 +27,8 @@ EXPORTS = (
     "dc_hip_extract", "dc_hip_extract_device", "dc_hip_extract_device_noalloc",
     "dc_hip_result_counts", "dc_hip_copy_result",
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
-    "dc_hip_triangulate",
+    "dc_hip_triangulate", "dc_hip_postprocess", "dc_hip_postprocess_copy",
+    "dc_hip_postprocess_device",
     "dc_hip_last_error", "dc_hip_status_string",
 )
 
@@ -51,6 +52,15 @@ class DcHipStats(ctypes.Structure):
                 ("qef_fallback_count", ctypes.c_int64), ("invalid_count", ctypes.c_int64),
                 ("normals_ms", ctypes.c_float), ("mark_kernel_ms", ctypes.c_float),
                 ("graph_mode", ctypes.c_int32)]
+
+
+class DcHipPostStats(ctypes.Structure):
+    _fields_ = [("welded_vertices", ctypes.c_int64), ("welded_faces", ctypes.c_int64),
+                ("components", ctypes.c_int64), ("kept_faces", ctypes.c_int64),
+                ("boundary_edges", ctypes.c_int64), ("loops_filled", ctypes.c_int64),
+                ("added_vertices", ctypes.c_int64), ("added_faces", ctypes.c_int64),
+                ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
+                ("walk_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
 
 
 _hip = None
@@ -99,6 +109,10 @@ def load_hip():
         "dc_hip_autotune": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, ctypes.c_int32,
                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                 ctypes.POINTER(ctypes.c_float)]),
+        "dc_hip_postprocess": (I, [P, P, I64, P, I64, I, I, ctypes.POINTER(I64),
+                                   ctypes.POINTER(I64), ctypes.POINTER(DcHipPostStats)]),
+        "dc_hip_postprocess_copy": (I, [P, P, P]),
+        "dc_hip_postprocess_device": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P)]),
         "dc_hip_last_error": (ctypes.c_char_p, [P]),
         "dc_hip_status_string": (ctypes.c_char_p, [I]),
     }

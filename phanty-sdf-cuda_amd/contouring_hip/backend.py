@@ -114,6 +114,27 @@ def _check_tensor(t, name, numel, device, dtype="float32"):
         raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
 
 
+def _mesh_array(x, name, per, dtype, device):
+    """(pointer, item count, the array kept alive) of a mesh array: a CUDA tensor on `device`, or
+    anything NumPy converts (CPU tensors included); `per` values per item."""
+    if hasattr(x, "data_ptr") and getattr(x, "is_cuda", False):
+        import torch
+        t = x.contiguous()
+        if t.dtype != getattr(torch, dtype):
+            raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+        if (t.device.index or 0) != device:
+            raise ValueError(f"{name} must live on cuda:{device}, got {t.device}")
+        if t.numel() % per:
+            raise ValueError(f"{name} must hold {per} values per item")
+        return ctypes.c_void_p(t.data_ptr()), t.numel() // per, t
+    if hasattr(x, "detach"):
+        x = x.detach().cpu().numpy()
+    a = np.ascontiguousarray(np.asarray(x, dtype).reshape(-1))
+    if a.size % per:
+        raise ValueError(f"{name} must hold {per} values per item")
+    return a.ctypes.data_as(ctypes.c_void_p), a.size // per, a
+
+
 class HipDualContouringBackend:
     """MI355X dense dual contouring (replaces CudaDualContouringBackend, dc_cuda.h:31-35).
 
@@ -326,6 +347,42 @@ class HipDualContouringBackend:
             self._check(self._lib.dc_hip_triangulate(self._ctx, ctypes.c_void_p(T.data_ptr())))
         return T
 
+    def postprocess(self, mesh=None, close_holes=True, remove_floaters=True, device=False):
+        """postprocess_mesh (src/cpp/dc_backend.cpp:217-558) on the GPU: weld, keep the largest
+        component (remove_floaters), close boundary loops (close_holes); the result equals the
+        reference's. mesh: a DualContouringMesh whose vertices / faces are NumPy arrays (or
+        sequences) or CUDA tensors, or None for this backend's last extraction (single grid), which
+        stays valid for backward. Returns (DualContouringMesh, stats dict); device=True returns
+        the arrays as CUDA tensors instead of NumPy."""
+        st = _lib.DcHipPostStats()
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        if mesh is None:
+            args = (None, 0, None, 0)
+        else:
+            vp, n_v, vkeep = _mesh_array(mesh.vertices, "vertices", 3, "float32", self.device)
+            fp, n_f, fkeep = _mesh_array(mesh.faces, "faces", 4, "int32", self.device)
+            args = (vp, n_v, fp, n_f)
+            if not isinstance(vkeep, np.ndarray) or not isinstance(fkeep, np.ndarray):
+                self._use_torch_stream()  # device inputs: ordered after the caller's work
+        self._check(self._lib.dc_hip_postprocess(self._ctx, *args, int(bool(close_holes)),
+                                                 int(bool(remove_floaters)), ctypes.byref(nv),
+                                                 ctypes.byref(nf), ctypes.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in _lib.DcHipPostStats._fields_}
+        if device:
+            import torch
+            dev = torch.device("cuda", self.device)
+            Vo = torch.empty(3 * nv.value, dtype=torch.float32, device=dev)
+            Fo = torch.empty(4 * nf.value, dtype=torch.int32, device=dev)
+            self._use_torch_stream()
+            self._check(self._lib.dc_hip_postprocess_copy(
+                self._ctx, ctypes.c_void_p(Vo.data_ptr()), ctypes.c_void_p(Fo.data_ptr())))
+        else:
+            Vo = np.empty(3 * nv.value, np.float32)
+            Fo = np.empty(4 * nf.value, np.int32)
+            self._check(self._lib.dc_hip_postprocess_copy(
+                self._ctx, Vo.ctypes.data_as(ctypes.c_void_p), Fo.ctypes.data_as(ctypes.c_void_p)))
+        return DualContouringMesh(Vo, Fo), stats
+
     def backward(self, grad_vertices, grad_phi):
         """dL/dphi (written into grad_phi: a contiguous float32 CUDA tensor with one element per
         sample of the last forward, all grids' samples concatenated after a batch) for dL/dV
@@ -345,3 +402,17 @@ class HipDualContouringBackend:
                                               ctypes.c_void_p(grad_phi.data_ptr()),
                                               ctypes.byref(st)))
         return DualContouringStats.from_c(st)
+
+
+_post_backend = None
+
+
+def postprocess_mesh(mesh, close_holes=True, remove_floaters=True):
+    """The reference's module function postprocess_mesh(mesh, close_holes=True,
+    remove_floaters=True) (src/cpp/bindings.cpp:145-146), run on the GPU (device 0): returns a new
+    DualContouringMesh with NumPy vertices and faces."""
+    global _post_backend
+    if _post_backend is None:
+        _post_backend = HipDualContouringBackend()
+    out, _ = _post_backend.postprocess(mesh, close_holes, remove_floaters)
+    return out

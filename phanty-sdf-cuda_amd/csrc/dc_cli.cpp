@@ -1,7 +1,9 @@
 // dc_cli.cpp — command-line front end with the reference's flags (src/cpp/dc_cli.cpp:205-380),
 // running the MI355X backend: -b hip (also the default). Input: a .sdf file
 // (int32 nx,ny,nz; float vx,vy,vz; float ox,oy,oz; float values, x fastest; dc_cli.cpp:31-47)
-// or --generate {sphere,box,plane,sphere-unit}; output: OBJ with 1-based quads (:17-29).
+// or --generate {sphere,box,plane,sphere-unit} (--save-sdf writes the grid in the same layout);
+// --close-holes / --remove-floaters post-process the mesh on the GPU (:334-338); output: OBJ with
+// 1-based quads (:17-29).
 // --benchmark prints the reference's stats block (:342-377) so stdout scrapers keep working.
 
 #include <cstdint>
@@ -55,6 +57,19 @@ bool read_sdf(const std::string& path, DenseSdfGrid& g) {
   return in.good();
 }
 
+// The .sdf layout read_sdf reads (the reference's writer: scripts/test_glb_benchmark.py:20-25).
+bool write_sdf(const std::string& path, const DenseSdfGrid& g) {
+  std::ofstream out(path, std::ios::binary);
+  if (!out) return false;
+  const int32_t dims[3] = {g.nx, g.ny, g.nz};
+  const float meta[6] = {g.vx, g.vy, g.vz, g.ox, g.oy, g.oz};
+  out.write(reinterpret_cast<const char*>(dims), sizeof(dims));
+  out.write(reinterpret_cast<const char*>(meta), sizeof(meta));
+  out.write(reinterpret_cast<const char*>(g.values.data()),
+            (std::streamsize)(g.values.size() * sizeof(float)));
+  return out.good();
+}
+
 bool generate(const std::string& kind, int res, DenseSdfGrid& g) {
   g.nx = g.ny = g.nz = res;
   g.values.resize((size_t)res * res * res);
@@ -77,7 +92,7 @@ bool generate(const std::string& kind, int res, DenseSdfGrid& g) {
 }  // namespace
 
 int main(int argc, char* argv[]) {
-  std::string input, output, backend_type = "hip", gen, precision = "plain-fp64";
+  std::string input, output, backend_type = "hip", gen, precision = "plain-fp64", save_sdf;
   bool benchmark = false, close_holes = false, remove_floaters = false;
   int res = 64;
   for (int i = 1; i < argc; ++i) {
@@ -89,6 +104,7 @@ int main(int argc, char* argv[]) {
     else if (a == "-g" || a == "--res" || a == "--grid-size") res = std::stoi(next());
     else if (a == "--generate") gen = next();
     else if (a == "--precision") precision = next();
+    else if (a == "--save-sdf") save_sdf = next();
     else if (a == "--benchmark") benchmark = true;
     else if (a == "--close-holes") close_holes = true;
     else if (a == "--remove-floaters") remove_floaters = true;
@@ -108,7 +124,7 @@ int main(int argc, char* argv[]) {
     std::cerr << "Usage: " << argv[0] << " -i <input.sdf> -o <output.obj> [options]\n"
               << "Options: -b/--backend hip, -g/--res/--grid-size <int>, --generate "
                  "<sphere|box|plane|sphere-unit>, --precision <plain-fp64|fp32-fallback>, "
-                 "--benchmark\n";
+                 "--close-holes, --remove-floaters, --save-sdf <grid.sdf>, --benchmark\n";
     return 1;
   }
   if (backend_type != "hip") {
@@ -116,9 +132,6 @@ int main(int argc, char* argv[]) {
               << "' is not part of this build; the dense MI355X backend is '-b hip'.\n";
     return 1;
   }
-  if (close_holes || remove_floaters)
-    std::cerr << "Warning: mesh post-processing is not part of this build; writing the raw "
-                 "dual-contouring mesh.\n";
   int prec;
   if (precision == "plain-fp64") prec = DC_HIP_PLAIN_FP64;
   else if (precision == "fp32-fallback") prec = DC_HIP_FP32_FALLBACK;
@@ -138,11 +151,22 @@ int main(int argc, char* argv[]) {
     return 1;
   }
 
+  if (!save_sdf.empty() && !write_sdf(save_sdf, grid)) {
+    std::cerr << "Failed to write SDF to: " << save_sdf << "\n";
+    return 1;
+  }
+
   DualContouringStats stats;
   DualContouringMesh mesh;
   try {
     HipDualContouringBackend backend(0, prec);
     mesh = backend.extract(grid, stats);
+    // post-processing after the stats are taken, as the reference does (dc_cli.cpp:334-338)
+    if (close_holes || remove_floaters) {
+      mesh = backend.postprocess_last(close_holes, remove_floaters);
+      stats.vertex_count = (int)(mesh.vertices.size() / 3);
+      stats.face_count = (int)(mesh.faces.size() / 4);
+    }
   } catch (const std::exception& e) {
     std::cerr << "Error: " << e.what() << "\n";
     return 1;

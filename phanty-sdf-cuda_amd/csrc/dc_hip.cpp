@@ -20,6 +20,7 @@
 #include "dc_hip.h"
 #include "dc_internal.h"
 #include "dc_kernels.h"
+#include "dc_post.h"
 
 namespace {
 
@@ -174,6 +175,7 @@ struct dc_hip_context {
   int bwd_lanes1 = 0, bwd_lanes2 = 0;  // options "adjoint_lanes", "adjoint_face_lanes" (0: auto)
   int bwd_lanes64 = 0;                 // option "adjoint_fallback_lanes" (0: auto)
   std::vector<DcGridDev> hgrids;
+  DcPost* post = nullptr;  // mesh post-processing state and its last result
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, cw_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_cw = 0,
           n_sg = 0;
@@ -818,6 +820,7 @@ void dc_hip_destroy(dc_hip_context* c) {
                     &c->qstate, &c->qstate64, &c->tris, &c->reorient};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
+  delete c->post;
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -1141,6 +1144,61 @@ int dc_hip_result_device(dc_hip_context* c, const float** vertices, const double
   if (vertices) *vertices = as<float>(c->v32);
   if (vertices_f64) *vertices_f64 = (c->precision == DC_HIP_PLAIN_FP64) ? as<double>(c->v64) : nullptr;
   if (faces) *faces = as<int32_t>(c->faces);
+  return DC_HIP_OK;
+}
+
+int dc_hip_postprocess(dc_hip_context* c, const float* vertices, int64_t n_vertices,
+                       const int32_t* faces, int64_t n_faces, int close_holes,
+                       int remove_floaters, int64_t* out_n_vertices, int64_t* out_n_faces,
+                       dc_hip_post_stats* stats) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (!vertices && !faces && n_vertices == 0 && n_faces == 0) {  // the last extraction's mesh
+    if (!c->have) return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful extraction");
+    if (c->hgrids.size() != 1)
+      return fail(c, DC_HIP_ERR_INVALID_ARG,
+                  "postprocess of the last result needs a single-grid extraction (a batch's face "
+                  "indices are local to each grid)");
+    vertices = as<float>(c->v32);
+    faces = as<int32_t>(c->faces);
+    n_vertices = c->nv;
+    n_faces = c->nf;
+  }
+  if (!c->post) c->post = new DcPost();
+  hipStream_t s;
+  int rc;
+  if ((rc = work_stream(c, &s))) return rc;
+  rc = c->post->run(vertices, n_vertices, faces, n_faces, close_holes != 0, remove_floaters != 0,
+                    c->cu_count, s, stats, &c->err);
+  if (rc != DC_HIP_OK) return rc;
+  if (out_n_vertices) *out_n_vertices = c->post->n_vertices();
+  if (out_n_faces) *out_n_faces = c->post->n_faces();
+  return DC_HIP_OK;
+}
+
+int dc_hip_postprocess_copy(dc_hip_context* c, float* vertices, int32_t* faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->post || !c->post->have())
+    return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful postprocess");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  const int64_t nv = c->post->n_vertices(), nf = c->post->n_faces();
+  // hipMemcpyDefault: destinations may be host or device memory (unified addressing)
+  if (vertices && nv)
+    DC_TRY(c, hipMemcpyAsync(vertices, c->post->vertices(), nv * 12, hipMemcpyDefault, c->stream),
+           "copy vertices");
+  if (faces && nf)
+    DC_TRY(c, hipMemcpyAsync(faces, c->post->faces(), nf * 16, hipMemcpyDefault, c->stream),
+           "copy faces");
+  DC_TRY(c, hipStreamSynchronize(c->stream), "download");
+  return DC_HIP_OK;
+}
+
+int dc_hip_postprocess_device(dc_hip_context* c, const float** vertices, const int32_t** faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->post || !c->post->have())
+    return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful postprocess");
+  if (vertices) *vertices = c->post->vertices();
+  if (faces) *faces = c->post->faces();
   return DC_HIP_OK;
 }
 

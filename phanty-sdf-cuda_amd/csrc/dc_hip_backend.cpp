@@ -99,3 +99,28 @@ std::vector<int> HipDualContouringBackend::last_triangles() {
   if (nf) check(dc_hip_triangulate(ctx_, t.data()));
   return t;
 }
+
+DualContouringMesh HipDualContouringBackend::post_copy(int64_t nv, int64_t nf) {
+  DualContouringMesh m;
+  m.vertices.resize(3 * (size_t)nv);
+  m.faces.resize(4 * (size_t)nf);
+  check(dc_hip_postprocess_copy(ctx_, m.vertices.data(), m.faces.data()));
+  return m;
+}
+
+DualContouringMesh HipDualContouringBackend::postprocess(const DualContouringMesh& mesh,
+                                                         bool close_holes, bool remove_floaters) {
+  int64_t nv = 0, nf = 0;
+  check(dc_hip_postprocess(ctx_, mesh.vertices.data(), (int64_t)(mesh.vertices.size() / 3),
+                           mesh.faces.data(), (int64_t)(mesh.faces.size() / 4), close_holes ? 1 : 0,
+                           remove_floaters ? 1 : 0, &nv, &nf, nullptr));
+  return post_copy(nv, nf);
+}
+
+DualContouringMesh HipDualContouringBackend::postprocess_last(bool close_holes,
+                                                              bool remove_floaters) {
+  int64_t nv = 0, nf = 0;
+  check(dc_hip_postprocess(ctx_, nullptr, 0, nullptr, 0, close_holes ? 1 : 0,
+                           remove_floaters ? 1 : 0, &nv, &nf, nullptr));
+  return post_copy(nv, nf);
+}

@@ -1,9 +1,10 @@
-"""Each plausible mistake in the oracle is caught by a test of tests/test_oracle_pinned.py (CPU).
+"""Each plausible mistake in the oracle is caught by a pinning test (CPU).
 
-A copy of oracle/dc_oracle.cpp with one mistake planted is built into a temporary library, and the
-test that names that mistake is run against it in a subprocess (ORACLE_LIB): it must fail. The
-unmodified copy, built the same way, must pass all of them (the control). Every mutation here
-fails an assertion (none crashes the mutated library).
+A copy of the oracle's sources (oracle/dc_oracle.cpp, oracle/post_oracle.cpp) with one mistake
+planted in one of them is built into a temporary library, and the test that names that mistake
+(tests/test_oracle_pinned.py, tests/test_post_oracle_pinned.py) is run against it in a subprocess
+(ORACLE_LIB): it must fail. The unmodified copy, built the same way, must pass all of them (the
+control). Every mutation here fails an assertion (none crashes the mutated library).
 """
 import os
 import subprocess
@@ -12,44 +13,70 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "oracle", "dc_oracle.cpp")
+SOURCES = ("dc_oracle.cpp", "post_oracle.cpp")
 PINNED = os.path.join(ROOT, "tests", "test_oracle_pinned.py")
+POST_PINNED = os.path.join(ROOT, "tests", "test_post_oracle_pinned.py")
 
-# (name, text in the oracle, replacement, test that must fail)
+# (name, oracle source, text in it, replacement, test file, test that must fail)
 MUTATIONS = [
-    ("trilinear_weight", "(S101 - S001) * omv * w", "(S101 - S001) * omu * w",
-     "test_trilinear_field_hermite_points_and_normals_in_closed_form"),
-    ("both_endpoints_inside",
+    ("trilinear_weight", "dc_oracle.cpp", "(S101 - S001) * omv * w", "(S101 - S001) * omu * w",
+     PINNED, "test_trilinear_field_hermite_points_and_normals_in_closed_form"),
+    ("both_endpoints_inside", "dc_oracle.cpp",
      "if (!inside(i0, j0, k0) && !inside(i1, j1, k1)) return 0;",
      "if (!inside(i0, j0, k0) || !inside(i1, j1, k1)) return 0;",
-     "test_edge_with_one_border_endpoint_is_averaged"),
-    ("average_active_only", "if (vids[q] < 0) return 0;", "if (vids[q] < 0) vids[q] = v0;",
-     "test_zero_endpoint_edge_averaged_only_with_four_active_cells"),
-    ("quad_order", "std::sort(keys.begin(), keys.end());", "std::sort(keys.rbegin(), keys.rend());",
+     PINNED, "test_edge_with_one_border_endpoint_is_averaged"),
+    ("average_active_only", "dc_oracle.cpp", "if (vids[q] < 0) return 0;",
+     "if (vids[q] < 0) vids[q] = v0;",
+     PINNED, "test_zero_endpoint_edge_averaged_only_with_four_active_cells"),
+    ("quad_order", "dc_oracle.cpp", "std::sort(keys.begin(), keys.end());",
+     "std::sort(keys.rbegin(), keys.rend());", PINNED,
      "test_quad_order_and_orientation_brute_force"),
-    ("gradient_border_step", "const double dx = coord(g.ox, ip, g.vx) - coord(g.ox, im, g.vx);",
-     "const double dx = 2.0 * g.vx;", "test_quad_order_and_orientation_brute_force"),
+    ("gradient_border_step", "dc_oracle.cpp",
+     "const double dx = coord(g.ox, ip, g.vx) - coord(g.ox, im, g.vx);",
+     "const double dx = 2.0 * g.vx;", PINNED, "test_quad_order_and_orientation_brute_force"),
+    ("weld_eps", "post_oracle.cpp", "diag * 1e-5f", "diag * 1e-4f", POST_PINNED,
+     "test_weld_matches_brute_force_keys"),
+    ("weld_key_axis", "post_oracle.cpp", "(int64_t)std::round(y * inv)",
+     "(int64_t)std::round(x * inv)", POST_PINNED, "test_weld_matches_brute_force_keys"),
+    ("weld_keeps_two_distinct", "post_oracle.cpp", "if (distinct < 3) continue;",
+     "if (distinct < 2) continue;", POST_PINNED, "test_weld_matches_brute_force_keys"),
+    ("component_tie_last", "post_oracle.cpp", "if (best < 0 || size[c] > size[best])",
+     "if (best < 0 || size[c] >= size[best])", POST_PINNED,
+     "test_edge_of_three_faces_joins_none_and_ties_keep_the_lowest_face"),
+    ("nonmanifold_edge_links", "post_oracle.cpp", "} else if (run == 2) {",
+     "} else if (run >= 2) {", POST_PINNED,
+     "test_edge_of_three_faces_joins_none_and_ties_keep_the_lowest_face"),
+    ("loop_not_reversed", "post_oracle.cpp",
+     "fill.run(std::vector<int>(walk.rbegin(), walk.rend()));", "fill.run(walk);", POST_PINNED,
+     "test_single_quad_becomes_a_closed_pillow"),
+    ("ring_step_rule", "post_oracle.cpp", "const float step = d > mean ? mean : d * 0.5f;",
+     "const float step = d > mean ? d * 0.5f : mean;", POST_PINNED,
+     "test_ring_fill_of_regular_octagons_in_closed_form"),
+    ("long_loops_filled", "post_oracle.cpp",
+     "if (closed && walk.size() >= 3 && walk.size() <= 4096)",
+     "if (closed && walk.size() >= 3)", POST_PINNED, "test_loops_longer_than_4096_stay_open"),
 ]
 
 
-def _build(tmp_path, name, old=None, new=None):
-    src = open(SRC).read()
-    if old is not None:
-        assert src.count(old) == 1, (name, src.count(old))
-        src = src.replace(old, new)
+def _build(tmp_path, name, source=None, old=None, new=None):
     d = tmp_path / name
     d.mkdir()
-    (d / "dc_oracle.cpp").write_text(src)
+    for src in SOURCES:
+        text = open(os.path.join(ROOT, "oracle", src)).read()
+        if src == source:
+            assert text.count(old) == 1, (name, text.count(old))
+            text = text.replace(old, new)
+        (d / src).write_text(text)
     lib = str(d / "liboracle_dc.so")
     r = subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math",
-                        "-fopenmp", "-fPIC", "-shared", "-o", lib, str(d / "dc_oracle.cpp")],
+                        "-fopenmp", "-fPIC", "-shared", "-o", lib,
+                        *(str(d / src) for src in SOURCES)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return lib
 
 
-def _run_pinned(lib, test=None):
-    target = PINNED if test is None else f"{PINNED}::{test}"
+def _run_pinned(lib, target):
     env = dict(os.environ, ORACLE_LIB=lib)
     return subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
                            target], cwd=ROOT, env=env, capture_output=True, text=True,
@@ -57,11 +84,13 @@ def _run_pinned(lib, test=None):
 
 
 def test_unmodified_copy_passes(tmp_path):
-    r = _run_pinned(_build(tmp_path, "control"))
-    assert r.returncode == 0, r.stdout[-2000:]
+    lib = _build(tmp_path, "control")
+    for target in (PINNED, POST_PINNED):
+        r = _run_pinned(lib, target)
+        assert r.returncode == 0, r.stdout[-2000:]
 
 
-@pytest.mark.parametrize("name,old,new,test", MUTATIONS, ids=[m[0] for m in MUTATIONS])
-def test_mutation_is_caught(tmp_path, name, old, new, test):
-    r = _run_pinned(_build(tmp_path, name, old, new), test)
+@pytest.mark.parametrize("name,source,old,new,path,test", MUTATIONS, ids=[m[0] for m in MUTATIONS])
+def test_mutation_is_caught(tmp_path, name, source, old, new, path, test):
+    r = _run_pinned(_build(tmp_path, name, source, old, new), f"{path}::{test}")
     assert r.returncode != 0, f"mutation {name} survived {test}:\n{r.stdout[-1500:]}"
