@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--no-host-input", action="store_true",
                     help="skip the host-input (upload + download) timing")
     ap.add_argument("--host-steps", type=int, default=5)
+    ap.add_argument("--post-runs", type=int, default=10,
+                    help="timed post-processing calls (median reported)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
     ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"),
@@ -188,6 +190,8 @@ def main():
         f"{fallback}/{nv}, vertex err max {err_max:.2e} h rms {err_rms:.2e} h, "
         f"bwd {bst.backward_ms:.3f} ms, quads oriented unlike plain: {flips}")
 
+    post = bench_postprocess(be, lib, cg, prec, st, args)
+
     Ns = n ** 3
     Nc = (n - 1) ** 3
     # sign pass: the algorithmic bytes are the samples it reads (4 N_s); its 16-byte sign entries
@@ -272,8 +276,39 @@ def main():
                           "frac": (stage_gbs / HBM_PEAK_GBS) if stage_gbs else None},
         "cpu_baseline": cpu,
         "host_input": host_input,
+        "postprocess": post,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_postprocess(be, lib, cg, prec, st, args):
+    """postprocess_mesh (weld, floaters, holes) of the plain extraction's mesh on the device: host
+    wall time per call (median; it synchronises for its counts and the boundary walk), beside the
+    oracle's single-thread restatement on the same mesh (skipped with --no-cpu)."""
+    be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st)))
+    times = []
+    for q in range(args.post_runs + 2):
+        t = time.perf_counter()
+        out, pst = be.postprocess(None, True, True)
+        if q >= 2:
+            times.append((time.perf_counter() - t) * 1e3)
+    rec = {"gpu_ms": float(np.median(times)), "runs": len(times), "close_holes": True,
+           "remove_floaters": True, "faces_in": int(st.face_count),
+           "faces_out": int(pst["face_count"]), "walk_ms": float(pst["walk_ms"]),
+           "boundary_edges": int(pst["boundary_edges"]), "loops_filled": int(pst["loops_filled"])}
+    log(f"[bench] postprocess: {rec['gpu_ms']:.2f} ms (median of {len(times)}), "
+        f"{rec['faces_in']} -> {rec['faces_out']} quads")
+    if not args.no_cpu:
+        import oracle
+        mesh = be._download(False)
+        t = time.perf_counter()
+        V2, F2, _ = oracle.postprocess(mesh.vertices.reshape(-1, 3), mesh.faces.reshape(-1, 4))
+        rec["cpu_ms"] = (time.perf_counter() - t) * 1e3
+        rec["cpu_kind"] = "port (oracle/post_oracle.cpp, 1 thread)"
+        rec["equal_to_oracle"] = bool(np.array_equal(V2.ravel(), out.vertices) and
+                                      np.array_equal(F2.ravel(), out.faces))
+        log(f"[bench] postprocess oracle: {rec['cpu_ms']:.0f} ms, equal: {rec['equal_to_oracle']}")
+    return rec
 
 
 def host_cpu():

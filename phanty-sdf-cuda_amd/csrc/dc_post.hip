@@ -175,15 +175,30 @@ __global__ void __launch_bounds__(kThreads)
 // reference's FP32 operations (dc_backend.cpp:144-148). The square root and the division are done
 // in FP64 and rounded once to FP32: FP64 carries more than 2 x 24 + 2 bits, so that equals the
 // correctly rounded FP32 result.
-__global__ void k_weld_step(const float* __restrict__ part, int nb, float* __restrict__ inv) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kThreads)
+    k_weld_step(const float* __restrict__ part, int nb, float* __restrict__ inv) {
+  __shared__ float s[6][kThreads];
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; ++b)
+  for (int b = threadIdx.x; b < nb; b += kThreads)
     for (int c = 0; c < 3; ++c) {
       mn[c] = fminf(mn[c], part[6 * b + c]);
       mx[c] = fmaxf(mx[c], part[6 * b + 3 + c]);
     }
-  const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+  for (int c = 0; c < 3; ++c) {
+    s[c][threadIdx.x] = mn[c];
+    s[3 + c][threadIdx.x] = mx[c];
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+        s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const float dx = s[3][0] - s[0][0], dy = s[4][0] - s[1][0], dz = s[5][0] - s[2][0];
   const float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
   const float sum = (dx2 + dy2) + dz2;
   const float diag = (float)sqrt((double)sum);
@@ -323,14 +338,41 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Side classes, one word per edge slot (written over the slot's table index): the other slot of
+// an edge used exactly twice, or one of these.
+constexpr uint32_t kSideMany = 0xFFFFFFFDu;  // edge used by more than two slots
+constexpr uint32_t kSideOne = 0xFFFFFFFEu;   // edge used by this slot alone
+// (kNone: the slot's two ends are one vertex)
+__global__ void k_edge_classify(uint32_t* __restrict__ eslot, int64_t nslots,
+                                const uint32_t* __restrict__ ecnt,
+                                const uint32_t* __restrict__ epair) {
+  GRID_STRIDE(s, nslots) {
+    const uint32_t h = eslot[s];
+    if (h == kNone) continue;
+    const uint32_t c = ecnt[h];
+    uint32_t v;
+    if (c == 1)
+      v = kSideOne;
+    else if (c > 2)
+      v = kSideMany;
+    else {
+      const uint32_t a = epair[2 * h], b = epair[2 * h + 1];
+      v = a == (uint32_t)s ? b : a;
+    }
+    eslot[s] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Components (dc_backend.cpp:313-370): faces joined by edges used exactly twice
 // ---------------------------------------------------------------------------------------------
+// Parent pointers are read and halved with relaxed, wave-scoped (cached) accesses; a stale value
+// is an earlier ancestor, which only costs a retry of the hooking CAS (performed at the L2).
 __device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 __device__ __forceinline__ void st_rlx(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 // parent[x] < x for every non-root x (roots hook under smaller roots, halving moves a pointer to
 // an ancestor), so the walk ends and each root is the lowest face of its tree.
@@ -360,50 +402,118 @@ __device__ __forceinline__ void uf_union(uint32_t* parent, uint32_t a, uint32_t 
   }
 }
 
-__global__ void k_uf_init(uint32_t* __restrict__ parent, int64_t n) {
-  GRID_STRIDE(i, n) parent[i] = (uint32_t)i;
+// Each face starts under its lowest neighbour across an edge used exactly twice (or itself): the
+// pointers still decrease, so this is a forest of faces in one component each, and its trees are
+// already shallow before the unions.
+__global__ void k_uf_init(const uint32_t* __restrict__ side, uint32_t* __restrict__ parent,
+                          int64_t n) {
+  GRID_STRIDE(f, n) {
+    const uint4 w = reinterpret_cast<const uint4*>(side)[f];
+    uint32_t p = (uint32_t)f;
+    if (w.x < kSideMany) p = min(p, w.x >> 2);
+    if (w.y < kSideMany) p = min(p, w.y >> 2);
+    if (w.z < kSideMany) p = min(p, w.z >> 2);
+    if (w.w < kSideMany) p = min(p, w.w >> 2);
+    parent[f] = p;
+  }
 }
 
 // Each edge used by exactly two slots joins their faces (taken once, by the lower slot).
-__global__ void k_uf_union(const uint32_t* __restrict__ eslot, const uint32_t* __restrict__ ecnt,
-                           const uint32_t* __restrict__ epair, int64_t nslots,
-                           uint32_t* parent) {
+__global__ void k_uf_union(const uint32_t* __restrict__ side, int64_t nslots, uint32_t* parent) {
   GRID_STRIDE(s, nslots) {
-    const uint32_t h = eslot[s];
-    if (h == kNone || ecnt[h] != 2) continue;
-    const uint32_t a = epair[2 * h], b = epair[2 * h + 1];
-    if ((uint32_t)s != min(a, b)) continue;
-    uf_union(parent, a >> 2, b >> 2);
+    const uint32_t o = side[s];
+    if (o >= kSideMany || o < (uint32_t)s) continue;
+    uf_union(parent, (uint32_t)(s >> 2), o >> 2);
   }
 }
 
-// parent[f] = root of f (the component's lowest face); component sizes and count.
-__global__ void k_uf_label(uint32_t* parent, int64_t n, uint32_t* __restrict__ csize,
-                           uint32_t* __restrict__ ncomp) {
+// Shortens every path (halving). Its final pointers need not be roots: a thread halving through x
+// may store an older ancestor after x's own store; the labels are taken read-only below.
+__global__ void k_uf_compress(uint32_t* parent, int64_t n) {
+  GRID_STRIDE(i, n) (void)uf_find(parent, (uint32_t)i);
+}
+// label[i] = root of i; nothing writes parent meanwhile, so every walk ends at the true root.
+__global__ void k_uf_label(const uint32_t* __restrict__ parent, int64_t n,
+                           uint32_t* __restrict__ label) {
   GRID_STRIDE(i, n) {
-    const uint32_t r = uf_find(parent, (uint32_t)i);
-    atomicAdd(csize + r, 1u);
-    if (r == (uint32_t)i) atomicAdd(ncomp, 1u);
+    uint32_t x = (uint32_t)i, p = parent[x];
+    while (p != x) {
+      x = p;
+      p = parent[x];
+    }
+    label[i] = x;
   }
 }
-__global__ void k_uf_root_label(uint32_t* parent, int64_t n) {
-  GRID_STRIDE(i, n) parent[i] = uf_find(parent, (uint32_t)i);
+
+// Component sizes and count from the root labels. Nearly every face of a mesh can belong to one
+// component, so adding 1 per face at its root would serialise on one address: each wave first
+// merges its lanes by root (one ballot per distinct root), each block adds the merged counts into
+// an LDS table of roots (a contiguous share of the faces holds few roots), and the table is
+// flushed with one global atomic per entry. Roots beyond the table's capacity go straight to
+// global memory.
+constexpr int kRootSlots = 256;
+__global__ void __launch_bounds__(kThreads)
+    k_comp_size(const uint32_t* __restrict__ label, int64_t n, uint32_t* __restrict__ csize,
+                uint32_t* __restrict__ ncomp) {
+  __shared__ uint32_t s_key[kRootSlots], s_cnt[kRootSlots];
+  for (int k = threadIdx.x; k < kRootSlots; k += kThreads) {
+    s_key[k] = kNone;
+    s_cnt[k] = 0;
+  }
+  __syncthreads();
+  uint64_t lo, hi;
+  dccommon::block_share((uint64_t)n, lo, hi);
+  const int lane = threadIdx.x & 63;
+  uint32_t roots = 0;
+  for (uint64_t base = lo; base < hi; base += kThreads) {
+    const uint64_t i = base + threadIdx.x;
+    const bool valid = i < hi;
+    const uint32_t r = valid ? label[i] : kNone;
+    roots += (valid && r == (uint32_t)i) ? 1u : 0u;
+    uint64_t pending = __ballot(valid);
+    while (pending) {  // uniform across the wave
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t lr = (uint32_t)__shfl((int)r, leader);
+      const uint64_t m = __ballot(valid && r == lr) & pending;
+      if (lane == leader) {
+        const uint32_t c = (uint32_t)__popcll(m);
+        uint32_t h = (lr * 2654435761u) & (kRootSlots - 1);
+        bool done = false;
+        for (int probe = 0; probe < kRootSlots && !done; ++probe) {
+          const uint32_t old = atomicCAS(s_key + h, kNone, lr);
+          if (old == kNone || old == lr) {
+            atomicAdd(s_cnt + h, c);
+            done = true;
+          }
+          h = (h + 1) & (kRootSlots - 1);
+        }
+        if (!done) atomicAdd(csize + lr, c);
+      }
+      pending &= ~m;
+    }
+  }
+  // roots counted per wave, one atomic per wave that saw any
+  for (int off = 32; off > 0; off >>= 1) roots += (uint32_t)__shfl_down((int)roots, off);
+  if (lane == 0 && roots) atomicAdd(ncomp, roots);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kRootSlots; k += kThreads)
+    if (s_key[k] != kNone) atomicAdd(csize + s_key[k], s_cnt[k]);
 }
 
 // The largest component, ties to the lowest first face (the reference keeps the first component
 // of maximum size in first-face order, dc_backend.cpp:351-358).
-__global__ void k_uf_best(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ csize,
+__global__ void k_uf_best(const uint32_t* __restrict__ label, const uint32_t* __restrict__ csize,
                           int64_t n, unsigned long long* __restrict__ best) {
-  GRID_STRIDE(i, n) if (parent[i] == (uint32_t)i) {
+  GRID_STRIDE(i, n) if (label[i] == (uint32_t)i) {
     const unsigned long long key =
         ((unsigned long long)csize[i] << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)i);
     atomicMax(best, key);
   }
 }
-__global__ void k_uf_keep(const uint32_t* __restrict__ parent, int64_t n,
+__global__ void k_uf_keep(const uint32_t* __restrict__ label, int64_t n,
                           const unsigned long long* __restrict__ best, uint32_t* __restrict__ keep) {
   const uint32_t root = 0xFFFFFFFFu - (uint32_t)(*best & 0xFFFFFFFFull);
-  GRID_STRIDE(i, n) keep[i] = parent[i] == root ? 1u : 0u;
+  GRID_STRIDE(i, n) keep[i] = label[i] == root ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -416,17 +526,13 @@ struct BEdge {
 
 // An edge slot is a boundary edge when its edge has one slot, or (with floater removal) more than
 // two; with floater removal only slots of kept faces count.
-__global__ void k_bnd_flags(const uint32_t* __restrict__ eslot, const uint32_t* __restrict__ ecnt,
-                            int64_t nslots, const uint32_t* __restrict__ keep, int floaters,
+__global__ void k_bnd_flags(const uint32_t* __restrict__ side, int64_t nslots,
+                            const uint32_t* __restrict__ keep, int floaters,
                             uint32_t* __restrict__ flag) {
   GRID_STRIDE(s, nslots) {
-    const uint32_t h = eslot[s];
-    uint32_t b = 0;
-    if (h != kNone) {
-      const uint32_t c = ecnt[h];
-      b = (c == 1 || (c > 2 && floaters)) ? 1u : 0u;
-      if (b && floaters && !keep[s >> 2]) b = 0;
-    }
+    const uint32_t o = side[s];
+    uint32_t b = (o == kSideOne || (o == kSideMany && floaters)) ? 1u : 0u;
+    if (b && floaters && !keep[s >> 2]) b = 0;
     flag[s] = b;
   }
 }
@@ -755,7 +861,7 @@ int DcPost::weld(const float* v, int64_t nv, const int32_t* f, int64_t nf, int o
   uint32_t* cnt = at<uint32_t>(b_[kCnt].p);
   hipLaunchKernelGGL(k_bbox, dim3((unsigned)nb), dim3(kThreads), 0, s_, v, nv, part);
   POST_TRY(hipGetLastError(), "k_bbox");
-  hipLaunchKernelGGL(k_weld_step, dim3(1), dim3(64), 0, s_, part, nb, inv);
+  hipLaunchKernelGGL(k_weld_step, dim3(1), dim3(kThreads), 0, s_, part, nb, inv);
   POST_TRY(hipGetLastError(), "k_weld_step");
   LAUNCH(k_fill32, (int64_t)tsize, table, (int64_t)tsize, kNone);
   LAUNCH(k_weld_insert, nv, v, nv, inv, table, (uint64_t)(tsize - 1), slot);
@@ -843,6 +949,8 @@ int DcPost::run(const float* vertices, int64_t nv, const int32_t* faces, int64_t
   LAUNCH(k_fill64, (int64_t)esize, etab, (int64_t)esize, kEmptyKey);
   LAUNCH(k_fill32, (int64_t)esize, ecnt, (int64_t)esize, 0u);
   LAUNCH(k_edge_insert, nslots, fw, nslots, etab, ecnt, epair, (uint64_t)(esize - 1), eslot);
+  LAUNCH(k_edge_classify, nslots, eslot, nslots, ecnt, epair);
+  const uint32_t* side = eslot;
 
   // floaters: keep flags of the largest component
   POST_RC(grow(kFlag, (size_t)std::max<int64_t>(nslots, nfw) * 4));
@@ -860,21 +968,27 @@ int DcPost::run(const float* vertices, int64_t nv, const int32_t* faces, int64_t
     uint32_t* csize = at<uint32_t>(b_[kCSize].p);
     unsigned long long* best = at<unsigned long long>(b_[kBest].p);
     keep = at<uint32_t>(b_[kKeep].p);
-    LAUNCH(k_uf_init, nfw, parent, nfw);
+    LAUNCH(k_uf_init, nfw, side, parent, nfw);
     LAUNCH(k_fill32, nfw, csize, nfw, 0u);
     LAUNCH(k_fill64, (int64_t)1, best, (int64_t)1, 0ull);
-    LAUNCH(k_uf_union, nslots, eslot, ecnt, epair, nslots, parent);
-    LAUNCH(k_uf_label, nfw, parent, nfw, csize, cnt + kCntComp);
-    LAUNCH(k_uf_root_label, nfw, parent, nfw);
-    LAUNCH(k_uf_best, nfw, parent, csize, nfw, best);
-    LAUNCH(k_uf_keep, nfw, parent, nfw, best, keep);
+    LAUNCH(k_uf_compress, nfw, parent, nfw);  // short paths before the unions walk them
+    LAUNCH(k_uf_union, nslots, side, nslots, parent);
+    LAUNCH(k_uf_compress, nfw, parent, nfw);
+    uint32_t* label = flag;  // free until the boundary flags
+    LAUNCH(k_uf_label, nfw, parent, nfw, label);
+    // a few hundred blocks: one flush per block and root
+    hipLaunchKernelGGL(k_comp_size, dim3((unsigned)std::min<int64_t>(blocks_for(nfw, cu_), 2 * cu_)),
+                       dim3(kThreads), 0, s_, label, nfw, csize, cnt + kCntComp);
+    POST_TRY(hipGetLastError(), "k_comp_size");
+    LAUNCH(k_uf_best, nfw, label, csize, nfw, best);
+    LAUNCH(k_uf_keep, nfw, label, nfw, best, keep);
   }
 
   // boundary edges, then the loops on the host
   HoleFill hf;
   hf.n_welded = nw;
   if (close_holes) {
-    LAUNCH(k_bnd_flags, nslots, eslot, ecnt, nslots, keep, remove_floaters ? 1 : 0, flag);
+    LAUNCH(k_bnd_flags, nslots, side, nslots, keep, remove_floaters ? 1 : 0, flag);
     POST_RC(scan(flag, nslots, idx, kCntBnd));
     uint32_t h[kCntSlots];
     POST_RC(read_counts(h, kCntSlots));
