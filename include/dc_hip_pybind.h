@@ -8,7 +8,9 @@
 //   bind_hip_backend(m);
 //
 // The class mirrors the reference's CudaDualContouringBackend binding (bindings.cpp:216-228):
-// extract(grid) and extract_device(grid) return (mesh, stats). DenseSdfGridDevice is registered
+// extract(grid) and extract_device(grid) return (mesh, stats); postprocess(mesh, close_holes,
+// remove_floaters) is the reference's postprocess_mesh (bindings.cpp:145-146) on the GPU, and
+// postprocess_last the same on the last extraction's mesh in device memory. DenseSdfGridDevice is registered
 // here only when the build has no CUDA backends (with DC_ENABLE_CUDA the reference registers it,
 // bindings.cpp:195-207); this registration adds d_values as an integer device address, so a
 // torch tensor's data_ptr() can be passed.
@@ -56,6 +58,10 @@ inline void bind_hip_backend(pybind11::module_& m) {
              DualContouringMesh mesh = self.extract_device(grid, stats);
              return py::make_tuple(mesh, stats);
            })
+      .def("postprocess", &HipDualContouringBackend::postprocess, py::arg("mesh"),
+           py::arg("close_holes") = true, py::arg("remove_floaters") = true)
+      .def("postprocess_last", &HipDualContouringBackend::postprocess_last,
+           py::arg("close_holes") = true, py::arg("remove_floaters") = true)
       .def("last_vertices_f64", &HipDualContouringBackend::last_vertices_f64)
       .def("last_triangles", &HipDualContouringBackend::last_triangles)
       .def("set_option", &HipDualContouringBackend::set_option)
