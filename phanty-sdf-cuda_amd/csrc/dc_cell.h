@@ -187,8 +187,8 @@ __device__ __forceinline__ void load_values(const DcGridDev& G, int i, int j, in
 }
 
 // FP64: global frame (the CPU backend's coordinates).
-__device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<double>& c) {
-  load_values(G, i, j, k, c);
+__device__ __forceinline__ void cell_coords(const DcGridDev& G, int i, int j, int k,
+                                            Cell<double>& c) {
   c.X0 = coordd(G.ox, i, G.vx);
   c.X1 = coordd(G.ox, i + 1, G.vx);
   c.Y0 = coordd(G.oy, j, G.vy);
@@ -198,14 +198,20 @@ __device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int 
 }
 
 // FP32: the cell's own frame, min corner at 0 (edge lengths from the FP32 grid coordinates).
-__device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<float>& c) {
-  load_values(G, i, j, k, c);
+__device__ __forceinline__ void cell_coords(const DcGridDev& G, int i, int j, int k,
+                                            Cell<float>& c) {
   c.X0 = 0.f;
   c.Y0 = 0.f;
   c.Z0 = 0.f;
   c.X1 = (float)(coordd(G.ox, i + 1, G.vx) - coordd(G.ox, i, G.vx));
   c.Y1 = (float)(coordd(G.oy, j + 1, G.vy) - coordd(G.oy, j, G.vy));
   c.Z1 = (float)(coordd(G.oz, k + 1, G.vz) - coordd(G.oz, k, G.vz));
+}
+
+template <typename T>
+__device__ __forceinline__ void load_cell(const DcGridDev& G, int i, int j, int k, Cell<T>& c) {
+  load_values(G, i, j, k, c);
+  cell_coords(G, i, j, k, c);
 }
 
 // Cell.cpp:80-155 gradientAtPointUsingTrilinear
@@ -365,6 +371,74 @@ __device__ __forceinline__ void group_sum(const DcGridDev& G, int axis, int x0, 
         acc[1] = acc[1] + n[1];
         acc[2] = acc[2] + n[2];
       }
+    }
+  }
+}
+
+// The corner block shared by the four members of the axis-A group at base cell (x0,y0,z0):
+// 2x3x3 samples (x, y, z) for an x edge, 3x2x3 for y, 3x3x2 for z, loaded row by row (9, 6 and 6
+// row loads instead of 16 per-cell pair loads). All four members must lie inside the grid (true
+// for a face's edge).
+template <int A>
+struct GroupBlock {
+  static constexpr int NX = A == 0 ? 2 : 3, NY = A == 1 ? 2 : 3, NZ = A == 2 ? 2 : 3;
+  float R[NZ][NY][NX];
+  int xl, yl, zl;
+  __device__ __forceinline__ void load(const DcGridDev& G, int x0, int y0, int z0) {
+    xl = A == 0 ? x0 : x0 - 1;
+    yl = A == 1 ? y0 : y0 - 1;
+    zl = A == 2 ? z0 : z0 - 1;
+#pragma unroll
+    for (int zz = 0; zz < NZ; ++zz)
+#pragma unroll
+      for (int yy = 0; yy < NY; ++yy) {
+        const gfloat p = sample_ptr(G, xl, yl + yy, zl + zz);
+#pragma unroll
+        for (int xx = 0; xx < NX; ++xx) R[zz][yy][xx] = p[xx];
+      }
+  }
+  // member q (group_member order: x edge (x0, y0-m1, z0-m2), y edge (x0-m1, y0, z0-m2), z edge
+  // (x0-m1, y0-m2, z0)) with the values load_cell gives it
+  template <typename T>
+  __device__ __forceinline__ void member(const DcGridDev& G, int q, Cell<T>& c) const {
+    const int m1 = q & 1, m2 = (q >> 1) & 1;
+    const int ox = A == 0 ? 0 : 1 - m1;
+    const int oy = A == 0 ? 1 - m1 : (A == 1 ? 0 : 1 - m2);
+    const int oz = A == 2 ? 0 : 1 - m2;
+    T* S = c.S;
+    S[0] = (T)R[oz][oy][ox];
+    S[1] = (T)R[oz][oy][ox + 1];
+    S[2] = (T)R[oz][oy + 1][ox + 1];
+    S[3] = (T)R[oz][oy + 1][ox];
+    S[4] = (T)R[oz + 1][oy][ox];
+    S[5] = (T)R[oz + 1][oy][ox + 1];
+    S[6] = (T)R[oz + 1][oy + 1][ox + 1];
+    S[7] = (T)R[oz + 1][oy + 1][ox];
+    const int x = xl + ox, y = yl + oy, z = zl + oz;
+    c.base = sample_ptr(G, x, y, z);
+    c.sy = G.nx;
+    c.sz = (int64_t)G.nx * G.ny;
+    cell_coords(G, x, y, z, c);
+  }
+};
+
+// group_sum from the shared corner block: the same sum in the same order.
+template <int A, typename T>
+__device__ __forceinline__ void group_sum_block(const DcGridDev& G, int x0, int y0, int z0,
+                                                T acc[3]) {
+  GroupBlock<A> B;
+  B.load(G, x0, y0, z0);
+  acc[0] = acc[1] = acc[2] = T(0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    Cell<T> c;
+    B.template member<T>(G, q, c);
+    T n[3];
+    raw_normal(c, group_edge(A, q), n);
+    if (sq3(n[0], n[1], n[2]) > T(0)) {
+      acc[0] = acc[0] + n[0];
+      acc[1] = acc[1] + n[1];
+      acc[2] = acc[2] + n[2];
     }
   }
 }

@@ -104,7 +104,13 @@ __global__ void __launch_bounds__(256)
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t face, uint64_t) {
               if (face >= capF) return;
               T n[3];
-              group_sum<T, true>(G, a, i, j, k, n);
+              // the four members from their shared corner block
+              if (a == 0)
+                group_sum_block<0>(G, i, j, k, n);
+              else if (a == 1)
+                group_sum_block<1>(G, i, j, k, n);
+              else
+                group_sum_block<2>(G, i, j, k, n);
               const T len = tsqrt(sq3(n[0], n[1], n[2]));
               normalize3(n);
               nrm[4 * face + 0] = n[0];
@@ -295,6 +301,105 @@ __device__ __forceinline__ void grid_gradient_from(const DcGridDev& G, int i, in
   }
 }
 
+// Loads of a few consecutive values at the alignment of their element (global memory takes
+// dword-aligned multi-dword loads): one instruction where element loads would take several.
+typedef float f3a4 __attribute__((ext_vector_type(3), aligned(4)));
+typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+typedef unsigned long long u64x2a8 __attribute__((ext_vector_type(2), aligned(8)));
+typedef unsigned int u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
+template <typename V, typename E>
+__device__ __forceinline__ V load_vec(const E* p) {
+  return *(const __attribute__((address_space(1))) V*)(p);  // a global-memory load
+}
+
+// gradient_samples at both endpoints of the axis-A quad edge at (i,j,k) (f0 at (i,j,k), f1 one step
+// along A), by rows: a quad edge's four cells lie inside the grid, so only the along-axis
+// neighbours are clamped, and the samples of both stencils come from 5 (x edge) or 8 (y, z edge)
+// loads instead of 12. An x edge on a grid under 4 samples wide takes the element loads.
+template <int A>
+__device__ __forceinline__ void edge_gradient_samples(const DcGridDev& G, int i, int j, int k,
+                                                      float f0[6], float f1[6]) {
+  if constexpr (A == 0) {
+    // the window starts at i-1 inside, at 0 for i = 0 and at i-2 for i = nx-2 (an x quad edge
+    // has i <= nx-2); with nx >= 4 the two border cases exclude each other. Narrower grids take
+    // the element loads.
+    const bool wide = G.nx >= 4;
+    const int xs = wide ? min(max(i - 1, 0), G.nx - 4) : 0;
+    const bool left = i == 0, right = i + 2 > G.nx - 1;
+    const gfloat yp = sample_ptr(G, i, j + 1, k), ym = sample_ptr(G, i, j - 1, k);
+    const gfloat zp = sample_ptr(G, i, j, k + 1), zm = sample_ptr(G, i, j, k - 1);
+    f0[2] = yp[0];
+    f1[2] = yp[1];
+    f0[3] = ym[0];
+    f1[3] = ym[1];
+    f0[4] = zp[0];
+    f1[4] = zp[1];
+    f0[5] = zm[0];
+    f1[5] = zm[1];
+    if (wide) {
+      const f4a4 w = load_vec<f4a4>(&sample_ptr(G, xs, j, k)[0]);
+      f0[0] = left ? w.y : (right ? w.w : w.z);  // i+1
+      f0[1] = right ? w.y : w.x;                 // max(i-1, 0)
+      f1[0] = left ? w.z : w.w;                  // min(i+2, nx-1)
+      f1[1] = left ? w.x : (right ? w.z : w.y);  // i
+    } else {
+      const gfloat r = sample_ptr(G, 0, j, k);
+      f0[0] = r[i + 1];
+      f0[1] = r[max(i - 1, 0)];
+      f1[0] = r[min(i + 2, G.nx - 1)];
+      f1[1] = r[i];
+    }
+  } else {
+    // rows through both endpoints give the x neighbours and the endpoints themselves
+    const int j1 = j + (A == 1), k1 = k + (A == 2);
+    const f3a4 r0 = load_vec<f3a4>(&sample_ptr(G, i - 1, j, k)[0]);
+    const f3a4 r1 = load_vec<f3a4>(&sample_ptr(G, i - 1, j1, k1)[0]);
+    f0[0] = r0.z;
+    f0[1] = r0.x;
+    f1[0] = r1.z;
+    f1[1] = r1.x;
+    if constexpr (A == 1) {
+      const int jm0 = max(j - 1, 0), jp1 = min(j + 2, G.ny - 1);
+      f0[2] = r1.y;                         // (i, j+1, k)
+      f0[3] = *sample_ptr(G, i, jm0, k);
+      f1[2] = *sample_ptr(G, i, jp1, k);
+      f1[3] = r0.y;                         // (i, j, k)
+      f0[4] = *sample_ptr(G, i, j, k + 1);
+      f0[5] = *sample_ptr(G, i, j, k - 1);
+      f1[4] = *sample_ptr(G, i, j1, k + 1);
+      f1[5] = *sample_ptr(G, i, j1, k - 1);
+    } else {
+      const int km0 = max(k - 1, 0), kp1 = min(k + 2, G.nz - 1);
+      f0[2] = *sample_ptr(G, i, j + 1, k);
+      f0[3] = *sample_ptr(G, i, j - 1, k);
+      f1[2] = *sample_ptr(G, i, j + 1, k1);
+      f1[3] = *sample_ptr(G, i, j - 1, k1);
+      f0[4] = r1.y;                         // (i, j, k+1)
+      f0[5] = *sample_ptr(G, i, j, km0);
+      f1[4] = *sample_ptr(G, i, j, kp1);
+      f1[5] = r0.y;                         // (i, j, k)
+    }
+  }
+}
+
+// Vertex ids (grid-local) of the active cells (i-1, j, k) and (i, j, k), i >= 1, from one load of
+// the two act words at and after cell i-1's and one of their prefixes. The act and prefix buffers
+// hold one word past their end, so the second word can always be loaded; it is used only when
+// cell i starts a new word (then it is in the same row).
+__device__ __forceinline__ void vertex_id_pair(const DcGridDev& G,
+                                               const uint64_t* __restrict__ act,
+                                               const uint32_t* __restrict__ wprefix,
+                                               uint32_t vbase, int i, int j, int k, int32_t& lo,
+                                               int32_t& hi) {
+  const int64_t w = G.act_off + ((int64_t)k * G.cy + j) * G.W + ((i - 1) >> 6);
+  const u64x2a8 a = load_vec<u64x2a8>(act + w);
+  const u32x2a4 p = load_vec<u32x2a4>(wprefix + w);
+  lo = (int32_t)(p.x - vbase + (uint32_t)__popcll(a.x & ((1ull << ((i - 1) & 63)) - 1ull)));
+  hi = (i & 63) ? (int32_t)(p.x - vbase + (uint32_t)__popcll(a.x & ((1ull << (i & 63)) - 1ull)))
+                : (int32_t)(p.y - vbase);
+}
+
 // Vertex id (grid-local) of active cell (i,j,k): rank in (k,j,i) order.
 __device__ __forceinline__ int32_t vertex_id(const DcGridDev& G, const uint64_t* __restrict__ act,
                                              const uint32_t* __restrict__ wprefix, uint32_t vbase,
@@ -359,6 +464,79 @@ __device__ __forceinline__ void quad_cell(int a, int i, int j, int k, int q, int
 constexpr double kVertexTol = 1e-4;
 constexpr double kTieMargin = 8.0;
 
+// One quad: every load issued before its arithmetic (the stencil rows of both endpoints, the four
+// cells' act words and prefixes, their vertices), then the orientation test of contouring.cpp:
+// 607-729; FP32 mode lists the quad for k_reorient when the test is within its error bound.
+template <int A>
+__device__ __forceinline__ void emit_face(const DcGridDev& G, int i, int j, int k, uint64_t pos,
+                                          uint64_t entry, const uint64_t* __restrict__ act,
+                                          const uint32_t* __restrict__ wprefix,
+                                          const double* __restrict__ v64,
+                                          const float* __restrict__ v32, int precision,
+                                          DcCounters* __restrict__ ctr, uint64_t capF,
+                                          int32_t* __restrict__ faces,
+                                          uint64_t* __restrict__ reorient) {
+  const int e1x = i + (A == 0), e1y = j + (A == 1), e1z = k + (A == 2);  // far endpoint
+  float f0[6], f1[6];
+  edge_gradient_samples<A>(G, i, j, k, f0, f1);
+  const uint32_t vbase = wprefix[G.act_off];
+  const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
+  int c[4];
+  if constexpr (A == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int x, y, z;
+      quad_cell(A, i, j, k, q, x, y, z);
+      c[q] = vertex_id(G, act, wprefix, vbase, x, y, z);
+    }
+  } else {
+    // cells (i-1, i) in two rows: c0 = (i-1, r0), c1 = (i, r0), c2 = (i, r1), c3 = (i-1, r1)
+    const int jr0 = A == 2 ? j - 1 : j, kr0 = A == 1 ? k - 1 : k;
+    vertex_id_pair(G, act, wprefix, vbase, i, jr0, kr0, c[0], c[1]);
+    vertex_id_pair(G, act, wprefix, vbase, i, j, k, c[3], c[2]);
+  }
+  double pa[3], pb[3], pc[3], pd[3];
+  double* dst[4] = {pa, pb, pc, pd};
+  if (precision == DC_PREC_FP64) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double* v = v64 + 3 * (vglob + c[q]);
+      const d2a8 xy = load_vec<d2a8>(v);
+      dst[q][0] = xy.x;
+      dst[q][1] = xy.y;
+      dst[q][2] = v[2];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f3a4 xyz = load_vec<f3a4>(v32 + 3 * (vglob + c[q]));
+      dst[q][0] = (double)xyz.x;
+      dst[q][1] = (double)xyz.y;
+      dst[q][2] = (double)xyz.z;
+    }
+  }
+  double g0[3], g1[3];
+  grid_gradient_from(G, i, j, k, f0, g0);
+  grid_gradient_from(G, e1x, e1y, e1z, f1, g1);
+  const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
+  int c1 = c[1], c3 = c[3];
+  double q1sq = 0.0;
+  const double dot = orient_quad(pa, pb, pc, pd, gn0, gn1, gn2, c1, c3, &q1sq);
+  if (precision != DC_PREC_FP64 && reorient) {
+    const double ba = tsqrt(sq3(pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]));
+    const double ca = tsqrt(sq3(pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]));
+    const double da = tsqrt(sq3(pd[0] - pa[0], pd[1] - pa[1], pd[2] - pa[2]));
+    const double gl = tsqrt(sq3(gn0, gn1, gn2));
+    const double err =
+        kTieMargin * 4.0 * kVertexTol * fmin(fmin(G.vx, G.vy), G.vz) * (ba + ca + da) + 2e-12;
+    if (gl > 0.0 && (tsqrt(q1sq) <= 1e-12 + err || fabs(dot) <= err * gl)) {
+      const uint64_t slot = atomicAdd((unsigned long long*)&ctr->n_reorient, 1ull);
+      if (slot < capF) reorient[slot] = entry;
+    }
+  }
+  reinterpret_cast<int4*>(faces)[pos] = make_int4(c[0], c1, c[2], c3);
+}
+
 __global__ void __launch_bounds__(256)
     k_emit(const DcGridDev* __restrict__ grids, int n_grids, const int64_t* __restrict__ ft_bounds,
            const uint64_t* __restrict__ flist, const uint64_t* __restrict__ ft,
@@ -371,49 +549,15 @@ __global__ void __launch_bounds__(256)
   if (ctr->n_active > capA || ctr->n_faces > capF) return;
   each_face(grids, n_grids, ft_bounds, flist, ft, pw, ctr->n_faces, &ctr->error,
             [&](const DcGridDev& G, int a, int i, int j, int k, uint64_t pos, uint64_t entry) {
-    const int e1x = i + (a == 0), e1y = j + (a == 1), e1z = k + (a == 2);  // far endpoint
-    // all loads of the quad are issued before its arithmetic: the stencil samples of both
-    // endpoints, then the four cells' act words and prefixes
-    float f0[6], f1[6];
-    gradient_samples(G, i, j, k, f0);
-    gradient_samples(G, e1x, e1y, e1z, f1);
-    const uint32_t vbase = wprefix[G.act_off];
-    const int64_t vglob = (int64_t)vbase;  // grid's first vertex in the concatenated arrays
-    int c[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int x, y, z;
-      quad_cell(a, i, j, k, q, x, y, z);
-      c[q] = vertex_id(G, act, wprefix, vbase, x, y, z);
-    }
-    double g0[3], g1[3];
-    grid_gradient_from(G, i, j, k, f0, g0);
-    grid_gradient_from(G, e1x, e1y, e1z, f1, g1);
-    const double gn0 = g0[0] + g1[0], gn1 = g0[1] + g1[1], gn2 = g0[2] + g1[2];
-    double pa[3], pb[3], pc[3], pd[3];
-    double* dst[4] = {pa, pb, pc, pd};
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        dst[q][d] = (precision == DC_PREC_FP64) ? v64[3 * (vglob + c[q]) + d]
-                                                : (double)v32[3 * (vglob + c[q]) + d];
-    int c1 = c[1], c3 = c[3];
-    double q1sq = 0.0;
-    const double dot = orient_quad(pa, pb, pc, pd, gn0, gn1, gn2, c1, c3, &q1sq);
-    if (precision != DC_PREC_FP64 && reorient) {
-      const double ba = tsqrt(sq3(pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]));
-      const double ca = tsqrt(sq3(pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]));
-      const double da = tsqrt(sq3(pd[0] - pa[0], pd[1] - pa[1], pd[2] - pa[2]));
-      const double gl = tsqrt(sq3(gn0, gn1, gn2));
-      const double err =
-          kTieMargin * 4.0 * kVertexTol * fmin(fmin(G.vx, G.vy), G.vz) * (ba + ca + da) + 2e-12;
-      if (gl > 0.0 && (tsqrt(q1sq) <= 1e-12 + err || fabs(dot) <= err * gl)) {
-        const uint64_t slot = atomicAdd((unsigned long long*)&ctr->n_reorient, 1ull);
-        if (slot < capF) reorient[slot] = entry;
-      }
-    }
-    reinterpret_cast<int4*>(faces)[pos] = make_int4(c[0], c1, c[2], c3);
+    if (a == 0)
+      emit_face<0>(G, i, j, k, pos, entry, act, wprefix, v64, v32, precision, ctr, capF, faces,
+                   reorient);
+    else if (a == 1)
+      emit_face<1>(G, i, j, k, pos, entry, act, wprefix, v64, v32, precision, ctr, capF, faces,
+                   reorient);
+    else
+      emit_face<2>(G, i, j, k, pos, entry, act, wprefix, v64, v32, precision, ctr, capF, faces,
+                   reorient);
   });
 }
 

@@ -403,9 +403,10 @@ int run_once(dc_hip_context* c, int precision) {
   if ((rc = work_stream(c, &s))) return rc;
   if ((rc = ensure(c, c->grids, n * sizeof(DcGridDev), "grid descriptors"))) return rc;
   if ((rc = ensure(c, c->bounds, 7 * (n + 1) * sizeof(int64_t), "grid bounds"))) return rc;
-  if ((rc = ensure(c, c->act, c->n_words * 8, "active plane"))) return rc;
+  // act and wprefix hold one word past their end: quad emission loads two words at a time
+  if ((rc = ensure(c, c->act, (c->n_words + 1) * 8, "active plane"))) return rc;
   if ((rc = ensure(c, c->sg, c->n_sg * 16, "sign planes"))) return rc;
-  if ((rc = ensure(c, c->wprefix, c->n_words * 4, "word prefix"))) return rc;
+  if ((rc = ensure(c, c->wprefix, (c->n_words + 1) * 4, "word prefix"))) return rc;
   if ((rc = ensure(c, c->ft, c->n_ft * 8, "face planes"))) return rc;
   if ((rc = ensure(c, c->pcount, c->n_pen * 4, "pencil counts"))) return rc;
   if ((rc = ensure(c, c->pbase, c->n_pen * 4, "pencil bases"))) return rc;
@@ -971,9 +972,9 @@ int dc_hip_extract_device_noalloc(dc_hip_context* c, const dc_hip_grid* grid, in
     b.grids = cv.take<DcGridDev>(sizeof(DcGridDev));
     b.bounds = cv.take<int64_t>(7 * 2 * sizeof(int64_t));
     b.ctr = cv.take<DcCounters>(sizeof(DcCounters));
-    b.act = cv.take<uint64_t>(c->n_words * 8);
+    b.act = cv.take<uint64_t>((c->n_words + 1) * 8);  // one word past the end, as above
     b.sg = cv.take<uint4>(c->n_sg * 16);
-    b.wprefix = cv.take<uint32_t>(c->n_words * 4);
+    b.wprefix = cv.take<uint32_t>((c->n_words + 1) * 4);
     b.ft = cv.take<uint64_t>(c->n_ft * 8);
     b.pcount = cv.take<uint32_t>(c->n_pen * 4);
     b.pbase = cv.take<uint32_t>(c->n_pen * 4);
