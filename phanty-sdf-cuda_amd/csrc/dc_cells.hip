@@ -69,7 +69,9 @@ hipError_t dc_launch_triangulate(const int32_t* faces, uint64_t nf, int n_grids,
 
 hipError_t dc_launch_gather(const DcBwdArgs& a, hipStream_t s) {
   if (a.n_sr <= 0) return hipGetLastError();
-  const dim3 grid((unsigned)((a.n_sr + 3) / 4));
+  // one wave per row group, or (option "gather_blocks") that many blocks looping over the groups
+  const int64_t need = (a.n_sr + 3) / 4;
+  const dim3 grid((unsigned)(a.gather_blocks > 0 ? std::min<int64_t>(need, a.gather_blocks) : need));
   const size_t lds = 4 * ((size_t)a.gather_lds_row + 2 * kGatherCap) * 4;
 #define DC_GATHER(R)                                                                              \
   hipLaunchKernelGGL(k_bwd_gather<R>, grid, dim3(256), lds, s, a.grids, a.n_grids, a.sr_bounds, \

@@ -1017,129 +1017,131 @@ __global__ void __launch_bounds__(256)
   extern __shared__ float s_gather[];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t grp = (int64_t)blockIdx.x * 4 + wv;
-  if (grp >= n_grp) return;  // uniform per wave
   float* buf = s_gather + (size_t)wv * (lds_row + 2 * kGatherCap);
   uint2* cells = reinterpret_cast<uint2*>(buf + lds_row);
-  const int g = find_bucket(grp_bounds, n_grids, grp);
-  const DcGridDev G = grids[g];
-  const int64_t lr0 = (grp - G.sr_off) * R;
-  const int nrow = (int)min<int64_t>(R, (int64_t)G.ny * G.nz - lr0);
-  int ys[R], zs[R];
-  {
-    // lr0 < ny * nz <= 2^32 (dimensions are at most 65536): 32-bit division suffices
-    const uint32_t l32 = (uint32_t)lr0, ny32 = (uint32_t)G.ny;
-    int y = (int)(l32 % ny32), z = (int)(l32 / ny32);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      ys[r] = y;
-      zs[r] = z;
-      if (++y == G.ny) {
-        y = 0;
-        ++z;
-      }
-    }
-  }
-  const uint32_t lane_meta = ((uint32_t)(lane >> 4) << 12) | (uint32_t)(64 * (lane & 15));
-  // rows start on 16-byte boundaries: row buffers, stores and zero fills move float4s
-  const bool vec4 = (G.nx & 3) == 0 && (G.samp_off & 3) == 0 && ((uintptr_t)grad & 15) == 0;
-  float carry[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) carry[r] = 0.f;
-  for (int s0 = 0; s0 < G.Ws; s0 += kGatherStep) {
-    const int nstep = min(kGatherStep, G.Ws - s0);
-    const int nsamp = min(G.nx - 64 * s0, 64 * nstep);
-    uint64_t wl[R];
-    uint32_t pl[R], off[R], cnt[R], base[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      wl[r] = 0ull;
-      pl[r] = 0u;
-      if (r < nrow) gather_words(G, act, wprefix, ys[r], zs[r], s0, lane, wl[r], pl[r]);
-    }
-    uint32_t total = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint32_t c = (uint32_t)__popcll(wl[r]);
-      const uint32_t inc = wave_scan_incl(c, lane);
-      cnt[r] = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-      base[r] = total;
-      off[r] = total + inc - c;
-      total += cnt[r];
-    }
-    auto init_row = [&](int r) {
-      float4* b4 = reinterpret_cast<float4*>(buf);
-      for (int x = lane; x <= 16 * nstep; x += 64) b4[x] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (lane == 0) buf[0] = carry[r];
-    };
-    auto store_row = [&](int r, bool summed) {
-      float* out = grad + G.samp_off + ((int64_t)zs[r] * G.ny + ys[r]) * G.nx + 64 * s0;
-      if (vec4) {
-        float4* o4 = reinterpret_cast<float4*>(out);
-        const float4* b4 = reinterpret_cast<const float4*>(buf);
-        if (summed)
-          for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, b4[x]);
-        else
-          for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, make_float4(0.f, 0.f, 0.f, 0.f));
-      } else {
-        if (summed)
-          for (int x = lane; x < nsamp; x += 64) gstore(out + x, buf[x]);
-        else
-          for (int x = lane; x < nsamp; x += 64) gstore(out + x, 0.f);
-      }
-      if (summed) carry[r] = buf[64 * nstep];
-    };
-    if (total <= (uint32_t)kGatherCap) {  // uniform: every listed cell loaded at once
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        list_cells(cells, wl[r], pl[r], off[r], 0u, total, lane_meta | ((uint32_t)r << 14));
-      wave_lds_sync();
-      uint2 d[kGatherLoads];
-      float2 v[kGatherLoads];
-#pragma unroll
-      for (int u = 0; u < kGatherLoads; ++u) {
-        d[u] = make_uint2(0u, 0u);
-        v[u] = make_float2(0.f, 0.f);
-        if (64u * u < total) {  // uniform
-          d[u] = cells[min(64u * u + lane, total - 1u)];
-          v[u] = cell_pair(contrib, d[u]);
+  // one row group per wave and pass; a grid smaller than the groups' waves loops over them
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  for (int64_t grp = (int64_t)blockIdx.x * 4 + wv; grp < n_grp; grp += stride) {  // uniform
+    const int g = find_bucket(grp_bounds, n_grids, grp);
+    const DcGridDev G = grids[g];
+    const int64_t lr0 = (grp - G.sr_off) * R;
+    const int nrow = (int)min<int64_t>(R, (int64_t)G.ny * G.nz - lr0);
+    int ys[R], zs[R];
+    {
+      // lr0 < ny * nz <= 2^32 (dimensions are at most 65536): 32-bit division suffices
+      const uint32_t l32 = (uint32_t)lr0, ny32 = (uint32_t)G.ny;
+      int y = (int)(l32 % ny32), z = (int)(l32 / ny32);
+  #pragma unroll
+      for (int r = 0; r < R; ++r) {
+        ys[r] = y;
+        zs[r] = z;
+        if (++y == G.ny) {
+          y = 0;
+          ++z;
         }
       }
-#pragma unroll
+    }
+    const uint32_t lane_meta = ((uint32_t)(lane >> 4) << 12) | (uint32_t)(64 * (lane & 15));
+    // rows start on 16-byte boundaries: row buffers, stores and zero fills move float4s
+    const bool vec4 = (G.nx & 3) == 0 && (G.samp_off & 3) == 0 && ((uintptr_t)grad & 15) == 0;
+    float carry[R];
+  #pragma unroll
+    for (int r = 0; r < R; ++r) carry[r] = 0.f;
+    for (int s0 = 0; s0 < G.Ws; s0 += kGatherStep) {
+      const int nstep = min(kGatherStep, G.Ws - s0);
+      const int nsamp = min(G.nx - 64 * s0, 64 * nstep);
+      uint64_t wl[R];
+      uint32_t pl[R], off[R], cnt[R], base[R];
+  #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (r >= nrow) break;  // uniform
-        const bool summed = cnt[r] != 0u || carry[r] != 0.f;
-        if (summed) {
-          wave_lds_sync();
-          init_row(r);
-#pragma unroll
-          for (int u = 0; u < kGatherLoads; ++u)
-            if (64u * u < total) add_cells(buf, d[u], v[u], 64u * u + lane < total, r);
-          wave_lds_sync();
-        }
-        store_row(r, summed);
+        wl[r] = 0ull;
+        pl[r] = 0u;
+        if (r < nrow) gather_words(G, act, wprefix, ys[r], zs[r], s0, lane, wl[r], pl[r]);
       }
-    } else {  // a dense step: one row at a time, in passes of kGatherCap cells
-#pragma unroll
+      uint32_t total = 0;
+  #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (r >= nrow) break;  // uniform
-        const bool summed = cnt[r] != 0u || carry[r] != 0.f;
-        if (summed) {
-          wave_lds_sync();
-          init_row(r);
-          for (uint32_t lo = base[r]; lo < base[r] + cnt[r]; lo += kGatherCap) {
-            const uint32_t hi = min(base[r] + cnt[r], lo + (uint32_t)kGatherCap);
-            wave_lds_sync();
-            list_cells(cells, wl[r], pl[r], off[r], lo, hi, lane_meta | ((uint32_t)r << 14));
-            wave_lds_sync();
-            for (uint32_t k0 = 0; k0 < hi - lo; k0 += 64) {
-              const uint2 d = cells[min(k0 + lane, hi - lo - 1u)];
-              add_cells(buf, d, cell_pair(contrib, d), k0 + lane < hi - lo, r);
-            }
+        const uint32_t c = (uint32_t)__popcll(wl[r]);
+        const uint32_t inc = wave_scan_incl(c, lane);
+        cnt[r] = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        base[r] = total;
+        off[r] = total + inc - c;
+        total += cnt[r];
+      }
+      auto init_row = [&](int r) {
+        float4* b4 = reinterpret_cast<float4*>(buf);
+        for (int x = lane; x <= 16 * nstep; x += 64) b4[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 0) buf[0] = carry[r];
+      };
+      auto store_row = [&](int r, bool summed) {
+        float* out = grad + G.samp_off + ((int64_t)zs[r] * G.ny + ys[r]) * G.nx + 64 * s0;
+        if (vec4) {
+          float4* o4 = reinterpret_cast<float4*>(out);
+          const float4* b4 = reinterpret_cast<const float4*>(buf);
+          if (summed)
+            for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, b4[x]);
+          else
+            for (int x = lane; x < (nsamp >> 2); x += 64) gstore(o4 + x, make_float4(0.f, 0.f, 0.f, 0.f));
+        } else {
+          if (summed)
+            for (int x = lane; x < nsamp; x += 64) gstore(out + x, buf[x]);
+          else
+            for (int x = lane; x < nsamp; x += 64) gstore(out + x, 0.f);
+        }
+        if (summed) carry[r] = buf[64 * nstep];
+      };
+      if (total <= (uint32_t)kGatherCap) {  // uniform: every listed cell loaded at once
+  #pragma unroll
+        for (int r = 0; r < R; ++r)
+          list_cells(cells, wl[r], pl[r], off[r], 0u, total, lane_meta | ((uint32_t)r << 14));
+        wave_lds_sync();
+        uint2 d[kGatherLoads];
+        float2 v[kGatherLoads];
+  #pragma unroll
+        for (int u = 0; u < kGatherLoads; ++u) {
+          d[u] = make_uint2(0u, 0u);
+          v[u] = make_float2(0.f, 0.f);
+          if (64u * u < total) {  // uniform
+            d[u] = cells[min(64u * u + lane, total - 1u)];
+            v[u] = cell_pair(contrib, d[u]);
           }
-          wave_lds_sync();
         }
-        store_row(r, summed);
+  #pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r >= nrow) break;  // uniform
+          const bool summed = cnt[r] != 0u || carry[r] != 0.f;
+          if (summed) {
+            wave_lds_sync();
+            init_row(r);
+  #pragma unroll
+            for (int u = 0; u < kGatherLoads; ++u)
+              if (64u * u < total) add_cells(buf, d[u], v[u], 64u * u + lane < total, r);
+            wave_lds_sync();
+          }
+          store_row(r, summed);
+        }
+      } else {  // a dense step: one row at a time, in passes of kGatherCap cells
+  #pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r >= nrow) break;  // uniform
+          const bool summed = cnt[r] != 0u || carry[r] != 0.f;
+          if (summed) {
+            wave_lds_sync();
+            init_row(r);
+            for (uint32_t lo = base[r]; lo < base[r] + cnt[r]; lo += kGatherCap) {
+              const uint32_t hi = min(base[r] + cnt[r], lo + (uint32_t)kGatherCap);
+              wave_lds_sync();
+              list_cells(cells, wl[r], pl[r], off[r], lo, hi, lane_meta | ((uint32_t)r << 14));
+              wave_lds_sync();
+              for (uint32_t k0 = 0; k0 < hi - lo; k0 += 64) {
+                const uint2 d = cells[min(k0 + lane, hi - lo - 1u)];
+                add_cells(buf, d, cell_pair(contrib, d), k0 + lane < hi - lo, r);
+              }
+            }
+            wave_lds_sync();
+          }
+          store_row(r, summed);
+        }
       }
     }
   }

@@ -146,6 +146,7 @@ struct dc_hip_context {
   int mark_rows = 16;  // default sign-pass tile: 16 sample rows x 4 waves per block (fastest at 512^3)
   int mark_waves = 4;
   int gather_rows = 1;
+  int64_t gather_blocks = 0;  // option "gather_blocks"
   int eff_gather_rows = 1;  // the value the last forward's row groups were laid out with
   int gather_lds_row = 128;
   bool use_graph = true;
@@ -855,6 +856,9 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
     if (value != 1 && value != 2 && value != 4)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_waves must be 1, 2 or 4");
     c->mark_waves = (int)value;
+  } else if (k == "gather_blocks") {
+    if (value < 0) return fail(c, DC_HIP_ERR_INVALID_ARG, "gather_blocks must be >= 0");
+    c->gather_blocks = value;
   } else if (k == "gather_rows") {
     if (value != 1 && value != 2 && value != 4 && value != 8)
       return fail(c, DC_HIP_ERR_INVALID_ARG, "gather_rows must be 1, 2, 4 or 8");
@@ -1242,6 +1246,7 @@ int dc_hip_backward(dc_hip_context* c, const float* grad_vertices, float* grad_v
   a.sr_bounds = d_tile_b + 4 * (n + 1);
   a.n_sr = c->n_sr;
   a.gather_rows = c->eff_gather_rows;
+  a.gather_blocks = c->gather_blocks;
   a.gather_lds_row = c->gather_lds_row;
   a.act = as<uint64_t>(c->act);
   a.wprefix = as<uint32_t>(c->wprefix);
