@@ -135,6 +135,7 @@ struct DcBwdArgs {
   int64_t n_fallback;       // FP32 mode's FP64 cells (host copy)
   int lanes1, lanes2;       // lanes per active cell in the phase 1 / phase 2 kernels: 1, 4 or 16
   int lanes64;              // lanes per FP64 cell of FP32 mode: 1 or 16
+  int64_t gather_blocks;    // blocks of the gradient gather (0: one wave per row group)
   const float* grad_vertices;
   const uint64_t* ft;
   const uint32_t* pw;
