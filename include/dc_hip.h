@@ -85,7 +85,9 @@ void dc_hip_destroy(dc_hip_context* ctx);
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_rows" (4, 8, 16, 32, default 16: sample rows per wave tile of the marking sign
  * pass), "mark_waves" (1, 2, 4, default 4: waves per marking block), "gather_rows" (1, 2, 4, 8:
- * sample rows per wave of the adjoint's gradient gather; applies from the next forward), "timing" (0/1, default 1: per-pass
+ * sample rows per wave of the adjoint's gradient gather; applies from the next forward),
+ * "gather_blocks" (0, the default: one wave per row group of the gather; N > 0: at most N blocks
+ * of 4 waves, each looping over row groups; the same gradient bit for bit), "timing" (0/1, default 1: per-pass
  * event timings in the stats), "graph" (0/1, default 1: with timing off, a forward plan that runs
  * twice in a row is captured as a hipGraph and replayed while its grids, sizes, pointers,
  * precision, options and stream stay the same), "qef_blocks" (persistent blocks of the solve),

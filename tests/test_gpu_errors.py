@@ -78,7 +78,8 @@ def test_invalid_arguments_and_options():
     assert lib.dc_hip_extract(ctx, ctypes.byref(grid), 7, ctypes.byref(st)) == \
         _lib.DC_HIP_ERR_INVALID_ARG  # unknown precision
     for key, value in ((b"mark_rows", 12), (b"mark_layers", 8), (b"mark_waves", 3),
-                       (b"gather_rows", 3), (b"qef_blocks", -1), (b"fallback_state", -2),
+                       (b"gather_rows", 3), (b"gather_blocks", -1), (b"qef_blocks", -1),
+                       (b"fallback_state", -2),
                        (b"adjoint_lanes", 2), (b"adjoint_fallback_lanes", 4),
                        (b"no_such_option", 1)):
         assert lib.dc_hip_set_option(ctx, key, value) == _lib.DC_HIP_ERR_INVALID_ARG, key
