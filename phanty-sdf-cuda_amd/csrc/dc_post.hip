@@ -26,14 +26,15 @@
 #include <vector>
 
 #include "dc_common.h"
+#include "dc_flagscan.h"
+#include "dc_unionfind.h"
 
 namespace {
 
 using dccommon::wave_scan_incl;
+using namespace dcscan;
+using namespace dcuf;
 
-constexpr int kThreads = 256;
-constexpr int kPer = 8;                        // flags per thread of a scan tile
-constexpr int64_t kTile = kThreads * kPer;     // flags per scan tile
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr unsigned long long kEmptyKey = ~0ull;
 constexpr int64_t kMaxFaces = int64_t(1) << 28;     // 4 * faces positions fit in 31 bits
@@ -51,75 +52,6 @@ size_t pow2_at_least(size_t n) {
   size_t p = 1024;
   while (p < n) p <<= 1;
   return p;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Exclusive scan of uint32 flags: tile sums, a one-block scan of the tile sums, then each tile
-// scanned again with its base.
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t block_excl(uint32_t x, uint32_t* s_w, uint32_t* total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t incl = wave_scan_incl(x, lane);
-  if (lane == 63) s_w[w] = incl;
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-#pragma unroll
-  for (int q = 0; q < kThreads / 64; ++q) {
-    if (q < w) base += s_w[q];
-    tot += s_w[q];
-  }
-  __syncthreads();
-  *total = tot;
-  return base + incl - x;
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_scan_sum(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ tsum) {
-  __shared__ uint32_t s_w[kThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x * kPer;
-  uint32_t x = 0;
-#pragma unroll
-  for (int k = 0; k < kPer; ++k)
-    if (base + k < n) x += in[base + k];
-  uint32_t tot;
-  block_excl(x, s_w, &tot);
-  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_scan_tiles(uint32_t* __restrict__ tsum, int64_t nt, uint32_t* __restrict__ total) {
-  __shared__ uint32_t s_w[kThreads / 64];
-  uint32_t carry = 0;
-  for (int64_t off = 0; off < nt; off += kThreads) {
-    const int64_t q = off + threadIdx.x;
-    const uint32_t x = q < nt ? tsum[q] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_excl(x, s_w, &tot);
-    if (q < nt) tsum[q] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_scan_apply(const uint32_t* __restrict__ in, int64_t n, const uint32_t* __restrict__ tbase,
-                 uint32_t* __restrict__ out) {
-  __shared__ uint32_t s_w[kThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x * kPer;
-  uint32_t v[kPer], x = 0;
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0u;
-    x += v[k];
-  }
-  uint32_t tot;
-  uint32_t run = tbase[blockIdx.x] + block_excl(x, s_w, &tot);
-#pragma unroll
-  for (int k = 0; k < kPer; ++k)
-    if (base + k < n) {
-      out[base + k] = run;
-      run += v[k];
-    }
 }
 
 __global__ void k_fill32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
@@ -366,42 +298,6 @@ __global__ void k_edge_classify(uint32_t* __restrict__ eslot, int64_t nslots,
 // ---------------------------------------------------------------------------------------------
 // Components (dc_backend.cpp:313-370): faces joined by edges used exactly twice
 // ---------------------------------------------------------------------------------------------
-// Parent pointers are read and halved with relaxed, wave-scoped (cached) accesses; a stale value
-// is an earlier ancestor, which only costs a retry of the hooking CAS (performed at the L2).
-__device__ __forceinline__ uint32_t ld_rlx(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-}
-__device__ __forceinline__ void st_rlx(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-}
-// parent[x] < x for every non-root x (roots hook under smaller roots, halving moves a pointer to
-// an ancestor), so the walk ends and each root is the lowest face of its tree.
-__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
-  while (true) {
-    const uint32_t p = ld_rlx(parent + x);
-    if (p == x) return x;
-    const uint32_t g = ld_rlx(parent + p);
-    if (g == p) return p;
-    st_rlx(parent + x, g);  // path halving
-    x = g;
-  }
-}
-__device__ __forceinline__ void uf_union(uint32_t* parent, uint32_t a, uint32_t b) {
-  while (true) {
-    a = uf_find(parent, a);
-    b = uf_find(parent, b);
-    if (a == b) return;
-    if (a > b) {
-      const uint32_t t = a;
-      a = b;
-      b = t;
-    }
-    const uint32_t old = atomicCAS(parent + b, b, a);  // hook the larger root under the smaller
-    if (old == b) return;
-    b = old;  // b was hooked meanwhile: continue from its new parent
-  }
-}
-
 // Each face starts under its lowest neighbour across an edge used exactly twice (or itself): the
 // pointers still decrease, so this is a forest of faces in one component each, and its trees are
 // already shallow before the unions.
@@ -813,22 +709,10 @@ static T* at(void* p) {
 }
 
 int DcPost::scan(const uint32_t* flags, int64_t n, uint32_t* out, int slot) {
-  const int64_t nt = std::max<int64_t>(1, (n + kTile - 1) / kTile);
-  POST_RC(grow(kTSum, (size_t)nt * 4));
-  uint32_t* tsum = at<uint32_t>(b_[kTSum].p);
-  uint32_t* cnt = at<uint32_t>(b_[kCnt].p);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scan_sum, dim3((unsigned)nt), dim3(kThreads), 0, s_, flags, n, tsum);
-    POST_TRY(hipGetLastError(), "scan");
-  }
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s_, tsum, n > 0 ? nt : 0,
-                     cnt + slot);
-  POST_TRY(hipGetLastError(), "scan");
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kThreads), 0, s_, flags, n, tsum,
-                       out);
-    POST_TRY(hipGetLastError(), "scan");
-  }
+  POST_RC(grow(kTSum, (size_t)std::max<int64_t>(1, scan_tiles(n)) * 4));
+  POST_TRY(exclusive_scan(flags, n, out, at<uint32_t>(b_[kTSum].p),
+                          at<uint32_t>(b_[kCnt].p) + slot, s_),
+           "scan");
   return DC_HIP_OK;
 }
 
