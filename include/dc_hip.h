@@ -81,7 +81,8 @@ typedef struct dc_hip_context dc_hip_context;
 int dc_hip_create(int device, dc_hip_context** out);
 void dc_hip_destroy(dc_hip_context* ctx);
 /* Runs all later work on `stream` (a hipStream_t). NULL, the default, is the null stream: each
- * call runs there and returns when its work is complete. */
+ * call runs there and returns when its work is complete, except dc_hip_backward with the
+ * "timing" option off, which returns once its work is enqueued (see there). */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
 /* Options: "mark_rows" (4, 8, 16, 32, default 16: sample rows per wave tile of the marking sign
  * pass), "mark_waves" (1, 2, 4, default 4: waves per marking block), "gather_rows" (1, 2, 4, 8:
@@ -155,9 +156,10 @@ int dc_hip_extract_batch(dc_hip_context* ctx, const dc_hip_grid* grids, int32_t 
  * grad_values[N_s] (device, overwritten) = dL/dvalues for grad_vertices[3*n_vertices]
  * (device) = dL/dvertices. The adjoint reads the last forward's samples again: they must be alive
  * and unchanged, and grad_values must not overlap them (DC_HIP_ERR_INVALID_ARG otherwise). Deterministic: per-cell stencil contributions are gathered in a
- * fixed order, no floating-point atomics. Returns when the gradient is complete, except with
- * the "timing" option off on a stream set by dc_hip_set_stream: then it returns once the work is
- * enqueued on that stream (later work on the stream sees the gradient). */
+ * fixed order, no floating-point atomics. With the "timing" option on (the default) it returns
+ * when the gradient is complete; with it off it returns once the work is enqueued on the
+ * context's stream, the null stream included: later work on that stream sees the gradient, and
+ * a host read must synchronise the stream first. */
 int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad_values,
                     dc_hip_stats* stats);
 
