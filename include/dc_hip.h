@@ -211,6 +211,44 @@ int dc_hip_postprocess_copy(dc_hip_context* ctx, float* vertices, int32_t* faces
 /* Device pointers of the last post-processing result, valid until the next dc_hip_postprocess. */
 int dc_hip_postprocess_device(dc_hip_context* ctx, const float** vertices, const int32_t** faces);
 
+/* Counts and times of one mesh-to-SDF call (dc_hip_mesh_to_sdf). */
+typedef struct dc_hip_m2s_stats {
+  int64_t triangles;    /* non-degenerate triangles (the others have no normal and are skipped) */
+  int64_t bin_entries;  /* (bin, triangle) pairs: each triangle in the bins its grown box covers */
+  int64_t bins;         /* max(1, nx/8) * max(1, ny/8) * max(1, nz/8) */
+  int64_t active_bins;  /* bins with a triangle in their 3x3x3 neighbourhood (evaluated) */
+  int32_t flood_passes; /* launches of the exterior flood (the last one changed nothing) */
+  int32_t voxel_res;    /* the voxelize option's clamped resolution, else 0 */
+  float eval_ms;        /* device time of the distance evaluation (far fill included) */
+  float total_ms;       /* host wall time of the call */
+} dc_hip_m2s_stats;
+
+/* Replaces compute_mesh_sdf_device_cuda / compute_mesh_sdf_cuda (include/mesh_to_sdf.h:6-26,
+ * src/cuda/mesh_to_sdf_cuda.cu:740-1002; bound at src/cpp/bindings.cpp:270-285 and called by
+ * dc_cli's mesh input, src/cpp/dc_cli.cpp:120-203): the signed distance of the triangle mesh
+ * (vertices: 3 * num_vertices floats; faces: 3 * num_faces int32 vertex indices; host or device
+ * memory) sampled at (ox + i*vx, oy + j*vy, oz + k*vz) into values[i + nx*(j + ny*k)] (host or
+ * device memory, nx*ny*nz floats). Samples within 2.5 bins (bin = (n*v)/max(1, n/8) per axis) of
+ * a triangle of their 3x3x3 bin neighbourhood get the signed distance, its sign from the
+ * angle-weighted normals of the nearest triangles; the others get +10 (exterior) or -10. Without
+ * voxelize_first, far samples joined to the grid's border or to a positive sample through far
+ * samples are exterior, and four relaxation passes follow (|f| > 1.5 min(v) takes
+ * min(|f|, |f_neighbour| + v)). With voxelize_first, far samples are -10 inside the solid of a
+ * voxelization at max(32, min(2048, voxel_res))^3 (shell voxels within 0.866 voxel diagonals of
+ * a triangle, grown by one voxel, plus every voxel the border cannot reach) and +10 elsewhere.
+ * Equal to the reference's result for the same input, with two choices where the reference's
+ * result varies from run to run: each bin's triangles are visited in ascending order (the
+ * reference fills bins with atomics), and floods run to their fixpoint (the reference stops its
+ * sample flood after 40 unordered passes). A triangle index outside [0, num_vertices) returns
+ * DC_HIP_ERR_INVALID_ARG, a non-finite vertex DC_HIP_ERR_NONFINITE, nx/ny/nz outside
+ * [1, 4096] or a non-positive spacing DC_HIP_ERR_INVALID_GRID. stats may be NULL. Runs on the
+ * context's stream and returns when values is complete. */
+int dc_hip_mesh_to_sdf(dc_hip_context* ctx, const float* vertices, int64_t num_vertices,
+                       const int32_t* faces, int64_t num_faces, int32_t nx, int32_t ny,
+                       int32_t nz, float ox, float oy, float oz, float vx, float vy, float vz,
+                       int voxelize_first, int32_t voxel_res, float* values,
+                       dc_hip_m2s_stats* stats);
+
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);
 

@@ -48,6 +48,15 @@ def load():
     lib.oracle_post_free.argtypes = [P]
     lib.oracle_weld.restype = P
     lib.oracle_weld.argtypes = [P, i64, P, i64]
+    m2s = [P, i64, P, i64, I, I, I, F, F, F, F, F, F]
+    lib.oracle_mesh_to_sdf.restype = I
+    lib.oracle_mesh_to_sdf.argtypes = m2s + [I, I, P]
+    lib.oracle_mesh_to_sdf_raw.restype = I
+    lib.oracle_mesh_to_sdf_raw.argtypes = m2s + [P]
+    lib.oracle_m2s_voxels.restype = I
+    lib.oracle_m2s_voxels.argtypes = m2s + [I, P]
+    lib.oracle_point_triangle.restype = F
+    lib.oracle_point_triangle.argtypes = [P, P, P, P, P]
     _lib = lib
     return lib
 
@@ -185,3 +194,50 @@ def _post_result(h):
     finally:
         lib.oracle_post_free(h)
     return V2, F2, dict(zip(POST_INFO, (int(x) for x in info)))
+
+
+def _mesh_args(V, F):
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    F = np.ascontiguousarray(F, np.int32).reshape(-1, 3)
+    return V, F
+
+
+def mesh_to_sdf(V, F, dims, origin, spacing, voxelize_first=False, voxel_res=256, raw=False):
+    """The reference's compute_mesh_sdf_device_cuda (src/cuda/mesh_to_sdf_cuda.cu:740-973)
+    restated on the CPU: the sampled SDF of triangle mesh (V, F) as a (nz, ny, nx) float32 array.
+    raw=True: the evaluation alone (FLT_MAX beyond 2.5 bins), before the flood and relaxation."""
+    lib = load()
+    V, F = _mesh_args(V, F)
+    nx, ny, nz = (int(d) for d in dims)
+    out = np.empty((nz, ny, nx), np.float32)
+    args = [_p(V), V.shape[0], _p(F), F.shape[0], nx, ny, nz, *(float(o) for o in origin),
+            *(float(v) for v in spacing)]
+    if raw:
+        rc = lib.oracle_mesh_to_sdf_raw(*args, _p(out))
+    else:
+        rc = lib.oracle_mesh_to_sdf(*args, int(bool(voxelize_first)), int(voxel_res), _p(out))
+    if rc != 0:
+        raise ValueError("oracle mesh_to_sdf: a triangle names a vertex out of range")
+    return out
+
+
+def m2s_voxels(V, F, dims, origin, spacing, voxel_res=256):
+    """The voxelize option's solid voxels as a (vres, vres, vres) uint8 array (z, y, x)."""
+    lib = load()
+    V, F = _mesh_args(V, F)
+    nx, ny, nz = (int(d) for d in dims)
+    args = [_p(V), V.shape[0], _p(F), F.shape[0], nx, ny, nz, *(float(o) for o in origin),
+            *(float(v) for v in spacing), int(voxel_res)]
+    vres = lib.oracle_m2s_voxels(*args, None)
+    solid = np.empty((vres, vres, vres), np.uint8)
+    lib.oracle_m2s_voxels(*args, _p(solid))
+    return solid
+
+
+def point_triangle(p, a, b, c):
+    """(squared distance, closest point) of p to triangle (a, b, c), the oracle's FP32 routine."""
+    lib = load()
+    arrs = [np.ascontiguousarray(x, np.float32).reshape(3) for x in (p, a, b, c)]
+    q = np.empty(3, np.float32)
+    d = lib.oracle_point_triangle(*(_p(x) for x in arrs), _p(q))
+    return float(d), q

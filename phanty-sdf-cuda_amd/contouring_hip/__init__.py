@@ -13,6 +13,8 @@ from .backend import (  # noqa: F401
     DualContouringStats,
     HipDualContouringBackend,
     PRECISIONS,
+    mesh_to_sdf_device_hip,
+    mesh_to_sdf_hip,
     postprocess_mesh,
 )
 from . import synth  # noqa: F401

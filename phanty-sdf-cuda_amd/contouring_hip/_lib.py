@@ -28,7 +28,7 @@ EXPORTS = (
     "dc_hip_result_counts", "dc_hip_copy_result",
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
     "dc_hip_triangulate", "dc_hip_postprocess", "dc_hip_postprocess_copy",
-    "dc_hip_postprocess_device",
+    "dc_hip_postprocess_device", "dc_hip_mesh_to_sdf",
     "dc_hip_last_error", "dc_hip_status_string",
 )
 
@@ -61,6 +61,13 @@ class DcHipPostStats(ctypes.Structure):
                 ("added_vertices", ctypes.c_int64), ("added_faces", ctypes.c_int64),
                 ("vertex_count", ctypes.c_int64), ("face_count", ctypes.c_int64),
                 ("walk_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
+class DcHipM2sStats(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_int64), ("bin_entries", ctypes.c_int64),
+                ("bins", ctypes.c_int64), ("active_bins", ctypes.c_int64),
+                ("flood_passes", ctypes.c_int32), ("voxel_res", ctypes.c_int32),
+                ("eval_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
 
 
 _hip = None
@@ -113,6 +120,9 @@ def load_hip():
                                    ctypes.POINTER(I64), ctypes.POINTER(DcHipPostStats)]),
         "dc_hip_postprocess_copy": (I, [P, P, P]),
         "dc_hip_postprocess_device": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "dc_hip_mesh_to_sdf": (I, [P, P, I64, P, I64, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32] + [ctypes.c_float] * 6
+                               + [I, ctypes.c_int32, P, ctypes.POINTER(DcHipM2sStats)]),
         "dc_hip_last_error": (ctypes.c_char_p, [P]),
         "dc_hip_status_string": (ctypes.c_char_p, [I]),
     }

@@ -6,6 +6,8 @@ Reference interface (file:line in the reference tree):
   <Backend>().extract(grid) -> (mesh, stats)              src/cpp/bindings.cpp:186-222
   DenseSdfGridDevice, extract_device(grid)                include/dc_cuda.h:24-35,
                                                           src/cpp/bindings.cpp:204-227
+  mesh_to_sdf_cuda / mesh_to_sdf_device_cuda(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy,
+  vz) -> (grid, ms)                                       src/cpp/bindings.cpp:270-284
 HipDualContouringBackend takes the place of CudaDualContouringBackend. There is no CPU fallback:
 without the HIP library or a device every call raises.
 """
@@ -383,6 +385,39 @@ class HipDualContouringBackend:
                 self._ctx, Vo.ctypes.data_as(ctypes.c_void_p), Fo.ctypes.data_as(ctypes.c_void_p)))
         return DualContouringMesh(Vo, Fo), stats
 
+    def mesh_to_sdf(self, vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz,
+                    voxelize_first=False, voxel_res=256, out=None):
+        """compute_mesh_sdf_cuda (src/cuda/mesh_to_sdf_cuda.cu:740-1002) on this backend's GPU:
+        the signed distance of the triangle mesh sampled at (ox + i*vx, oy + j*vy, oz + k*vz).
+        vertices: (n, 3) float32 and faces: (m, 3) int32, NumPy arrays (or sequences) or CUDA
+        tensors. out: None for a new NumPy (nz, ny, nx) array, "device" for a new CUDA tensor of
+        that shape, or a contiguous float32 CUDA tensor of nx*ny*nz elements written in place.
+        Returns (values, stats dict); equal to the reference's result for the same input (each
+        bin's triangles in ascending order, floods to their fixpoint)."""
+        vp, n_v, vkeep = _mesh_array(vertices, "vertices", 3, "float32", self.device)
+        fp, n_f, fkeep = _mesh_array(faces, "faces", 3, "int32", self.device)
+        n = int(nx) * int(ny) * int(nz)
+        if out is None:
+            vals = np.empty((int(nz), int(ny), int(nx)), np.float32)
+            optr = vals.ctypes.data_as(ctypes.c_void_p)
+        else:
+            import torch
+            if isinstance(out, str) and out == "device":
+                vals = torch.empty((int(nz), int(ny), int(nx)), dtype=torch.float32,
+                                   device=torch.device("cuda", self.device))
+            else:
+                vals = out
+            _check_tensor(vals, "out", n, self.device)
+            optr = ctypes.c_void_p(vals.data_ptr())
+        if out is not None or not isinstance(vkeep, np.ndarray) or not isinstance(fkeep, np.ndarray):
+            self._use_torch_stream()  # device buffers: ordered after the caller's work
+        st = _lib.DcHipM2sStats()
+        self._check(self._lib.dc_hip_mesh_to_sdf(
+            self._ctx, vp, n_v, fp, n_f, int(nx), int(ny), int(nz), float(ox), float(oy),
+            float(oz), float(vx), float(vy), float(vz), int(bool(voxelize_first)),
+            int(voxel_res), optr, ctypes.byref(st)))
+        return vals, {f: getattr(st, f) for f, _ in _lib.DcHipM2sStats._fields_}
+
     def backward(self, grad_vertices, grad_phi):
         """dL/dphi (written into grad_phi: a contiguous float32 CUDA tensor with one element per
         sample of the last forward, all grids' samples concatenated after a batch) for dL/dV
@@ -416,3 +451,34 @@ def postprocess_mesh(mesh, close_holes=True, remove_floaters=True):
         _post_backend = HipDualContouringBackend()
     out, _ = _post_backend.postprocess(mesh, close_holes, remove_floaters)
     return out
+
+
+_m2s_backend = None
+
+
+def _m2s(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz, voxelize_first, voxel_res, out):
+    global _m2s_backend
+    if _m2s_backend is None:
+        _m2s_backend = HipDualContouringBackend()
+    vals, st = _m2s_backend.mesh_to_sdf(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz,
+                                        voxelize_first, voxel_res, out)
+    return vals, st["total_ms"]
+
+
+def mesh_to_sdf_hip(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz, voxelize_first=False,
+                    voxel_res=256):
+    """The reference module's mesh_to_sdf_cuda (src/cpp/bindings.cpp:270-276) on the GPU (device
+    0): (DenseSdfGrid with NumPy values, ms)."""
+    vals, ms = _m2s(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz, voxelize_first,
+                    voxel_res, None)
+    return DenseSdfGrid(nx, ny, nz, vx, vy, vz, ox, oy, oz, vals.reshape(-1)), ms
+
+
+def mesh_to_sdf_device_hip(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz,
+                           voxelize_first=False, voxel_res=256):
+    """The reference module's mesh_to_sdf_device_cuda (src/cpp/bindings.cpp:278-284): the grid
+    stays on the GPU as a DenseSdfGridDevice whose d_values is a CUDA tensor, ready for
+    extract_device; (grid, ms)."""
+    vals, ms = _m2s(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz, voxelize_first,
+                    voxel_res, "device")
+    return DenseSdfGridDevice(nx, ny, nz, vx, vy, vz, ox, oy, oz, d_values=vals.reshape(-1)), ms

@@ -201,3 +201,100 @@ def batch_specs(count=256, seed=777):
 def batch_grids(count=256, seed=777):
     return [unit_grid(n, _eval(n, prims, ops), f"batch{q}_{n}")
             for q, (n, prims, ops) in enumerate(batch_specs(count, seed))]
+
+
+# ---------------------------------------------------------------------------------------------
+# Triangle meshes (inputs of mesh_to_sdf): vertices (n, 3) float32, triangles (m, 3) int32,
+# counter-clockwise seen from outside (outward normals by the right-hand rule).
+# ---------------------------------------------------------------------------------------------
+def icosphere(level=3, radius=0.35, center=(0.5, 0.5, 0.5)):
+    """Subdivided icosahedron (level 0: 20 triangles, each level x4), vertices projected to the
+    sphere in FP64 and rounded to FP32."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, np.float64) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                mid[key] = len(V) - 1
+            return mid[key]
+        F2 = []
+        for a, b, c in F:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            F2 += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = F2
+    P = np.array(V) * radius + np.array(center, np.float64)
+    return P.astype(np.float32), np.array(F, np.int32)
+
+
+def box_mesh(lo=(0.3, 0.3, 0.3), hi=(0.7, 0.7, 0.7)):
+    """Axis-aligned box as 12 triangles (two per face, of equal area)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    V = np.array([(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0),
+                  (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)], np.float32)
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (2, 3, 7, 6), (1, 2, 6, 5), (0, 4, 7, 3)]
+    F = []
+    for a, b, c, d in quads:
+        F += [(a, b, c), (a, c, d)]
+    return V, np.array(F, np.int32)
+
+
+def torus_mesh(R=0.25, r=0.1, nu=48, nv=24, center=(0.5, 0.5, 0.5)):
+    """Torus around the z axis (major radius R, minor r) on an nu x nv parameter grid."""
+    u = np.arange(nu) * (2 * math.pi / nu)
+    v = np.arange(nv) * (2 * math.pi / nv)
+    U, W = np.meshgrid(u, v, indexing="ij")
+    X = (R + r * np.cos(W)) * np.cos(U) + center[0]
+    Y = (R + r * np.cos(W)) * np.sin(U) + center[1]
+    Z = r * np.sin(W) + center[2]
+    V = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32)
+    F = []
+    for i in range(nu):
+        for j in range(nv):
+            a = i * nv + j
+            b = ((i + 1) % nu) * nv + j
+            c = ((i + 1) % nu) * nv + (j + 1) % nv
+            d = i * nv + (j + 1) % nv
+            F += [(a, b, c), (a, c, d)]
+    return V, np.array(F, np.int32)
+
+
+def hollow_box_mesh(lo=(0.2, 0.2, 0.2), hi=(0.8, 0.8, 0.8), wall=0.2):
+    """A box with a closed inner cavity: the outer box and an inner box (inset by `wall`) with
+    its triangles reversed (normals into the cavity)."""
+    V1, F1 = box_mesh(lo, hi)
+    V2, F2 = box_mesh(tuple(a + wall for a in lo), tuple(b - wall for b in hi))
+    F2 = F2[:, ::-1] + len(V1)
+    return np.concatenate([V1, V2]), np.ascontiguousarray(np.concatenate([F1, F2]), np.int32)
+
+
+def random_soup(count=200, seed=99, size=0.08):
+    """Seeded random triangles (not a closed surface) in [0.2, 0.8]^3 (SplitMix64, in order:
+    centre xyz, then three corner offsets per triangle)."""
+    rng = SplitMix64(seed)
+    V = []
+    for _ in range(count):
+        c = [rng.uniform(0.2, 0.8) for _ in range(3)]
+        for _ in range(3):
+            V.append([c[d] + rng.uniform(-size, size) for d in range(3)])
+    V = np.array(V, np.float32)
+    return V, np.arange(3 * count, dtype=np.int32).reshape(-1, 3)
+
+
+def quads_to_triangles(faces):
+    """Quads (a, b, c, d) -> triangles (a, b, c), (a, c, d)."""
+    q = np.asarray(faces, np.int32).reshape(-1, 4)
+    t = np.empty((2 * len(q), 3), np.int32)
+    t[0::2] = q[:, [0, 1, 2]]
+    t[1::2] = q[:, [0, 2, 3]]
+    return t

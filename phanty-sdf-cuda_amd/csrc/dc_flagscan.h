@@ -8,7 +8,10 @@
 
 #include "dc_common.h"
 
+// The kernels have internal linkage: every translation unit that includes this header gets its
+// own copies.
 namespace dcscan {
+namespace {
 
 using dccommon::wave_scan_incl;
 
@@ -96,4 +99,5 @@ inline hipError_t exclusive_scan(const uint32_t* in, int64_t n, uint32_t* out, u
   return hipGetLastError();
 }
 
+}  // namespace
 }  // namespace dcscan

@@ -20,6 +20,7 @@
 #include "dc_hip.h"
 #include "dc_internal.h"
 #include "dc_kernels.h"
+#include "dc_m2s.h"
 #include "dc_post.h"
 
 namespace {
@@ -177,6 +178,7 @@ struct dc_hip_context {
   int bwd_lanes64 = 0;                 // option "adjoint_fallback_lanes" (0: auto)
   std::vector<DcGridDev> hgrids;
   DcPost* post = nullptr;  // mesh post-processing state and its last result
+  DcM2s* m2s = nullptr;    // mesh-to-SDF state
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, cw_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_cw = 0,
           n_sg = 0;
@@ -823,6 +825,7 @@ void dc_hip_destroy(dc_hip_context* c) {
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   delete c->post;
+  delete c->m2s;
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -1179,6 +1182,23 @@ int dc_hip_postprocess(dc_hip_context* c, const float* vertices, int64_t n_verti
   if (out_n_vertices) *out_n_vertices = c->post->n_vertices();
   if (out_n_faces) *out_n_faces = c->post->n_faces();
   return DC_HIP_OK;
+}
+
+int dc_hip_mesh_to_sdf(dc_hip_context* c, const float* vertices, int64_t num_vertices,
+                       const int32_t* faces, int64_t num_faces, int32_t nx, int32_t ny,
+                       int32_t nz, float ox, float oy, float oz, float vx, float vy, float vz,
+                       int voxelize_first, int32_t voxel_res, float* values,
+                       dc_hip_m2s_stats* stats) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (!c->m2s) c->m2s = new DcM2s();
+  hipStream_t s;
+  int rc;
+  if ((rc = work_stream(c, &s))) return rc;
+  const int32_t dims[3] = {nx, ny, nz};
+  const float origin[3] = {ox, oy, oz}, spacing[3] = {vx, vy, vz};
+  return c->m2s->run(vertices, num_vertices, faces, num_faces, dims, origin, spacing,
+                     voxelize_first != 0, voxel_res, values, c->cu_count, s, stats, &c->err);
 }
 
 int dc_hip_postprocess_copy(dc_hip_context* c, float* vertices, int32_t* faces) {
