@@ -360,11 +360,46 @@ __global__ void __launch_bounds__(256)
   if (!act[b]) out[x + (int64_t)g.nx * (y + (int64_t)g.ny * z)] = kFar;
 }
 
+// The box test. A triangle can matter to a sample only when its computed squared distance d is
+// within thr (the sample's running bound, or the band) : d <= thr. Its box [lo, hi] lies farther
+// than that from every point of the sample box [a, b] when the squared box distance b2 exceeds
+// lim_sq(thr) = ((sqrt(thr) + slack) * 1.0001)^2: slack (1e-5 of the largest coordinate) covers
+// the rounding of d, the factor that of b2 and of the hardware square root (no square root per
+// test: lim_sq is formed once per bound change).
+__device__ __forceinline__ float lim_sq(float thr, float slack) {
+  const float r = (__builtin_amdgcn_sqrtf(thr) + slack) * 1.0001f;
+  return r * r;
+}
+__device__ __forceinline__ bool beyond(float lox, float loy, float loz, float hix, float hiy,
+                                       float hiz, V3 a, V3 b, float limsq) {
+  const float bx = fmaxf(fmaxf(lox - b.x, a.x - hix), 0.0f);
+  const float by = fmaxf(fmaxf(loy - b.y, a.y - hiy), 0.0f);
+  const float bz = fmaxf(fmaxf(loz - b.z, a.z - hiz), 0.0f);
+  return bx * bx + by * by + bz * bz > limsq;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+// LDS writes of a wave visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int kEvalPer = 2;  // samples per thread and pass (a bin of 8^3 samples in one pass)
 
 struct SampleState {
   V3 p, q, n;
-  float best, bound;
+  float best, bound, limsq;  // bound: running bound on the final minimum; limsq: its box limit
 };
 
 // One triangle for one sample (:282-347), after the box test.
@@ -372,17 +407,15 @@ __device__ __forceinline__ void visit(SampleState& s, const float4* __restrict__
                                       float band, float slack) {
   // the box test: skip a triangle that cannot be within the tie tolerance of the final minimum
   const float4 r4 = r[4], r5 = r[5];
-  const float bx = fmaxf(fmaxf(r4.x - s.p.x, s.p.x - r4.w), 0.0f);
-  const float by = fmaxf(fmaxf(r4.y - s.p.y, s.p.y - r5.x), 0.0f);
-  const float bz = fmaxf(fmaxf(r4.z - s.p.z, s.p.z - r5.y), 0.0f);
-  const float rb = sqrtf(bx * bx + by * by + bz * bz);
-  const float rl = rb - (1e-4f * rb + slack);
-  if (rl > 0.0f && rl * rl > fminf(s.bound, band)) return;
+  if (beyond(r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, s.p, s.p, s.limsq)) return;
   const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
   const V3 v0 = {r0.x, r0.y, r0.z}, v1 = {r0.w, r1.x, r1.y}, v2 = {r1.z, r1.w, r2.x};
   V3 q;
   const float d = tri_dist_sq(s.p, v0, v1, v2, q);
-  s.bound = fminf(s.bound, (d + 1e-12f) * 1.0001f);
+  if ((d + 1e-12f) * 1.0001f < s.bound) {
+    s.bound = (d + 1e-12f) * 1.0001f;
+    s.limsq = lim_sq(fminf(s.bound, band), slack);
+  }
   float w = r3.x;
   const V3 e0 = sub(q, v0), e1 = sub(q, v1), e2 = sub(q, v2);
   float ang = -1.0f;
@@ -403,6 +436,10 @@ __device__ __forceinline__ void visit(SampleState& s, const float4* __restrict__
   }
 }
 
+// Samples of a bin are taken in 4x4x4 tiles, one tile per wave and pass, so the lanes that must
+// agree before a triangle is skipped lie close together.
+constexpr int kTile = 4;
+
 __global__ void __launch_bounds__(256)
     k_eval(Geom g, Axis ax, const uint32_t* __restrict__ act_list,
            const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ tri_of,
@@ -410,7 +447,11 @@ __global__ void __launch_bounds__(256)
   __shared__ float4 s_rec[256 * 6];
   __shared__ uint32_t s_beg[27], s_pre[28];
   __shared__ int s_nseg;
-  const int t = threadIdx.x;
+  __shared__ uint32_t s_wcnt[4];
+  __shared__ float s_wmax[4];
+  __shared__ uint8_t s_list[4][256];  // per wave: the chunk entries its samples may still need
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const uint32_t b = act_list[blockIdx.x];
   const int cx = (int)(b % (uint32_t)g.gx), cy = (int)((b / (uint32_t)g.gx) % (uint32_t)g.gy),
             cz = (int)(b / ((uint32_t)g.gx * (uint32_t)g.gy));
@@ -438,43 +479,110 @@ __global__ void __launch_bounds__(256)
   const uint32_t total = s_pre[nseg];
   const int x0 = ax.sx[cx], y0 = ax.sy[cy], z0 = ax.sz[cz];
   const int lx = ax.sx[cx + 1] - x0, ly = ax.sy[cy + 1] - y0, lz = ax.sz[cz + 1] - z0;
-  const int nbox = lx * ly * lz;
+  const int tx = (lx + kTile - 1) / kTile, ty = (ly + kTile - 1) / kTile;
+  const int nslots = tx * ty * ((lz + kTile - 1) / kTile) * 64;
   const float band = g.lim_sq_pad;
-  for (int base = 0; base < nbox; base += 256 * kEvalPer) {
+  for (int base = 0; base < nslots; base += 256 * kEvalPer) {
     SampleState S[kEvalPer];
     int64_t idx[kEvalPer];
     bool valid[kEvalPer];
+    V3 lo = {kFar, kFar, kFar}, hi = {-kFar, -kFar, -kFar};  // this pass's samples' box
 #pragma unroll
     for (int k = 0; k < kEvalPer; ++k) {
       const int s = base + k * 256 + t;
-      valid[k] = s < nbox;
-      const int ix = x0 + s % lx, iy = y0 + (s / lx) % ly, iz = z0 + s / (lx * ly);
+      const int tile = s >> 6, l = s & 63;
+      const int ix = x0 + kTile * (tile % tx) + (l & 3);
+      const int iy = y0 + kTile * ((tile / tx) % ty) + ((l >> 2) & 3);
+      const int iz = z0 + kTile * (tile / (tx * ty)) + (l >> 4);
+      valid[k] = s < nslots && ix < x0 + lx && iy < y0 + ly && iz < z0 + lz;
       idx[k] = ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz);
       S[k].p = {g.ox + (float)ix * g.vx, g.oy + (float)iy * g.vy, g.oz + (float)iz * g.vz};
       S[k].q = {0.0f, 0.0f, 0.0f};
       S[k].n = {0.0f, 1.0f, 0.0f};
       S[k].best = kFar;
       S[k].bound = kFar;
+      S[k].limsq = lim_sq(band, g.slack);
     }
+    {  // the pass's sample box: the bin's samples (every pass of a bin covers part of them)
+      const int xl = x0 + lx - 1, yl = y0 + ly - 1, zl = z0 + lz - 1;
+      lo = {g.ox + (float)x0 * g.vx, g.oy + (float)y0 * g.vy, g.oz + (float)z0 * g.vz};
+      hi = {g.ox + (float)xl * g.vx, g.oy + (float)yl * g.vy, g.oz + (float)zl * g.vz};
+    }
+    // this wave's samples' box (empty when none is valid)
+    V3 wlo = {kFar, kFar, kFar}, whi = {-kFar, -kFar, -kFar};
+#pragma unroll
+    for (int k = 0; k < kEvalPer; ++k)
+      if (valid[k]) {
+        wlo = {fminf(wlo.x, S[k].p.x), fminf(wlo.y, S[k].p.y), fminf(wlo.z, S[k].p.z)};
+        whi = {fmaxf(whi.x, S[k].p.x), fmaxf(whi.y, S[k].p.y), fmaxf(whi.z, S[k].p.z)};
+      }
+    wlo = {wave_min(wlo.x), wave_min(wlo.y), wave_min(wlo.z)};
+    whi = {wave_max(whi.x), wave_max(whi.y), wave_max(whi.z)};
+    const bool wave_any = whi.x >= wlo.x;
+    // box limits of the largest threshold of the block's samples so far, and of this wave's
+    float bmax = lim_sq(band, g.slack), wthr = bmax;
     for (uint32_t c0 = 0; c0 < total; c0 += 256) {
+      // the chunk's triangles that some sample of the bin may still need, compacted in order
       const uint32_t e = c0 + t;
+      bool keep = false;
+      uint32_t tri = 0;
+      float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f), r5 = r4;
       if (e < total) {
         int sg = 0;
         for (int q = 1; q < nseg; ++q)
           if (s_pre[q] <= e) sg = q;
-        const uint32_t tri = tri_of[s_beg[sg] + (e - s_pre[sg])];
+        tri = tri_of[s_beg[sg] + (e - s_pre[sg])];
+        r4 = rec[6 * (size_t)tri + 4];
+        r5 = rec[6 * (size_t)tri + 5];
+        keep = !beyond(r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, lo, hi, bmax);
+      }
+      const uint64_t m = __ballot(keep);
+      if (lane == 0) s_wcnt[wv] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t pos = (uint32_t)__popcll(m & below), cnt = 0;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) s_rec[6 * t + j] = rec[6 * (size_t)tri + j];
+      for (int q = 0; q < 4; ++q) {
+        if (q < wv) pos += s_wcnt[q];
+        cnt += s_wcnt[q];
+      }
+      if (keep) {
+        float4* d = s_rec + 6 * pos;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = rec[6 * (size_t)tri + j];
+        d[4] = r4;
+        d[5] = r5;
       }
       __syncthreads();
-      const int cnt = (int)min(256u, total - c0);
-      for (int i = 0; i < cnt; ++i) {
-        const float4* r = s_rec + 6 * i;
+      // this wave's list: the staged triangles within reach of its own samples, in order
+      uint32_t nw = 0;
+      if (wave_any)
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 64) {
+          const uint32_t i = i0 + lane;
+          bool need = false;
+          if (i < cnt) {
+            const float4 a = s_rec[6 * i + 4], c = s_rec[6 * i + 5];
+            need = !beyond(a.x, a.y, a.z, a.w, c.x, c.y, wlo, whi, wthr);
+          }
+          const uint64_t mw = __ballot(need);
+          if (need) s_list[wv][nw + (uint32_t)__popcll(mw & below)] = (uint8_t)i;
+          nw += (uint32_t)__popcll(mw);
+        }
+      wave_lds_sync();
+      for (uint32_t j = 0; j < nw; ++j) {
+        const float4* r = s_rec + 6 * (uint32_t)s_list[wv][j];
 #pragma unroll
         for (int k = 0; k < kEvalPer; ++k)
           if (valid[k]) visit(S[k], r, g.eps_sq, band, g.slack);
       }
+      float mine = 0.0f;  // (limsq grows with the bound)
+#pragma unroll
+      for (int k = 0; k < kEvalPer; ++k)
+        if (valid[k]) mine = fmaxf(mine, S[k].limsq);
+      mine = wave_max(mine);
+      wthr = mine;
+      if (lane == 0) s_wmax[wv] = mine;
       __syncthreads();
+      bmax = fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]));
     }
 #pragma unroll
     for (int k = 0; k < kEvalPer; ++k) {
