@@ -55,6 +55,8 @@ def load():
     lib.oracle_mesh_to_sdf_raw.argtypes = m2s + [P]
     lib.oracle_m2s_voxels.restype = I
     lib.oracle_m2s_voxels.argtypes = m2s + [I, P]
+    lib.oracle_m2s_sample_state.restype = None
+    lib.oracle_m2s_sample_state.argtypes = m2s + [I, I, I, P]
     lib.oracle_point_triangle.restype = F
     lib.oracle_point_triangle.argtypes = [P, P, P, P, P]
     _lib = lib
@@ -241,3 +243,14 @@ def point_triangle(p, a, b, c):
     q = np.empty(3, np.float32)
     d = lib.oracle_point_triangle(*(_p(x) for x in arrs), _p(q))
     return float(d), q
+
+
+def m2s_sample_state(V, F, dims, origin, spacing, ix, iy, iz):
+    """One sample's final (squared distance, closest point, summed weighted normal)."""
+    lib = load()
+    V, F = _mesh_args(V, F)
+    out = np.empty(7, np.float32)
+    lib.oracle_m2s_sample_state(_p(V), V.shape[0], _p(F), F.shape[0], *(int(d) for d in dims),
+                                *(float(o) for o in origin), *(float(v) for v in spacing),
+                                int(ix), int(iy), int(iz), _p(out))
+    return float(out[0]), out[1:4].copy(), out[4:7].copy()

@@ -182,8 +182,9 @@ void build_bins(const Input& in, Bins& B) {
   }
 }
 
-// The signed distance of one sample from the triangles of its 3x3x3 bins, or FLT_MAX.
-float eval_sample(const Input& in, const Bins& B, int ix, int iy, int iz) {
+// The signed distance of one sample from the triangles of its 3x3x3 bins, or FLT_MAX. state
+// (unless null) receives the final squared distance, closest point and summed normal.
+float eval_sample(const Input& in, const Bins& B, int ix, int iy, int iz, float* state = nullptr) {
   const V3 p = {in.ox + (float)ix * in.vx, in.oy + (float)iy * in.vy, in.oz + (float)iz * in.vz};
   const int cx = clamp_index((p.x - in.ox) / B.bdx, B.gx);
   const int cy = clamp_index((p.y - in.oy) / B.bdy, B.gy);
@@ -231,6 +232,10 @@ float eval_sample(const Input& in, const Bins& B, int ix, int iy, int iz) {
     for (int by = std::max(0, cy - 1); by <= std::min(B.gy - 1, cy + 1); ++by)
       for (int bx = std::max(0, cx - 1); bx <= std::min(B.gx - 1, cx + 1); ++bx)
         if (bx != cx || by != cy || bz != cz) visit(bx, by, bz);
+  if (state) {
+    const float st[7] = {best, best_q.x, best_q.y, best_q.z, best_n.x, best_n.y, best_n.z};
+    std::memcpy(state, st, sizeof(st));
+  }
   if (best == kFar) return kFar;
   float dist = std::sqrt(best);
   const float nl = std::sqrt(dot(best_n, best_n));
@@ -455,6 +460,17 @@ int oracle_m2s_voxels(const float* V, int64_t nv, const int32_t* F, int64_t nf, 
   voxelize(in, G.vres, G.vx, G.vy, G.vz, solid);
   std::memcpy(solid_out, solid.data(), solid.size());
   return G.vres;
+}
+
+// One sample's evaluation state: out[0] squared distance, out[1..3] closest point, out[4..6] the
+// summed weighted normal (before normalisation).
+void oracle_m2s_sample_state(const float* V, int64_t nv, const int32_t* F, int64_t nf, int nx,
+                             int ny, int nz, float ox, float oy, float oz, float vx, float vy,
+                             float vz, int ix, int iy, int iz, float* out) {
+  const Input in{V, nv, F, nf, nx, ny, nz, ox, oy, oz, vx, vy, vz};
+  Bins B;
+  build_bins(in, B);
+  eval_sample(in, B, ix, iy, iz, out);
 }
 
 // Squared distance from p to triangle (a, b, c) and the closest point (3 floats each).

@@ -1,8 +1,9 @@
 """Each plausible mistake in the oracle is caught by a pinning test (CPU).
 
-A copy of the oracle's sources (oracle/dc_oracle.cpp, oracle/post_oracle.cpp) with one mistake
-planted in one of them is built into a temporary library, and the test that names that mistake
-(tests/test_oracle_pinned.py, tests/test_post_oracle_pinned.py) is run against it in a subprocess
+A copy of the oracle's sources (oracle/dc_oracle.cpp, oracle/post_oracle.cpp,
+oracle/m2s_oracle.cpp) with one mistake planted in one of them is built into a temporary library,
+and the test that names that mistake (tests/test_oracle_pinned.py,
+tests/test_post_oracle_pinned.py, tests/test_m2s_oracle_pinned.py) is run against it in a subprocess
 (ORACLE_LIB): it must fail. The unmodified copy, built the same way, must pass all of them (the
 control). Every mutation here fails an assertion (none crashes the mutated library).
 """
@@ -13,9 +14,10 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SOURCES = ("dc_oracle.cpp", "post_oracle.cpp")
+SOURCES = ("dc_oracle.cpp", "post_oracle.cpp", "m2s_oracle.cpp")
 PINNED = os.path.join(ROOT, "tests", "test_oracle_pinned.py")
 POST_PINNED = os.path.join(ROOT, "tests", "test_post_oracle_pinned.py")
+M2S_PINNED = os.path.join(ROOT, "tests", "test_m2s_oracle_pinned.py")
 
 # (name, oracle source, text in it, replacement, test file, test that must fail)
 MUTATIONS = [
@@ -55,6 +57,38 @@ MUTATIONS = [
     ("long_loops_filled", "post_oracle.cpp",
      "if (closed && walk.size() >= 3 && walk.size() <= 4096)",
      "if (closed && walk.size() >= 3)", POST_PINNED, "test_loops_longer_than_4096_stay_open"),
+    ("corner_b_region", "m2s_oracle.cpp", "if (d3 >= 0.0f && d4 <= d3) {  // corner b",
+     "if (d3 >= 0.0f && d4 >= d3) {  // corner b", M2S_PINNED,
+     "test_point_triangle_closest_point"),
+    ("normal_flipped", "m2s_oracle.cpp", "if (dot(sub(p, best_q), best_n) < 0.0f) dist = -dist;",
+     "if (dot(sub(p, best_q), best_n) > 0.0f) dist = -dist;", M2S_PINNED,
+     "test_convex_band_distances_and_signs"),
+    ("no_tie_sum", "m2s_oracle.cpp", "} else if (std::fabs(d - best) <= tol) {",
+     "} else if (d == best) {", M2S_PINNED, "test_single_bin_signs_match_ray_parity"),
+    ("angle_weight", "m2s_oracle.cpp", "if (ang >= 0.0f) w = ang;", "if (ang >= 0.0f) w = len;",
+     M2S_PINNED, "test_weighted_normals_in_one_bin"),
+    ("bin_size", "m2s_oracle.cpp", "B.gx = std::max(1, in.nx / 8);",
+     "B.gx = std::max(1, in.nx / 4);", M2S_PINNED, "test_bins_follow_the_documented_rule"),
+    ("margin", "m2s_oracle.cpp", "2.0f * std::fmax(in.vx", "1.0f * std::fmax(in.vx", M2S_PINNED,
+     "test_bins_follow_the_documented_rule"),
+    ("band", "m2s_oracle.cpp", "const float lim = 2.5f * B.min_dim;",
+     "const float lim = 2.0f * B.min_dim;", M2S_PINNED, "test_bins_follow_the_documented_rule"),
+    ("cavity_sign", "m2s_oracle.cpp", "if (out[i] == kFar) out[i] = ext[i] ? 10.0f : -10.0f;",
+     "if (out[i] == kFar) out[i] = 10.0f;", M2S_PINNED, "test_flood_exterior_and_cavity"),
+    ("border_not_seed", "m2s_oracle.cpp",
+     "ext[i] = f > 0.0f && (on_border(x, y, z, nx, ny, nz) || f != kFar);",
+     "ext[i] = f > 0.0f && f != kFar;", M2S_PINNED, "test_flood_exterior_and_cavity"),
+    ("relax_passes", "m2s_oracle.cpp", "for (int pass = 0; pass < 4; ++pass) {",
+     "for (int pass = 0; pass < 2; ++pass) {", M2S_PINNED,
+     "test_relaxation_column_over_a_flat_face"),
+    ("relax_in_place", "m2s_oracle.cpp", "  float* dst = tmp.data();", "  float* dst = out;",
+     M2S_PINNED, "test_relaxation_column_over_a_flat_face"),
+    ("voxel_reach", "m2s_oracle.cpp", "* 0.8660254f;", "* 0.25f;", M2S_PINNED,
+     "test_voxelize_solid"),
+    ("no_dilation", "m2s_oracle.cpp", "        grown[x + (int64_t)vres * (y + (int64_t)vres * z)] = any;",
+     "        grown[x + (int64_t)vres * (y + (int64_t)vres * z)] =\n"
+     "            shell[x + (int64_t)vres * (y + (int64_t)vres * z)];", M2S_PINNED,
+     "test_voxelize_growth_seals_a_small_hole"),
 ]
 
 
@@ -85,7 +119,7 @@ def _run_pinned(lib, target):
 
 def test_unmodified_copy_passes(tmp_path):
     lib = _build(tmp_path, "control")
-    for target in (PINNED, POST_PINNED):
+    for target in (PINNED, POST_PINNED, M2S_PINNED):
         r = _run_pinned(lib, target)
         assert r.returncode == 0, r.stdout[-2000:]
 
