@@ -249,6 +249,13 @@ int dc_hip_mesh_to_sdf(dc_hip_context* ctx, const float* vertices, int64_t num_v
                        int voxelize_first, int32_t voxel_res, float* values,
                        dc_hip_m2s_stats* stats);
 
+/* Device memory for results that leave the library, such as a mesh_to_sdf grid kept on the
+ * device (the reference's compute_mesh_sdf_device_cuda returns cudaMalloc'd values,
+ * src/cuda/mesh_to_sdf_cuda.cu:759-762, freed by free_mesh_sdf_device_cuda, :1004-1009):
+ * allocated on the context's device, released with dc_hip_free. */
+int dc_hip_malloc(dc_hip_context* ctx, int64_t bytes, void** ptr);
+int dc_hip_free(void* ptr);
+
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);
 

@@ -54,8 +54,21 @@ class HipDualContouringBackend : public IDualContouringBackend {
                                  bool remove_floaters);
   // The same on the last extraction's mesh, without copying it to the host first (single grid).
   DualContouringMesh postprocess_last(bool close_holes, bool remove_floaters);
+  // compute_mesh_sdf_cuda / compute_mesh_sdf_device_cuda (include/mesh_to_sdf.h:6-26) on this
+  // backend's GPU: the triangle mesh's signed distance sampled on the grid, equal to the
+  // reference's result (dc_hip_mesh_to_sdf). The device form writes into `values`, a device
+  // buffer of nx*ny*nz floats the caller owns.
+  DenseSdfGrid mesh_to_sdf(const float* vertices, int num_vertices, const int* faces,
+                           int num_faces, int nx, int ny, int nz, float ox, float oy, float oz,
+                           float vx, float vy, float vz, float* out_ms = nullptr,
+                           bool voxelize_first = false, int voxel_res = 256);
+  void mesh_to_sdf_device(const float* vertices, int num_vertices, const int* faces,
+                          int num_faces, int nx, int ny, int nz, float ox, float oy, float oz,
+                          float vx, float vy, float vz, float* values, float* out_ms = nullptr,
+                          bool voxelize_first = false, int voxel_res = 256);
   void set_option(const std::string& key, int64_t value);
   int precision() const { return precision_; }
+  dc_hip_context* context() { return ctx_; }
 
  private:
   DualContouringMesh post_copy(int64_t nv, int64_t nf);
@@ -64,3 +77,19 @@ class HipDualContouringBackend : public IDualContouringBackend {
   dc_hip_context* ctx_ = nullptr;
   int precision_;
 };
+
+// include/mesh_to_sdf.h:6-28 with the backend's name: the mesh's signed distance grid on the host
+// (compute_mesh_sdf_hip), or on the device (compute_mesh_sdf_device_hip: d_values allocated with
+// dc_hip_malloc, released by free_mesh_sdf_device_hip). out_ms receives the call's wall time.
+// They run on device 0 through one process-wide context (calls are serialised). Throw
+// std::runtime_error on failure.
+DenseSdfGrid compute_mesh_sdf_hip(const float* vertices, int num_vertices, const int* faces,
+                                  int num_faces, int nx, int ny, int nz, float ox, float oy,
+                                  float oz, float vx, float vy, float vz, float* out_ms = nullptr,
+                                  bool voxelize_first = false, int voxel_res = 256);
+DenseSdfGridDevice compute_mesh_sdf_device_hip(const float* vertices, int num_vertices,
+                                               const int* faces, int num_faces, int nx, int ny,
+                                               int nz, float ox, float oy, float oz, float vx,
+                                               float vy, float vz, float* out_ms = nullptr,
+                                               bool voxelize_first = false, int voxel_res = 256);
+void free_mesh_sdf_device_hip(DenseSdfGridDevice& device_grid);

@@ -10,12 +10,16 @@
 // The class mirrors the reference's CudaDualContouringBackend binding (bindings.cpp:216-228):
 // extract(grid) and extract_device(grid) return (mesh, stats); postprocess(mesh, close_holes,
 // remove_floaters) is the reference's postprocess_mesh (bindings.cpp:145-146) on the GPU, and
-// postprocess_last the same on the last extraction's mesh in device memory. DenseSdfGridDevice is registered
+// postprocess_last the same on the last extraction's mesh in device memory. The module functions
+// mesh_to_sdf_hip / mesh_to_sdf_device_hip mirror mesh_to_sdf_cuda / mesh_to_sdf_device_cuda
+// (bindings.cpp:270-284): (vertices (n, 3) float32, faces (m, 3) int32, nx, ny, nz, ox, oy, oz,
+// vx, vy, vz) -> (grid, ms). DenseSdfGridDevice is registered
 // here only when the build has no CUDA backends (with DC_ENABLE_CUDA the reference registers it,
 // bindings.cpp:195-207); this registration adds d_values as an integer device address, so a
 // torch tensor's data_ptr() can be passed.
 #pragma once
 
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -66,4 +70,23 @@ inline void bind_hip_backend(pybind11::module_& m) {
       .def("last_triangles", &HipDualContouringBackend::last_triangles)
       .def("set_option", &HipDualContouringBackend::set_option)
       .def_property_readonly("precision", &HipDualContouringBackend::precision);
+  using Floats = py::array_t<float, py::array::c_style | py::array::forcecast>;
+  using Ints = py::array_t<int, py::array::c_style | py::array::forcecast>;
+  m.def("mesh_to_sdf_hip", [](Floats vertices, Ints faces, int nx, int ny, int nz, float ox,
+                              float oy, float oz, float vx, float vy, float vz) {
+    float ms = 0.0f;
+    DenseSdfGrid grid = compute_mesh_sdf_hip(vertices.data(), (int)(vertices.size() / 3),
+                                             faces.data(), (int)(faces.size() / 3), nx, ny, nz,
+                                             ox, oy, oz, vx, vy, vz, &ms);
+    return py::make_tuple(grid, ms);
+  });
+  m.def("mesh_to_sdf_device_hip", [](Floats vertices, Ints faces, int nx, int ny, int nz,
+                                     float ox, float oy, float oz, float vx, float vy, float vz) {
+    float ms = 0.0f;
+    DenseSdfGridDevice grid = compute_mesh_sdf_device_hip(
+        vertices.data(), (int)(vertices.size() / 3), faces.data(), (int)(faces.size() / 3), nx,
+        ny, nz, ox, oy, oz, vx, vy, vz, &ms);
+    return py::make_tuple(grid, ms);
+  });
+  m.def("free_mesh_sdf_device_hip", [](DenseSdfGridDevice& grid) { free_mesh_sdf_device_hip(grid); });
 }

@@ -28,7 +28,7 @@ EXPORTS = (
     "dc_hip_result_counts", "dc_hip_copy_result",
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
     "dc_hip_triangulate", "dc_hip_postprocess", "dc_hip_postprocess_copy",
-    "dc_hip_postprocess_device", "dc_hip_mesh_to_sdf",
+    "dc_hip_postprocess_device", "dc_hip_mesh_to_sdf", "dc_hip_malloc", "dc_hip_free",
     "dc_hip_last_error", "dc_hip_status_string",
 )
 
@@ -123,6 +123,8 @@ def load_hip():
         "dc_hip_mesh_to_sdf": (I, [P, P, I64, P, I64, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_int32] + [ctypes.c_float] * 6
                                + [I, ctypes.c_int32, P, ctypes.POINTER(DcHipM2sStats)]),
+        "dc_hip_malloc": (I, [P, I64, ctypes.POINTER(P)]),
+        "dc_hip_free": (I, [P]),
         "dc_hip_last_error": (ctypes.c_char_p, [P]),
         "dc_hip_status_string": (ctypes.c_char_p, [I]),
     }

@@ -1,13 +1,17 @@
 // dc_cli.cpp — command-line front end with the reference's flags (src/cpp/dc_cli.cpp:205-380),
 // running the MI355X backend: -b hip (also the default). Input: a .sdf file
-// (int32 nx,ny,nz; float vx,vy,vz; float ox,oy,oz; float values, x fastest; dc_cli.cpp:31-47)
-// or --generate {sphere,box,plane,sphere-unit} (--save-sdf writes the grid in the same layout);
+// (int32 nx,ny,nz; float vx,vy,vz; float ox,oy,oz; float values, x fastest; dc_cli.cpp:31-47),
+// a triangle mesh .obj sampled into a res^3 grid on the GPU (:120-203; --voxelize-first,
+// --voxel-res), or --generate {sphere,box,plane,sphere-unit} (--save-sdf writes the grid in the
+// .sdf layout);
 // --close-holes / --remove-floaters post-process the mesh on the GPU (:334-338); output: OBJ with
 // 1-based quads (:17-29).
 // --benchmark prints the reference's stats block (:342-377) so stdout scrapers keep working.
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <memory>
@@ -70,6 +74,70 @@ bool write_sdf(const std::string& path, const DenseSdfGrid& g) {
   return out.good();
 }
 
+// An OBJ's triangles as the reference's mesh input reads them (dc_cli.cpp:121-155): "v x y z"
+// lines, "f" lines of 1-based (negative: relative) indices up to the first '/', polygons split
+// into a fan from their first vertex.
+bool read_obj_triangles(const std::string& path, std::vector<float>& V, std::vector<int>& F) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.size() >= 2 && line[0] == 'v' && line[1] == ' ') {
+      float x, y, z;
+      if (std::sscanf(line.c_str() + 2, "%f %f %f", &x, &y, &z) == 3) {
+        V.push_back(x);
+        V.push_back(y);
+        V.push_back(z);
+      }
+    } else if (line.size() >= 2 && line[0] == 'f' && line[1] == ' ') {
+      std::vector<int> ids;
+      const char* c = line.c_str() + 2;
+      while (*c) {
+        while (*c == ' ' || *c == '\t') ++c;
+        if (!*c) break;
+        int k = std::atoi(c);
+        if (k != 0) {
+          if (k < 0) k = (int)(V.size() / 3) + k + 1;
+          ids.push_back(k - 1);
+        }
+        while (*c && *c != ' ' && *c != '\t') ++c;
+      }
+      for (size_t q = 1; q + 1 < ids.size(); ++q) {
+        F.push_back(ids[0]);
+        F.push_back(ids[q]);
+        F.push_back(ids[q + 1]);
+      }
+    }
+  }
+  return !V.empty() && !F.empty();
+}
+
+// The grid the reference samples a mesh on (dc_cli.cpp:157-182): the vertices' bounding box grown
+// by 8% of its diagonal (0.1 when the diagonal is 0) on every side, res samples per axis.
+void mesh_grid(const std::vector<float>& V, int res, DenseSdfGrid& g) {
+  float lo[3] = {V[0], V[1], V[2]}, hi[3] = {V[0], V[1], V[2]};
+  for (size_t i = 0; i < V.size() / 3; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], V[3 * i + a]);
+      hi[a] = std::max(hi[a], V[3 * i + a]);
+    }
+  const float diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) +
+                               (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                               (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  const float pad = diag > 0.0f ? diag * 0.08f : 0.1f;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] -= pad;
+    hi[a] += pad;
+  }
+  g.nx = g.ny = g.nz = res;
+  g.ox = lo[0];
+  g.oy = lo[1];
+  g.oz = lo[2];
+  g.vx = (hi[0] - lo[0]) / (res - 1);
+  g.vy = (hi[1] - lo[1]) / (res - 1);
+  g.vz = (hi[2] - lo[2]) / (res - 1);
+}
+
 bool generate(const std::string& kind, int res, DenseSdfGrid& g) {
   g.nx = g.ny = g.nz = res;
   g.values.resize((size_t)res * res * res);
@@ -93,8 +161,8 @@ bool generate(const std::string& kind, int res, DenseSdfGrid& g) {
 
 int main(int argc, char* argv[]) {
   std::string input, output, backend_type = "hip", gen, precision = "plain-fp64", save_sdf;
-  bool benchmark = false, close_holes = false, remove_floaters = false;
-  int res = 64;
+  bool benchmark = false, close_holes = false, remove_floaters = false, voxelize_first = false;
+  int res = 64, voxel_res = -1;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> std::string { return (i + 1 < argc) ? argv[++i] : std::string(); };
@@ -108,23 +176,26 @@ int main(int argc, char* argv[]) {
     else if (a == "--benchmark") benchmark = true;
     else if (a == "--close-holes") close_holes = true;
     else if (a == "--remove-floaters") remove_floaters = true;
+    else if (a == "-vf" || a == "--voxelize-first") voxelize_first = true;
+    else if (a == "-vr" || a == "--voxel-res") voxel_res = std::stoi(next());
     else if (a == "-oi" || a == "-ii" || a == "-dc" || a == "-mu" || a == "-hu" ||
-             a == "--brick-size" || a == "--chunk-size" || a == "--normal-mode" ||
-             a == "-vr" || a == "--voxel-res")
+             a == "--brick-size" || a == "--chunk-size" || a == "--normal-mode")
       next();  // accepted for compatibility; no effect on the dense backend
-    else if (a == "-vf" || a == "--voxelize-first") {
-    } else if (input.empty()) input = a;
+    else if (input.empty()) input = a;
     else if (output.empty()) output = a;
   }
+  if (voxel_res < 0) voxel_res = res;  // :260-261
+  voxel_res = std::max(32, std::min(2048, voxel_res));
   if (output.empty() && !input.empty()) {
     output = input;
     input.clear();
   }
   if (output.empty()) {
-    std::cerr << "Usage: " << argv[0] << " -i <input.sdf> -o <output.obj> [options]\n"
+    std::cerr << "Usage: " << argv[0] << " -i <input.obj|.sdf> -o <output.obj> [options]\n"
               << "Options: -b/--backend hip, -g/--res/--grid-size <int>, --generate "
                  "<sphere|box|plane|sphere-unit>, --precision <plain-fp64|fp32-fallback>, "
-                 "--close-holes, --remove-floaters, --save-sdf <grid.sdf>, --benchmark\n";
+                 "--close-holes, --remove-floaters, --voxelize-first, --voxel-res <int>, "
+                 "--save-sdf <grid.sdf>, --benchmark\n";
     return 1;
   }
   if (backend_type != "hip") {
@@ -140,19 +211,24 @@ int main(int argc, char* argv[]) {
     return 1;
   }
 
+  const bool is_obj = input.size() >= 4 && (input.compare(input.size() - 4, 4, ".obj") == 0 ||
+                                             input.compare(input.size() - 4, 4, ".OBJ") == 0);
   DenseSdfGrid grid;
+  std::vector<float> mesh_v;
+  std::vector<int> mesh_f;
   if (!gen.empty()) {
     if (res < 2 || !generate(gen, res, grid)) {
       std::cerr << "Unknown generator type or bad resolution: " << gen << "\n";
       return 1;
     }
+  } else if (is_obj) {
+    if (res < 2 || !read_obj_triangles(input, mesh_v, mesh_f)) {
+      std::cerr << "Failed to convert OBJ to SDF from: " << input << "\n";
+      return 1;
+    }
+    mesh_grid(mesh_v, res, grid);
   } else if (!read_sdf(input, grid)) {
     std::cerr << "Failed to read SDF from: " << input << "\n";
-    return 1;
-  }
-
-  if (!save_sdf.empty() && !write_sdf(save_sdf, grid)) {
-    std::cerr << "Failed to write SDF to: " << save_sdf << "\n";
     return 1;
   }
 
@@ -160,7 +236,34 @@ int main(int argc, char* argv[]) {
   DualContouringMesh mesh;
   try {
     HipDualContouringBackend backend(0, prec);
-    mesh = backend.extract(grid, stats);
+    if (is_obj && !save_sdf.empty()) {  // the grid is written out: sample it to the host
+      grid = backend.mesh_to_sdf(mesh_v.data(), (int)(mesh_v.size() / 3), mesh_f.data(),
+                                 (int)(mesh_f.size() / 3), grid.nx, grid.ny, grid.nz, grid.ox,
+                                 grid.oy, grid.oz, grid.vx, grid.vy, grid.vz, nullptr,
+                                 voxelize_first, voxel_res);
+      mesh = backend.extract(grid, stats);
+    } else if (is_obj) {
+      // the mesh's grid stays on the device from the mesh-to-SDF pass into the extraction
+      void* d = nullptr;
+      const int64_t n = (int64_t)grid.nx * grid.ny * grid.nz;
+      if (dc_hip_malloc(backend.context(), n * 4, &d) != DC_HIP_OK)
+        throw std::runtime_error(std::string("device allocation: ") +
+                                 dc_hip_last_error(backend.context()));
+      std::unique_ptr<void, int (*)(void*)> hold(d, dc_hip_free);
+      backend.mesh_to_sdf_device(mesh_v.data(), (int)(mesh_v.size() / 3), mesh_f.data(),
+                                 (int)(mesh_f.size() / 3), grid.nx, grid.ny, grid.nz, grid.ox,
+                                 grid.oy, grid.oz, grid.vx, grid.vy, grid.vz, static_cast<float*>(d),
+                                 nullptr, voxelize_first, voxel_res);
+      const DenseSdfGridDevice dg{grid.nx, grid.ny, grid.nz, grid.vx, grid.vy,
+                                  grid.vz, grid.ox, grid.oy, grid.oz, static_cast<const float*>(d)};
+      mesh = backend.extract_device(dg, stats);
+    } else {
+      mesh = backend.extract(grid, stats);
+    }
+    if (!save_sdf.empty() && !write_sdf(save_sdf, grid)) {
+      std::cerr << "Failed to write SDF to: " << save_sdf << "\n";
+      return 1;
+    }
     // post-processing after the stats are taken, as the reference does (dc_cli.cpp:334-338)
     if (close_holes || remove_floaters) {
       mesh = backend.postprocess_last(close_holes, remove_floaters);

@@ -1201,6 +1201,25 @@ int dc_hip_mesh_to_sdf(dc_hip_context* c, const float* vertices, int64_t num_ver
                      voxelize_first != 0, voxel_res, values, c->cu_count, s, stats, &c->err);
 }
 
+int dc_hip_malloc(dc_hip_context* c, int64_t bytes, void** ptr) {
+  if (!c || !ptr || bytes < 0) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument or bytes < 0");
+  *ptr = nullptr;
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  const hipError_t e = hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(c, e == hipErrorOutOfMemory ? DC_HIP_ERR_OUT_OF_MEMORY : DC_HIP_ERR_HIP,
+                std::string("device allocation: ") + hipGetErrorString(e));
+  }
+  return DC_HIP_OK;
+}
+
+int dc_hip_free(void* ptr) {
+  if (!ptr) return DC_HIP_OK;
+  return hipFree(ptr) == hipSuccess ? DC_HIP_OK : DC_HIP_ERR_HIP;
+}
+
 int dc_hip_postprocess_copy(dc_hip_context* c, float* vertices, int32_t* faces) {
   if (!c) return DC_HIP_ERR_INVALID_ARG;
   if (!c->post || !c->post->have())

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -123,4 +125,99 @@ DualContouringMesh HipDualContouringBackend::postprocess_last(bool close_holes,
   check(dc_hip_postprocess(ctx_, nullptr, 0, nullptr, 0, close_holes ? 1 : 0,
                            remove_floaters ? 1 : 0, &nv, &nf, nullptr));
   return post_copy(nv, nf);
+}
+
+void HipDualContouringBackend::mesh_to_sdf_device(const float* vertices, int num_vertices,
+                                                  const int* faces, int num_faces, int nx, int ny,
+                                                  int nz, float ox, float oy, float oz, float vx,
+                                                  float vy, float vz, float* values, float* out_ms,
+                                                  bool voxelize_first, int voxel_res) {
+  dc_hip_m2s_stats st{};
+  check(dc_hip_mesh_to_sdf(ctx_, vertices, num_vertices, faces, num_faces, nx, ny, nz, ox, oy, oz,
+                           vx, vy, vz, voxelize_first ? 1 : 0, voxel_res, values, &st));
+  if (out_ms) *out_ms = st.total_ms;
+}
+
+DenseSdfGrid HipDualContouringBackend::mesh_to_sdf(const float* vertices, int num_vertices,
+                                                   const int* faces, int num_faces, int nx, int ny,
+                                                   int nz, float ox, float oy, float oz, float vx,
+                                                   float vy, float vz, float* out_ms,
+                                                   bool voxelize_first, int voxel_res) {
+  DenseSdfGrid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.ox = ox;
+  g.oy = oy;
+  g.oz = oz;
+  g.vx = vx;
+  g.vy = vy;
+  g.vz = vz;
+  if (nx < 1 || ny < 1 || nz < 1)
+    throw std::runtime_error("HipDualContouringBackend: nx, ny and nz must be >= 1");
+  g.values.resize((size_t)nx * ny * nz);
+  mesh_to_sdf_device(vertices, num_vertices, faces, num_faces, nx, ny, nz, ox, oy, oz, vx, vy, vz,
+                     g.values.data(), out_ms, voxelize_first, voxel_res);
+  return g;
+}
+
+namespace {
+std::mutex g_m2s_mutex;
+std::unique_ptr<HipDualContouringBackend> g_m2s_backend;
+
+HipDualContouringBackend& m2s_backend() {
+  if (!g_m2s_backend) g_m2s_backend.reset(new HipDualContouringBackend(0));
+  return *g_m2s_backend;
+}
+}  // namespace
+
+DenseSdfGrid compute_mesh_sdf_hip(const float* vertices, int num_vertices, const int* faces,
+                                  int num_faces, int nx, int ny, int nz, float ox, float oy,
+                                  float oz, float vx, float vy, float vz, float* out_ms,
+                                  bool voxelize_first, int voxel_res) {
+  std::lock_guard<std::mutex> lock(g_m2s_mutex);
+  return m2s_backend().mesh_to_sdf(vertices, num_vertices, faces, num_faces, nx, ny, nz, ox, oy,
+                                   oz, vx, vy, vz, out_ms, voxelize_first, voxel_res);
+}
+
+DenseSdfGridDevice compute_mesh_sdf_device_hip(const float* vertices, int num_vertices,
+                                               const int* faces, int num_faces, int nx, int ny,
+                                               int nz, float ox, float oy, float oz, float vx,
+                                               float vy, float vz, float* out_ms,
+                                               bool voxelize_first, int voxel_res) {
+  std::lock_guard<std::mutex> lock(g_m2s_mutex);
+  HipDualContouringBackend& be = m2s_backend();
+  if (nx < 1 || ny < 1 || nz < 1)
+    throw std::runtime_error("compute_mesh_sdf_device_hip: nx, ny and nz must be >= 1");
+  void* p = nullptr;
+  const int rc = dc_hip_malloc(be.context(), (int64_t)nx * ny * nz * 4, &p);
+  if (rc != DC_HIP_OK)
+    throw std::runtime_error(std::string("compute_mesh_sdf_device_hip: ") +
+                             dc_hip_status_string(rc) + ": " + dc_hip_last_error(be.context()));
+  try {
+    be.mesh_to_sdf_device(vertices, num_vertices, faces, num_faces, nx, ny, nz, ox, oy, oz, vx,
+                          vy, vz, static_cast<float*>(p), out_ms, voxelize_first, voxel_res);
+  } catch (...) {
+    dc_hip_free(p);
+    throw;
+  }
+  DenseSdfGridDevice g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.vx = vx;
+  g.vy = vy;
+  g.vz = vz;
+  g.ox = ox;
+  g.oy = oy;
+  g.oz = oz;
+  g.d_values = static_cast<const float*>(p);
+  return g;
+}
+
+void free_mesh_sdf_device_hip(DenseSdfGridDevice& device_grid) {
+  if (device_grid.d_values) {
+    dc_hip_free(const_cast<float*>(device_grid.d_values));
+    device_grid.d_values = nullptr;
+  }
 }

@@ -108,3 +108,70 @@ def test_cli_postprocess_flags(tmp_path, flags):
     assert f"Face Count: {F2.shape[0]}" in res.stdout
     if "--close-holes" in flags:
         assert info["loops_filled"] == 1
+
+
+def _obj_grid(V, res):
+    """The grid dc_cli samples a mesh on (the reference's dc_cli.cpp:157-182), in FP32."""
+    f32 = np.float32
+    lo, hi = V.min(0).astype(f32), V.max(0).astype(f32)
+    d = hi - lo
+    diag = f32(np.sqrt(f32(f32(f32(d[0] * d[0]) + f32(d[1] * d[1])) + f32(d[2] * d[2]))))
+    pad = f32(diag * f32(0.08)) if diag > 0 else f32(0.1)
+    lo, hi = (lo - pad).astype(f32), (hi + pad).astype(f32)
+    spacing = ((hi - lo) / f32(res - 1)).astype(f32)
+    return (res, res, res), tuple(float(x) for x in lo), tuple(float(x) for x in spacing)
+
+
+@pytest.mark.parametrize("voxelize", [False, True])
+def test_cli_mesh_input(tmp_path, voxelize):
+    """-i mesh.obj: quads written as faces (split into a fan from their first vertex), tokens with
+    texture/normal indices and negative (relative) indices, sampled on the padded bounding-box
+    grid by the GPU mesh-to-SDF pass and extracted; equal to the oracle's mesh-to-SDF followed by
+    the oracle's extraction. --save-sdf writes the sampled grid."""
+    V, T = synth.torus_mesh(R=0.3, r=0.12, nu=40, nv=20)
+    nv = len(V)
+    lines = [f"v {x:.9g} {y:.9g} {z:.9g}" for x, y, z in V]
+    Q = T.reshape(-1, 2, 3)  # (a, b, c), (a, c, d) -> quad (a, b, c, d)
+    for q, (t0, t1) in enumerate(Q):
+        a, b, c, d = t0[0], t0[1], t0[2], t1[2]
+        if q % 3 == 0:
+            lines.append(f"f {a + 1}/1/1 {b + 1}//2 {c + 1}/3 {d + 1}")
+        elif q % 3 == 1:
+            lines.append(f"f {a - nv} {b - nv} {c - nv} {d - nv}")
+        else:
+            lines.append(f"f {a + 1} {b + 1} {c + 1} {d + 1}")
+    obj_in = tmp_path / "torus.obj"
+    obj_in.write_text("\n".join(lines) + "\n")
+    res_n = 48
+    dims, origin, spacing = _obj_grid(V, res_n)
+    out, sdf = tmp_path / "out.obj", tmp_path / "grid.sdf"
+    args = ["-i", obj_in, "-o", out, "-g", res_n, "--benchmark"]
+    if voxelize:
+        args += ["--voxelize-first", "--voxel-res", 40]
+    r = _cli(*args)
+    assert r.returncode == 0, r.stderr
+    want = oracle.mesh_to_sdf(V, T, dims, origin, spacing, voxelize, 40)
+    ref = oracle.run(want.ravel(), *dims, *spacing, *origin)
+    Vo, Fo = read_obj(out)
+    np.testing.assert_array_equal(Fo, ref.F)
+    np.testing.assert_allclose(Vo, ref.V64, rtol=0, atol=2e-6)
+    # the same through the host path with the grid written out
+    r = _cli(*args, "--save-sdf", sdf)
+    assert r.returncode == 0, r.stderr
+    g = io.load_sdf(sdf)
+    assert (g.nx, g.ny, g.nz) == dims
+    assert np.array_equal(np.asarray(g.values, np.float32).view(np.uint32),
+                          want.ravel().view(np.uint32))
+    Vo2, Fo2 = read_obj(out)
+    np.testing.assert_array_equal(Fo2, ref.F)
+
+
+def test_cli_mesh_input_errors(tmp_path):
+    bad = tmp_path / "bad.obj"
+    bad.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 7\n")
+    r = _cli("-i", bad, "-o", tmp_path / "x.obj", "-g", 16)
+    assert r.returncode == 1 and "outside" in r.stderr
+    empty = tmp_path / "empty.obj"
+    empty.write_text("v 0 0 0\n")
+    r = _cli("-i", empty, "-o", tmp_path / "x.obj", "-g", 16)
+    assert r.returncode == 1 and "Failed to convert OBJ" in r.stderr
