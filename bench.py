@@ -77,7 +77,7 @@ def main():
                     help="timed post-processing calls (median reported)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
-    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"),
+    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5", "m2s"),
                     help="c1: 64^3 spheres with oracle parity; "
                          "c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
                          "FP32+fallback; c4: 256 uneven grids, one batch vs sequential; c5: "
@@ -97,7 +97,8 @@ def main():
     from contouring_hip import DenseSdfGridDevice, HipDualContouringBackend, _lib, synth
 
     if args.config != "c2":
-        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+         "m2s": bench_m2s_config}[args.config](args)
         return
 
     n = args.size
@@ -191,6 +192,7 @@ def main():
         f"bwd {bst.backward_ms:.3f} ms, quads oriented unlike plain: {flips}")
 
     post = bench_postprocess(be, lib, cg, prec, st, args)
+    m2s = bench_mesh_to_sdf(be, lib, cg, prec, st, args)
 
     Ns = n ** 3
     Nc = (n - 1) ** 3
@@ -277,6 +279,7 @@ def main():
         "cpu_baseline": cpu,
         "host_input": host_input,
         "postprocess": post,
+        "mesh_to_sdf": m2s,
     }
     print(json.dumps(out), flush=True)
 
@@ -309,6 +312,78 @@ def bench_postprocess(be, lib, cg, prec, st, args):
                                       np.array_equal(F2.ravel(), out.faces))
         log(f"[bench] postprocess oracle: {rec['cpu_ms']:.0f} ms, equal: {rec['equal_to_oracle']}")
     return rec
+
+
+def bench_mesh_to_sdf(be, lib, cg, prec, st, args, cpu_n=32):
+    """The mesh-to-SDF pass (dc_hip_mesh_to_sdf) on the plain extraction's mesh (quads split
+    into triangles) sampled back on the same grid: host wall time per call (median of 5 after a
+    warm-up; the output stays on the device), beside the oracle's single-thread restatement and
+    the GPU on a small csg8 grid (cpu_n^3, its own DC mesh) where the oracle finishes in seconds
+    (skipped with --no-cpu)."""
+    import torch
+    from contouring_hip import DenseSdfGrid, synth
+    be._check(lib.dc_hip_extract_device(be._ctx, ctypes.byref(cg), prec, ctypes.byref(st)))
+    mesh = be._download(False)
+    V = torch.from_numpy(mesh.vertices.reshape(-1, 3)).cuda()
+    F = torch.from_numpy(synth.quads_to_triangles(mesh.faces)).cuda()
+    n = cg.nx
+    out = torch.empty(cg.nx * cg.ny * cg.nz, dtype=torch.float32, device="cuda:0")
+    times = []
+    for q in range(6):
+        t = time.perf_counter()
+        _, mst = be.mesh_to_sdf(V, F, cg.nx, cg.ny, cg.nz, cg.ox, cg.oy, cg.oz, cg.vx, cg.vy,
+                                cg.vz, out=out)
+        if q > 0:
+            times.append((time.perf_counter() - t) * 1e3)
+    rec = {"workload": f"csg8 {n}^3 DC mesh ({int(F.shape[0])} triangles) sampled on its "
+                       f"{n}^3 grid", "gpu_ms": float(np.median(times)), "runs": len(times),
+           "eval_ms": float(mst["eval_ms"]), "triangles": int(mst["triangles"]),
+           "bin_entries": int(mst["bin_entries"]), "active_bins": int(mst["active_bins"]),
+           "flood_passes": int(mst["flood_passes"]),
+           "samples_per_s": n ** 3 / (float(np.median(times)) * 1e-3)}
+    log(f"[bench] mesh_to_sdf {n}^3: {rec['gpu_ms']:.2f} ms (evaluation {rec['eval_ms']:.2f} ms)")
+    if not args.no_cpu:
+        import oracle
+        gs = synth.csg8(cpu_n, seed=args.seed)
+        ms_, _ = be.extract(DenseSdfGrid(gs.nx, gs.ny, gs.nz, gs.vx, gs.vy, gs.vz, gs.ox, gs.oy,
+                                         gs.oz, gs.values))
+        Vs = ms_.vertices.reshape(-1, 3)
+        Fs = synth.quads_to_triangles(ms_.faces)
+        dims, origin, spacing = (cpu_n,) * 3, (gs.ox, gs.oy, gs.oz), (gs.vx, gs.vy, gs.vz)
+        oracle.set_threads(1)
+        t = time.perf_counter()
+        want = oracle.mesh_to_sdf(Vs, Fs, dims, origin, spacing)
+        cpu_ms = (time.perf_counter() - t) * 1e3
+        gt = []
+        for q in range(6):
+            t = time.perf_counter()
+            got, _ = be.mesh_to_sdf(Vs, Fs, *dims, *origin, *spacing)
+            if q > 0:
+                gt.append((time.perf_counter() - t) * 1e3)
+        rec["small"] = {"grid": cpu_n, "triangles": int(Fs.shape[0]), "cpu_ms": cpu_ms,
+                        "cpu_kind": "port (oracle/m2s_oracle.cpp, 1 thread)",
+                        "gpu_ms": float(np.median(gt)),
+                        "equal_to_oracle": bool(np.array_equal(got.view(np.uint32),
+                                                               want.view(np.uint32)))}
+        log(f"[bench] mesh_to_sdf {cpu_n}^3: oracle {cpu_ms:.0f} ms (1 thread), GPU "
+            f"{rec['small']['gpu_ms']:.2f} ms, equal: {rec['small']['equal_to_oracle']}")
+    return rec
+
+
+def bench_m2s_config(args):
+    """The mesh-to-SDF leg alone: the csg8 grid of --size extracted (plain FP64), its mesh sampled
+    back on the same grid (the default run's mesh_to_sdf object)."""
+    import torch
+    from contouring_hip import HipDualContouringBackend, _lib, synth
+    g = synth.csg8(args.size, seed=args.seed)
+    phi = torch.from_numpy(g.values).to("cuda:0")
+    be = HipDualContouringBackend("plain-fp64", device=0)
+    cg = be._cgrid(g, phi.data_ptr())
+    st = _lib.DcHipStats()
+    rec = bench_mesh_to_sdf(be, be._lib, cg, _lib.DC_HIP_PLAIN_FP64, st, args)
+    print(json.dumps({"metric": "mesh to SDF ms (csg8 DC mesh on its grid)", "config": "m2s",
+                      "unit": "ms", "value": rec["gpu_ms"], "higher_is_better": False,
+                      "results": rec}), flush=True)
 
 
 def host_cpu():
