@@ -94,7 +94,7 @@ MUTATIONS = [
 
 def _build(tmp_path, name, source=None, old=None, new=None):
     d = tmp_path / name
-    d.mkdir()
+    d.mkdir(exist_ok=True)
     for src in SOURCES:
         text = open(os.path.join(ROOT, "oracle", src)).read()
         if src == source:
@@ -117,14 +117,24 @@ def _run_pinned(lib, target):
                           timeout=300)
 
 
-def test_unmodified_copy_passes(tmp_path):
-    lib = _build(tmp_path, "control")
+@pytest.fixture(scope="module")
+def libs(tmp_path_factory):
+    """Every library (the control and each mutant), built once, several compilers at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    root = tmp_path_factory.mktemp("mutants")
+    jobs = [("control", None, None, None)] + [(m[0], m[1], m[2], m[3]) for m in MUTATIONS]
+    with ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1))) as ex:
+        built = list(ex.map(lambda j: _build(root, *j), jobs))
+    return dict(zip((j[0] for j in jobs), built))
+
+
+def test_unmodified_copy_passes(libs):
     for target in (PINNED, POST_PINNED, M2S_PINNED):
-        r = _run_pinned(lib, target)
+        r = _run_pinned(libs["control"], target)
         assert r.returncode == 0, r.stdout[-2000:]
 
 
 @pytest.mark.parametrize("name,source,old,new,path,test", MUTATIONS, ids=[m[0] for m in MUTATIONS])
-def test_mutation_is_caught(tmp_path, name, source, old, new, path, test):
-    r = _run_pinned(_build(tmp_path, name, source, old, new), f"{path}::{test}")
+def test_mutation_is_caught(libs, name, source, old, new, path, test):
+    r = _run_pinned(libs[name], f"{path}::{test}")
     assert r.returncode != 0, f"mutation {name} survived {test}:\n{r.stdout[-1500:]}"
