@@ -256,6 +256,69 @@ int dc_hip_mesh_to_sdf(dc_hip_context* ctx, const float* vertices, int64_t num_v
 int dc_hip_malloc(dc_hip_context* ctx, int64_t bytes, void** ptr);
 int dc_hip_free(void* ptr);
 
+/* Normal of a crossing edge's hermite entry in the sparse extraction (the reference's
+ * NormalComputationMode, include/dc_cuda.h:5-9). */
+typedef enum dc_hip_normal_mode {
+  DC_HIP_NORMAL_FINITE_DIFFERENCE = 0,    /* gradient of the cell's trilinear interpolant */
+  DC_HIP_NORMAL_EDGE_GRADIENT = 1,        /* the ends' central differences, blended along the edge */
+  DC_HIP_NORMAL_PRECOMPUTED_GRADIENT = 2  /* the reference passes no gradient grid to its kernels
+                                             (dc_cuda_sparse.cu:1310, :1336): as mode 0 */
+} dc_hip_normal_mode;
+
+/* Counts and times of one sparse extraction, with the reference's DualContouringStats names
+ * (include/dc_backend.h:44-78) for the fields its sparse backends fill (dc_cuda_sparse.cu:
+ * 1436-1482). */
+typedef struct dc_hip_sparse_stats {
+  int32_t nx, ny, nz, brick_size;
+  int64_t total_cells;    /* (nx-1)(ny-1)(nz-1) */
+  int64_t active_cells;
+  int64_t total_bricks;   /* ceil((nx-1)/B) ceil((ny-1)/B) ceil((nz-1)/B) */
+  int64_t active_bricks;
+  int64_t vertex_count, face_count;
+  int64_t qef_fallback_count, clamp_count, invalid_count;
+  int64_t ambiguous_cells, multi_vertex_cells, one_vertex_cells, two_vertex_cells;
+  int64_t split_rejection_count, bad_qef_count, faces_skipped_due_to_missing_cluster;
+  float upload_ms;
+  float active_brick_marking_ms;     /* sample bit planes and brick flags (reads the grid) */
+  float active_brick_compaction_ms;  /* 0: the bricks stay a flag table, no list is compacted */
+  float active_cell_marking_ms;
+  float active_cell_compaction_ms;   /* active-plane scan (ranks and the cell list) */
+  float marking_ms, compaction_ms;
+  float qef_ms;                      /* vertices (multi-vertex form: with their scan) */
+  float face_count_ms, face_prefix_sum_ms, face_fill_ms, face_emission_ms;
+  float download_ms;                 /* 0 here: the mesh stays on the device */
+  float total_ms;                    /* host wall time of the call */
+  float sample_pass_ms;              /* the grid-reading kernel alone */
+} dc_hip_sparse_stats;
+
+/* Replaces CudaSparseDualContouringBackend::extract / ::extract_device and, with multi_vertex set,
+ * CudaSparseMvdcDualContouringBackend (include/dc_cuda.h:37-62, src/cuda/dc_cuda_sparse.cu:
+ * 1134-1485; bound at src/cpp/bindings.cpp:224-262, selected by dc_cli -b cuda-sparse /
+ * cuda-sparse-mvdc, src/cpp/dc_cli.cpp:317-321). Bricks of brick_size cells per axis are active
+ * when their samples hold a negative and a positive value; their cells are active when the 8
+ * corners hold both signs or one lies within 0.45 vx of zero; each active cell solves one FP32
+ * QEF (multi_vertex: two when its hermite normals split into two opposed clusters); a strict
+ * sign change or a thin edge (same sign, both ends within 0.45 vx, opposed gradients) on a
+ * cell's first-corner x, y or z edge emits a quad (a thin edge also its reverse). Vertices
+ * follow the active cells in linear cell order, faces the active cells' x, y, z edges: the
+ * result equals the reference's, bit for bit (the reference's nvcc contracts a*b+c into FMAs by
+ * default; its FP32 values match this result's where it is built with contraction off).
+ * grid->values: host memory (dc_hip_extract_sparse) or device memory (_device). brick_size >= 1,
+ * normal_mode a dc_hip_normal_mode, else DC_HIP_ERR_INVALID_ARG; dimensions under 2 or above
+ * 65536 DC_HIP_ERR_INVALID_GRID; a NaN or Inf sample DC_HIP_ERR_NONFINITE. The mesh stays on
+ * the device until the next sparse call; the reference's chunk_size has no effect there either.
+ * stats may be NULL. Runs on the context's stream and returns when the mesh is complete. */
+int dc_hip_extract_sparse(dc_hip_context* ctx, const dc_hip_grid* grid, int32_t brick_size,
+                          int normal_mode, int multi_vertex, dc_hip_sparse_stats* stats);
+int dc_hip_extract_sparse_device(dc_hip_context* ctx, const dc_hip_grid* grid, int32_t brick_size,
+                                 int normal_mode, int multi_vertex, dc_hip_sparse_stats* stats);
+/* The last sparse mesh: counts and device pointers (valid until the next sparse call) ... */
+int dc_hip_sparse_result_device(dc_hip_context* ctx, int64_t* n_vertices, int64_t* n_faces,
+                                const float** vertices, const int32_t** faces);
+/* ... or a copy into vertices (3 floats per vertex) and faces (4 int32 per quad), host or device
+ * memory (the reference's download, dc_cuda_sparse.cu:1395-1405). Either pointer may be NULL. */
+int dc_hip_sparse_copy_result(dc_hip_context* ctx, float* vertices, int32_t* faces);
+
 const char* dc_hip_last_error(dc_hip_context* ctx);
 const char* dc_hip_status_string(int status);
 

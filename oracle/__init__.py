@@ -59,6 +59,15 @@ def load():
     lib.oracle_m2s_sample_state.argtypes = m2s + [I, I, I, P]
     lib.oracle_point_triangle.restype = F
     lib.oracle_point_triangle.argtypes = [P, P, P, P, P]
+    lib.oracle_sparse_run.restype = P
+    lib.oracle_sparse_run.argtypes = [P, I, I, I, F, F, F, F, F, F, I, I, I]
+    lib.oracle_sparse_stats.argtypes = [P, P]
+    lib.oracle_sparse_copy.argtypes = [P, P, P, P, P, P]
+    lib.oracle_sparse_free.argtypes = [P]
+    lib.oracle_sparse_jacobi.restype = I
+    lib.oracle_sparse_jacobi.argtypes = [P, P, F, P]
+    lib.oracle_sparse_cell.restype = I
+    lib.oracle_sparse_cell.argtypes = [P, I, I, I, F, F, F, F, F, F, I, I, I, I, P, P]
     _lib = lib
     return lib
 
@@ -254,3 +263,65 @@ def m2s_sample_state(V, F, dims, origin, spacing, ix, iy, iz):
                                 *(float(o) for o in origin), *(float(v) for v in spacing),
                                 int(ix), int(iy), int(iz), _p(out))
     return float(out[0]), out[1:4].copy(), out[4:7].copy()
+
+
+# stats of the sparse brick / multi-vertex extraction, in the reference's DualContouringStats names
+SPARSE_STATS = ("total_cells", "active_cells", "total_bricks", "active_bricks", "vertex_count",
+                "face_count", "qef_fallback_count", "clamp_count", "invalid_count",
+                "ambiguous_cells", "multi_vertex_cells", "one_vertex_cells", "two_vertex_cells",
+                "split_rejection_count", "bad_qef_count", "faces_skipped_due_to_missing_cluster")
+
+
+def sparse(values, nx, ny, nz, vx, vy, vz, ox=0.0, oy=0.0, oz=0.0, brick_size=8, normal_mode=0,
+           multi_vertex=False):
+    """The reference's CudaSparse[Mvdc]DualContouringBackend on a grid (values x fastest).
+
+    Returns a dict: vertices (n, 3) float32, vnormals (n, 3) (split cells' vertex normals, else
+    zero), faces (m, 4) int32, active (linear cell ids, ascending), vcount (vertices per active
+    cell) and stats (SPARSE_STATS)."""
+    lib = load()
+    vals = np.ascontiguousarray(np.asarray(values, np.float32).reshape(-1))
+    if vals.size != nx * ny * nz:
+        raise ValueError("values do not match the grid size")
+    h = lib.oracle_sparse_run(_p(vals), nx, ny, nz, vx, vy, vz, ox, oy, oz, int(brick_size),
+                              int(normal_mode), int(bool(multi_vertex)))
+    if not h:
+        raise ValueError("invalid sparse extraction arguments")
+    try:
+        st = np.zeros(16, np.int64)
+        lib.oracle_sparse_stats(h, _p(st))
+        nv, nf, na = int(st[4]), int(st[5]), int(st[1])
+        V = np.empty((nv, 3), np.float32)
+        N = np.empty((nv, 3), np.float32)
+        Fc = np.empty((nf, 4), np.int32)
+        A = np.empty(na, np.int64)
+        C = np.empty(na, np.int32)
+        lib.oracle_sparse_copy(h, _p(V), _p(N), _p(Fc), _p(A), _p(C))
+    finally:
+        lib.oracle_sparse_free(h)
+    return {"vertices": V, "vnormals": N, "faces": Fc, "active": A, "vcount": C,
+            "stats": dict(zip(SPARSE_STATS, (int(x) for x in st)))}
+
+
+def sparse_grid(grid, **kw):
+    return sparse(grid.values, grid.nx, grid.ny, grid.nz, grid.vx, grid.vy, grid.vz, grid.ox,
+                  grid.oy, grid.oz, **kw)
+
+
+def sparse_jacobi(B, d, thr=0.01):
+    """The sparse backends' 3x3 Jacobi pseudo-inverse solve: (y, singular)."""
+    B = np.ascontiguousarray(np.asarray(B, np.float32).reshape(9))
+    d = np.ascontiguousarray(np.asarray(d, np.float32).reshape(3))
+    y = np.zeros(3, np.float32)
+    s = load().oracle_sparse_jacobi(_p(B), _p(d), float(thr), _p(y))
+    return y, bool(s)
+
+
+def sparse_cell(values, dims, spacing, origin, ijk, normal_mode=0):
+    """One cell's hermite entries (points, normals), (m, 3) float32 each."""
+    vals = np.ascontiguousarray(np.asarray(values, np.float32).reshape(-1))
+    P = np.zeros((12, 3), np.float32)
+    Nn = np.zeros((12, 3), np.float32)
+    m = load().oracle_sparse_cell(_p(vals), *(int(x) for x in dims), *spacing, *origin,
+                                  *(int(x) for x in ijk), int(normal_mode), _p(P), _p(Nn))
+    return P[:m], Nn[:m]

@@ -40,7 +40,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     for src in ("dc_kernels.hip", "dc_cells.hip", "dc_cells_fast.hip", "dc_post.hip",
-                "dc_m2s.hip", "dc_hip.cpp"):
+                "dc_m2s.hip", "dc_sparse.hip", "dc_hip.cpp"):
         obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         flags = list(HIP_FLAGS)
         if src == "dc_cells_fast.hip":  # results held to tolerances: FMA contraction allowed

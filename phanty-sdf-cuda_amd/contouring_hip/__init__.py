@@ -1,4 +1,5 @@
-"""MI355X dual contouring of dense SDF grids — drop-in for the reference's dense GPU backend.
+"""MI355X dual contouring of dense SDF grids — drop-in for the reference's dense GPU backend and its
+sparse brick / multi-vertex backends.
 
 Mirrors the reference Python module (src/python/contouring, src/cpp/bindings.cpp): grid, mesh
 and stats classes plus a backend class whose extract(grid) returns (mesh, stats). It is a ctypes
@@ -12,6 +13,9 @@ from .backend import (  # noqa: F401
     DualContouringMesh,
     DualContouringStats,
     HipDualContouringBackend,
+    HipSparseDualContouringBackend,
+    HipSparseMvdcDualContouringBackend,
+    NormalComputationMode,
     PRECISIONS,
     mesh_to_sdf_device_hip,
     mesh_to_sdf_hip,

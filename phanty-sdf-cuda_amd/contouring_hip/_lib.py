@@ -29,7 +29,8 @@ EXPORTS = (
     "dc_hip_result_device", "dc_hip_extract_batch", "dc_hip_backward", "dc_hip_autotune",
     "dc_hip_triangulate", "dc_hip_postprocess", "dc_hip_postprocess_copy",
     "dc_hip_postprocess_device", "dc_hip_mesh_to_sdf", "dc_hip_malloc", "dc_hip_free",
-    "dc_hip_last_error", "dc_hip_status_string",
+    "dc_hip_extract_sparse", "dc_hip_extract_sparse_device", "dc_hip_sparse_result_device",
+    "dc_hip_sparse_copy_result", "dc_hip_last_error", "dc_hip_status_string",
 )
 
 
@@ -68,6 +69,24 @@ class DcHipM2sStats(ctypes.Structure):
                 ("bins", ctypes.c_int64), ("active_bins", ctypes.c_int64),
                 ("flood_passes", ctypes.c_int32), ("voxel_res", ctypes.c_int32),
                 ("eval_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
+_SPARSE_INT = ("total_cells", "active_cells", "total_bricks", "active_bricks", "vertex_count",
+               "face_count", "qef_fallback_count", "clamp_count", "invalid_count",
+               "ambiguous_cells", "multi_vertex_cells", "one_vertex_cells", "two_vertex_cells",
+               "split_rejection_count", "bad_qef_count",
+               "faces_skipped_due_to_missing_cluster")
+_SPARSE_MS = ("upload_ms", "active_brick_marking_ms", "active_brick_compaction_ms",
+              "active_cell_marking_ms", "active_cell_compaction_ms", "marking_ms",
+              "compaction_ms", "qef_ms", "face_count_ms", "face_prefix_sum_ms", "face_fill_ms",
+              "face_emission_ms", "download_ms", "total_ms", "sample_pass_ms")
+
+
+class DcHipSparseStats(ctypes.Structure):
+    _fields_ = ([("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                 ("brick_size", ctypes.c_int32)]
+                + [(f, ctypes.c_int64) for f in _SPARSE_INT]
+                + [(f, ctypes.c_float) for f in _SPARSE_MS])
 
 
 _hip = None
@@ -124,6 +143,13 @@ def load_hip():
                                    ctypes.c_int32] + [ctypes.c_float] * 6
                                + [I, ctypes.c_int32, P, ctypes.POINTER(DcHipM2sStats)]),
         "dc_hip_malloc": (I, [P, I64, ctypes.POINTER(P)]),
+        "dc_hip_extract_sparse": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, I,
+                                      ctypes.POINTER(DcHipSparseStats)]),
+        "dc_hip_extract_sparse_device": (I, [P, ctypes.POINTER(DcHipGrid), ctypes.c_int32, I, I,
+                                             ctypes.POINTER(DcHipSparseStats)]),
+        "dc_hip_sparse_result_device": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(I64),
+                                            ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "dc_hip_sparse_copy_result": (I, [P, P, P]),
         "dc_hip_free": (I, [P]),
         "dc_hip_last_error": (ctypes.c_char_p, [P]),
         "dc_hip_status_string": (ctypes.c_char_p, [I]),

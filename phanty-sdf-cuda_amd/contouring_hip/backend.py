@@ -8,8 +8,14 @@ Reference interface (file:line in the reference tree):
                                                           src/cpp/bindings.cpp:204-227
   mesh_to_sdf_cuda / mesh_to_sdf_device_cuda(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy,
   vz) -> (grid, ms)                                       src/cpp/bindings.cpp:270-284
-HipDualContouringBackend takes the place of CudaDualContouringBackend. There is no CPU fallback:
-without the HIP library or a device every call raises.
+  NormalComputationMode                                   include/dc_cuda.h:5-9,
+                                                          src/cpp/bindings.cpp:209-213
+  CudaSparse[Mvdc]DualContouringBackend(brick_size, normal_mode, chunk_size[,
+  multi_vertex_cells]).extract / extract_device           include/dc_cuda.h:37-62,
+                                                          src/cpp/bindings.cpp:229-262
+HipDualContouringBackend takes the place of CudaDualContouringBackend,
+HipSparseDualContouringBackend and HipSparseMvdcDualContouringBackend the places of the sparse
+pair. There is no CPU fallback: without the HIP library or a device every call raises.
 """
 import ctypes
 
@@ -482,3 +488,105 @@ def mesh_to_sdf_device_hip(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz,
     vals, ms = _m2s(vertices, faces, nx, ny, nz, ox, oy, oz, vx, vy, vz, voxelize_first,
                     voxel_res, "device")
     return DenseSdfGridDevice(nx, ny, nz, vx, vy, vz, ox, oy, oz, d_values=vals.reshape(-1)), ms
+
+
+class NormalComputationMode:
+    """include/dc_cuda.h:5-9: the normal of a crossing edge's hermite entry in the sparse
+    backends."""
+    FiniteDifference = 0
+    EdgeGradient = 1
+    PrecomputedGradient = 2  # no gradient grid reaches the kernels: as FiniteDifference
+
+
+class HipSparseDualContouringBackend:
+    """MI355X sparse brick dual contouring (replaces CudaSparseDualContouringBackend,
+    include/dc_cuda.h:37-56, bound at src/cpp/bindings.cpp:229-243). extract(grid) and
+    extract_device(grid) return (mesh, stats) equal to the reference's: cells of active bricks
+    (bricks of brick_size cells whose samples hold both signs) with both signs or a corner within
+    0.45 vx of zero, one FP32 QEF vertex per cell (two for a split cell when multi_vertex_cells),
+    quads on strict sign changes and both windings on thin edges. chunk_size is kept as an
+    attribute and has no effect, as in the reference."""
+
+    _name = "hip-sparse"
+
+    def __init__(self, brick_size=8, normal_mode=NormalComputationMode.FiniteDifference,
+                 chunk_size=0, multi_vertex_cells=False, device=0):
+        self.brick_size = brick_size
+        self.normal_mode = normal_mode
+        self.chunk_size = chunk_size
+        self.multi_vertex_cells = multi_vertex_cells
+        self.device = device
+        self._be = HipDualContouringBackend(device=device)  # owns the context
+
+    def _stats(self, st, download_ms):
+        out = DualContouringStats()
+        out.backend = "hip-sparse-mvdc" if self.multi_vertex_cells else "hip-sparse"
+        for f in ("nx", "ny", "nz") + _lib._SPARSE_INT + _lib._SPARSE_MS:
+            if hasattr(out, f) or f in ("sample_pass_ms",):
+                setattr(out, f, getattr(st, f))
+        out.download_ms = download_ms
+        out.total_ms = st.total_ms + download_ms
+        return out
+
+    def result_counts(self):
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        self._be._check(self._be._lib.dc_hip_sparse_result_device(
+            self._be._ctx, ctypes.byref(nv), ctypes.byref(nf), None, None))
+        return nv.value, nf.value
+
+    def _download(self, st):
+        import time
+        t0 = time.perf_counter()
+        nv, nf = self.result_counts()
+        V = np.empty(3 * nv, np.float32)
+        F = np.empty(4 * nf, np.int32)
+        self._be._check(self._be._lib.dc_hip_sparse_copy_result(
+            self._be._ctx, V.ctypes.data_as(ctypes.c_void_p), F.ctypes.data_as(ctypes.c_void_p)))
+        return DualContouringMesh(V, F), self._stats(st, (time.perf_counter() - t0) * 1e3)
+
+    def extract(self, grid):
+        """Host grid -> (DualContouringMesh, DualContouringStats) (dc_cuda_sparse.cu:1134-1168)."""
+        vals = np.ascontiguousarray(np.asarray(grid.values, dtype=np.float32))
+        if vals.size != int(grid.nx) * int(grid.ny) * int(grid.nz):
+            raise ValueError("values size does not match nx*ny*nz")
+        cg = self._be._cgrid(grid, vals.ctypes.data)
+        st = _lib.DcHipSparseStats()
+        self._be.set_stream(None)
+        self._be._check(self._be._lib.dc_hip_extract_sparse(
+            self._be._ctx, ctypes.byref(cg), int(self.brick_size), int(self.normal_mode),
+            int(bool(self.multi_vertex_cells)), ctypes.byref(st)))
+        return self._download(st)
+
+    def extract_device(self, grid, download=True):
+        """Device grid -> (mesh, stats) (dc_cuda_sparse.cu:1170-1485). grid: DenseSdfGridDevice
+        whose d_values is a float32 CUDA tensor (checked) or an int device pointer. With
+        download=False the mesh stays on the device: (V [n,3] float32, F [m,4] int32) torch
+        tensors copied from the context's result, and the stats."""
+        n = int(grid.nx) * int(grid.ny) * int(grid.nz)
+        if not isinstance(grid.d_values, int):
+            _check_tensor(grid.d_values, "d_values", n, self.device)
+            self._be._use_torch_stream()
+        cg = self._be._cgrid(grid, _ptr(grid.d_values))
+        st = _lib.DcHipSparseStats()
+        self._be._check(self._be._lib.dc_hip_extract_sparse_device(
+            self._be._ctx, ctypes.byref(cg), int(self.brick_size), int(self.normal_mode),
+            int(bool(self.multi_vertex_cells)), ctypes.byref(st)))
+        if download:
+            return self._download(st)
+        import torch
+        nv, nf = self.result_counts()
+        dev = torch.device("cuda", self.device)
+        V = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        F = torch.empty((nf, 4), dtype=torch.int32, device=dev)
+        self._be._check(self._be._lib.dc_hip_sparse_copy_result(
+            self._be._ctx, ctypes.c_void_p(V.data_ptr()), ctypes.c_void_p(F.data_ptr())))
+        return (V, F), self._stats(st, 0.0)
+
+
+class HipSparseMvdcDualContouringBackend(HipSparseDualContouringBackend):
+    """The multi-vertex form (replaces CudaSparseMvdcDualContouringBackend,
+    include/dc_cuda.h:58-62, bound at src/cpp/bindings.cpp:245-262)."""
+
+    def __init__(self, brick_size=8, normal_mode=NormalComputationMode.FiniteDifference,
+                 chunk_size=0, device=0):
+        super().__init__(brick_size, normal_mode, chunk_size, True, device)

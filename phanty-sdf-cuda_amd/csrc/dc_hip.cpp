@@ -21,6 +21,7 @@
 #include "dc_internal.h"
 #include "dc_kernels.h"
 #include "dc_m2s.h"
+#include "dc_sparse.h"
 #include "dc_post.h"
 
 namespace {
@@ -179,6 +180,7 @@ struct dc_hip_context {
   std::vector<DcGridDev> hgrids;
   DcPost* post = nullptr;  // mesh post-processing state and its last result
   DcM2s* m2s = nullptr;    // mesh-to-SDF state
+  DcSparse* sparse = nullptr;  // sparse brick extraction state and its last mesh
   std::vector<int64_t> tile_b, act_b, pen_b, samp_b, sr_b, ft_b, cw_b, hbounds;
   int64_t n_words = 0, n_ft = 0, n_pen = 0, n_tiles = 0, n_samples = 0, n_sr = 0, n_cw = 0,
           n_sg = 0;
@@ -826,6 +828,7 @@ void dc_hip_destroy(dc_hip_context* c) {
     if (b->p) hipFree(b->p);
   delete c->post;
   delete c->m2s;
+  delete c->sparse;
   for (int q = 0; q < kNumEvents; ++q)
     if (c->ev[q]) hipEventDestroy(c->ev[q]);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -1199,6 +1202,60 @@ int dc_hip_mesh_to_sdf(dc_hip_context* c, const float* vertices, int64_t num_ver
   const float origin[3] = {ox, oy, oz}, spacing[3] = {vx, vy, vz};
   return c->m2s->run(vertices, num_vertices, faces, num_faces, dims, origin, spacing,
                      voxelize_first != 0, voxel_res, values, c->cu_count, s, stats, &c->err);
+}
+
+namespace {
+int run_sparse(dc_hip_context* c, const dc_hip_grid* grid, bool on_device, int32_t brick_size,
+               int normal_mode, int multi_vertex, dc_hip_sparse_stats* stats) {
+  if (!c || !grid) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  if (!c->sparse) c->sparse = new DcSparse();
+  hipStream_t s;
+  int rc;
+  if ((rc = work_stream(c, &s))) return rc;
+  return c->sparse->run(*grid, on_device, brick_size, normal_mode, multi_vertex != 0, s, stats,
+                        &c->err);
+}
+}  // namespace
+
+int dc_hip_extract_sparse(dc_hip_context* c, const dc_hip_grid* grid, int32_t brick_size,
+                          int normal_mode, int multi_vertex, dc_hip_sparse_stats* stats) {
+  return run_sparse(c, grid, false, brick_size, normal_mode, multi_vertex, stats);
+}
+
+int dc_hip_extract_sparse_device(dc_hip_context* c, const dc_hip_grid* grid, int32_t brick_size,
+                                 int normal_mode, int multi_vertex, dc_hip_sparse_stats* stats) {
+  return run_sparse(c, grid, true, brick_size, normal_mode, multi_vertex, stats);
+}
+
+int dc_hip_sparse_result_device(dc_hip_context* c, int64_t* n_vertices, int64_t* n_faces,
+                                const float** vertices, const int32_t** faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->sparse || !c->sparse->have())
+    return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful sparse extraction");
+  if (n_vertices) *n_vertices = c->sparse->n_vertices();
+  if (n_faces) *n_faces = c->sparse->n_faces();
+  if (vertices) *vertices = c->sparse->vertices();
+  if (faces) *faces = c->sparse->faces();
+  return DC_HIP_OK;
+}
+
+int dc_hip_sparse_copy_result(dc_hip_context* c, float* vertices, int32_t* faces) {
+  if (!c) return DC_HIP_ERR_INVALID_ARG;
+  if (!c->sparse || !c->sparse->have())
+    return fail(c, DC_HIP_ERR_NO_FORWARD, "no successful sparse extraction");
+  DC_TRY(c, hipSetDevice(c->device), "set device");
+  const int64_t nv = c->sparse->n_vertices(), nf = c->sparse->n_faces();
+  // hipMemcpyDefault: destinations may be host or device memory (unified addressing)
+  if (vertices && nv)
+    DC_TRY(c, hipMemcpyAsync(vertices, c->sparse->vertices(), nv * 12, hipMemcpyDefault,
+                             c->stream),
+           "copy vertices");
+  if (faces && nf)
+    DC_TRY(c, hipMemcpyAsync(faces, c->sparse->faces(), nf * 16, hipMemcpyDefault, c->stream),
+           "copy faces");
+  DC_TRY(c, hipStreamSynchronize(c->stream), "download");
+  return DC_HIP_OK;
 }
 
 int dc_hip_malloc(dc_hip_context* c, int64_t bytes, void** ptr) {
