@@ -1,9 +1,10 @@
 """Each plausible mistake in the oracle is caught by a pinning test (CPU).
 
 A copy of the oracle's sources (oracle/dc_oracle.cpp, oracle/post_oracle.cpp,
-oracle/m2s_oracle.cpp) with one mistake planted in one of them is built into a temporary library,
-and the test that names that mistake (tests/test_oracle_pinned.py,
-tests/test_post_oracle_pinned.py, tests/test_m2s_oracle_pinned.py) is run against it in a subprocess
+oracle/m2s_oracle.cpp, oracle/sparse_oracle.cpp) with one mistake planted in one of them is built
+into a temporary library, and the test that names that mistake (tests/test_oracle_pinned.py,
+tests/test_post_oracle_pinned.py, tests/test_m2s_oracle_pinned.py,
+tests/test_sparse_oracle_pinned.py) is run against it in a subprocess
 (ORACLE_LIB): it must fail. The unmodified copy, built the same way, must pass all of them (the
 control). Every mutation here fails an assertion (none crashes the mutated library).
 """
@@ -14,10 +15,11 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SOURCES = ("dc_oracle.cpp", "post_oracle.cpp", "m2s_oracle.cpp")
+SOURCES = ("dc_oracle.cpp", "post_oracle.cpp", "m2s_oracle.cpp", "sparse_oracle.cpp")
 PINNED = os.path.join(ROOT, "tests", "test_oracle_pinned.py")
 POST_PINNED = os.path.join(ROOT, "tests", "test_post_oracle_pinned.py")
 M2S_PINNED = os.path.join(ROOT, "tests", "test_m2s_oracle_pinned.py")
+SP_PINNED = os.path.join(ROOT, "tests", "test_sparse_oracle_pinned.py")
 
 # (name, oracle source, text in it, replacement, test file, test that must fail)
 MUTATIONS = [
@@ -89,6 +91,45 @@ MUTATIONS = [
      "        grown[x + (int64_t)vres * (y + (int64_t)vres * z)] =\n"
      "            shell[x + (int64_t)vres * (y + (int64_t)vres * z)];", M2S_PINNED,
      "test_voxelize_growth_seals_a_small_hole"),
+    ("jacobi_v_update", "sparse_oracle.cpp", "V[r * 3 + q] = s * vrp + c * vrq;",
+     "V[r * 3 + q] = s * vrp - c * vrq;", SP_PINNED,
+     "test_jacobi_pinv_matches_eigendecomposition"),
+    ("jacobi_transposed", "sparse_oracle.cpp", "w[0] = V[0] * d[0] + V[3] * d[1] + V[6] * d[2];",
+     "w[0] = V[0] * d[0] + V[1] * d[1] + V[2] * d[2];", SP_PINNED,
+     "test_jacobi_pinv_matches_eigendecomposition"),
+    ("jacobi_no_cut", "sparse_oracle.cpp", "if (S[i] > thr) {", "if (S[i] > 0.0f) {", SP_PINNED,
+     "test_jacobi_pinv_drops_small_singular_values"),
+    ("sparse_trilinear_weight", "sparse_oracle.cpp", "(f[5] - f[4]) * ov * w",
+     "(f[5] - f[4]) * ou * w", SP_PINNED, "test_trilinear_normals_away_from_the_cell_centre"),
+    ("edge_blend", "sparse_oracle.cpp", "n = add(mul(1.0f - t, ga), mul(t, gb));",
+     "n = add(mul(t, ga), mul(1.0f - t, gb));", SP_PINNED,
+     "test_edge_gradient_of_a_quadratic_is_the_gradient_at_the_point"),
+    ("crossing_t", "sparse_oracle.cpp", "crossing ? fa / (fa - fb + 1e-12f)",
+     "crossing ? fb / (fb - fa + 1e-12f)", SP_PINNED,
+     "test_crossing_points_are_linear_zeros_of_their_edge"),
+    ("brick_either_sign", "sparse_oracle.cpp", "brick[b] = neg && pos;", "brick[b] = neg || pos;",
+     SP_PINNED, "test_activation_rule_brute_force"),
+    ("brick_extent", "sparse_oracle.cpp", "for (int k = bz * B; k < bz * B + B + 1 && k < g.nz; ++k)",
+     "for (int k = bz * B; k < bz * B + B && k < g.nz; ++k)", SP_PINNED,
+     "test_activation_rule_brute_force"),
+    ("near_rule", "sparse_oracle.cpp", "close = close || fabsf(v) <= near;",
+     "close = close || fabsf(v) < 0.5f * near;", SP_PINNED, "test_activation_rule_brute_force"),
+    ("thin_threshold", "sparse_oracle.cpp", "return dot(ga, gb) < -0.3f;",
+     "return dot(ga, gb) < -0.2f;", SP_PINNED, "test_thin_edges_need_opposed_gradients"),
+    ("split_side", "sparse_oracle.cpp", "if (dot(cd.n[q], nA) >= dot(cd.n[q], nB)) {",
+     "if (dot(cd.n[q], nA) <= dot(cd.n[q], nB)) {", SP_PINNED, "test_thin_sheet_in_closed_form"),
+    ("split_threshold", "sparse_oracle.cpp", "if (*ia < 0 || *ib < 0 || lo >= -0.3f) {",
+     "if (*ia < 0 || *ib < 0 || lo >= -1.5f) {", SP_PINNED, "test_thin_sheet_in_closed_form"),
+    ("mass_point", "sparse_oracle.cpp", "c = mul(1.0f / (float)cnt, c);",
+     "c = mul(1.0f / (float)(cnt + 1), c);", SP_PINNED, "test_thin_sheet_in_closed_form"),
+    ("face_orientation", "sparse_oracle.cpp",
+     "if (dot(gn, gn) > 0.0f && dot(qn, qn) > 0.0f && dot(qn, gn) < 0.0f) {",
+     "if (dot(gn, gn) > 0.0f && dot(qn, qn) > 0.0f && dot(qn, gn) > 0.0f) {", SP_PINNED,
+     "test_smooth_topology_equals_the_dense_oracle"),
+    ("face_cycle", "sparse_oracle.cpp",
+     "const int t[4][3] = {{i, j - 1, k - 1}, {i, j, k - 1}, {i, j, k}, {i, j - 1, k}};",
+     "const int t[4][3] = {{i, j - 1, k - 1}, {i, j, k}, {i, j, k - 1}, {i, j - 1, k}};",
+     SP_PINNED, "test_smooth_topology_equals_the_dense_oracle"),
 ]
 
 
@@ -129,7 +170,7 @@ def libs(tmp_path_factory):
 
 
 def test_unmodified_copy_passes(libs):
-    for target in (PINNED, POST_PINNED, M2S_PINNED):
+    for target in (PINNED, POST_PINNED, M2S_PINNED, SP_PINNED):
         r = _run_pinned(libs["control"], target)
         assert r.returncode == 0, r.stdout[-2000:]
 
