@@ -77,11 +77,13 @@ def main():
                     help="timed post-processing calls (median reported)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the mark tile shapes on this grid first and use the fastest")
-    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5", "m2s"),
+    ap.add_argument("--config", default="c2",
+                    choices=("c1", "c2", "c3", "c4", "c5", "m2s", "sparse"),
                     help="c1: 64^3 spheres with oracle parity; "
                          "c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
                          "FP32+fallback; c4: 256 uneven grids, one batch vs sequential; c5: "
-                         "forward+adjoint at 128^3 and 256^3")
+                         "forward+adjoint at 128^3 and 256^3; m2s: mesh to SDF of the csg8 DC mesh; "
+                         "sparse: the sparse brick backends on csg8")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one elementwise kernel over the grid (known bytes: reads and writes "
                          "4*N_s) for scaling the HBM byte counters")
@@ -98,7 +100,7 @@ def main():
 
     if args.config != "c2":
         {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-         "m2s": bench_m2s_config}[args.config](args)
+         "m2s": bench_m2s_config, "sparse": bench_sparse_config}[args.config](args)
         return
 
     n = args.size
@@ -193,6 +195,7 @@ def main():
 
     post = bench_postprocess(be, lib, cg, prec, st, args)
     m2s = bench_mesh_to_sdf(be, lib, cg, prec, st, args)
+    sparse = bench_sparse(g, phi, args)
 
     Ns = n ** 3
     Nc = (n - 1) ** 3
@@ -280,6 +283,7 @@ def main():
         "host_input": host_input,
         "postprocess": post,
         "mesh_to_sdf": m2s,
+        "sparse": sparse,
     }
     print(json.dumps(out), flush=True)
 
@@ -368,6 +372,84 @@ def bench_mesh_to_sdf(be, lib, cg, prec, st, args, cpu_n=32):
         log(f"[bench] mesh_to_sdf {cpu_n}^3: oracle {cpu_ms:.0f} ms (1 thread), GPU "
             f"{rec['small']['gpu_ms']:.2f} ms, equal: {rec['small']['equal_to_oracle']}")
     return rec
+
+
+def bench_sparse(g, phi, args, cpu_n=128):
+    """The sparse brick backends (dc_hip_extract_sparse_device, brick 8, finite-difference
+    normals) on the same device grid: host wall time per call with the mesh left on the device
+    (median of 5 after a warm-up; each call synchronises for its counts), the per-stage device
+    times and the sample pass's read rate (4 bytes per sample), for the one-vertex and the
+    multi-vertex form; beside the oracle's single-thread restatement and the GPU on a csg8 grid of
+    cpu_n^3 (skipped with --no-cpu)."""
+    from contouring_hip import (DenseSdfGridDevice, HipSparseDualContouringBackend,
+                                HipSparseMvdcDualContouringBackend, _lib, synth)
+    n = g.nx
+    recs = {}
+    for mv in (False, True):
+        sp = HipSparseMvdcDualContouringBackend() if mv else HipSparseDualContouringBackend()
+        be = sp._be
+        be._use_torch_stream()
+        cg = be._cgrid(g, phi.data_ptr())
+        st = _lib.DcHipSparseStats()
+        times = []
+        for q in range(6):
+            t = time.perf_counter()
+            be._check(be._lib.dc_hip_extract_sparse_device(be._ctx, ctypes.byref(cg), 8, 0,
+                                                           int(mv), ctypes.byref(st)))
+            if q > 0:
+                times.append((time.perf_counter() - t) * 1e3)
+        ms = float(np.median(times))
+        rec = {"workload": f"csg8 {n}^3, brick 8, finite-difference normals", "gpu_ms": ms,
+               "runs": len(times), "active_cells_per_s": st.active_cells / (ms * 1e-3),
+               "sample_pass_gbs": 4.0 * n ** 3 / (st.sample_pass_ms * 1e-3) / 1e9
+               if st.sample_pass_ms > 0 else None}
+        for f in _lib._SPARSE_INT + _lib._SPARSE_MS:
+            rec[f] = getattr(st, f)
+        log(f"[bench] {'mvdc' if mv else 'sparse'} {n}^3: {ms:.3f} ms, {st.active_bricks}/"
+            f"{st.total_bricks} bricks, {st.active_cells} cells, {st.vertex_count} vertices, "
+            f"{st.face_count} quads; sample pass {st.sample_pass_ms * 1e3:.1f} us "
+            f"({rec['sample_pass_gbs']:.0f} GB/s), cells {st.active_cell_marking_ms * 1e3:.1f} us, "
+            f"scan {st.active_cell_compaction_ms * 1e3:.1f} us, vertices {st.qef_ms * 1e3:.1f} us, "
+            f"faces {st.face_emission_ms * 1e3:.1f} us")
+        if not args.no_cpu:
+            import oracle
+            gs = synth.csg8(cpu_n, seed=args.seed)
+            t = time.perf_counter()
+            want = oracle.sparse_grid(gs, multi_vertex=mv)
+            cpu_ms = (time.perf_counter() - t) * 1e3
+            mesh, _ = sp.extract(gs)
+            gt = []
+            for q in range(6):
+                t = time.perf_counter()
+                mesh, _ = sp.extract(gs)
+                if q > 0:
+                    gt.append((time.perf_counter() - t) * 1e3)
+            rec["small"] = {
+                "grid": cpu_n, "cpu_ms": cpu_ms,
+                "cpu_kind": "port (oracle/sparse_oracle.cpp, 1 thread)",
+                "gpu_ms_host_grid": float(np.median(gt)),
+                "equal_to_oracle": bool(
+                    np.array_equal(mesh.vertices.view(np.uint32),
+                                   want["vertices"].ravel().view(np.uint32)) and
+                    np.array_equal(mesh.faces, want["faces"].ravel()))}
+            log(f"[bench] {'mvdc' if mv else 'sparse'} {cpu_n}^3: oracle {cpu_ms:.0f} ms "
+                f"(1 thread), GPU {rec['small']['gpu_ms_host_grid']:.2f} ms with upload and "
+                f"download, equal: {rec['small']['equal_to_oracle']}")
+        recs["mvdc" if mv else "sparse"] = rec
+        del sp
+    return recs
+
+
+def bench_sparse_config(args):
+    """The sparse leg alone on the csg8 grid of --size (the default run's sparse object)."""
+    import torch
+    from contouring_hip import synth
+    g = synth.csg8(args.size, seed=args.seed)
+    phi = torch.from_numpy(g.values).to("cuda:0")
+    recs = bench_sparse(g, phi, args)
+    print(json.dumps({"metric": "sparse brick extraction ms (csg8, brick 8)", "config": "sparse",
+                      "unit": "ms", "value": recs["mvdc"]["gpu_ms"], "higher_is_better": False,
+                      "results": recs}), flush=True)
 
 
 def bench_m2s_config(args):

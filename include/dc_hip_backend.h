@@ -78,6 +78,50 @@ class HipDualContouringBackend : public IDualContouringBackend {
   int precision_;
 };
 
+// include/dc_cuda.h:37-56 (CudaSparseDualContouringBackend, src/cuda/dc_cuda_sparse.cu:1134-1485):
+// the sparse brick extraction (dc_hip_extract_sparse) behind IDualContouringBackend, with the
+// reference's public fields. normal_mode takes the values of the reference's NormalComputationMode
+// (dc_hip_normal_mode). The mesh and every counter equal the reference's; chunk_size has no
+// effect, as in the reference. Throws std::runtime_error on failure.
+class HipSparseDualContouringBackend : public IDualContouringBackend {
+ public:
+  explicit HipSparseDualContouringBackend(int brick_size = 8,
+                                          int normal_mode = DC_HIP_NORMAL_FINITE_DIFFERENCE,
+                                          int chunk_size = 0, bool multi_vertex_cells = false,
+                                          int device = 0);
+  ~HipSparseDualContouringBackend() override;
+  HipSparseDualContouringBackend(const HipSparseDualContouringBackend&) = delete;
+  HipSparseDualContouringBackend& operator=(const HipSparseDualContouringBackend&) = delete;
+
+  // dc_cuda_sparse.cu:1134-1168 — host grid in, host mesh out.
+  DualContouringMesh extract(const DenseSdfGrid& grid, DualContouringStats& stats) override;
+  // dc_cuda_sparse.cu:1170-1485 — device grid in, host mesh out.
+  DualContouringMesh extract_device(const DenseSdfGridDevice& grid, DualContouringStats& stats);
+  // postprocess_mesh (src/cpp/dc_backend.cpp:217-558) on the last sparse mesh, on the GPU,
+  // without copying the mesh to the host first.
+  DualContouringMesh postprocess_last(bool close_holes, bool remove_floaters);
+  dc_hip_context* context() { return ctx_; }
+
+  int brick_size;
+  int normal_mode;
+  int chunk_size;
+  bool multi_vertex_cells;
+
+ private:
+  DualContouringMesh finish(const dc_hip_sparse_stats& st, DualContouringStats& stats);
+  void check(int rc);
+  dc_hip_context* ctx_ = nullptr;
+};
+
+// include/dc_cuda.h:58-62 (CudaSparseMvdcDualContouringBackend): the multi-vertex form.
+class HipSparseMvdcDualContouringBackend : public HipSparseDualContouringBackend {
+ public:
+  explicit HipSparseMvdcDualContouringBackend(int brick_size = 8,
+                                              int normal_mode = DC_HIP_NORMAL_FINITE_DIFFERENCE,
+                                              int chunk_size = 0, int device = 0)
+      : HipSparseDualContouringBackend(brick_size, normal_mode, chunk_size, true, device) {}
+};
+
 // include/mesh_to_sdf.h:6-28 with the backend's name: the mesh's signed distance grid on the host
 // (compute_mesh_sdf_hip), or on the device (compute_mesh_sdf_device_hip: d_values allocated with
 // dc_hip_malloc, released by free_mesh_sdf_device_hip). out_ms receives the call's wall time.

@@ -7,7 +7,9 @@
 //   ...
 //   bind_hip_backend(m);
 //
-// The class mirrors the reference's CudaDualContouringBackend binding (bindings.cpp:216-228):
+// HipSparseDualContouringBackend and HipSparseMvdcDualContouringBackend mirror the reference's
+// sparse pair (bindings.cpp:229-262). The dense class mirrors the reference's
+// CudaDualContouringBackend binding (bindings.cpp:216-228):
 // extract(grid) and extract_device(grid) return (mesh, stats); postprocess(mesh, close_holes,
 // remove_floaters) is the reference's postprocess_mesh (bindings.cpp:145-146) on the GPU, and
 // postprocess_last the same on the last extraction's mesh in device memory. The module functions
@@ -70,6 +72,34 @@ inline void bind_hip_backend(pybind11::module_& m) {
       .def("last_triangles", &HipDualContouringBackend::last_triangles)
       .def("set_option", &HipDualContouringBackend::set_option)
       .def_property_readonly("precision", &HipDualContouringBackend::precision);
+  // bindings.cpp:229-262 — normal_mode as the integer value of the reference's
+  // NormalComputationMode (0 FiniteDifference, 1 EdgeGradient, 2 PrecomputedGradient)
+  py::class_<HipSparseDualContouringBackend>(m, "HipSparseDualContouringBackend")
+      .def(py::init<int, int, int, bool, int>(), py::arg("brick_size") = 8,
+           py::arg("normal_mode") = 0, py::arg("chunk_size") = 0,
+           py::arg("multi_vertex_cells") = false, py::arg("device") = 0)
+      .def_readwrite("brick_size", &HipSparseDualContouringBackend::brick_size)
+      .def_readwrite("normal_mode", &HipSparseDualContouringBackend::normal_mode)
+      .def_readwrite("chunk_size", &HipSparseDualContouringBackend::chunk_size)
+      .def_readwrite("multi_vertex_cells", &HipSparseDualContouringBackend::multi_vertex_cells)
+      .def("extract",
+           [](HipSparseDualContouringBackend& self, const DenseSdfGrid& grid) {
+             DualContouringStats stats;
+             DualContouringMesh mesh = self.extract(grid, stats);
+             return py::make_tuple(mesh, stats);
+           })
+      .def("extract_device",
+           [](HipSparseDualContouringBackend& self, const DenseSdfGridDevice& grid) {
+             DualContouringStats stats;
+             DualContouringMesh mesh = self.extract_device(grid, stats);
+             return py::make_tuple(mesh, stats);
+           })
+      .def("postprocess_last", &HipSparseDualContouringBackend::postprocess_last,
+           py::arg("close_holes") = true, py::arg("remove_floaters") = true);
+  py::class_<HipSparseMvdcDualContouringBackend, HipSparseDualContouringBackend>(
+      m, "HipSparseMvdcDualContouringBackend")
+      .def(py::init<int, int, int, int>(), py::arg("brick_size") = 8, py::arg("normal_mode") = 0,
+           py::arg("chunk_size") = 0, py::arg("device") = 0);
   using Floats = py::array_t<float, py::array::c_style | py::array::forcecast>;
   using Ints = py::array_t<int, py::array::c_style | py::array::forcecast>;
   m.def("mesh_to_sdf_hip", [](Floats vertices, Ints faces, int nx, int ny, int nz, float ox,

@@ -1,5 +1,6 @@
 // dc_cli.cpp — command-line front end with the reference's flags (src/cpp/dc_cli.cpp:205-380),
-// running the MI355X backend: -b hip (also the default). Input: a .sdf file
+// running the MI355X backends: -b hip (dense, the default), -b hip-sparse and -b hip-sparse-mvdc
+// (or mvdc; with --brick-size, --normal-mode, --chunk-size). Input: a .sdf file
 // (int32 nx,ny,nz; float vx,vy,vz; float ox,oy,oz; float values, x fastest; dc_cli.cpp:31-47),
 // a triangle mesh .obj sampled into a res^3 grid on the GPU (:120-203; --voxelize-first,
 // --voxel-res), or --generate {sphere,box,plane,sphere-unit} (--save-sdf writes the grid in the
@@ -161,8 +162,9 @@ bool generate(const std::string& kind, int res, DenseSdfGrid& g) {
 
 int main(int argc, char* argv[]) {
   std::string input, output, backend_type = "hip", gen, precision = "plain-fp64", save_sdf;
+  std::string normal_mode_str = "finite-difference";
   bool benchmark = false, close_holes = false, remove_floaters = false, voxelize_first = false;
-  int res = 64, voxel_res = -1;
+  int res = 64, voxel_res = -1, brick_size = 8, chunk_size = 0;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> std::string { return (i + 1 < argc) ? argv[++i] : std::string(); };
@@ -178,9 +180,11 @@ int main(int argc, char* argv[]) {
     else if (a == "--remove-floaters") remove_floaters = true;
     else if (a == "-vf" || a == "--voxelize-first") voxelize_first = true;
     else if (a == "-vr" || a == "--voxel-res") voxel_res = std::stoi(next());
-    else if (a == "-oi" || a == "-ii" || a == "-dc" || a == "-mu" || a == "-hu" ||
-             a == "--brick-size" || a == "--chunk-size" || a == "--normal-mode")
-      next();  // accepted for compatibility; no effect on the dense backend
+    else if (a == "--brick-size") brick_size = std::stoi(next());
+    else if (a == "--chunk-size") chunk_size = std::stoi(next());
+    else if (a == "--normal-mode") normal_mode_str = next();
+    else if (a == "-oi" || a == "-ii" || a == "-dc" || a == "-mu" || a == "-hu")
+      next();  // the reference ignores these too (dc_cli.cpp:231-232)
     else if (input.empty()) input = a;
     else if (output.empty()) output = a;
   }
@@ -192,17 +196,27 @@ int main(int argc, char* argv[]) {
   }
   if (output.empty()) {
     std::cerr << "Usage: " << argv[0] << " -i <input.obj|.sdf> -o <output.obj> [options]\n"
-              << "Options: -b/--backend hip, -g/--res/--grid-size <int>, --generate "
-                 "<sphere|box|plane|sphere-unit>, --precision <plain-fp64|fp32-fallback>, "
-                 "--close-holes, --remove-floaters, --voxelize-first, --voxel-res <int>, "
-                 "--save-sdf <grid.sdf>, --benchmark\n";
+              << "Options: -b/--backend <hip|hip-sparse|hip-sparse-mvdc>, -g/--res/--grid-size "
+                 "<int>, --generate <sphere|box|plane|sphere-unit>, --precision "
+                 "<plain-fp64|fp32-fallback>, --brick-size <int>, --normal-mode "
+                 "<finite-difference|edge-gradient|precomputed-gradient>, --close-holes, "
+                 "--remove-floaters, --voxelize-first, --voxel-res <int>, --save-sdf <grid.sdf>, "
+                 "--benchmark\n";
     return 1;
   }
-  if (backend_type != "hip") {
+  // dc_cli.cpp:311-321: the sparse pair and the reference's "mvdc" alias; unknown normal modes
+  // are finite differences, as there
+  const bool sparse_backend = backend_type == "hip-sparse" || backend_type == "hip-sparse-mvdc" ||
+                              backend_type == "mvdc";
+  if (backend_type != "hip" && !sparse_backend) {
     std::cerr << "Backend '" << backend_type
-              << "' is not part of this build; the dense MI355X backend is '-b hip'.\n";
+              << "' is not part of this build; the MI355X backends are '-b hip' (dense), "
+                 "'-b hip-sparse' and '-b hip-sparse-mvdc'.\n";
     return 1;
   }
+  int n_mode = DC_HIP_NORMAL_FINITE_DIFFERENCE;
+  if (normal_mode_str == "edge-gradient") n_mode = DC_HIP_NORMAL_EDGE_GRADIENT;
+  else if (normal_mode_str == "precomputed-gradient") n_mode = DC_HIP_NORMAL_PRECOMPUTED_GRADIENT;
   int prec;
   if (precision == "plain-fp64") prec = DC_HIP_PLAIN_FP64;
   else if (precision == "fp32-fallback") prec = DC_HIP_FP32_FALLBACK;
@@ -236,12 +250,25 @@ int main(int argc, char* argv[]) {
   DualContouringMesh mesh;
   try {
     HipDualContouringBackend backend(0, prec);
+    std::unique_ptr<HipSparseDualContouringBackend> sparse;
+    if (sparse_backend) {
+      if (backend_type == "hip-sparse")
+        sparse.reset(new HipSparseDualContouringBackend(brick_size, n_mode, chunk_size));
+      else
+        sparse.reset(new HipSparseMvdcDualContouringBackend(brick_size, n_mode, chunk_size));
+    }
+    auto extract_host = [&](const DenseSdfGrid& g) {
+      return sparse ? sparse->extract(g, stats) : backend.extract(g, stats);
+    };
+    auto extract_dev = [&](const DenseSdfGridDevice& g) {
+      return sparse ? sparse->extract_device(g, stats) : backend.extract_device(g, stats);
+    };
     if (is_obj && !save_sdf.empty()) {  // the grid is written out: sample it to the host
       grid = backend.mesh_to_sdf(mesh_v.data(), (int)(mesh_v.size() / 3), mesh_f.data(),
                                  (int)(mesh_f.size() / 3), grid.nx, grid.ny, grid.nz, grid.ox,
                                  grid.oy, grid.oz, grid.vx, grid.vy, grid.vz, nullptr,
                                  voxelize_first, voxel_res);
-      mesh = backend.extract(grid, stats);
+      mesh = extract_host(grid);
     } else if (is_obj) {
       // the mesh's grid stays on the device from the mesh-to-SDF pass into the extraction
       void* d = nullptr;
@@ -256,9 +283,9 @@ int main(int argc, char* argv[]) {
                                  nullptr, voxelize_first, voxel_res);
       const DenseSdfGridDevice dg{grid.nx, grid.ny, grid.nz, grid.vx, grid.vy,
                                   grid.vz, grid.ox, grid.oy, grid.oz, static_cast<const float*>(d)};
-      mesh = backend.extract_device(dg, stats);
+      mesh = extract_dev(dg);
     } else {
-      mesh = backend.extract(grid, stats);
+      mesh = extract_host(grid);
     }
     if (!save_sdf.empty() && !write_sdf(save_sdf, grid)) {
       std::cerr << "Failed to write SDF to: " << save_sdf << "\n";
@@ -266,7 +293,8 @@ int main(int argc, char* argv[]) {
     }
     // post-processing after the stats are taken, as the reference does (dc_cli.cpp:334-338)
     if (close_holes || remove_floaters) {
-      mesh = backend.postprocess_last(close_holes, remove_floaters);
+      mesh = sparse ? sparse->postprocess_last(close_holes, remove_floaters)
+                    : backend.postprocess_last(close_holes, remove_floaters);
       stats.vertex_count = (int)(mesh.vertices.size() / 3);
       stats.face_count = (int)(mesh.faces.size() / 4);
     }
@@ -304,6 +332,15 @@ int main(int argc, char* argv[]) {
               << "QEF Fallback Count: " << stats.qef_fallback_count << "\n"
               << "Out-of-Cell Clamps: " << stats.clamp_count << "\n"
               << "Invalid/NaN Count: " << stats.invalid_count << "\n";
+    if (stats.ambiguous_cells > 0 || stats.multi_vertex_cells > 0)  // :362-370
+      std::cout << "Ambiguous Cells: " << stats.ambiguous_cells << "\n"
+                << "Multi-Vertex Cells: " << stats.multi_vertex_cells << "\n"
+                << "One-Vertex Cells: " << stats.one_vertex_cells << "\n"
+                << "Two-Vertex Cells: " << stats.two_vertex_cells << "\n"
+                << "Split Rejection Count: " << stats.split_rejection_count << "\n"
+                << "Bad QEF Count: " << stats.bad_qef_count << "\n"
+                << "Faces Skipped (Missing Cluster): "
+                << stats.faces_skipped_due_to_missing_cluster << "\n";
   }
   return 0;
 }

@@ -1,6 +1,7 @@
 // dc_hip_backend.cpp — IDualContouringBackend over the C ABI. Compiles against the reference's
 // include/dc_backend.h (inside the reference tree) or this repository's standalone restatement,
-// and defines only HipDualContouringBackend's members.
+// and defines only the members of HipDualContouringBackend and HipSparseDualContouringBackend and
+// the mesh-to-SDF free functions.
 
 #include "dc_hip_backend.h"
 
@@ -220,4 +221,114 @@ void free_mesh_sdf_device_hip(DenseSdfGridDevice& device_grid) {
     dc_hip_free(const_cast<float*>(device_grid.d_values));
     device_grid.d_values = nullptr;
   }
+}
+
+HipSparseDualContouringBackend::HipSparseDualContouringBackend(int brick_size_, int normal_mode_,
+                                                               int chunk_size_,
+                                                               bool multi_vertex_cells_,
+                                                               int device)
+    : brick_size(brick_size_),
+      normal_mode(normal_mode_),
+      chunk_size(chunk_size_),
+      multi_vertex_cells(multi_vertex_cells_) {
+  const int rc = dc_hip_create(device, &ctx_);
+  if (rc != DC_HIP_OK)
+    throw std::runtime_error(std::string("HipSparseDualContouringBackend: ") +
+                             dc_hip_status_string(rc));
+}
+
+HipSparseDualContouringBackend::~HipSparseDualContouringBackend() { dc_hip_destroy(ctx_); }
+
+void HipSparseDualContouringBackend::check(int rc) {
+  if (rc != DC_HIP_OK)
+    throw std::runtime_error(std::string("HipSparseDualContouringBackend: ") +
+                             dc_hip_status_string(rc) + ": " + dc_hip_last_error(ctx_));
+}
+
+DualContouringMesh HipSparseDualContouringBackend::finish(const dc_hip_sparse_stats& st,
+                                                          DualContouringStats& stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  DualContouringMesh mesh;
+  mesh.vertices.resize(3 * (size_t)st.vertex_count);
+  mesh.faces.resize(4 * (size_t)st.face_count);
+  check(dc_hip_sparse_copy_result(ctx_, mesh.vertices.empty() ? nullptr : mesh.vertices.data(),
+                                  mesh.faces.empty() ? nullptr : mesh.faces.data()));
+  const float dl =
+      std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // dc_cuda_sparse.cu:1436-1482
+  stats = DualContouringStats();
+  stats.backend = multi_vertex_cells ? "hip-sparse-mvdc" : "hip-sparse";
+  stats.nx = st.nx;
+  stats.ny = st.ny;
+  stats.nz = st.nz;
+  stats.total_cells = (int)st.total_cells;
+  stats.active_cells = (int)st.active_cells;
+  stats.total_bricks = (int)st.total_bricks;
+  stats.active_bricks = (int)st.active_bricks;
+  stats.upload_ms = st.upload_ms;
+  stats.active_brick_marking_ms = st.active_brick_marking_ms;
+  stats.active_brick_compaction_ms = st.active_brick_compaction_ms;
+  stats.active_cell_marking_ms = st.active_cell_marking_ms;
+  stats.active_cell_compaction_ms = st.active_cell_compaction_ms;
+  stats.marking_ms = st.marking_ms;
+  stats.compaction_ms = st.compaction_ms;
+  stats.qef_ms = st.qef_ms;
+  stats.face_count_ms = st.face_count_ms;
+  stats.face_prefix_sum_ms = st.face_prefix_sum_ms;
+  stats.face_fill_ms = st.face_fill_ms;
+  stats.face_emission_ms = st.face_emission_ms;
+  stats.download_ms = dl;
+  stats.total_ms = st.total_ms + dl;
+  stats.vertex_count = (int)st.vertex_count;
+  stats.face_count = (int)st.face_count;
+  stats.qef_fallback_count = (int)st.qef_fallback_count;
+  stats.clamp_count = (int)st.clamp_count;
+  stats.invalid_count = (int)st.invalid_count;
+  stats.ambiguous_cells = (int)st.ambiguous_cells;
+  stats.multi_vertex_cells = (int)st.multi_vertex_cells;
+  stats.one_vertex_cells = (int)st.one_vertex_cells;
+  stats.two_vertex_cells = (int)st.two_vertex_cells;
+  stats.split_rejection_count = (int)st.split_rejection_count;
+  stats.bad_qef_count = (int)st.bad_qef_count;
+  stats.faces_skipped_due_to_missing_cluster = (int)st.faces_skipped_due_to_missing_cluster;
+  return mesh;
+}
+
+DualContouringMesh HipSparseDualContouringBackend::extract(const DenseSdfGrid& grid,
+                                                           DualContouringStats& stats) {
+  if ((size_t)grid.nx * grid.ny * grid.nz != grid.values.size())
+    throw std::runtime_error("HipSparseDualContouringBackend: values size does not match nx*ny*nz");
+  dc_hip_grid g{grid.nx, grid.ny, grid.nz, grid.vx, grid.vy, grid.vz,
+                grid.ox, grid.oy, grid.oz, grid.values.data()};
+  dc_hip_sparse_stats st{};
+  check(dc_hip_extract_sparse(ctx_, &g, brick_size, normal_mode, multi_vertex_cells ? 1 : 0, &st));
+  return finish(st, stats);
+}
+
+DualContouringMesh HipSparseDualContouringBackend::extract_device(const DenseSdfGridDevice& grid,
+                                                                  DualContouringStats& stats) {
+  dc_hip_grid g{grid.nx, grid.ny, grid.nz, grid.vx, grid.vy, grid.vz,
+                grid.ox, grid.oy, grid.oz, grid.d_values};
+  dc_hip_sparse_stats st{};
+  check(dc_hip_extract_sparse_device(ctx_, &g, brick_size, normal_mode,
+                                     multi_vertex_cells ? 1 : 0, &st));
+  return finish(st, stats);
+}
+
+DualContouringMesh HipSparseDualContouringBackend::postprocess_last(bool close_holes,
+                                                                    bool remove_floaters) {
+  int64_t nv = 0, nf = 0;
+  const float* dv = nullptr;
+  const int32_t* df = nullptr;
+  check(dc_hip_sparse_result_device(ctx_, &nv, &nf, &dv, &df));
+  if (nv == 0 && nf == 0) return DualContouringMesh();  // an empty mesh stays empty
+  int64_t onv = 0, onf = 0;
+  check(dc_hip_postprocess(ctx_, dv, nv, df, nf, close_holes ? 1 : 0, remove_floaters ? 1 : 0,
+                           &onv, &onf, nullptr));
+  DualContouringMesh m;
+  m.vertices.resize(3 * (size_t)onv);
+  m.faces.resize(4 * (size_t)onf);
+  check(dc_hip_postprocess_copy(ctx_, m.vertices.empty() ? nullptr : m.vertices.data(),
+                                m.faces.empty() ? nullptr : m.faces.data()));
+  return m;
 }

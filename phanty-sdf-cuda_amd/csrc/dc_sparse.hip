@@ -51,8 +51,6 @@ enum {
   kFallbacks,
   kClamps,
   kInvalid,
-  kOne,
-  kTwo,
   kSplitRej,
   kSkipped,
   kVertTotal,
@@ -69,11 +67,14 @@ struct SpGrid {
   const float* f;  // the samples (global memory)
   int nx, ny, nz;
   int cx, cy, cz, W;
+  int64_t layer;   // nx * ny
   float vx, vy, vz, ox, oy, oz;
   float near;  // 0.45 vx
-  __device__ __forceinline__ float at(int i, int j, int k) const {
-    return dccommon::to_global(f)[(int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k)];
+  __device__ __forceinline__ int64_t idx(int i, int j, int k) const {
+    return (int64_t)k * layer + (int64_t)j * nx + i;
   }
+  __device__ __forceinline__ float at(int64_t q) const { return dccommon::to_global(f)[q]; }
+  __device__ __forceinline__ float at(int i, int j, int k) const { return at(idx(i, j, k)); }
 };
 
 struct F3 {
@@ -104,9 +105,10 @@ __device__ __forceinline__ F3 sample_gradient(const SpGrid& g, int i, int j, int
   const int im = i > 0 ? i - 1 : 0, ip = i < g.nx - 1 ? i + 1 : g.nx - 1;
   const int jm = j > 0 ? j - 1 : 0, jp = j < g.ny - 1 ? j + 1 : g.ny - 1;
   const int km = k > 0 ? k - 1 : 0, kp = k < g.nz - 1 ? k + 1 : g.nz - 1;
-  const float fim = g.at(im, j, k), fip = g.at(ip, j, k);
-  const float fjm = g.at(i, jm, k), fjp = g.at(i, jp, k);
-  const float fkm = g.at(i, j, km), fkp = g.at(i, j, kp);
+  const int64_t b = g.idx(i, j, k);
+  const float fim = g.at(b + (im - i)), fip = g.at(b + (ip - i));
+  const float fjm = g.at(b + (int64_t)(jm - j) * g.nx), fjp = g.at(b + (int64_t)(jp - j) * g.nx);
+  const float fkm = g.at(b + (km - k) * g.layer), fkp = g.at(b + (kp - k) * g.layer);
   F3 r = f3(0.f, 0.f, 0.f);
   const float dx = (float)(ip - im) * g.vx, dy = (float)(jp - jm) * g.vy,
               dz = (float)(kp - km) * g.vz;
@@ -153,55 +155,61 @@ struct Sym3 {
   float a00, a01, a02, a11, a12, a22;
 };
 
-// One Jacobi rotation in the (P, Q) plane (R: the third index), dc_cuda_sparse.cu:214-242.
-template <int P, int Q>
-__device__ __forceinline__ void rotate(float A[3][3], float V[3][3]) {
-  constexpr int R = 3 - P - Q;
-  const float app = A[P][P], aqq = A[Q][Q], apq = A[P][Q];
-  const float phi = 0.5f * (aqq - app) / apq;
-  float t = 1.0f / (fabsf(phi) + sqrtf(phi * phi + 1.0f));
-  if (phi < 0.0f) t = -t;
-  const float c = 1.0f / sqrtf(t * t + 1.0f);
-  const float s = t * c;
-  const float arp = A[R][P], arq = A[R][Q];
-  A[R][P] = A[P][R] = c * arp - s * arq;
-  A[R][Q] = A[Q][R] = s * arp + c * arq;
-  A[P][P] = app - t * apq;
-  A[Q][Q] = aqq + t * apq;
-  A[P][Q] = A[Q][P] = 0.0f;
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const float vrp = V[r][P], vrq = V[r][Q];
-    V[r][P] = c * vrp - s * vrq;
-    V[r][Q] = s * vrp + c * vrq;
-  }
-}
-
 // dc_cuda_sparse.cu:178-269: at most 15 rotations on the largest upper off-diagonal entry (stop
 // below 1e-7), then y = V S+ V^T d dropping singular values <= 0.01. Returns "every value dropped".
+// The rotation plane (p, q) of a sweep is one of (0,1), (0,2), (1,2); its entries are picked and
+// written back with selects (one code path for the three planes), with the reference's
+// arithmetic on them (:214-242).
 __device__ __forceinline__ bool jacobi_pinv(const Sym3& B, const float d[3], float y[3]) {
-  float A[3][3] = {{B.a00, B.a01, B.a02}, {B.a01, B.a11, B.a12}, {B.a02, B.a12, B.a22}};
+  float a00 = B.a00, a01 = B.a01, a02 = B.a02, a11 = B.a11, a12 = B.a12, a22 = B.a22;
   float V[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
   for (int it = 0; it < 15; ++it) {
     int sel = 0;
-    float big = fabsf(A[0][1]);
-    if (fabsf(A[0][2]) > big) {
+    float big = fabsf(a01);
+    if (fabsf(a02) > big) {
       sel = 1;
-      big = fabsf(A[0][2]);
+      big = fabsf(a02);
     }
-    if (fabsf(A[1][2]) > big) {
+    if (fabsf(a12) > big) {
       sel = 2;
-      big = fabsf(A[1][2]);
+      big = fabsf(a12);
     }
     if (big < 1e-7f) break;
-    if (sel == 0)
-      rotate<0, 1>(A, V);
-    else if (sel == 1)
-      rotate<0, 2>(A, V);
-    else
-      rotate<1, 2>(A, V);
+    const bool s0 = sel == 0, s1 = sel == 1;
+    // (p, q, r) = (0, 1, 2), (0, 2, 1), (1, 2, 0)
+    const float app = sel < 2 ? a00 : a11;
+    const float aqq = s0 ? a11 : a22;
+    const float apq = s0 ? a01 : (s1 ? a02 : a12);
+    const float arp = s0 ? a02 : a01;
+    const float arq = s0 ? a12 : (s1 ? a12 : a02);
+    const float phi = 0.5f * (aqq - app) / apq;
+    float t = 1.0f / (fabsf(phi) + sqrtf(phi * phi + 1.0f));
+    if (phi < 0.0f) t = -t;
+    const float c = 1.0f / sqrtf(t * t + 1.0f);
+    const float s = t * c;
+    const float rp = c * arp - s * arq;
+    const float rq = s * arp + c * arq;
+    const float pp = app - t * apq;
+    const float qq = aqq + t * apq;
+    if (s0) {
+      a00 = pp; a11 = qq; a01 = 0.0f; a02 = rp; a12 = rq;
+    } else if (s1) {
+      a00 = pp; a22 = qq; a02 = 0.0f; a01 = rp; a12 = rq;
+    } else {
+      a11 = pp; a22 = qq; a12 = 0.0f; a01 = rp; a02 = rq;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const float vrp = sel < 2 ? V[r][0] : V[r][1];
+      const float vrq = s0 ? V[r][1] : V[r][2];
+      const float np = c * vrp - s * vrq;
+      const float nq = s * vrp + c * vrq;
+      V[r][0] = sel < 2 ? np : V[r][0];
+      V[r][1] = s0 ? nq : (s1 ? V[r][1] : np);
+      V[r][2] = s0 ? V[r][2] : nq;
+    }
   }
-  const float S[3] = {A[0][0], A[1][1], A[2][2]};
+  const float S[3] = {a00, a11, a22};
   float Si[3];
   int kept = 0;
 #pragma unroll
@@ -246,8 +254,9 @@ __device__ __forceinline__ F3 corner_pos(const Cell& c, int q) {
 // dc_cuda_sparse.cu:488-563 / :682-758: the entries of cell (i, j, k).
 template <int MODE>
 __device__ __forceinline__ void cell_entries(const SpGrid& g, int i, int j, int k, Cell& c) {
+  const int64_t b = g.idx(i, j, k);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) c.f[q] = g.at(i + cox(q), j + coy(q), k + coz(q));
+  for (int q = 0; q < 8; ++q) c.f[q] = g.at(b + cox(q) + (int64_t)coy(q) * g.nx + coz(q) * g.layer);
   c.x0 = g.ox + (float)i * g.vx;
   c.x1 = g.ox + (float)(i + 1) * g.vx;
   c.y0 = g.oy + (float)j * g.vy;
@@ -320,7 +329,7 @@ __device__ __forceinline__ void cell_entries(const SpGrid& g, int i, int j, int 
 }
 
 struct Tally {
-  uint32_t fallbacks = 0, clamps = 0, invalid = 0, one = 0, two = 0, split_rej = 0;
+  uint32_t fallbacks = 0, clamps = 0, invalid = 0, split_rej = 0;
 };
 
 // dc_cuda_sparse.cu:380-443: the QEF of the entries in (tA, tB) about their mass point.
@@ -420,21 +429,37 @@ __device__ __forceinline__ F3 pick_slot(const Cell& c, int s) {
   return r;
 }
 
+// Active-list entry -> cell (act words number fewer than 2^26: 32-bit arithmetic).
 __device__ __forceinline__ void decode_cell(uint64_t entry, int W, int cy, int& i, int& j, int& k) {
-  const int64_t word = (int64_t)(entry >> 6);
-  const int64_t row = word / W;
-  i = (int)(word - row * W) * 64 + (int)(entry & 63);
-  j = (int)(row % cy);
-  k = (int)(row / cy);
+  const uint32_t word = (uint32_t)(entry >> 6);
+  const uint32_t row = word / (uint32_t)W;
+  i = (int)(word - row * (uint32_t)W) * 64 + (int)(entry & 63);
+  j = (int)(row % (uint32_t)cy);
+  k = (int)(row / (uint32_t)cy);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_incl(v, threadIdx.x & 63), 63);
 }
 
-__device__ __forceinline__ void flush(uint32_t v, uint32_t* dst) {
-  const uint32_t s = wave_sum(v);
-  if (s && (threadIdx.x & 63) == 0) atomicAdd(dst, s);
+// Adds each of the block's N per-thread counts to its counter: one atomic per block and counter
+// (none when the sum is zero). Every thread of the block calls it.
+template <int N>
+__device__ __forceinline__ void block_flush(const uint32_t (&v)[N], uint32_t* const (&dst)[N]) {
+  __shared__ uint32_t s_part[N][kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    const uint32_t t = wave_sum(v[q]);
+    if (lane == 0) s_part[q][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int x = 0; x < kThreads / 64; ++x) t += s_part[threadIdx.x][x];
+    if (t) atomicAdd(dst[threadIdx.x], t);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -485,24 +510,44 @@ __global__ void __launch_bounds__(kThreads)
   if (__ballot(bad) != 0 && lane == 0) atomicOr(ctr + kNonfinite, 1u);
 }
 
-// Brick rows: thread (bz, by, w) ORs word w of N and of P over the brick row's sample rows.
+// Brick rows, in two steps. Thread (k, by, w) ORs word w of N and of P over the brick row's
+// sample rows j in [by*B, by*B + B] of layer k; then thread (bz, by, w) ORs those over the
+// layers k in [bz*B, bz*B + B].
 __global__ void __launch_bounds__(kThreads)
-    k_sp_brick_rows(const uint64_t* __restrict__ N, const uint64_t* __restrict__ P, int ny, int nz,
-                    int Ws, int B, int nby, int64_t n, uint64_t* __restrict__ RN,
-                    uint64_t* __restrict__ RP) {
+    k_sp_rows_j(const uint64_t* __restrict__ N, const uint64_t* __restrict__ P, int ny, int Ws,
+                int B, int nby, int64_t n, uint64_t* __restrict__ JN, uint64_t* __restrict__ JP) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= n) return;
+  const int w = (int)(t % Ws);
+  const int64_t r = t / Ws;
+  const int by = (int)(r % nby);
+  const int64_t k = r / nby;
+  const int j0 = by * B, j1 = min(j0 + B + 1, ny);
+  uint64_t on = 0, op = 0;
+  for (int j = j0; j < j1; ++j) {
+    const int64_t q = (k * ny + j) * Ws + w;
+    on |= N[q];
+    op |= P[q];
+  }
+  JN[t] = on;
+  JP[t] = op;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_sp_rows_k(const uint64_t* __restrict__ JN, const uint64_t* __restrict__ JP, int nz, int Ws,
+                int B, int nby, int64_t n, uint64_t* __restrict__ RN, uint64_t* __restrict__ RP) {
   const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (t >= n) return;
   const int w = (int)(t % Ws);
   const int64_t r = t / Ws;
   const int by = (int)(r % nby), bz = (int)(r / nby);
-  const int j0 = by * B, j1 = min(j0 + B + 1, ny), k0 = bz * B, k1 = min(k0 + B + 1, nz);
+  const int k0 = bz * B, k1 = min(k0 + B + 1, nz);
   uint64_t on = 0, op = 0;
-  for (int k = k0; k < k1; ++k)
-    for (int j = j0; j < j1; ++j) {
-      const int64_t q = ((int64_t)k * ny + j) * Ws + w;
-      on |= N[q];
-      op |= P[q];
-    }
+  for (int k = k0; k < k1; ++k) {
+    const int64_t q = ((int64_t)k * nby + by) * Ws + w;
+    on |= JN[q];
+    op |= JP[q];
+  }
   RN[t] = on;
   RP[t] = op;
 }
@@ -518,9 +563,9 @@ __global__ void __launch_bounds__(kThreads)
     k_sp_brick_flags(const uint64_t* __restrict__ RN, const uint64_t* __restrict__ RP, int nx,
                      int Ws, int B, int nbx, int64_t n_bricks, uint8_t* __restrict__ flag,
                      uint32_t* __restrict__ ctr) {
-  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  bool act = false;
-  if (b < n_bricks) {
+  uint32_t cnt = 0;
+  for (int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x; b < n_bricks;
+       b += (int64_t)gridDim.x * kThreads) {
     const int bx = (int)(b % nbx);
     const int64_t r = b / nbx;
     const int x0 = bx * B, x1 = min(x0 + B + 1, nx);
@@ -530,11 +575,13 @@ __global__ void __launch_bounds__(kThreads)
       an |= RN[r * Ws + w] & m;
       ap |= RP[r * Ws + w] & m;
     }
-    act = an != 0 && ap != 0;
+    const bool act = an != 0 && ap != 0;
     flag[b] = act ? 1 : 0;
+    cnt += act;
   }
-  const uint32_t cnt = (uint32_t)__popcll(__ballot(act));
-  if (cnt && (threadIdx.x & 63) == 0) atomicAdd(ctr + kBrickCount, cnt);
+  const uint32_t v[1] = {cnt};
+  uint32_t* const dst[1] = {ctr + kBrickCount};
+  block_flush<1>(v, dst);
 }
 
 // Active cells of the brick row (by, bz) in act word w: the cells of active bricks.
@@ -672,13 +719,11 @@ __global__ void __launch_bounds__(kThreads)
     }
     if (live) {
       if (nv == 1) {
-        ++tl.one;
         const F3 x = solve_cluster(c, all, allB, g, tl);
         spos[6 * r] = x.x;
         spos[6 * r + 1] = x.y;
         spos[6 * r + 2] = x.z;
       } else {
-        ++tl.two;
         const uint32_t t1A = all & ~t0A, t1B = allB & ~t0B;
         const F3 x0 = solve_cluster(c, t0A, t0B, g, tl);
         const F3 x1 = solve_cluster(c, t1A, t1B, g, tl);
@@ -709,14 +754,10 @@ __global__ void __launch_bounds__(kThreads)
       vcnt[r] = (uint32_t)nv;
     }
   }
-  flush(tl.fallbacks, ctr + kFallbacks);
-  flush(tl.clamps, ctr + kClamps);
-  flush(tl.invalid, ctr + kInvalid);
-  if (MV) {
-    flush(tl.one, ctr + kOne);
-    flush(tl.two, ctr + kTwo);
-    flush(tl.split_rej, ctr + kSplitRej);
-  }
+  // one- and two-vertex cells follow from the counts (two = vertices - cells)
+  const uint32_t v[4] = {tl.fallbacks, tl.clamps, tl.invalid, tl.split_rej};
+  uint32_t* const dst[4] = {ctr + kFallbacks, ctr + kClamps, ctr + kInvalid, ctr + kSplitRej};
+  block_flush<4>(v, dst);
 }
 
 // Multi-vertex form: the vertices in id order from the slots.
@@ -750,40 +791,29 @@ struct FaceArgs {
   uint32_t* ctr;
 };
 
-// dc_cuda_sparse.cu:829-851: the vertex of cell (i, j, k) for a face whose summed edge
-// gradients are gn: -1 when the cell is not active or (split cell) no vertex normal is within
-// 120 degrees of gn; else the id, and its position in *pos.
-template <bool MV, bool POS>
-__device__ __forceinline__ int cell_vertex(const SpGrid& g, const FaceArgs& a, int i, int j, int k,
-                                           F3 gn, F3* pos) {
-  const int64_t w = ((int64_t)k * g.cy + j) * g.W + (i >> 6);
+// Rank of cell (i, j, k) in the active list, or -1 when it is not active.
+__device__ __forceinline__ int cell_rank(const SpGrid& g, const FaceArgs& a, int i, int j, int k) {
+  const uint32_t w = ((uint32_t)k * (uint32_t)g.cy + (uint32_t)j) * (uint32_t)g.W + (uint32_t)(i >> 6);
   const uint64_t word = a.act[w];
   const int bit = i & 63;
   if (!((word >> bit) & 1)) return -1;
-  const uint32_t rank = a.wpre[w] + (uint32_t)__popcll(word & ((1ull << bit) - 1ull));
-  if (!MV) {
-    if (POS) *pos = f3(a.verts[3 * (int64_t)rank], a.verts[3 * (int64_t)rank + 1],
-                       a.verts[3 * (int64_t)rank + 2]);
-    return (int)rank;
-  }
-  const uint32_t n = a.vcnt[rank];
-  int h = 0;
-  if (n == 2) {
-    const float* nr = a.snrm + 6 * (int64_t)rank;
-    const float d0 = dot(gn, f3(nr[0], nr[1], nr[2]));
-    const float d1 = dot(gn, f3(nr[3], nr[4], nr[5]));
-    h = d0 >= d1 ? 0 : 1;
-    if (fmaxf(d0, d1) < -0.5f) return -1;
-  }
-  if (POS) {
-    const float* p = a.spos + 6 * (int64_t)rank + 3 * h;
-    *pos = f3(p[0], p[1], p[2]);
-  }
-  return (int)(a.vstart[rank] + (uint32_t)h);
+  return (int)(a.wpre[w] + (uint32_t)__popcll(word & ((1ull << bit) - 1ull)));
+}
+
+// dc_cuda_sparse.cu:829-851: which of a split cell's two vertices a face whose summed edge
+// gradients are gn takes (the one whose normal is nearer gn), or -1 when neither normal is within
+// 120 degrees of gn.
+__device__ __forceinline__ int pick_vertex(const FaceArgs& a, int rank, F3 gn) {
+  const float* nr = a.snrm + 6 * (int64_t)rank;
+  const float d0 = dot(gn, f3(nr[0], nr[1], nr[2]));
+  const float d1 = dot(gn, f3(nr[3], nr[4], nr[5]));
+  if (fmaxf(d0, d1) < -0.5f) return -1;
+  return d0 >= d1 ? 0 : 1;
 }
 
 // dc_cuda_sparse.cu:853-1102: per active cell its x, y and z edge from its first corner. EMIT
-// false: the quads per cell; EMIT true: the quads at the cell's first quad.
+// false: the quads per cell (the edge gradients only where a thin-edge test or a split cell needs
+// them); EMIT true: the quads at the cell's first quad, oriented along the summed gradients.
 template <bool MV, bool EMIT>
 __global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
   const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -793,20 +823,27 @@ __global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
     decode_cell(a.alist[r], g.W, g.cy, i, j, k);
     uint32_t nf = 0;
     int64_t out = EMIT ? (int64_t)a.fstart[r] : 0;
+    const int64_t b = g.idx(i, j, k);
+    const float f0 = g.at(b);
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
       const bool has = axis == 0 ? (j >= 1 && k >= 1) : axis == 1 ? (i >= 1 && k >= 1)
                                                                    : (i >= 1 && j >= 1);
       if (!has) continue;
       const int i1 = i + (axis == 0), j1 = j + (axis == 1), k1 = k + (axis == 2);
-      const float f0 = g.at(i, j, k), f1 = g.at(i1, j1, k1);
+      const float f1 = g.at(b + (axis == 0 ? 1 : axis == 1 ? (int64_t)g.nx : g.layer));
       const bool crossing = f0 * f1 < 0.0f;
       const bool maybe = !(f0 * f1 <= 0.0f) && !(fabsf(f0) > g.near || fabsf(f1) > g.near);
       if (!crossing && !maybe) continue;
-      const F3 g0 = sample_gradient(g, i, j, k), g1 = sample_gradient(g, i1, j1, k1);
+      F3 g0 = f3(0.f, 0.f, 0.f), g1 = f3(0.f, 0.f, 0.f);
+      bool have_g = false;
+      if (maybe) {
+        g0 = sample_gradient(g, i, j, k);
+        g1 = sample_gradient(g, i1, j1, k1);
+        have_g = true;
+      }
       const bool thin = !crossing && dot(g0, g1) < -0.3f;
       if (!crossing && !thin) continue;
-      const F3 gn = add(g0, g1);
       // the four cells around the edge (:871-874, :894-897, :917-920)
       int ci[4], cj[4], ck[4];
       if (axis == 0) {
@@ -825,13 +862,29 @@ __global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
         ci[2] = i;     cj[2] = j;     ck[2] = k;
         ci[3] = i - 1; cj[3] = j;     ck[3] = k;
       }
-      int v[4];
-      F3 p[4];
-      bool ok = true;
+      int rk[4];
+      bool ok = true, split = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        v[q] = cell_vertex<MV, EMIT>(g, a, ci[q], cj[q], ck[q], gn, &p[q]);
-        ok = ok && v[q] >= 0;
+        rk[q] = cell_rank(g, a, ci[q], cj[q], ck[q]);
+        ok = ok && rk[q] >= 0;
+        if (MV && rk[q] >= 0) split = split || a.vcnt[rk[q]] == 2;
+      }
+      if (ok && (EMIT || split) && !have_g) {
+        g0 = sample_gradient(g, i, j, k);
+        g1 = sample_gradient(g, i1, j1, k1);
+      }
+      const F3 gn = add(g0, g1);
+      int v[4], h[4] = {0, 0, 0, 0};
+      if (ok) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (MV && a.vcnt[rk[q]] == 2) {
+            h[q] = pick_vertex(a, rk[q], gn);
+            ok = ok && h[q] >= 0;
+          }
+          v[q] = MV ? (int)a.vstart[rk[q]] + h[q] : rk[q];
+        }
       }
       if (!ok) {
         ++skipped;
@@ -839,6 +892,12 @@ __global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
       }
       nf += thin ? 2 : 1;
       if (EMIT) {
+        F3 p[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float* src = MV ? a.spos + 6 * (int64_t)rk[q] + 3 * h[q] : a.verts + 3 * (int64_t)rk[q];
+          p[q] = f3(src[0], src[1], src[2]);
+        }
         F3 qn = cross(sub(p[1], p[0]), sub(p[3], p[0]));
         if (dot(qn, qn) < 1e-12f) qn = cross(sub(p[2], p[0]), sub(p[3], p[0]));
         if (dot(gn, gn) > 0.0f && dot(qn, qn) > 0.0f && dot(qn, gn) < 0.0f) {
@@ -855,7 +914,11 @@ __global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
   }
   // the reference counts an edge without its four vertices in its emission pass, which skips
   // cells without quads (:953, :1008-1010)
-  if (MV && EMIT) flush(skipped, a.ctr + kSkipped);
+  if (MV && EMIT) {
+    const uint32_t v[1] = {skipped};
+    uint32_t* const dst[1] = {a.ctr + kSkipped};
+    block_flush<1>(v, dst);
+  }
 }
 
 int64_t blocks_for(int64_t n, int64_t per_block) { return (n + per_block - 1) / per_block; }
@@ -965,7 +1028,7 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
   }
   SP_TRY(mark(1), "event");
 
-  SpGrid g;
+  SpGrid g{};
   g.f = vals;
   g.nx = nx;
   g.ny = ny;
@@ -974,6 +1037,7 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
   g.cy = cy;
   g.cz = cz;
   g.W = W;
+  g.layer = (int64_t)nx * ny;
   g.vx = grid.vx;
   g.vy = grid.vy;
   g.vz = grid.vz;
@@ -983,7 +1047,8 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
   g.near = 0.45f * grid.vx;
 
   SP_RC(grow(kPlanes, (size_t)n_sw * 3 * 8));
-  SP_RC(grow(kRows, (size_t)n_rows * 2 * 8));
+  const int64_t n_jrows = (int64_t)nz * nby * Ws;  // first step of the brick rows
+  SP_RC(grow(kRows, (size_t)(n_rows + n_jrows) * 2 * 8));
   SP_RC(grow(kBricks, (size_t)n_bricks));
   SP_RC(grow(kAct, (size_t)(n_words + 1) * 8));
   SP_RC(grow(kWpre, (size_t)(n_words + 1) * 4));
@@ -1006,10 +1071,13 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
   const int64_t plane_waves = (n_sw + kPlaneWords - 1) / kPlaneWords;
   SP_LAUNCH(k_sp_planes, blocks_for(plane_waves, kThreads / 64), g, Ws, n_sw, N, P, C, ctr);
   SP_TRY(mark(2), "event");
-  SP_LAUNCH(k_sp_brick_rows, blocks_for(n_rows, kThreads), N, P, ny, nz, Ws, B, nby, n_rows, rows,
-            rows + n_rows);
-  SP_LAUNCH(k_sp_brick_flags, blocks_for(n_bricks, kThreads), rows, rows + n_rows, nx, Ws, B, nbx,
-            n_bricks, bricks, ctr);
+  uint64_t* jrows = rows + 2 * n_rows;
+  SP_LAUNCH(k_sp_rows_j, blocks_for(n_jrows, kThreads), N, P, ny, Ws, B, nby, n_jrows, jrows,
+            jrows + n_jrows);
+  SP_LAUNCH(k_sp_rows_k, blocks_for(n_rows, kThreads), jrows, jrows + n_jrows, nz, Ws, B, nby,
+            n_rows, rows, rows + n_rows);
+  SP_LAUNCH(k_sp_brick_flags, std::min<int64_t>(blocks_for(n_bricks, kThreads), 512), rows,
+            rows + n_rows, nx, Ws, B, nbx, n_bricks, bricks, ctr);
   SP_TRY(mark(3), "event");
   const int chunks = (cy + kCellRows - 1) / kCellRows;
   const int64_t n_cell_threads = (int64_t)cz * chunks * W;
@@ -1150,10 +1218,11 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
     st->qef_fallback_count = h_ctr_[kFallbacks];
     st->clamp_count = h_ctr_[kClamps];
     st->invalid_count = h_ctr_[kInvalid];
-    st->two_vertex_cells = multi_vertex ? h_ctr_[kTwo] : 0;
+    // the multi-vertex form: every active cell has one or two vertices
+    st->two_vertex_cells = multi_vertex ? nv - na : 0;
     st->ambiguous_cells = st->two_vertex_cells;
     st->multi_vertex_cells = st->two_vertex_cells;
-    st->one_vertex_cells = multi_vertex ? h_ctr_[kOne] : 0;
+    st->one_vertex_cells = multi_vertex ? na - st->two_vertex_cells : 0;
     st->split_rejection_count = multi_vertex ? h_ctr_[kSplitRej] : 0;
     st->bad_qef_count = 0;
     st->faces_skipped_due_to_missing_cluster = multi_vertex ? h_ctr_[kSkipped] : 0;

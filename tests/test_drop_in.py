@@ -85,11 +85,20 @@ def test_pybind_module_builds_and_imports(tmp_path):
         "    print('extracted', st.vertex_count, len(mesh.faces))\n"
         "except RuntimeError as e:\n"
         "    print('raised', e)\n"
+        "try:\n"
+        "    sp = m.HipSparseMvdcDualContouringBackend(brick_size=4, normal_mode=1)\n"
+        "    assert sp.multi_vertex_cells and sp.brick_size == 4\n"
+        "    mesh, st = sp.extract(g)\n"
+        "    print('sparse', st.backend, st.vertex_count)\n"
+        "except RuntimeError as e:\n"
+        "    print('sparse raised', e)\n"
     )
     r = subprocess.run([sys.executable, "-c", code, str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    out = r.stdout.strip()
+    out, sp = r.stdout.strip().split("\n")
     # without a device the constructor raises; with one, the all-positive grid has no vertices
     assert out.startswith("raised HipDualContouringBackend: no HIP device") or \
         out == "extracted 0 0", out
+    assert sp.startswith("sparse raised HipSparseDualContouringBackend: no HIP device") or \
+        sp == "sparse hip-sparse-mvdc 0", sp

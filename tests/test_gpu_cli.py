@@ -175,3 +175,58 @@ def test_cli_mesh_input_errors(tmp_path):
     empty.write_text("v 0 0 0\n")
     r = _cli("-i", empty, "-o", tmp_path / "x.obj", "-g", 16)
     assert r.returncode == 1 and "Failed to convert OBJ" in r.stderr
+
+
+@pytest.mark.parametrize("backend,mode,brick", [("hip-sparse", "finite-difference", 8),
+                                                ("hip-sparse-mvdc", "edge-gradient", 4),
+                                                ("mvdc", "precomputed-gradient", 3)])
+def test_cli_sparse_backends(tmp_path, backend, mode, brick):
+    """-b hip-sparse / hip-sparse-mvdc / mvdc with --brick-size and --normal-mode
+    (src/cpp/dc_cli.cpp:311-321): OBJ and the stats block against the oracle's sparse backends."""
+    out = tmp_path / "m.obj"
+    r = _cli("--generate", "box", "-g", 40, "-b", backend, "--normal-mode", mode, "--brick-size",
+             brick, "--chunk-size", 1000, "--benchmark", "-o", out)
+    assert r.returncode == 0, r.stderr
+    mv = backend != "hip-sparse"
+    modes = {"finite-difference": 0, "edge-gradient": 1, "precomputed-gradient": 2}
+    want = oracle.sparse_grid(synth.cli_generate("box", 40), brick_size=brick,
+                              normal_mode=modes[mode], multi_vertex=mv)
+    V, F = read_obj(out)
+    np.testing.assert_array_equal(F, want["faces"])
+    np.testing.assert_allclose(V, want["vertices"], rtol=0, atol=2e-6)
+    assert ("Backend: hip-sparse-mvdc" if mv else "Backend: hip-sparse") in r.stdout
+    st = want["stats"]
+    assert f"Active Bricks: {st['active_bricks']} " in r.stdout
+    assert f"Vertex Count: {st['vertex_count']}" in r.stdout
+    assert f"Face Count: {st['face_count']}" in r.stdout
+    if st["two_vertex_cells"]:
+        assert f"Two-Vertex Cells: {st['two_vertex_cells']}" in r.stdout
+
+
+def test_cli_mesh_input_sparse_mvdc_and_postprocess(tmp_path):
+    """The web app's call (webapp/server.py:63-78): dc_cli -i in.obj -o out.obj -g res -b <mvdc>
+    --brick-size 8 with the post-processing flags: mesh-to-SDF on the GPU, the multi-vertex
+    extraction on the device grid, then the GPU post-processing of the sparse mesh; equal to the
+    oracle's three steps."""
+    V, T = synth.torus_mesh(R=0.3, r=0.12, nu=40, nv=20)
+    obj_in = tmp_path / "torus.obj"
+    obj_in.write_text("\n".join([f"v {x:.9g} {y:.9g} {z:.9g}" for x, y, z in V] +
+                                [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in T]) + "\n")
+    res_n = 48
+    dims, origin, spacing = _obj_grid(V, res_n)
+    grid = oracle.mesh_to_sdf(V, T, dims, origin, spacing)
+    want = oracle.sparse(grid.ravel(), *dims, *spacing, *origin, brick_size=8, multi_vertex=True)
+    out = tmp_path / "out.obj"
+    r = _cli("-i", obj_in, "-o", out, "-g", res_n, "-b", "hip-sparse-mvdc", "--brick-size", 8)
+    assert r.returncode == 0, r.stderr
+    Vo, Fo = read_obj(out)
+    np.testing.assert_array_equal(Fo, want["faces"])
+    np.testing.assert_allclose(Vo, want["vertices"], rtol=0, atol=2e-6)
+    V2, F2, _ = oracle.postprocess(want["vertices"], want["faces"], True, True)
+    r = _cli("-i", obj_in, "-o", out, "-g", res_n, "-b", "hip-sparse-mvdc", "--close-holes",
+             "--remove-floaters", "--benchmark")
+    assert r.returncode == 0, r.stderr
+    Vo, Fo = read_obj(out)
+    np.testing.assert_array_equal(Fo, F2)
+    np.testing.assert_allclose(Vo, V2, rtol=0, atol=2e-6)
+    assert f"Face Count: {F2.shape[0]}" in r.stdout
