@@ -1213,8 +1213,8 @@ int run_sparse(dc_hip_context* c, const dc_hip_grid* grid, bool on_device, int32
   hipStream_t s;
   int rc;
   if ((rc = work_stream(c, &s))) return rc;
-  return c->sparse->run(*grid, on_device, brick_size, normal_mode, multi_vertex != 0, s, stats,
-                        &c->err);
+  return c->sparse->run(*grid, on_device, brick_size, normal_mode, multi_vertex != 0, c->cu_count,
+                        s, stats, &c->err);
 }
 }  // namespace
 

@@ -20,7 +20,8 @@ class DcSparse {
   // says) on stream s; the mesh stays in device memory (vertices(), faces()) until the next call.
   // Returns a dc_hip_status; on failure *err names the cause. Synchronises s.
   int run(const dc_hip_grid& grid, bool on_device, int brick_size, int normal_mode,
-          bool multi_vertex, hipStream_t s, dc_hip_sparse_stats* st, std::string* err);
+          bool multi_vertex, int cu_count, hipStream_t s, dc_hip_sparse_stats* st,
+          std::string* err);
 
   bool have() const { return have_; }
   int64_t n_vertices() const { return nv_; }
@@ -55,6 +56,7 @@ class DcSparse {
 
   Buf b_[kNumBufs];
   hipStream_t s_ = nullptr;
+  int cu_ = 256;  // compute units of the device (grid-stride launch sizes)
   std::string* err_ = nullptr;
   uint32_t* h_ctr_ = nullptr;  // pinned host copy of the counters
   hipEvent_t ev_[20] = {};

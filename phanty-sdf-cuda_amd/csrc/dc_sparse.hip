@@ -385,50 +385,6 @@ __device__ __forceinline__ F3 solve_cluster(const Cell& c, uint32_t tA, uint32_t
   return x;
 }
 
-// Normal of slot s (static after unrolling).
-__device__ __forceinline__ F3 slot_normal(const Cell& c, int s) {
-  return (s & 1) ? c.gc[eb(s >> 1)] : c.nA[s >> 1];
-}
-
-// dc_cuda_sparse.cu:577-589: the first most opposed pair of entries (strictly smallest dot, in
-// entry order); SLOTS = 1 looks at the slots 2e only (no cell of the wave has a thin edge).
-template <int SLOTS>
-__device__ __forceinline__ float opposed_pair(const Cell& c, int& ia, int& ib) {
-  const uint64_t use = [&] {
-    uint64_t u = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) {
-      u |= (uint64_t)((c.useA >> e) & 1) << (2 * e);
-      u |= (uint64_t)((c.useB >> e) & 1) << (2 * e + 1);
-    }
-    return u;
-  }();
-  float lo = 1.0f;
-  ia = ib = -1;
-#pragma unroll
-  for (int a = 0; a < 24; a += 3 - SLOTS) {
-#pragma unroll
-    for (int b = a + 3 - SLOTS; b < 24; b += 3 - SLOTS) {
-      if (((use >> a) & (use >> b) & 1) == 0) continue;
-      const float t = dot(slot_normal(c, a), slot_normal(c, b));
-      if (t < lo) {
-        lo = t;
-        ia = a;
-        ib = b;
-      }
-    }
-  }
-  return lo;
-}
-
-__device__ __forceinline__ F3 pick_slot(const Cell& c, int s) {
-  F3 r = f3(0.f, 0.f, 0.f);
-#pragma unroll
-  for (int q = 0; q < 24; ++q)
-    if (q == s) r = slot_normal(c, q);
-  return r;
-}
-
 // Active-list entry -> cell (act words number fewer than 2^26: 32-bit arithmetic).
 __device__ __forceinline__ void decode_cell(uint64_t entry, int W, int cy, int& i, int& j, int& k) {
   const uint32_t word = (uint32_t)(entry >> 6);
@@ -652,10 +608,11 @@ __global__ void __launch_bounds__(kThreads)
 // form the split (:565-620) and one solve per cluster with the clusters' mean normals
 // (:786-826), else one solve.
 template <int MODE, bool MV>
-__global__ void __launch_bounds__(kThreads)
-    k_sp_vertices(SpGrid g, const uint64_t* __restrict__ alist, int64_t na,
-                  float* __restrict__ verts, float* __restrict__ spos, float* __restrict__ snrm,
-                  uint32_t* __restrict__ vcnt, uint32_t* __restrict__ ctr) {
+__device__ __forceinline__ void sp_vertices(SpGrid g, const uint64_t* __restrict__ alist,
+                                            int64_t na, float* __restrict__ verts,
+                                            float* __restrict__ spos, float* __restrict__ snrm,
+                                            uint32_t* __restrict__ vcnt,
+                                            uint32_t* __restrict__ ctr) {
   const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   Tally tl;
   const bool live = r < na;
@@ -677,41 +634,71 @@ __global__ void __launch_bounds__(kThreads)
       verts[3 * r + 2] = x.z;
     }
   } else {
-    // the split decision
+    // the split decision (:565-620): the entry normals in entry order go to LDS (one column per
+    // thread), so the pair search and the sides run over this cell's m entries only
+    __shared__ float s_n[3 * 12 * kThreads];
+    auto put = [&](int q, F3 n) {
+      s_n[(3 * q) * kThreads + threadIdx.x] = n.x;
+      s_n[(3 * q + 1) * kThreads + threadIdx.x] = n.y;
+      s_n[(3 * q + 2) * kThreads + threadIdx.x] = n.z;
+    };
+    auto get = [&](int q) {
+      return f3(s_n[(3 * q) * kThreads + threadIdx.x], s_n[(3 * q + 1) * kThreads + threadIdx.x],
+                s_n[(3 * q + 2) * kThreads + threadIdx.x]);
+    };
     int nv = 1;
     uint32_t t0A = all, t0B = allB;
-    int ia = -1, ib = -1;
+    F3 vA = f3(0.f, 0.f, 0.f), vB = f3(0.f, 0.f, 0.f);
     if (live && c.m >= 2) {
-      const float lo = __ballot(allB != 0) ? opposed_pair<2>(c, ia, ib) : opposed_pair<1>(c, ia, ib);
-      if (ia >= 0 && ib >= 0 && !(lo >= -0.3f)) {
-        const F3 vA = pick_slot(c, ia), vB = pick_slot(c, ib);
-        uint32_t sA = 0, sB = 0;
-        int n0 = 0, n1 = 0;
+      int q = 0;
 #pragma unroll
-        for (int e = 0; e < 12; ++e) {
-          if ((all >> e) & 1) {
-            const F3 n = c.nA[e];
-            if (dot(n, vA) >= dot(n, vB)) {
-              sA |= 1u << e;
-              ++n0;
-            } else {
-              ++n1;
-            }
-          }
-          if ((allB >> e) & 1) {
-            const F3 n = c.gc[eb(e)];
-            if (dot(n, vA) >= dot(n, vB)) {
-              sB |= 1u << e;
-              ++n0;
-            } else {
-              ++n1;
-            }
+      for (int e = 0; e < 12; ++e) {
+        if ((all >> e) & 1) put(q++, c.nA[e]);
+        if ((allB >> e) & 1) put(q++, c.gc[eb(e)]);
+      }
+      float lo = 1.0f;
+      int ia = -1, ib = -1;
+      for (int x = 0; x < c.m; ++x) {
+        const F3 nx_ = get(x);
+        for (int y = x + 1; y < c.m; ++y) {
+          const float t = dot(nx_, get(y));
+          if (t < lo) {
+            lo = t;
+            ia = x;
+            ib = y;
           }
         }
-        if (n0 < 2 || n1 < 2) {
+      }
+      if (ia >= 0 && ib >= 0 && !(lo >= -0.3f)) {
+        vA = get(ia);
+        vB = get(ib);
+        uint32_t side = 0;  // bit q: entry q is nearer nA
+        int n0 = 0;
+        for (int x = 0; x < c.m; ++x) {
+          const F3 n = get(x);
+          if (dot(n, vA) >= dot(n, vB)) {
+            side |= 1u << x;
+            ++n0;
+          }
+        }
+        if (n0 < 2 || c.m - n0 < 2) {
           ++tl.split_rej;
         } else {
           nv = 2;
+          // entry bits back to slot masks (entries are in slot order)
+          uint32_t sA = 0, sB = 0;
+          int x = 0;
+#pragma unroll
+          for (int e = 0; e < 12; ++e) {
+            if ((all >> e) & 1) {
+              sA |= ((side >> x) & 1u) << e;
+              ++x;
+            }
+            if ((allB >> e) & 1) {
+              sB |= ((side >> x) & 1u) << e;
+              ++x;
+            }
+          }
           t0A = sA;
           t0B = sB;
         }
@@ -736,8 +723,8 @@ __global__ void __launch_bounds__(kThreads)
           if ((t1B >> e) & 1) s1 = add(s1, c.gc[eb(e)]);
         }
         const float l0 = sqrtf(dot(s0, s0)), l1 = sqrtf(dot(s1, s1));
-        const F3 n0 = l0 > 1e-6f ? mul(1.0f / l0, s0) : pick_slot(c, ia);
-        const F3 n1 = l1 > 1e-6f ? mul(1.0f / l1, s1) : pick_slot(c, ib);
+        const F3 n0 = l0 > 1e-6f ? mul(1.0f / l0, s0) : vA;
+        const F3 n1 = l1 > 1e-6f ? mul(1.0f / l1, s1) : vB;
         spos[6 * r] = x0.x;
         spos[6 * r + 1] = x0.y;
         spos[6 * r + 2] = x0.z;
@@ -758,6 +745,23 @@ __global__ void __launch_bounds__(kThreads)
   const uint32_t v[4] = {tl.fallbacks, tl.clamps, tl.invalid, tl.split_rej};
   uint32_t* const dst[4] = {ctr + kFallbacks, ctr + kClamps, ctr + kInvalid, ctr + kSplitRej};
   block_flush<4>(v, dst);
+}
+
+// The one-vertex form held to 128 registers (four waves per SIMD); the multi-vertex form keeps
+// its split state in registers beside the LDS normals.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+    k_sp_vertices1(SpGrid g, const uint64_t* __restrict__ alist, int64_t na,
+                   float* __restrict__ verts, uint32_t* __restrict__ ctr) {
+  sp_vertices<MODE, false>(g, alist, na, verts, nullptr, nullptr, nullptr, ctr);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_sp_vertices2(SpGrid g, const uint64_t* __restrict__ alist, int64_t na,
+                   float* __restrict__ spos, float* __restrict__ snrm, uint32_t* __restrict__ vcnt,
+                   uint32_t* __restrict__ ctr) {
+  sp_vertices<MODE, true>(g, alist, na, nullptr, spos, snrm, vcnt, ctr);
 }
 
 // Multi-vertex form: the vertices in id order from the slots.
@@ -982,9 +986,11 @@ static T* at(void* p) {
 }
 
 int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int normal_mode,
-                  bool multi_vertex, hipStream_t s, dc_hip_sparse_stats* st, std::string* err) {
+                  bool multi_vertex, int cu_count, hipStream_t s, dc_hip_sparse_stats* st,
+                  std::string* err) {
   const auto t_start = std::chrono::steady_clock::now();
   s_ = s;
+  cu_ = std::max(1, cu_count);
   err_ = err;
   have_ = false;
   nv_ = nf_ = 0;
@@ -1128,22 +1134,19 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
     uint32_t* tsum = at<uint32_t>(b_[kTsum].p);
     const int64_t blk = blocks_for(na, kThreads);
     SP_TRY(mark(6), "event");
-#define SP_VERTS(M, V) \
-  SP_LAUNCH((k_sp_vertices<M, V>), blk, g, alist, na, verts, spos, snrm, vcnt, ctr)
     if (multi_vertex) {
       if (normal_mode == 1)
-        SP_VERTS(1, true);
+        SP_LAUNCH(k_sp_vertices2<1>, blk, g, alist, na, spos, snrm, vcnt, ctr);
       else
-        SP_VERTS(0, true);
+        SP_LAUNCH(k_sp_vertices2<0>, blk, g, alist, na, spos, snrm, vcnt, ctr);
       SP_TRY(dcscan::exclusive_scan(vcnt, na, vstart, tsum, ctr + kVertTotal, s_), "vertex scan");
       SP_LAUNCH(k_sp_place, blk, spos, vcnt, vstart, na, verts);
     } else {
       if (normal_mode == 1)
-        SP_VERTS(1, false);
+        SP_LAUNCH(k_sp_vertices1<1>, blk, g, alist, na, verts, ctr);
       else
-        SP_VERTS(0, false);
+        SP_LAUNCH(k_sp_vertices1<0>, blk, g, alist, na, verts, ctr);
     }
-#undef SP_VERTS
     SP_TRY(mark(7), "event");
     FaceArgs fa;
     fa.act = act;
