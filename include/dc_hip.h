@@ -285,7 +285,10 @@ typedef struct dc_hip_sparse_stats {
   float active_cell_compaction_ms;   /* active-plane scan (ranks and the cell list) */
   float marking_ms, compaction_ms;
   float qef_ms;                      /* vertices (multi-vertex form: with their scan) */
-  float face_count_ms, face_prefix_sum_ms, face_fill_ms, face_emission_ms;
+  float face_count_ms;               /* the quad pass: each wave's quads into its staging run */
+  float face_prefix_sum_ms;          /* scan of the per-wave quad counts */
+  float face_fill_ms;                /* the staging runs copied into face order */
+  float face_emission_ms;            /* the three above */
   float download_ms;                 /* 0 here: the mesh stays on the device */
   float total_ms;                    /* host wall time of the call */
   float sample_pass_ms;              /* the grid-reading kernel alone */

@@ -46,7 +46,8 @@ class DcSparse {
     kVerts,                // vertices (3 floats each)
     kSlotPos, kSlotNrm,    // multi-vertex form: two position and normal slots per active cell
     kVcnt, kVstart,        // multi-vertex form: vertices per active cell and their scan
-    kFcnt, kFstart,        // faces per active cell and their scan
+    kFcnt, kFstart,        // quads per wave of active cells and their scan
+    kStage,                // per-wave staging runs of quads
     kTsum,                 // tile sums of the uint32 scans
     kFaces,                // quads (4 int32 each)
     kNumBufs

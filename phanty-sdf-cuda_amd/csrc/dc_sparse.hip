@@ -789,9 +789,6 @@ struct FaceArgs {
   const float* snrm;
   const uint32_t* vcnt;
   const uint32_t* vstart;
-  uint32_t* fcnt;
-  const uint32_t* fstart;
-  int32_t* faces;
   uint32_t* ctr;
 };
 
@@ -815,114 +812,132 @@ __device__ __forceinline__ int pick_vertex(const FaceArgs& a, int rank, F3 gn) {
   return d0 >= d1 ? 0 : 1;
 }
 
-// dc_cuda_sparse.cu:853-1102: per active cell its x, y and z edge from its first corner. EMIT
-// false: the quads per cell (the edge gradients only where a thin-edge test or a split cell needs
-// them); EMIT true: the quads at the cell's first quad, oriented along the summed gradients.
-template <bool MV, bool EMIT>
-__global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a) {
-  const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  uint32_t skipped = 0;
-  if (r < a.na && (!EMIT || a.fcnt[r] != 0)) {
-    int i, j, k;
-    decode_cell(a.alist[r], g.W, g.cy, i, j, k);
-    uint32_t nf = 0;
-    int64_t out = EMIT ? (int64_t)a.fstart[r] : 0;
-    const int64_t b = g.idx(i, j, k);
-    const float f0 = g.at(b);
+constexpr int kFacesPerCell = 6;                       // 3 edges, 2 quads for a thin edge
+constexpr int kStagePerWave = 64 * kFacesPerCell;      // quads a wave can emit
+
+// dc_cuda_sparse.cu:853-1102: the quads of active cell r: its x, y and z edge from its first
+// corner, a strict sign change giving a quad and a thin edge the quad and its reverse, each of
+// the four cells around the edge giving a vertex (:871-874, :894-897, :917-920), oriented along
+// the summed end gradients. *skipped counts the edges that lacked a vertex.
+template <bool MV>
+__device__ __forceinline__ int cell_faces(const SpGrid& g, const FaceArgs& a, int64_t r,
+                                          int4 (&fq)[kFacesPerCell], uint32_t* skipped) {
+  int i, j, k;
+  decode_cell(a.alist[r], g.W, g.cy, i, j, k);
+  int nf = 0;
+  const int64_t b = g.idx(i, j, k);
+  const float f0 = g.at(b);
 #pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      const bool has = axis == 0 ? (j >= 1 && k >= 1) : axis == 1 ? (i >= 1 && k >= 1)
-                                                                   : (i >= 1 && j >= 1);
-      if (!has) continue;
-      const int i1 = i + (axis == 0), j1 = j + (axis == 1), k1 = k + (axis == 2);
-      const float f1 = g.at(b + (axis == 0 ? 1 : axis == 1 ? (int64_t)g.nx : g.layer));
-      const bool crossing = f0 * f1 < 0.0f;
-      const bool maybe = !(f0 * f1 <= 0.0f) && !(fabsf(f0) > g.near || fabsf(f1) > g.near);
-      if (!crossing && !maybe) continue;
-      F3 g0 = f3(0.f, 0.f, 0.f), g1 = f3(0.f, 0.f, 0.f);
-      bool have_g = false;
-      if (maybe) {
-        g0 = sample_gradient(g, i, j, k);
-        g1 = sample_gradient(g, i1, j1, k1);
-        have_g = true;
-      }
-      const bool thin = !crossing && dot(g0, g1) < -0.3f;
-      if (!crossing && !thin) continue;
-      // the four cells around the edge (:871-874, :894-897, :917-920)
-      int ci[4], cj[4], ck[4];
-      if (axis == 0) {
-        ci[0] = i; cj[0] = j - 1; ck[0] = k - 1;
-        ci[1] = i; cj[1] = j;     ck[1] = k - 1;
-        ci[2] = i; cj[2] = j;     ck[2] = k;
-        ci[3] = i; cj[3] = j - 1; ck[3] = k;
-      } else if (axis == 1) {
-        ci[0] = i - 1; cj[0] = j; ck[0] = k - 1;
-        ci[1] = i;     cj[1] = j; ck[1] = k - 1;
-        ci[2] = i;     cj[2] = j; ck[2] = k;
-        ci[3] = i - 1; cj[3] = j; ck[3] = k;
-      } else {
-        ci[0] = i - 1; cj[0] = j - 1; ck[0] = k;
-        ci[1] = i;     cj[1] = j - 1; ck[1] = k;
-        ci[2] = i;     cj[2] = j;     ck[2] = k;
-        ci[3] = i - 1; cj[3] = j;     ck[3] = k;
-      }
-      int rk[4];
-      bool ok = true, split = false;
+  for (int axis = 0; axis < 3; ++axis) {
+    const bool has = axis == 0 ? (j >= 1 && k >= 1) : axis == 1 ? (i >= 1 && k >= 1)
+                                                                 : (i >= 1 && j >= 1);
+    if (!has) continue;
+    const int i1 = i + (axis == 0), j1 = j + (axis == 1), k1 = k + (axis == 2);
+    const float f1 = g.at(b + (axis == 0 ? 1 : axis == 1 ? (int64_t)g.nx : g.layer));
+    const bool crossing = f0 * f1 < 0.0f;
+    const bool maybe = !(f0 * f1 <= 0.0f) && !(fabsf(f0) > g.near || fabsf(f1) > g.near);
+    if (!crossing && !maybe) continue;
+    const F3 g0 = sample_gradient(g, i, j, k), g1 = sample_gradient(g, i1, j1, k1);
+    const bool thin = !crossing && dot(g0, g1) < -0.3f;
+    if (!crossing && !thin) continue;
+    const F3 gn = add(g0, g1);
+    int ci[4], cj[4], ck[4];
+    if (axis == 0) {
+      ci[0] = i; cj[0] = j - 1; ck[0] = k - 1;
+      ci[1] = i; cj[1] = j;     ck[1] = k - 1;
+      ci[2] = i; cj[2] = j;     ck[2] = k;
+      ci[3] = i; cj[3] = j - 1; ck[3] = k;
+    } else if (axis == 1) {
+      ci[0] = i - 1; cj[0] = j; ck[0] = k - 1;
+      ci[1] = i;     cj[1] = j; ck[1] = k - 1;
+      ci[2] = i;     cj[2] = j; ck[2] = k;
+      ci[3] = i - 1; cj[3] = j; ck[3] = k;
+    } else {
+      ci[0] = i - 1; cj[0] = j - 1; ck[0] = k;
+      ci[1] = i;     cj[1] = j - 1; ck[1] = k;
+      ci[2] = i;     cj[2] = j;     ck[2] = k;
+      ci[3] = i - 1; cj[3] = j;     ck[3] = k;
+    }
+    int rk[4], v[4], h[4] = {0, 0, 0, 0};
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      rk[q] = cell_rank(g, a, ci[q], cj[q], ck[q]);
+      ok = ok && rk[q] >= 0;
+    }
+    if (ok) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        rk[q] = cell_rank(g, a, ci[q], cj[q], ck[q]);
-        ok = ok && rk[q] >= 0;
-        if (MV && rk[q] >= 0) split = split || a.vcnt[rk[q]] == 2;
-      }
-      if (ok && (EMIT || split) && !have_g) {
-        g0 = sample_gradient(g, i, j, k);
-        g1 = sample_gradient(g, i1, j1, k1);
-      }
-      const F3 gn = add(g0, g1);
-      int v[4], h[4] = {0, 0, 0, 0};
-      if (ok) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (MV && a.vcnt[rk[q]] == 2) {
-            h[q] = pick_vertex(a, rk[q], gn);
-            ok = ok && h[q] >= 0;
-          }
-          v[q] = MV ? (int)a.vstart[rk[q]] + h[q] : rk[q];
+        if (MV && a.vcnt[rk[q]] == 2) {
+          h[q] = pick_vertex(a, rk[q], gn);
+          ok = ok && h[q] >= 0;
         }
-      }
-      if (!ok) {
-        ++skipped;
-        continue;
-      }
-      nf += thin ? 2 : 1;
-      if (EMIT) {
-        F3 p[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float* src = MV ? a.spos + 6 * (int64_t)rk[q] + 3 * h[q] : a.verts + 3 * (int64_t)rk[q];
-          p[q] = f3(src[0], src[1], src[2]);
-        }
-        F3 qn = cross(sub(p[1], p[0]), sub(p[3], p[0]));
-        if (dot(qn, qn) < 1e-12f) qn = cross(sub(p[2], p[0]), sub(p[3], p[0]));
-        if (dot(gn, gn) > 0.0f && dot(qn, qn) > 0.0f && dot(qn, gn) < 0.0f) {
-          const int t = v[1];
-          v[1] = v[3];
-          v[3] = t;
-        }
-        int4* dst = reinterpret_cast<int4*>(a.faces);
-        dst[out++] = make_int4(v[0], v[1], v[2], v[3]);
-        if (thin) dst[out++] = make_int4(v[0], v[3], v[2], v[1]);
+        v[q] = MV ? (int)a.vstart[rk[q]] + h[q] : rk[q];
       }
     }
-    if (!EMIT) a.fcnt[r] = nf;
+    if (!ok) {
+      ++*skipped;
+      continue;
+    }
+    F3 p[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float* src = MV ? a.spos + 6 * (int64_t)rk[q] + 3 * h[q] : a.verts + 3 * (int64_t)rk[q];
+      p[q] = f3(src[0], src[1], src[2]);
+    }
+    F3 qn = cross(sub(p[1], p[0]), sub(p[3], p[0]));
+    if (dot(qn, qn) < 1e-12f) qn = cross(sub(p[2], p[0]), sub(p[3], p[0]));
+    if (dot(gn, gn) > 0.0f && dot(qn, qn) > 0.0f && dot(qn, gn) < 0.0f) {
+      const int t = v[1];
+      v[1] = v[3];
+      v[3] = t;
+    }
+    // static slots: the quads of edge `axis` go to slots 2 axis and 2 axis + 1, then move down
+    fq[2 * axis] = make_int4(v[0], v[1], v[2], v[3]);
+    fq[2 * axis + 1] = make_int4(v[0], v[3], v[2], v[1]);
+    nf |= (thin ? 3 : 1) << (2 * axis);
   }
+  return nf;  // bits 2a, 2a + 1: slots of edge a in use
+}
+
+// The quads of every active cell in one pass: each wave writes its cells' quads, in cell order,
+// to its own staging run (kStagePerWave quads) and its count to wcount; a scan of the counts and
+// k_sp_face_copy then place the runs. The reference counts, scans and emits in three passes
+// (:1359-1392); the order of the quads is the same.
+template <bool MV>
+__global__ void __launch_bounds__(kThreads) k_sp_faces(SpGrid g, FaceArgs a, int4* __restrict__ stage,
+                                                       uint32_t* __restrict__ wcount) {
+  const int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t skipped = 0;
+  int4 fq[kFacesPerCell];
+  int used = 0;
+  if (r < a.na) used = cell_faces<MV>(g, a, r, fq, &skipped);
+  const uint32_t n = (uint32_t)__popc(used);
+  const uint32_t incl = wave_scan_incl(n, lane);
+  const int64_t wave = r >> 6;
+  int4* out = stage + wave * kStagePerWave + (incl - n);
+  int q = 0;
+#pragma unroll
+  for (int s = 0; s < kFacesPerCell; ++s)
+    if ((used >> s) & 1) out[q++] = fq[s];
+  if (lane == 63 && wave * 64 < a.na) wcount[wave] = incl;
   // the reference counts an edge without its four vertices in its emission pass, which skips
   // cells without quads (:953, :1008-1010)
-  if (MV && EMIT) {
-    const uint32_t v[1] = {skipped};
+  if (MV) {
+    const uint32_t v[1] = {n ? skipped : 0u};
     uint32_t* const dst[1] = {a.ctr + kSkipped};
     block_flush<1>(v, dst);
   }
+}
+
+// One block of 64 threads per staging run: the run's quads to their place in the face order.
+__global__ void __launch_bounds__(64)
+    k_sp_face_copy(const int4* __restrict__ stage, const uint32_t* __restrict__ wcount,
+                   const uint32_t* __restrict__ wbase, int4* __restrict__ faces) {
+  const int64_t w = blockIdx.x;
+  const uint32_t n = wcount[w], base = wbase[w];
+  for (uint32_t q = threadIdx.x; q < n; q += 64) faces[base + q] = stage[w * kStagePerWave + q];
 }
 
 int64_t blocks_for(int64_t n, int64_t per_block) { return (n + per_block - 1) / per_block; }
@@ -1117,8 +1132,10 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
   float t_q = 0.f, t_fc = 0.f, t_fs = 0.f, t_ff = 0.f;
   if (na > 0) {
     SP_RC(grow(kVerts, (size_t)(multi_vertex ? 2 : 1) * na * 12));
-    SP_RC(grow(kFcnt, (size_t)na * 4));
-    SP_RC(grow(kFstart, (size_t)na * 4));
+    const int64_t n_waves = (na + 63) / 64;
+    SP_RC(grow(kFcnt, (size_t)n_waves * 4));
+    SP_RC(grow(kFstart, (size_t)n_waves * 4));
+    SP_RC(grow(kStage, (size_t)n_waves * kStagePerWave * 16));
     SP_RC(grow(kTsum, (size_t)std::max<int64_t>(1, dcscan::scan_tiles(na)) * 4));
     if (multi_vertex) {
       SP_RC(grow(kSlotPos, (size_t)na * 24));
@@ -1158,18 +1175,16 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
     fa.snrm = snrm;
     fa.vcnt = vcnt;
     fa.vstart = vstart;
-    fa.fcnt = at<uint32_t>(b_[kFcnt].p);
-    fa.fstart = at<uint32_t>(b_[kFstart].p);
-    fa.faces = nullptr;
     fa.ctr = ctr;
+    int4* stage = at<int4>(b_[kStage].p);
+    uint32_t* wcount = at<uint32_t>(b_[kFcnt].p);
+    uint32_t* wbase = at<uint32_t>(b_[kFstart].p);
     if (multi_vertex)
-      SP_LAUNCH((k_sp_faces<true, false>), blk, g, fa);
+      SP_LAUNCH(k_sp_faces<true>, blk, g, fa, stage, wcount);
     else
-      SP_LAUNCH((k_sp_faces<false, false>), blk, g, fa);
+      SP_LAUNCH(k_sp_faces<false>, blk, g, fa, stage, wcount);
     SP_TRY(mark(8), "event");
-    SP_TRY(dcscan::exclusive_scan(fa.fcnt, na, at<uint32_t>(b_[kFstart].p), tsum, ctr + kFaceTotal,
-                                  s_),
-           "face scan");
+    SP_TRY(dcscan::exclusive_scan(wcount, n_waves, wbase, tsum, ctr + kFaceTotal, s_), "face scan");
     SP_TRY(mark(9), "event");
     SP_TRY(hipMemcpyAsync(h_ctr_, ctr, kNumCtr * 4, hipMemcpyDeviceToHost, s_), "read counts");
     SP_TRY(hipStreamSynchronize(s_), "face count");
@@ -1177,13 +1192,11 @@ int DcSparse::run(const dc_hip_grid& grid, bool on_device, int brick_size, int n
     if (multi_vertex) nv = h_ctr_[kVertTotal];
     if (nf >= (1ll << 31) / 4) return fail(DC_HIP_ERR_INVALID_GRID, "mesh exceeds 32-bit face ids");
     SP_RC(grow(kFaces, (size_t)std::max<int64_t>(nf, 1) * 16));
-    fa.faces = at<int32_t>(b_[kFaces].p);
     SP_TRY(mark(10), "event");
     if (nf > 0) {
-      if (multi_vertex)
-        SP_LAUNCH((k_sp_faces<true, true>), blk, g, fa);
-      else
-        SP_LAUNCH((k_sp_faces<false, true>), blk, g, fa);
+      hipLaunchKernelGGL(k_sp_face_copy, dim3((unsigned)n_waves), dim3(64), 0, s_, stage, wcount,
+                         wbase, at<int4>(b_[kFaces].p));
+      SP_TRY(hipGetLastError(), "k_sp_face_copy");
     }
     SP_TRY(mark(11), "event");
     SP_TRY(hipMemcpyAsync(h_ctr_, ctr, kNumCtr * 4, hipMemcpyDeviceToHost, s_), "read counters");
