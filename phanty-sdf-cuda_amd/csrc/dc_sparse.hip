@@ -757,7 +757,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
     k_sp_vertices2(SpGrid g, const uint64_t* __restrict__ alist, int64_t na,
                    float* __restrict__ spos, float* __restrict__ snrm, uint32_t* __restrict__ vcnt,
                    uint32_t* __restrict__ ctr) {
