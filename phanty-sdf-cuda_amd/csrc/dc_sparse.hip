@@ -422,48 +422,66 @@ __device__ __forceinline__ void block_flush(const uint32_t (&v)[N], uint32_t* co
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
-// The sample bit planes: one wave per kPlaneWords consecutive sample words, one sample per lane
-// per word (256-byte coalesced rows), ballots as the plane words; non-finite samples flagged.
+// The sample bit planes: one wave per kPlaneWords consecutive sample words (one sample per lane
+// per word: 256-byte coalesced rows), all loads in flight before the ballots; non-finite samples
+// flagged. The wave's position and row pointers are wave-uniform (scalar registers), so the vector
+// work per word is the load, three compares and the finiteness test; the ballots reach the lanes
+// that store them through LDS.
 __global__ void __launch_bounds__(kThreads)
     k_sp_planes(SpGrid g, int Ws, int64_t n_words, uint64_t* __restrict__ N,
                 uint64_t* __restrict__ P, uint64_t* __restrict__ C, uint32_t* __restrict__ ctr) {
   const int lane = threadIdx.x & 63;
-  const int64_t q0 = ((int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * kPlaneWords;
-  if (q0 >= n_words) return;
-  int64_t row = q0 / Ws;
-  int w = (int)(q0 - row * Ws);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t q0 = ((int64_t)blockIdx.x * (kThreads / 64) + wv) * kPlaneWords;
+  if (q0 >= n_words) return;  // uniform per wave
+  const int nw = (int)min<int64_t>(kPlaneWords, n_words - q0);
+  const int64_t row0 = q0 / Ws;
+  int w = (int)(q0 - row0 * Ws);
+  gfloat rp = dccommon::to_global(g.f) + row0 * g.nx;  // the current row (uniform)
   float v[kPlaneWords];
-  bool ok[kPlaneWords];
+  uint64_t inm[kPlaneWords];  // lanes whose sample of word u lies in the row
 #pragma unroll
   for (int u = 0; u < kPlaneWords; ++u) {
-    const int x = 64 * w + lane;
-    ok[u] = q0 + u < n_words && x < g.nx;
-    v[u] = ok[u] ? dccommon::to_global(g.f)[row * g.nx + x] : 1.0f;
+    inm[u] = 0;
+    v[u] = 1.0f;
+    if (u < nw) {  // uniform
+      const int x = 64 * w + lane;
+      inm[u] = __ballot(x < g.nx);
+      v[u] = rp[min(x, g.nx - 1)];
+    }
     if (++w == Ws) {
       w = 0;
-      ++row;
+      rp += g.nx;
     }
   }
-  uint64_t mine = 0;
-  bool bad = false;
+  // the wave's plane words go through LDS to the lanes that store them: lane p*8 + u stores
+  // word u of plane p (N, P, C)
+  __shared__ uint64_t s_words[kThreads / 64][3 * kPlaneWords];
+  uint64_t* sw = s_words[wv];
+  uint64_t bad = 0;
 #pragma unroll
   for (int u = 0; u < kPlaneWords; ++u) {
-    const uint64_t bn = __ballot(ok[u] && v[u] < 0.0f);
-    const uint64_t bp = __ballot(ok[u] && v[u] > 0.0f);
-    const uint64_t bc = __ballot(ok[u] && fabsf(v[u]) <= g.near);
-    bad = bad || (ok[u] && !isfinite(v[u]));
-    if (lane == u) mine = bn;
-    if (lane == kPlaneWords + u) mine = bp;
-    if (lane == 2 * kPlaneWords + u) mine = bc;
-  }
-  if (lane < 3 * kPlaneWords) {
-    const int u = lane % kPlaneWords;
-    if (q0 + u < n_words) {
-      uint64_t* dst = lane < kPlaneWords ? N : (lane < 2 * kPlaneWords ? P : C);
-      dst[q0 + u] = mine;
+    const uint64_t bn = __ballot(v[u] < 0.0f) & inm[u];
+    const uint64_t bp = __ballot(v[u] > 0.0f) & inm[u];
+    const uint64_t bc = __ballot(fabsf(v[u]) <= g.near) & inm[u];
+    bad |= __ballot(!isfinite(v[u])) & inm[u];
+    if (lane == 0) {
+      sw[u] = bn;
+      sw[kPlaneWords + u] = bp;
+      sw[2 * kPlaneWords + u] = bc;
     }
   }
-  if (__ballot(bad) != 0 && lane == 0) atomicOr(ctr + kNonfinite, 1u);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < 3 * kPlaneWords) {
+    const int u = lane % kPlaneWords;
+    if (u < nw) {
+      uint64_t* dst = lane < kPlaneWords ? N : (lane < 2 * kPlaneWords ? P : C);
+      dst[q0 + u] = sw[lane];
+    }
+  }
+  if (bad != 0 && lane == 0) atomicOr(ctr + kNonfinite, 1u);
 }
 
 // Brick rows, in two steps. Thread (k, by, w) ORs word w of N and of P over the brick row's
