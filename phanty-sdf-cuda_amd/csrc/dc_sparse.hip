@@ -60,7 +60,7 @@ enum {
 };
 
 constexpr int kThreads = 256;
-constexpr int kPlaneWords = 8;  // sample words per wave in the plane pass
+constexpr int kPlaneWords = 16;  // sample words per wave in the plane pass
 constexpr int kCellRows = 8;    // cell rows (along y) per thread in the cell pass
 
 struct SpGrid {
