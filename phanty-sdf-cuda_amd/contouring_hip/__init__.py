@@ -23,4 +23,4 @@ from .backend import (  # noqa: F401
 )
 from . import synth  # noqa: F401
 from .io import load_obj, load_sdf, save_obj, save_sdf  # noqa: F401
-from .autotune import autotune_mark  # noqa: F401
+from .autotune import autotune_launch, autotune_mark  # noqa: F401

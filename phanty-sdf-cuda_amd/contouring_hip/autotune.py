@@ -1,4 +1,5 @@
-"""Autotuner for the marking sign pass's tile shape (sample rows per wave tile, waves per block).
+"""Autotuners: the marking sign pass's tile shape (sample rows per wave tile, waves per block), and
+the per-cell kernels' launch shapes (solve blocks per CU, gather rows per wave).
 
 The reference CUDA backend launches its marking kernel with one fixed block shape
 (src/cuda/dc_cuda.cu:687-692). Here the shape is picked per grid size by the library's own sweep
@@ -98,3 +99,107 @@ def autotune_mark(backend, grids, repeats=3, cache_path=None, use_cache=True):
         t[key] = winner
         _store(cache_path, t)
     return winner, table
+
+
+# Launch shapes of the per-cell kernels, chosen by whole timed steps (after the mark tile): the
+# solve kernels' blocks per CU (option qef_blocks: the FP64 solve's grid-stride loop, the FP32
+# solve's contiguous shares and twice as many for the FP64 fallback; 0: the library's default, 8
+# per CU), and, for the adjoint, sample rows per wave of the gradient gather. Neither changes the mesh; the gather's
+# row count may change the order of each sample's (at most 8) corner terms, within 1e-6 relative.
+QEF_BLOCKS_PER_CU = (0, 4, 16, 32)
+GATHER_ROWS = (1, 2, 4)
+
+
+def _cu_count(device):
+    try:
+        import torch
+        return int(torch.cuda.get_device_properties(device).multi_processor_count)
+    except Exception:
+        return 256
+
+
+def _median_step_ms(step, repeats, warmup=2, per=5):
+    import time
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    samples = []
+    for _ in range(max(repeats, 1)):
+        t0 = time.perf_counter()
+        for _ in range(per):
+            step()
+        torch.cuda.synchronize()
+        samples.append((time.perf_counter() - t0) * 1e3 / per)
+    samples.sort()
+    return samples[len(samples) // 2]
+
+
+def autotune_launch(backend, grid, repeats=5, adjoint=False, cache_path=None, use_cache=True):
+    """Pick the per-cell kernels' launch shapes for `grid` (a DenseSdfGridDevice whose values are a
+    CUDA tensor on the backend's device) by timing whole extractions (and, with adjoint=True,
+    extraction + adjoint steps) with the launch sequences replayed, and set them on `backend`.
+
+    Returns (best, table): best = {"qef_blocks", "gather_rows", "step_ms"} (qef_blocks in blocks of
+    256 threads, 0 = the library's default); table = every candidate timed (empty when the cached
+    choice was used). Per-pass timing is switched off while timing and restored afterwards.
+    """
+    import torch
+    from .backend import PRECISIONS
+    cache_path = cache_path or _default_cache()
+    key = "launch|" + _key(backend.device, [grid]) + f"|{backend.precision}|{int(bool(adjoint))}"
+    if use_cache:
+        hit = _load(cache_path).get(key)
+        if hit:
+            backend.set_option("qef_blocks", hit["qef_blocks"])
+            backend.set_option("gather_rows", hit["gather_rows"])
+            return hit, []
+    backend._use_torch_stream()
+    cg = backend._cgrid(grid, grid.d_values.data_ptr())
+    prec = PRECISIONS[backend.precision]
+    lib = backend._lib
+    st = _lib.DcHipStats()
+    bst = _lib.DcHipStats()
+    timing = backend.option("timing", 1)
+    backend.set_option("timing", 0)
+    state = {}
+
+    def fwd():
+        backend._check(lib.dc_hip_extract_device(backend._ctx, ctypes.byref(cg), prec,
+                                                 ctypes.byref(st)))
+
+    def step():
+        fwd()
+        if adjoint:
+            if "gv" not in state or state["gv"].shape[0] != int(st.vertex_count):
+                gen = torch.Generator(device=grid.d_values.device)
+                gen.manual_seed(0)
+                state["gv"] = torch.randn((max(int(st.vertex_count), 1), 3),
+                                          device=grid.d_values.device, generator=gen)
+                state["g"] = torch.empty_like(grid.d_values)
+            backend._check(lib.dc_hip_backward(
+                backend._ctx, ctypes.c_void_p(state["gv"].data_ptr()),
+                ctypes.c_void_p(state["g"].data_ptr()), ctypes.byref(bst)))
+
+    cu = _cu_count(backend.device)
+    table = []
+    try:
+        for per_cu in QEF_BLOCKS_PER_CU:
+            for rows in (GATHER_ROWS if adjoint else (1,)):
+                blocks = per_cu * cu
+                backend.set_option("qef_blocks", blocks)
+                backend.set_option("gather_rows", rows)
+                fwd()  # lays out the gather's row groups for this row count
+                table.append({"qef_blocks": blocks, "gather_rows": rows,
+                              "step_ms": _median_step_ms(step, repeats)})
+    finally:
+        backend.set_option("timing", timing)
+    table.sort(key=lambda c: c["step_ms"])
+    best = dict(table[0])
+    backend.set_option("qef_blocks", best["qef_blocks"])
+    backend.set_option("gather_rows", best["gather_rows"])
+    if use_cache:
+        t = _load(cache_path)
+        t[key] = best
+        _store(cache_path, t)
+    return best, table

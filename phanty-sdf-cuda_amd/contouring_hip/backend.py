@@ -184,12 +184,23 @@ class HipDualContouringBackend:
 
     def set_option(self, key, value):
         self._check(self._lib.dc_hip_set_option(self._ctx, key.encode(), int(value)))
+        self.__dict__.setdefault("_options", {})[key] = int(value)
+
+    def option(self, key, default=None):
+        """The value last set for `key` through set_option (else `default`)."""
+        return self.__dict__.get("_options", {}).get(key, default)
 
     def autotune(self, grids, **kw):
         """Time every mark tile shape on `grids` (device grids) and keep the fastest; see
         contouring_hip.autotune.autotune_mark."""
         from .autotune import autotune_mark
         return autotune_mark(self, grids, **kw)
+
+    def autotune_launch(self, grid, **kw):
+        """Time the per-cell kernels' launch shapes on `grid` (a device grid) by whole steps and
+        keep the fastest; see contouring_hip.autotune.autotune_launch."""
+        from .autotune import autotune_launch
+        return autotune_launch(self, grid, **kw)
 
     def set_stream(self, stream_handle):
         self._check(self._lib.dc_hip_set_stream(self._ctx, ctypes.c_void_p(stream_handle or 0)))
