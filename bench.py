@@ -82,7 +82,7 @@ def main():
                     help="c1: 64^3 spheres with oracle parity; "
                          "c2: the 512^3 CSG headline (default); c3: 1024^3 narrow band, plain vs "
                          "FP32+fallback; c4: 256 uneven grids, one batch vs sequential; c5: "
-                         "forward+adjoint at 128^3 and 256^3; m2s: mesh to SDF of the csg8 DC mesh; "
+                         "forward+adjoint at 128^3, 256^3 and 512^3; m2s: mesh to SDF of the csg8 DC mesh; "
                          "sparse: the sparse brick backends on csg8")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one elementwise kernel over the grid (known bytes: reads and writes "
@@ -586,9 +586,10 @@ def _fwd_bwd_ms(n, steps, warmup, options=None):
 
 
 def bench_c5(args):
-    """Forward + adjoint on 8-box CSG grids at 128^3 and 256^3 (loss = sum w_i v_i, w ~ N(0,1))."""
+    """Forward + adjoint on 8-box CSG grids at 128^3, 256^3 and 512^3 (loss = sum w_i v_i,
+    w ~ N(0,1))."""
     rows = {}
-    for n in (128, 256):
+    for n in (128, 256, 512):
         plain, both, nv = _fwd_bwd_ms(n, args.steps, args.warmup)
         rows[n] = {"plain_fwd_ms": plain, "fp32_fwd_bwd_ms": both, "ratio": both / plain,
                    "vertices": nv}
