@@ -84,8 +84,11 @@ void dc_hip_destroy(dc_hip_context* ctx);
  * call runs there and returns when its work is complete, except dc_hip_backward with the
  * "timing" option off, which returns once its work is enqueued (see there). */
 int dc_hip_set_stream(dc_hip_context* ctx, void* stream);
-/* Options: "mark_rows" (4, 8, 16, 32, default 16: sample rows per wave tile of the marking sign
- * pass), "mark_waves" (1, 2, 4, default 4: waves per marking block), "gather_rows" (1, 2, 4, 8:
+/* Options: "mark_rows" (the marking sign pass's wave tile: 4, 8, 16 or 32 sample rows of 64
+ * samples; 1: whole rows, 8 samples per lane, where every grid's nx is a multiple of 8 and its
+ * values are 16-byte aligned, else 16; 0, the default: whole rows where 16-row tiles would give
+ * fewer than 8 waves per CU, else 16), "mark_waves" (1, 2, 4, default 4: waves per marking
+ * block, for tiles of 4-32 rows), "gather_rows" (1, 2, 4, 8:
  * sample rows per wave of the adjoint's gradient gather; applies from the next forward),
  * "gather_blocks" (0, the default: one wave per row group of the gather; N > 0: at most N blocks
  * of 4 waves, each looping over row groups; the same gradient bit for bit), "timing" (0/1, default 1: per-pass
@@ -165,7 +168,8 @@ int dc_hip_backward(dc_hip_context* ctx, const float* grad_vertices, float* grad
 
 /* Tile-shape sweep of the marking pass (no counterpart in the reference, whose marking kernel has
  * one fixed launch shape, src/cuda/dc_cuda.cu:241-281 launched at :687-692). Every
- * (mark_rows, mark_waves) in {4, 8, 16, 32} x {1, 2, 4} is timed on these device-resident grids
+ * (mark_rows, mark_waves) in {4, 8, 16, 32} x {1, 2, 4} (the whole-row form is not a candidate) is
+ * timed on these device-resident grids
  * with the per-pass events (best of `repeats` runs after one warm-up run); the fastest is kept on
  * the context and returned. table_ms (12 floats, rows-major; may be NULL) receives every
  * candidate's marking time (sign and cell passes). The context's "timing" option is restored afterwards; the last

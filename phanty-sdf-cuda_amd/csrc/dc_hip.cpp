@@ -145,7 +145,10 @@ struct dc_hip_context {
   hipStream_t own = nullptr;     // captures sequences meant for the null stream
   hipStream_t stream = nullptr;
   std::string err;
-  int mark_rows = 16;  // default sign-pass tile: 16 sample rows x 4 waves per block (fastest at 512^3)
+  // sign-pass tile (option "mark_rows"): 0 picks per plan (see plan), kMarkWideRows whole rows,
+  // else that many 64-sample rows per wave (4 waves per block by default)
+  int mark_rows = 0;
+  int eff_mark_rows = 16;  // the plan's choice
   int mark_waves = 4;
   int gather_rows = 1;
   int64_t gather_blocks = 0;  // option "gather_blocks"
@@ -245,6 +248,21 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     for (int q = 0; q < n; ++q) max_ws = std::max(max_ws, std::min((grids[q].nx + 63) / 64, 16));
     c->gather_lds_row = 64 * max_ws + 64;
     c->eff_gather_rows = c->gather_rows;
+    // The sign pass: the whole-row form reads 8 samples per lane as two 16-byte loads (rows of
+    // 8k samples, 16-byte aligned values). By default it is used where the 16-row form would
+    // have fewer than 8 waves per CU (below ~512^3: at 64^3-256^3 it takes the sign pass from
+    // 43-46 us to 13-16 us, at 384^3 the two are equal, at 512^3 and 1024^3 the 16-row form is
+    // 16% and 7% faster).
+    bool wide_ok = true;
+    int64_t t16 = 0;
+    for (int q = 0; q < n; ++q) {
+      const dc_hip_grid& s = grids[q];
+      wide_ok = wide_ok && s.nx % 8 == 0 && ((uintptr_t)s.values & 15) == 0;
+      t16 += (int64_t)((s.nx + 63) / 64) * ((s.ny + 15) / 16) * ((s.nz + 63) / 64);
+    }
+    int rows = c->mark_rows;
+    if (rows == 0) rows = t16 < 8ll * c->cu_count ? kMarkWideRows : 16;
+    c->eff_mark_rows = rows == kMarkWideRows && !wide_ok ? 16 : rows;
   }
   c->hgrids.resize(n);
   c->tile_b.assign(n + 1, 0);
@@ -299,7 +317,7 @@ int plan(dc_hip_context* c, const dc_hip_grid* grids, int n) {
     words += (int64_t)g.W * g.cy * g.cz;
     ftw += 3ll * g.Wk * g.cy * g.cx;
     pen += 3ll * g.cy * g.cx;
-    tiles += dc_mark_tiles(g, c->mark_rows);
+    tiles += dc_mark_tiles(g, c->eff_mark_rows);
     cw += dc_cell_waves(g);
     sgn += (int64_t)g.Wks * g.ny * 64 * g.Ws;
     samp += (int64_t)g.nx * g.ny * g.nz;
@@ -547,7 +565,7 @@ int enqueue(dc_hip_context* c, int precision, hipStream_t s, const Bufs& b) {
   ma.n_grids = n;
   ma.tile_bounds = d_tile_b;
   ma.n_tiles = c->n_tiles;
-  ma.rows = c->mark_rows;
+  ma.rows = c->eff_mark_rows;
   ma.waves_per_block = c->mark_waves;
   ma.sg = b.sg;
   ma.cw_bounds = d_tile_b + 6 * (n + 1);
@@ -855,8 +873,10 @@ int dc_hip_set_option(dc_hip_context* c, const char* key, int64_t value) {
   if (!c || !key) return DC_HIP_ERR_INVALID_ARG;
   const std::string k(key);
   if (k == "mark_rows") {
-    if (value != 4 && value != 8 && value != 16 && value != 32)
-      return fail(c, DC_HIP_ERR_INVALID_ARG, "mark_rows must be 4, 8, 16 or 32");
+    if (value != 0 && value != kMarkWideRows && value != 4 && value != 8 && value != 16 &&
+        value != 32)
+      return fail(c, DC_HIP_ERR_INVALID_ARG,
+                  "mark_rows must be 0 (by grid size), 1 (whole rows), 4, 8, 16 or 32");
     c->mark_rows = (int)value;
   } else if (k == "mark_waves") {
     if (value != 1 && value != 2 && value != 4)

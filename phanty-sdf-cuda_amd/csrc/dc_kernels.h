@@ -159,6 +159,10 @@ hipError_t dc_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s)
 hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, int n,
                                   const uint32_t* wprefix, const uint32_t* pbase, uint32_t* out,
                                   hipStream_t s);
+// Sign-pass tile rows: 4, 8, 16 or 32 sample rows of 64 samples per wave (k_mark), or
+// kMarkWideRows: one 512-sample row segment per wave with 8 samples per lane (k_mark_wide; grids
+// with nx % 8 == 0 and 16-byte aligned values).
+constexpr int kMarkWideRows = 1;
 int64_t dc_mark_tiles(const DcGridDev& g, int rows);  // sign-pass wave tiles of a grid
 int64_t dc_cell_waves(const DcGridDev& g);            // cell-pass waves of a grid
 // k_mark, the sign pass (sign planes); then dc_launch_mark_finish, the cell pass: the active plane

@@ -161,6 +161,91 @@ __global__ void __launch_bounds__(256)
   if (__ballot(badacc != badacc) && lane == 0) atomicOr(&ctr->nonfinite, 1u);
 }
 
+// The sign pass over whole rows (grids with nx % 8 == 0 and 16-byte aligned rows): lane = 8
+// consecutive samples along x, read as two 16-byte loads per row and layer, so a wave's load
+// covers 1 KiB of one row; one wave per (512-sample row segment, sample row, block of 64 layers),
+// kMarkWideDepth layers in flight. The sign entries are k_mark's, bit for bit (pencils past nx
+// are not written: no cell reads them).
+constexpr int kMarkWideDepth = 8;
+
+__global__ void __launch_bounds__(256)
+    k_mark_wide(const DcGridDev* __restrict__ grids, int n_grids,
+                const int64_t* __restrict__ tile_bounds, int64_t n_tiles, uint4* __restrict__ sg,
+                DcCounters* __restrict__ ctr) {
+  constexpr int D = kMarkWideDepth;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpb = blockDim.x / kWave;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t tile = blk * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (tile >= n_tiles) return;  // uniform per wave
+  const int g = find_bucket(tile_bounds, n_grids, tile);
+  const DcGridDev G = grids[g];
+  int64_t lt = tile - tile_bounds[g];
+  const int nseg = (G.nx + 511) / 512;
+  const int xs = (int)(lt % nseg);
+  lt /= nseg;
+  const int j = (int)(lt % G.ny);
+  const int kb = (int)(lt / G.ny);
+  const int nl = min(64, G.nz - 64 * kb);  // sample layers of this block
+  const int64_t lfloats = (int64_t)G.nx * G.ny;
+  const uint32_t lbytes = (uint32_t)(lfloats * 4);  // nx * ny < 2^30 - 64 (checked on the host)
+  const float* base = G.val + (int64_t)64 * kb * lfloats;
+  const int x0 = 512 * xs + 8 * lane;
+  // lanes past the row read the next row (pencils no cell uses) or, past the layer's end, 0
+  const uint32_t voff = ((uint32_t)j * (uint32_t)G.nx + (uint32_t)x0) * 4u;
+  auto load_layer = [&](int t, float (&v)[8]) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(base + (int64_t)t * lfloats), (short)0, t < nl ? (int)lbytes : 0, 0x00020000);
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, 0, 0);
+    const u4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff + 16, 0, 0);
+    v[0] = __uint_as_float(a.x);
+    v[1] = __uint_as_float(a.y);
+    v[2] = __uint_as_float(a.z);
+    v[3] = __uint_as_float(a.w);
+    v[4] = __uint_as_float(b.x);
+    v[5] = __uint_as_float(b.y);
+    v[6] = __uint_as_float(b.z);
+    v[7] = __uint_as_float(b.w);
+  };
+  float buf[D][8];
+  uint32_t nlo[8], nhi[8], plo[8], phi[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) nlo[q] = nhi[q] = plo[q] = phi[q] = 0u;
+  float badacc = 0.f;  // becomes NaN once a NaN or Inf sample has been read
+  auto consume = [&](const float (&v)[8], uint32_t (&nm)[8], uint32_t (&pm)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      badacc = __builtin_fmaf(v[q], 0.f, badacc);
+      shift_in_sign(nm[q], pm[q], v[q]);
+    }
+  };
+  static_for<D>([&](auto dc) { load_layer(decltype(dc)::value, buf[decltype(dc)::value]); });
+  for (int t0 = 0; t0 < 32; t0 += D) {
+    static_for<D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      consume(buf[d], nlo, plo);
+      load_layer(t0 + d + D, buf[d]);
+    });
+  }
+  for (int t0 = 32; t0 < 64 - D; t0 += D) {
+    static_for<D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      consume(buf[d], nhi, phi);
+      load_layer(t0 + d + D, buf[d]);
+    });
+  }
+  static_for<D>([&](auto dc) { consume(buf[decltype(dc)::value], nhi, phi); });
+  const int64_t nxp = 64 * (int64_t)G.Ws;
+  uint4* out = sg + G.sg_off + ((int64_t)kb * G.ny + j) * nxp + x0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (x0 + q < G.nx)
+      out[q] = make_uint4(__builtin_bitreverse32(nlo[q]), __builtin_bitreverse32(nhi[q]),
+                          __builtin_bitreverse32(plo[q]), __builtin_bitreverse32(phi[q]));
+  if (__ballot(badacc != badacc) && lane == 0) atomicOr(&ctr->nonfinite, 1u);
+}
+
 // 64x64 bit-matrix transpose across a wave: lane l holds row l (bit b = column b) in, column l
 // out (six block-swap steps).
 __device__ __forceinline__ uint64_t transpose64(uint64_t x) {
@@ -582,6 +667,7 @@ hipError_t dc_launch_grid_offsets(const int64_t* act_b, const int64_t* pen_b, in
 }
 
 int64_t dc_mark_tiles(const DcGridDev& g, int rows) {
+  if (rows == kMarkWideRows) return (int64_t)((g.nx + 511) / 512) * g.ny * g.Wks;
   return (int64_t)g.Ws * ((g.ny + rows - 1) / rows) * g.Wks;
 }
 
@@ -597,6 +683,10 @@ hipError_t dc_launch_mark(const DcMarkArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mark<R>, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds, a.n_tiles,  \
                      a.sg, a.ctr)
   switch (a.rows) {
+    case kMarkWideRows:
+      hipLaunchKernelGGL(k_mark_wide, grid, block, 0, s, a.grids, a.n_grids, a.tile_bounds,
+                         a.n_tiles, a.sg, a.ctr);
+      break;
     case 4: DC_MARK(4); break;
     case 8: DC_MARK(8); break;
     case 16: DC_MARK(16); break;

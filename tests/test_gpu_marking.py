@@ -5,7 +5,7 @@ next block's bit 0 for a cell's upper layer and the next word's pencil for its x
 These grids put the surface across those boundaries: nz = 65 (the next block holds one layer),
 nx = 65 and 129 (the last cell's x + 1 corner is the first pencil of the next sign word), and
 ny values that are not multiples of the row tile. Every sign-pass tile shape must give the same
-mesh.
+mesh, and so must the whole-row form of the sign pass (8 samples per lane, rows of 8k samples).
 """
 import numpy as np
 import pytest
@@ -58,3 +58,26 @@ def test_every_tile_shape_gives_the_same_mesh(dims):
             else:
                 assert torch.equal(V, ref[0]) and torch.equal(F, ref[1])
                 assert torch.equal(V64, ref[2])
+
+
+# rows of 8k samples: the whole-row sign pass applies (one or two 512-sample row segments)
+WIDE_GRIDS = [(72, 17, 65, 0.4), (520, 9, 66, 0.5), (64, 64, 130, 0.4), (8, 3, 70, 0.6)]
+
+
+@pytest.mark.parametrize("dims", WIDE_GRIDS)
+def test_whole_row_sign_pass_matches_oracle_and_tiles(dims):
+    import torch
+    g = ragged_sphere(*dims[:3], r=dims[3])
+    print(f"[marking] whole rows {g.name}", flush=True)
+    ref = oracle.run_grid(g)
+    got = {}
+    for rows in (1, 16, 0):
+        be = HipDualContouringBackend("plain-fp64", mark_rows=rows)
+        (V, F, V64), st = _extract(be, g)
+        assert st.vertex_count == ref.n_vertices and st.face_count == ref.n_faces, rows
+        got[rows] = (V, F, V64)
+    np.testing.assert_array_equal(got[1][1].cpu().numpy(), ref.F.reshape(-1, 4))
+    if ref.n_vertices:
+        np.testing.assert_array_equal(got[1][0].cpu().numpy(), ref.V32)
+    for rows in (16, 0):
+        assert all(torch.equal(a, b) for a, b in zip(got[rows], got[1])), rows
