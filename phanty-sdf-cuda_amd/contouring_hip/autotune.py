@@ -136,8 +136,8 @@ def _median_step_ms(step, repeats, warmup=2, per=5):
 
 
 def autotune_launch(backend, grid, repeats=5, adjoint=False, cache_path=None, use_cache=True):
-    """Pick the per-cell kernels' launch shapes for `grid` (a DenseSdfGridDevice whose values are a
-    CUDA tensor on the backend's device) by timing whole extractions (and, with adjoint=True,
+    """Pick the per-cell kernels' launch shapes for `grid` (a DenseSdfGridDevice: values as a
+    float32 CUDA tensor on the backend's device, or a device pointer) by timing whole extractions (and, with adjoint=True,
     extraction + adjoint steps) with the launch sequences replayed, and set them on `backend`.
 
     Returns (best, table): best = {"qef_blocks", "gather_rows", "step_ms"} (qef_blocks in blocks of
@@ -145,7 +145,7 @@ def autotune_launch(backend, grid, repeats=5, adjoint=False, cache_path=None, us
     choice was used). Per-pass timing is switched off while timing and restored afterwards.
     """
     import torch
-    from .backend import PRECISIONS
+    from .backend import PRECISIONS, _check_tensor, _ptr
     cache_path = cache_path or _default_cache()
     key = "launch|" + _key(backend.device, [grid]) + f"|{backend.precision}|{int(bool(adjoint))}"
     if use_cache:
@@ -154,8 +154,12 @@ def autotune_launch(backend, grid, repeats=5, adjoint=False, cache_path=None, us
             backend.set_option("qef_blocks", hit["qef_blocks"])
             backend.set_option("gather_rows", hit["gather_rows"])
             return hit, []
+    n_samples = int(grid.nx) * int(grid.ny) * int(grid.nz)
+    if not isinstance(grid.d_values, int):
+        _check_tensor(grid.d_values, "d_values", n_samples, backend.device)
     backend._use_torch_stream()
-    cg = backend._cgrid(grid, grid.d_values.data_ptr())
+    dev = torch.device("cuda", backend.device)
+    cg = backend._cgrid(grid, _ptr(grid.d_values))
     prec = PRECISIONS[backend.precision]
     lib = backend._lib
     st = _lib.DcHipStats()
@@ -172,11 +176,11 @@ def autotune_launch(backend, grid, repeats=5, adjoint=False, cache_path=None, us
         fwd()
         if adjoint:
             if "gv" not in state or state["gv"].shape[0] != int(st.vertex_count):
-                gen = torch.Generator(device=grid.d_values.device)
+                gen = torch.Generator(device=dev)
                 gen.manual_seed(0)
-                state["gv"] = torch.randn((max(int(st.vertex_count), 1), 3),
-                                          device=grid.d_values.device, generator=gen)
-                state["g"] = torch.empty_like(grid.d_values)
+                state["gv"] = torch.randn((max(int(st.vertex_count), 1), 3), device=dev,
+                                          generator=gen)
+                state["g"] = torch.empty(n_samples, dtype=torch.float32, device=dev)
             backend._check(lib.dc_hip_backward(
                 backend._ctx, ctypes.c_void_p(state["gv"].data_ptr()),
                 ctypes.c_void_p(state["g"].data_ptr()), ctypes.byref(bst)))

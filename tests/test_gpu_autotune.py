@@ -70,3 +70,8 @@ def test_autotune_launch_keeps_results_and_picks_fastest(tmp_path):
     assert rel <= 1e-6, rel
     best2, table2 = be.autotune_launch(dev, adjoint=True, cache_path=cache)
     assert table2 == [] and best2 == best
+    # a device pointer in place of the tensor, forward shapes only
+    dev_ptr = DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz,
+                                 d_values=phi.data_ptr())
+    best3, table3 = be.autotune_launch(dev_ptr, repeats=1, cache_path=str(tmp_path / "t3.json"))
+    assert len(table3) == 4 and best3["gather_rows"] == 1
