@@ -81,3 +81,22 @@ def test_whole_row_sign_pass_matches_oracle_and_tiles(dims):
         np.testing.assert_array_equal(got[1][0].cpu().numpy(), ref.V32)
     for rows in (16, 0):
         assert all(torch.equal(a, b) for a, b in zip(got[rows], got[1])), rows
+
+
+def test_whole_row_sign_pass_falls_back_for_unaligned_values():
+    """Values that start 4 bytes past a 16-byte boundary cannot take the whole-row form's 16-byte
+    loads: the 16-row form runs instead, with the oracle's mesh."""
+    import torch
+    g = ragged_sphere(72, 17, 65, r=0.4)
+    n = g.nx * g.ny * g.nz
+    base = torch.zeros(n + 4, dtype=torch.float32, device="cuda")
+    base[1:1 + n] = torch.from_numpy(np.ascontiguousarray(g.values, np.float32)).cuda()
+    view = base[1:1 + n]
+    assert view.data_ptr() % 16 == 4
+    be = HipDualContouringBackend("plain-fp64", mark_rows=1)
+    (V, F, V64), st = be.extract_tensor(view.reshape(g.nz, g.ny, g.nx), (g.vx, g.vy, g.vz),
+                                        (g.ox, g.oy, g.oz), with_f64=True)
+    ref = oracle.run_grid(g)
+    assert st.vertex_count == ref.n_vertices and st.face_count == ref.n_faces
+    np.testing.assert_array_equal(F.cpu().numpy(), ref.F.reshape(-1, 4))
+    np.testing.assert_array_equal(V.cpu().numpy(), ref.V32)
