@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dc_decode.h"
@@ -167,6 +168,8 @@ struct dc_hip_context {
   DcCounters* h_counters = nullptr;
   unsigned char* h_stage = nullptr;  // pinned: descriptors of the allocation-free entry point
   uint32_t* h_goff = nullptr;  // pinned: per-grid first vertex / first face of the last batch
+  char* h_ring[2] = {nullptr, nullptr};  // pinned: staging chunks of a large host-grid upload
+  hipEvent_t ring_ev[2] = {nullptr, nullptr};
   int h_goff_cap = 0;
   hipEvent_t ev[kNumEvents] = {};
   hipStream_t side = nullptr;    // second stream for independent work beside the main one
@@ -855,6 +858,10 @@ void dc_hip_destroy(dc_hip_context* c) {
   if (c->h_counters) hipHostFree(c->h_counters);
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->h_goff) hipHostFree(c->h_goff);
+  for (int q = 0; q < 2; ++q) {
+    if (c->h_ring[q]) hipHostFree(c->h_ring[q]);
+    if (c->ring_ev[q]) hipEventDestroy(c->ring_ev[q]);
+  }
   if (c->side) {
     hipStreamSynchronize(c->side);
     hipStreamDestroy(c->side);
@@ -922,6 +929,48 @@ int dc_hip_extract_device(dc_hip_context* c, const dc_hip_grid* grid, int precis
   return run_pipeline(c, grid, 1, precision, stats);
 }
 
+namespace {
+// A large host grid is uploaded through two pinned chunks: host threads copy chunk q into one
+// while the copy engine moves chunk q - 1 from the other, so the upload runs at the lower of the
+// two rates instead of the runtime's single-threaded staging of pageable memory.
+constexpr size_t kRingChunk = size_t(64) << 20;
+
+void copy_threads(char* dst, const char* src, size_t len) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned t = std::max(1u, std::min(8u, hw ? hw / 2 : 1u));
+  const size_t per = ((len + t - 1) / t + 63) & ~size_t(63);
+  std::vector<std::thread> pool;
+  for (unsigned q = 1; q < t && q * per < len; ++q)
+    pool.emplace_back([=] { std::memcpy(dst + q * per, src + q * per, std::min(per, len - q * per)); });
+  std::memcpy(dst, src, std::min(per, len));
+  for (auto& th : pool) th.join();
+}
+
+int upload_pipelined(dc_hip_context* c, const float* host, size_t bytes) {
+  for (int q = 0; q < 2; ++q) {
+    if (!c->h_ring[q] && hipHostMalloc((void**)&c->h_ring[q], kRingChunk, hipHostMallocDefault) !=
+                             hipSuccess) {
+      c->h_ring[q] = nullptr;
+      return fail(c, DC_HIP_ERR_OUT_OF_MEMORY, "pinned upload chunks");
+    }
+    if (!c->ring_ev[q])
+      DC_TRY(c, hipEventCreateWithFlags(&c->ring_ev[q], hipEventDisableTiming), "upload event");
+  }
+  const char* src = reinterpret_cast<const char*>(host);
+  char* dst = static_cast<char*>(c->upload.p);
+  size_t q = 0;
+  for (size_t off = 0; off < bytes; off += kRingChunk, ++q) {
+    const size_t len = std::min(kRingChunk, bytes - off);
+    if (q >= 2) DC_TRY(c, hipEventSynchronize(c->ring_ev[q & 1]), "upload");  // chunk q - 2 sent
+    copy_threads(c->h_ring[q & 1], src + off, len);
+    DC_TRY(c, hipMemcpyAsync(dst + off, c->h_ring[q & 1], len, hipMemcpyHostToDevice, c->stream),
+           "upload");
+    DC_TRY(c, hipEventRecord(c->ring_ev[q & 1], c->stream), "upload");
+  }
+  return DC_HIP_OK;
+}
+}  // namespace
+
 int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
                    dc_hip_stats* stats) {
   if (!c || !grid || !grid->values) return fail(c, DC_HIP_ERR_INVALID_ARG, "NULL argument");
@@ -937,8 +986,12 @@ int dc_hip_extract(dc_hip_context* c, const dc_hip_grid* grid, int precision,
   int rc = ensure(c, c->upload, bytes, "upload buffer", false);  // in the plan key as values
   if (rc) return rc;
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[8], c->stream), "event");
-  DC_TRY(c, hipMemcpyAsync(c->upload.p, grid->values, bytes, hipMemcpyHostToDevice, c->stream),
-         "upload");
+  if (bytes < 2 * kRingChunk) {
+    DC_TRY(c, hipMemcpyAsync(c->upload.p, grid->values, bytes, hipMemcpyHostToDevice, c->stream),
+           "upload");
+  } else if ((rc = upload_pipelined(c, grid->values, bytes))) {
+    return rc;
+  }
   if (c->timing) DC_TRY(c, hipEventRecord(c->ev[9], c->stream), "event");
   DC_TRY(c, hipStreamSynchronize(c->stream), "upload");
   dc_hip_grid dg = *grid;
