@@ -1286,8 +1286,28 @@ int run_sparse(dc_hip_context* c, const dc_hip_grid* grid, bool on_device, int32
   hipStream_t s;
   int rc;
   if ((rc = work_stream(c, &s))) return rc;
-  return c->sparse->run(*grid, on_device, brick_size, normal_mode, multi_vertex != 0, c->cu_count,
-                        s, stats, &c->err);
+  const bool sane = grid->values && grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2 &&
+                    grid->nx <= 65536 && grid->ny <= 65536 && grid->nz <= 65536;
+  const size_t bytes = sane ? (size_t)grid->nx * grid->ny * grid->nz * sizeof(float) : 0;
+  if (on_device || bytes < 2 * kRingChunk)
+    return c->sparse->run(*grid, on_device, brick_size, normal_mode, multi_vertex != 0,
+                          c->cu_count, s, stats, &c->err);
+  // a large host grid: the pipelined upload into the context's upload buffer, then the device path
+  const auto t0 = std::chrono::steady_clock::now();
+  if ((rc = ensure(c, c->upload, bytes, "upload buffer", false))) return rc;
+  if ((rc = upload_pipelined(c, grid->values, bytes))) return rc;
+  DC_TRY(c, hipStreamSynchronize(c->stream), "upload");
+  const float up_ms =
+      std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  dc_hip_grid dg = *grid;
+  dg.values = as<float>(c->upload);
+  rc = c->sparse->run(dg, true, brick_size, normal_mode, multi_vertex != 0, c->cu_count, s, stats,
+                      &c->err);
+  if (rc == DC_HIP_OK && stats) {
+    stats->upload_ms = up_ms;
+    stats->total_ms += up_ms;
+  }
+  return rc;
 }
 }  // namespace
 

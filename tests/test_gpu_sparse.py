@@ -147,3 +147,25 @@ def test_errors(backends):
     be.brick_size, be.normal_mode = 8, 0
     # the context still works afterwards
     _check(backends, g, False)
+
+
+@pytest.mark.timeout(240)
+def test_large_host_grid_matches_device_input(backends):
+    """A host grid of 128 MiB or more goes up through the pinned, pipelined upload (then the device
+    path): the same mesh and counters as the device-resident grid, bit for bit."""
+    import torch
+    g = synth.csg8(384, seed=1234)  # 226 MB of samples: four 64 MiB chunks
+    t = torch.from_numpy(np.asarray(g.values, np.float32).copy()).cuda()
+    dg = DenseSdfGridDevice(g.nx, g.ny, g.nz, g.vx, g.vy, g.vz, g.ox, g.oy, g.oz, t)
+    for mv in (False, True):
+        _say(f"[sparse] host vs device input, csg8 384^3 mv={mv}")
+        be = backends[mv]
+        be.brick_size, be.normal_mode = 8, 0
+        m_host, st_host = be.extract(_dense(g))
+        m_dev, st_dev = be.extract_device(dg)
+        assert np.array_equal(np.asarray(m_host.vertices).view(np.uint32),
+                              np.asarray(m_dev.vertices).view(np.uint32))
+        assert np.array_equal(np.asarray(m_host.faces), np.asarray(m_dev.faces))
+        assert st_host.vertex_count == st_dev.vertex_count > 0
+        assert st_host.face_count == st_dev.face_count
+        assert st_host.upload_ms > 0
