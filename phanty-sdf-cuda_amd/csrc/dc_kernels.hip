@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256)
 // ever waits on another, and no counter is shared by all blocks).
 // ---------------------------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
+constexpr int kScanItems = 8;  // 2048-word tiles: half the tile sums of 4 for the one-block scan
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kTileScanThreads = 1024;
 
